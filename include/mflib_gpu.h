@@ -1,0 +1,158 @@
+/*
+ * mflib_gpu.h -- C ABI of the MI355X matched-filter engine.
+ *
+ * This is the drop-in boundary: plain pointers, sizes and int status codes,
+ * no C++ or GPU-runtime types.  The C++ classes in include/mflib/ (the
+ * reference's public API) hold one mf_handle each; pymflib sits on those.
+ * Every entry point below names the reference interface it replaces.
+ *
+ * Status codes: MF_OK (0) or a negative MF_ERR_*.  After a failure,
+ * mf_last_error() returns a thread-local message.  The C++ facade maps
+ * MF_ERR_INVALID_ARGUMENT to std::invalid_argument and MF_ERR_BAD_STATE
+ * to std::runtime_error, as the reference throws them
+ * (src/matchedFilter.cpp:513-516, 829-834, 876-881, 977-980, 1322-1329).
+ *
+ * Threading: a handle is not thread-safe (neither is the reference's
+ * MatchedFilter); distinct handles are independent.  Each handle owns one
+ * HIP stream.  mf_apply() only enqueues work; every call that returns
+ * results to the host waits for it.
+ */
+#ifndef MFLIB_GPU_H
+#define MFLIB_GPU_H
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MF_OK 0
+#define MF_ERR_INVALID_ARGUMENT (-1)
+#define MF_ERR_BAD_STATE (-2)
+#define MF_ERR_OUT_OF_MEMORY (-3)
+#define MF_ERR_DEVICE (-4)
+#define MF_ERR_UNSUPPORTED (-5)
+
+/* Arithmetic type of signals and correlations: MatchedFilter<float> /
+ * MatchedFilter<double> (src/matchedFilter.cpp:1434-1435). */
+#define MF_PRECISION_FLOAT 0
+#define MF_PRECISION_DOUBLE 1
+
+/* MatchedFilterDetectionMode (include/mflib/enums.hpp:14-21), plus the
+ * batched layout in which several templates share one channel. */
+#define MF_MODE_SINGLE_CHANNEL 0
+#define MF_MODE_MULTI_CHANNEL 1
+#define MF_MODE_BATCHED 2
+
+typedef struct mf_handle mf_handle;
+
+/* Thread-local message for the last failed call on this thread. */
+const char *mf_last_error(void);
+
+/* Library version (include/mflib/version.hpp.in). */
+int mf_version(int *major, int *minor, int *patch);
+
+/* ---- instance lifecycle: MatchedFilter<T>::MatchedFilter / ~MatchedFilter
+ *      / clear (include/mflib/matchedFilter.hpp:27-41) ------------------- */
+int mf_create(mf_handle **h, int precision, int mode);
+int mf_destroy(mf_handle *h);
+/* MatchedFilter<T>::clear(): drops the plan, templates and device memory. */
+int mf_clear(mf_handle *h);
+
+/* ---- parameters: MatchedFilterParameters (include/mflib/matchedFilterParameters.hpp)
+ * setSignalSize (src/matchedFilterParameters.cpp:318-327). */
+int mf_set_signal_size(mf_handle *h, int n);
+/* addTemplate (src/matchedFilterParameters.cpp:199-219) for all templates
+ * at once, as initialize() reads them (src/matchedFilter.cpp:560-593).
+ * samples[i] holds lengths[i] FP64 samples (WaveformTemplate stores FP64).
+ * shifts/weights: the moveout nshift = int((travel-onset)*fs + 0.5) and
+ * the shift-and-stack weight of each template (src/matchedFilter.cpp:577-592);
+ * pass NULL shifts when any template lacks onset or travel time
+ * (shiftAndStack then fails as the reference's does, 1326-1329). */
+int mf_set_templates(mf_handle *h, int nTemplates, const int *lengths,
+                     const double *const *samples, const int *shifts,
+                     const double *weights);
+/* setStackAbsoluteValues (src/matchedFilterParameters.cpp:338-348). */
+int mf_set_stack_absolute_values(mf_handle *h, int absval);
+/* No reference counterpart (MF_MODE_BATCHED only): channel lengths and
+ * the channel / stacking group of every template row. */
+int mf_set_batch_layout(mf_handle *h, int nChannels, const int *channelLengths,
+                        const int *templateChannel, const int *templateGroup,
+                        int nGroups);
+/* FFT length: 0 lets the autotuner choose (the default); a power of two
+ * forces it.  The reference's MatchedFilter ignores setFFTLength and uses
+ * computeOptimalFFTAndBlockLength (src/matchedFilter.cpp:519-525). */
+int mf_set_fft_length(mf_handle *h, int nfft);
+
+/* MatchedFilter<T>::initialize (src/matchedFilter.cpp:506-821): template
+ * demean/normalize (FP64), plans, template spectra, device buffers. */
+int mf_initialize(mf_handle *h);
+int mf_is_initialized(const mf_handle *h, int *yes);
+/* getNumberOfTemplates / getSignalLength / getFilteredSignalLength
+ * (src/matchedFilter.cpp:1366-1402).  For MF_MODE_BATCHED the lengths are
+ * those of channel 0 / group 0. */
+int mf_get_number_of_templates(const mf_handle *h, int *n);
+int mf_get_signal_length(const mf_handle *h, int *n);
+int mf_get_filtered_length(const mf_handle *h, int row, int *n);
+/* The block plan of the channel that template row `row` reads. */
+int mf_get_plan(const mf_handle *h, int row, int *nfft, int *blockLength);
+
+/* setSignal(it, n, x) (src/matchedFilter.cpp:862-921): copies n host
+ * samples of the handle's precision into channel `channel` and applies
+ * the dead-trace rule |min| == |max| (894).  In single-channel mode the
+ * channel is 0 (src/singleChannel/matchedFilter.cpp:61-67). */
+int mf_set_signal(mf_handle *h, int channel, int n, const void *x);
+/* No reference counterpart: the same from device memory (already in HBM). */
+int mf_set_signal_device(mf_handle *h, int channel, int n, const void *dx);
+/* zeroSignal(it) (src/matchedFilter.cpp:823-845). */
+int mf_zero_signal(mf_handle *h, int channel);
+/* getSignalPointer (923-938) reads the caller-side copy the C++ facade
+ * keeps; this reads the device copy back. */
+int mf_get_signal(mf_handle *h, int channel, void *out, int cap);
+
+/* apply() (src/matchedFilter.cpp:971-1316): correlation + normalization
+ * of every template row.  Enqueues work on the handle's stream. */
+int mf_apply(mf_handle *h);
+int mf_synchronize(mf_handle *h);
+int mf_have_matched_filtered_signals(const mf_handle *h, int *yes);
+
+/* getMatchedFilteredSignal(it) (940-949): copies the row's
+ * getFilteredSignalLength() correlation samples to host memory. */
+int mf_get_rho(mf_handle *h, int row, void *out, int cap);
+/* No reference counterpart: device pointer of row `row` (valid until the
+ * next mf_initialize / mf_clear / mf_destroy). */
+int mf_get_rho_device(mf_handle *h, int row, const void **dptr);
+
+/* shiftAndStack() (src/matchedFilter.cpp:1318-1364) of stacking group
+ * `group` (0 for single/multi-channel).  *len receives the stack length
+ * n - nb + 1 - max(shift); out must hold that many values. */
+int mf_shift_and_stack(mf_handle *h, int group, void *out, int cap, int *len);
+/* No reference counterpart: stack into device memory. */
+int mf_shift_and_stack_device(mf_handle *h, int group, void *dout, int cap, int *len);
+
+/* getSpectraOfTemplate(it) (src/matchedFilter.cpp:1411-1431).  Writes
+ * nfft/2+1 complex values (re,im interleaved, handle precision) of the
+ * spectrum of the demeaned, unit-norm, time-reversed template, zero padded
+ * to the row's FFT length, in the reference's convention. */
+int mf_template_spectrum(mf_handle *h, int row, void *outInterleaved, int cap, int *len);
+
+/* PeakFinder<T>::apply (src/peakFinder.cpp:468-586) on n samples of the
+ * given precision: threshold and local-maximum pass on the GPU, then the
+ * greedy minimum-distance pruning.  Ties in value are kept in ascending
+ * index order.  idx must hold cap entries; *nPeaks receives the count. */
+int mf_peaks(const void *x, int n, int precision, double threshold, int minDist,
+             int *idx, int cap, int *nPeaks);
+/* No reference counterpart: the same on a device-resident signal. */
+int mf_peaks_device(const void *dx, int n, int precision, double threshold,
+                    int minDist, int *idx, int cap, int *nPeaks);
+
+/* computeOptimalFFTAndBlockLength(nb, nx)
+ * (include/mflib/matchedFilterParameters.hpp:208, src/matchedFilterParameters.cpp:371-385). */
+int mf_compute_optimal_fft_and_block_length(int nb, int nx, int *nfft, int *blockLength);
+
+/* No reference counterpart: per-apply kernel timings (ms) measured with
+ * HIP events on the handle's stream: correlation kernels and the stack
+ * kernel of the last mf_shift_and_stack*. */
+int mf_get_timings(mf_handle *h, double *corrMs, double *stackMs);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

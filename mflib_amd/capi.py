@@ -1,0 +1,282 @@
+"""ctypes binding of the C ABI in include/mflib_gpu.h (libmflib_gpu.so).
+
+This is the layer the GPU parity tests and bench.py drive.  It raises
+ValueError for MF_ERR_INVALID_ARGUMENT and RuntimeError for every other
+failure, mirroring the std::invalid_argument / std::runtime_error split of
+the reference.  There is no CPU fallback: loading fails loudly when the
+library is missing, and every compute call fails when no GPU is present.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import re
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "lib", "libmflib_gpu.so")
+HEADER_PATH = os.path.join(os.path.dirname(_HERE), "include", "mflib_gpu.h")
+
+MF_OK = 0
+MF_ERR_INVALID_ARGUMENT = -1
+MF_ERR_BAD_STATE = -2
+MF_ERR_OUT_OF_MEMORY = -3
+MF_ERR_DEVICE = -4
+MF_ERR_UNSUPPORTED = -5
+PRECISION = {np.dtype(np.float32): 0, np.dtype(np.float64): 1}
+MODES = {"single": 0, "multi": 1, "batched": 2}
+
+_lib = None
+_vp = ctypes.c_void_p
+_ip = ctypes.POINTER(ctypes.c_int)
+_dp = ctypes.POINTER(ctypes.c_double)
+
+_SIGNATURES = {
+    "mf_last_error": ([], ctypes.c_char_p),
+    "mf_version": ([_ip, _ip, _ip], ctypes.c_int),
+    "mf_create": ([ctypes.POINTER(_vp), ctypes.c_int, ctypes.c_int], ctypes.c_int),
+    "mf_destroy": ([_vp], ctypes.c_int),
+    "mf_clear": ([_vp], ctypes.c_int),
+    "mf_set_signal_size": ([_vp, ctypes.c_int], ctypes.c_int),
+    "mf_set_templates": ([_vp, ctypes.c_int, _ip, ctypes.POINTER(_dp), _ip, _dp], ctypes.c_int),
+    "mf_set_stack_absolute_values": ([_vp, ctypes.c_int], ctypes.c_int),
+    "mf_set_batch_layout": ([_vp, ctypes.c_int, _ip, _ip, _ip, ctypes.c_int], ctypes.c_int),
+    "mf_set_fft_length": ([_vp, ctypes.c_int], ctypes.c_int),
+    "mf_initialize": ([_vp], ctypes.c_int),
+    "mf_is_initialized": ([_vp, _ip], ctypes.c_int),
+    "mf_get_number_of_templates": ([_vp, _ip], ctypes.c_int),
+    "mf_get_signal_length": ([_vp, _ip], ctypes.c_int),
+    "mf_get_filtered_length": ([_vp, ctypes.c_int, _ip], ctypes.c_int),
+    "mf_get_plan": ([_vp, ctypes.c_int, _ip, _ip], ctypes.c_int),
+    "mf_set_signal": ([_vp, ctypes.c_int, ctypes.c_int, _vp], ctypes.c_int),
+    "mf_set_signal_device": ([_vp, ctypes.c_int, ctypes.c_int, _vp], ctypes.c_int),
+    "mf_zero_signal": ([_vp, ctypes.c_int], ctypes.c_int),
+    "mf_get_signal": ([_vp, ctypes.c_int, _vp, ctypes.c_int], ctypes.c_int),
+    "mf_apply": ([_vp], ctypes.c_int),
+    "mf_synchronize": ([_vp], ctypes.c_int),
+    "mf_have_matched_filtered_signals": ([_vp, _ip], ctypes.c_int),
+    "mf_get_rho": ([_vp, ctypes.c_int, _vp, ctypes.c_int], ctypes.c_int),
+    "mf_get_rho_device": ([_vp, ctypes.c_int, ctypes.POINTER(_vp)], ctypes.c_int),
+    "mf_shift_and_stack": ([_vp, ctypes.c_int, _vp, ctypes.c_int, _ip], ctypes.c_int),
+    "mf_shift_and_stack_device": ([_vp, ctypes.c_int, _vp, ctypes.c_int, _ip], ctypes.c_int),
+    "mf_template_spectrum": ([_vp, ctypes.c_int, _vp, ctypes.c_int, _ip], ctypes.c_int),
+    "mf_peaks": ([_vp, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_int, _ip,
+                  ctypes.c_int, _ip], ctypes.c_int),
+    "mf_peaks_device": ([_vp, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_int, _ip,
+                         ctypes.c_int, _ip], ctypes.c_int),
+    "mf_compute_optimal_fft_and_block_length": ([ctypes.c_int, ctypes.c_int, _ip, _ip], ctypes.c_int),
+    "mf_get_timings": ([_vp, _dp, _dp], ctypes.c_int),
+}
+
+
+def header_symbols() -> list:
+    """Every function declared in include/mflib_gpu.h."""
+    text = open(HEADER_PATH).read()
+    return sorted(set(re.findall(r"^\s*(?:const\s+)?\w+\s*\*?\s*(mf_\w+)\s*\(", text, re.M)))
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: run `python -m mflib_amd.build` "
+                               "(the engine has no CPU fallback)")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (args, res) in _SIGNATURES.items():
+            f = getattr(L, name)
+            f.argtypes = args
+            f.restype = res
+        _lib = L
+    return _lib
+
+
+def last_error() -> str:
+    return (lib().mf_last_error() or b"").decode()
+
+
+def check(rc: int, what: str = "") -> None:
+    if rc == MF_OK:
+        return
+    msg = f"{what}: {last_error()}" if what else last_error()
+    if rc == MF_ERR_INVALID_ARGUMENT:
+        raise ValueError(msg)
+    raise RuntimeError(f"[{rc}] {msg}")
+
+
+def compute_optimal_fft_and_block_length(nb: int, nx: int):
+    a, b = ctypes.c_int(), ctypes.c_int()
+    check(lib().mf_compute_optimal_fft_and_block_length(nb, nx, ctypes.byref(a), ctypes.byref(b)),
+          "computeOptimalFFTAndBlockLength")
+    return a.value, b.value
+
+
+def peaks(x, threshold: float, min_dist: int):
+    """PeakFinder on the GPU for a host numpy array (float32 or float64)."""
+    x = np.ascontiguousarray(x)
+    if x.dtype not in PRECISION:
+        x = x.astype(np.float64)
+    idx = np.empty(max(len(x), 1), dtype=np.int32)
+    n = ctypes.c_int()
+    check(lib().mf_peaks(x.ctypes.data_as(_vp), len(x), PRECISION[x.dtype], float(threshold),
+                         int(min_dist), idx.ctypes.data_as(_ip), len(idx), ctypes.byref(n)), "peaks")
+    return idx[:n.value].copy()
+
+
+def peaks_device(ptr: int, n: int, dtype, threshold: float, min_dist: int, cap: int | None = None):
+    cap = cap or max(n, 1)
+    idx = np.empty(cap, dtype=np.int32)
+    cnt = ctypes.c_int()
+    check(lib().mf_peaks_device(_vp(ptr), n, PRECISION[np.dtype(dtype)], float(threshold),
+                                int(min_dist), idx.ctypes.data_as(_ip), cap, ctypes.byref(cnt)),
+          "peaks_device")
+    return idx[:cnt.value].copy()
+
+
+class Handle:
+    """One engine instance (one mf_handle)."""
+
+    def __init__(self, dtype=np.float32, mode: str = "multi"):
+        self.dtype = np.dtype(dtype)
+        self._h = _vp()
+        check(lib().mf_create(ctypes.byref(self._h), PRECISION[self.dtype], MODES[mode]), "mf_create")
+
+    def close(self):
+        if self._h:
+            lib().mf_destroy(self._h)
+            self._h = _vp()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    # ---- configuration
+    def set_signal_size(self, n: int):
+        check(lib().mf_set_signal_size(self._h, int(n)), "set_signal_size")
+
+    def set_templates(self, templates, shifts=None, weights=None):
+        self._tmpl = [np.ascontiguousarray(t, dtype=np.float64) for t in templates]
+        nT = len(self._tmpl)
+        lens = np.array([len(t) for t in self._tmpl], dtype=np.int32)
+        ptrs = (_dp * nT)(*[t.ctypes.data_as(_dp) for t in self._tmpl])
+        sh = None if shifts is None else np.ascontiguousarray(shifts, dtype=np.int32)
+        w = None if weights is None else np.ascontiguousarray(weights, dtype=np.float64)
+        check(lib().mf_set_templates(self._h, nT, lens.ctypes.data_as(_ip), ptrs,
+                                     None if sh is None else sh.ctypes.data_as(_ip),
+                                     None if w is None else w.ctypes.data_as(_dp)), "set_templates")
+
+    def set_stack_absolute_values(self, absval: bool):
+        check(lib().mf_set_stack_absolute_values(self._h, 1 if absval else 0))
+
+    def set_batch_layout(self, channel_lengths, template_channel, template_group, n_groups: int):
+        cl = np.ascontiguousarray(channel_lengths, dtype=np.int32)
+        tc = np.ascontiguousarray(template_channel, dtype=np.int32)
+        tg = np.ascontiguousarray(template_group, dtype=np.int32)
+        check(lib().mf_set_batch_layout(self._h, len(cl), cl.ctypes.data_as(_ip), tc.ctypes.data_as(_ip),
+                                        tg.ctypes.data_as(_ip), int(n_groups)), "set_batch_layout")
+
+    def set_fft_length(self, nfft: int):
+        check(lib().mf_set_fft_length(self._h, int(nfft)), "set_fft_length")
+
+    def initialize(self):
+        check(lib().mf_initialize(self._h), "initialize")
+
+    # ---- queries
+    def _int(self, fn, *args):
+        v = ctypes.c_int()
+        check(fn(self._h, *args, ctypes.byref(v)))
+        return v.value
+
+    def is_initialized(self) -> bool:
+        return bool(self._int(lib().mf_is_initialized))
+
+    def number_of_templates(self) -> int:
+        return self._int(lib().mf_get_number_of_templates)
+
+    def signal_length(self) -> int:
+        return self._int(lib().mf_get_signal_length)
+
+    def filtered_length(self, row: int = 0) -> int:
+        return self._int(lib().mf_get_filtered_length, int(row))
+
+    def plan(self, row: int = 0):
+        a, b = ctypes.c_int(), ctypes.c_int()
+        check(lib().mf_get_plan(self._h, int(row), ctypes.byref(a), ctypes.byref(b)))
+        return a.value, b.value
+
+    # ---- signals
+    def set_signal(self, channel: int, x):
+        x = np.ascontiguousarray(x, dtype=self.dtype)
+        check(lib().mf_set_signal(self._h, int(channel), len(x), x.ctypes.data_as(_vp)), "set_signal")
+
+    def set_signal_device(self, channel: int, ptr: int, n: int):
+        check(lib().mf_set_signal_device(self._h, int(channel), int(n), _vp(ptr)), "set_signal_device")
+
+    def zero_signal(self, channel: int):
+        check(lib().mf_zero_signal(self._h, int(channel)), "zero_signal")
+
+    def get_signal(self, channel: int, n: int):
+        out = np.empty(n, dtype=self.dtype)
+        check(lib().mf_get_signal(self._h, int(channel), out.ctypes.data_as(_vp), n), "get_signal")
+        return out
+
+    # ---- compute
+    def apply(self):
+        check(lib().mf_apply(self._h), "apply")
+
+    def synchronize(self):
+        check(lib().mf_synchronize(self._h), "synchronize")
+
+    def have_matched_filtered_signals(self) -> bool:
+        return bool(self._int(lib().mf_have_matched_filtered_signals))
+
+    def rho(self, row: int):
+        n = self.filtered_length(row)
+        out = np.empty(n, dtype=self.dtype)
+        check(lib().mf_get_rho(self._h, int(row), out.ctypes.data_as(_vp), n), "get_rho")
+        return out
+
+    def rho_device_ptr(self, row: int) -> int:
+        p = _vp()
+        check(lib().mf_get_rho_device(self._h, int(row), ctypes.byref(p)))
+        return p.value
+
+    def shift_and_stack(self, group: int = 0):
+        ln = ctypes.c_int()
+        cap = self.filtered_length(0) + 1
+        out = np.empty(max(cap, 1), dtype=self.dtype)
+        # first call obtains the length when the group's rows are longer
+        rc = lib().mf_shift_and_stack(self._h, int(group), out.ctypes.data_as(_vp), len(out), ctypes.byref(ln))
+        if rc == MF_ERR_INVALID_ARGUMENT and ln.value > len(out):
+            out = np.empty(ln.value, dtype=self.dtype)
+            rc = lib().mf_shift_and_stack(self._h, int(group), out.ctypes.data_as(_vp), len(out),
+                                          ctypes.byref(ln))
+        check(rc, "shift_and_stack")
+        return out[:ln.value].copy()
+
+    def shift_and_stack_device(self, group: int, dptr: int, cap: int) -> int:
+        ln = ctypes.c_int()
+        check(lib().mf_shift_and_stack_device(self._h, int(group), _vp(dptr), int(cap), ctypes.byref(ln)),
+              "shift_and_stack_device")
+        return ln.value
+
+    def template_spectrum(self, row: int):
+        nfft, _ = self.plan(row)
+        out = np.empty(2 * (nfft // 2 + 1), dtype=self.dtype)
+        ln = ctypes.c_int()
+        check(lib().mf_template_spectrum(self._h, int(row), out.ctypes.data_as(_vp), nfft // 2 + 1,
+                                         ctypes.byref(ln)), "template_spectrum")
+        return out[0::2] + 1j * out[1::2]
+
+    def timings(self):
+        a, b = ctypes.c_double(), ctypes.c_double()
+        check(lib().mf_get_timings(self._h, ctypes.byref(a), ctypes.byref(b)))
+        return a.value, b.value

@@ -1,0 +1,1215 @@
+// Host side of the C ABI declared in include/mflib_gpu.h.
+//
+// A handle owns: the template rows (FP64 host copies), the channel layout,
+// one HIP stream, and after mf_initialize the device state:
+//   channels   [C][n_c]            samples in the handle precision
+//   templates  [rows][Lt]          FP64 demeaned unit-norm templates
+//   spectra    [rows][nfft/2+1]    conj(T^)/(2 nfft), handle precision
+//   rho        [rows][ldOut]       correlation rows (ldOut = 64-multiple)
+// plus the small descriptor tables the kernels read (lanes, rows, work
+// items) and the two-level twiddle tables of every FFT size in use.
+//
+// Work layout: a lane is (channel, FFT size); its blocks are the work
+// items.  Items are ordered so that all blocks of a lane land on one XCD
+// (workgroup i runs on XCD i mod 8), which keeps that channel's template
+// spectra resident in that XCD's L2.
+#include "mflib_gpu.h"
+#include "mf_kernels.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <new>
+#include <numeric>
+#include <stdexcept>
+#include <string>
+#include <utility>
+#include <vector>
+
+namespace {
+
+thread_local std::string g_lastError;
+
+struct MfError {
+    int code;
+    std::string msg;
+};
+
+[[noreturn]] void fail(int code, const std::string &msg) { throw MfError{code, msg}; }
+
+#define MF_HIP(expr)                                                                        \
+    do {                                                                                    \
+        hipError_t e_ = (expr);                                                             \
+        if (e_ != hipSuccess) {                                                             \
+            fail(e_ == hipErrorOutOfMemory ? MF_ERR_OUT_OF_MEMORY : MF_ERR_DEVICE,          \
+                 std::string(#expr) + " failed: " + hipGetErrorString(e_));                 \
+        }                                                                                   \
+    } while (0)
+
+template <class F>
+int guard(F &&f)
+{
+    try {
+        f();
+        return MF_OK;
+    } catch (const MfError &e) {
+        g_lastError = e.msg;
+        return e.code;
+    } catch (const std::bad_alloc &) {
+        g_lastError = "out of host memory";
+        return MF_ERR_OUT_OF_MEMORY;
+    } catch (const std::exception &e) {
+        g_lastError = e.what();
+        return MF_ERR_DEVICE;
+    } catch (...) {
+        g_lastError = "unknown failure";
+        return MF_ERR_DEVICE;
+    }
+}
+
+// RAII device allocation.
+struct DevBuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf &) = delete;
+    DevBuf &operator=(const DevBuf &) = delete;
+    DevBuf(DevBuf &&o) noexcept : p(o.p), bytes(o.bytes) { o.p = nullptr; o.bytes = 0; }
+    DevBuf &operator=(DevBuf &&o) noexcept
+    {
+        if (this != &o) { release(); p = o.p; bytes = o.bytes; o.p = nullptr; o.bytes = 0; }
+        return *this;
+    }
+    ~DevBuf() { release(); }
+    void release()
+    {
+        if (p) { (void)hipFree(p); }
+        p = nullptr;
+        bytes = 0;
+    }
+    void alloc(size_t nb)
+    {
+        release();
+        if (nb == 0) { return; }
+        hipError_t e = hipMalloc(&p, nb);
+        if (e != hipSuccess) {
+            p = nullptr;
+            fail(e == hipErrorOutOfMemory ? MF_ERR_OUT_OF_MEMORY : MF_ERR_DEVICE,
+                 "hipMalloc of " + std::to_string(nb) + " bytes failed: " + hipGetErrorString(e));
+        }
+        bytes = nb;
+    }
+    template <typename T>
+    T *as() const { return static_cast<T *>(p); }
+};
+
+size_t roundUp(size_t v, size_t m) { return (v + m - 1) / m * m; }
+
+// Same table and rule as computeWindowLength
+// (src/matchedFilterParameters.cpp:47-104, 371-385).
+const double kFftCost[21] = {18,       59,       138,      303,      660,     1441,    3150,
+                             6875,     14952,    32373,    69762,    149647,  319644,  680105,
+                             1441974,  3047619,  6422736,  13500637, 28311786, 59244791,
+                             59244791 * 2.09};
+
+int referenceWindowLength(int nb, int nx, int *nfft, int *L)
+{
+    if (nb < 1) { fail(MF_ERR_INVALID_ARGUMENT, "Filter length must be positive"); }
+    if (nx <= nb) {
+        fail(MF_ERR_INVALID_ARGUMENT, "Signal length = " + std::to_string(nx) +
+                                          " must exceed filter length = " + std::to_string(nb));
+    }
+    int first = -1;
+    for (int i = 0; i < 21; ++i) {
+        if ((2LL << i) > nb - 1) { first = i; break; }
+    }
+    if (first < 0) {
+        long long p = 1;
+        while (p < nx) { p <<= 1; }
+        if (p > 0x7fffffffLL) { fail(MF_ERR_INVALID_ARGUMENT, "Overflow error"); }
+        *nfft = static_cast<int>(p);
+        *L = static_cast<int>(p) - (nb - 1);
+        return 0;
+    }
+    double best = 1e308;
+    for (int i = first; i < 21; ++i) {
+        long long len = 2LL << i;
+        long long lt = len - (nb - 1);
+        double obj = std::ceil(nx / static_cast<double>(lt)) * kFftCost[i];
+        if (obj < best) {
+            best = obj;
+            *nfft = static_cast<int>(len);
+            *L = static_cast<int>(lt);
+        }
+    }
+    return 0;
+}
+
+// Two-level twiddle tables for W_{2M}: lo[i] = W^i (i < 64),
+// hi[j] = W^(64 j) (j < 2M/64), FP64 from cos/sin, rounded to T.
+template <typename T>
+std::vector<T> twiddleTable(int logM, bool high)
+{
+    const long long N = 2LL << logM;
+    const int count = high ? static_cast<int>(N / 64) : 64;
+    std::vector<T> v(2 * count);
+    for (int i = 0; i < count; ++i) {
+        const long long e = high ? 64LL * i : i;
+        const double ang = -2.0 * M_PI * static_cast<double>(e) / static_cast<double>(N);
+        v[2 * i] = static_cast<T>(std::cos(ang));
+        v[2 * i + 1] = static_cast<T>(std::sin(ang));
+    }
+    return v;
+}
+
+struct TwTables {
+    DevBuf lo, hi, lo64, hi64;
+};
+
+}  // namespace
+
+struct mf_handle {
+    int precision = MF_PRECISION_FLOAT;
+    int mode = MF_MODE_MULTI_CHANNEL;
+    size_t tsize = 4;
+    int signalSize = 0;
+    int absval = 1;
+    int userNfft = 0;
+    // templates
+    std::vector<std::vector<double>> tmpl;
+    bool haveShifts = false;
+    std::vector<int> shifts;
+    std::vector<double> weights;
+    // batched layout
+    bool haveLayout = false;
+    int nChannels = 0;
+    int nGroupsIn = 0;
+    std::vector<int> chLenIn, rowChIn, rowGroupIn;
+    // derived layout
+    struct Channel {
+        DevBuf buf;
+        long long n = 0;
+        bool dead = true;
+        bool set = false;
+    };
+    struct RowP {
+        int ch = 0, group = 0, Lt = 0;
+        int nOut = 0;
+        int lane = 0;
+        int logM = 0;
+        size_t specOff = 0;  // elements of complex T
+        size_t tmplOff = 0;  // doubles
+        float hmax = 0;
+    };
+    struct Lane {
+        int ch = 0, logM = 0, L = 0, nOut = 0;
+        std::vector<int> rows;  // sorted by Lt
+    };
+    struct Group {
+        std::vector<int> rows;
+        int nb = 0;
+        int nOut = 0;
+    };
+    std::vector<Channel> ch;
+    std::vector<RowP> rows;
+    std::vector<Lane> lanes;
+    std::vector<Group> groups;
+    size_t ldOut = 0;
+    bool initialized = false;
+    bool applied = false;
+    bool tablesDirty = true;
+    hipStream_t stream = nullptr;
+    hipEvent_t evA = nullptr, evB = nullptr, evC = nullptr, evD = nullptr;
+    double corrMs = 0, stackMs = 0;
+    // device state
+    DevBuf dRho, dSpec, dTmpl, dTmplRaw, dHmax;
+    DevBuf dSegs, dRowTab, dItems;
+    DevBuf dStackRows, dStackOut;
+    std::map<int, TwTables> tw;
+    struct Launch {
+        int logM, first, count;
+    };
+    std::vector<Launch> launches;
+    std::vector<mfk::Seg> segHost;
+    std::vector<int> rowTabIndex;  // row table position of each row
+
+    ~mf_handle()
+    {
+        if (evA) { (void)hipEventDestroy(evA); }
+        if (evB) { (void)hipEventDestroy(evB); }
+        if (evC) { (void)hipEventDestroy(evC); }
+        if (evD) { (void)hipEventDestroy(evD); }
+        if (stream) { (void)hipStreamSynchronize(stream); (void)hipStreamDestroy(stream); }
+    }
+};
+
+namespace {
+
+int maxLogM(int precision) { return precision == MF_PRECISION_FLOAT ? mfk::kMaxLogMFloat : mfk::kMaxLogMDouble; }
+
+void requireHandle(const mf_handle *h)
+{
+    if (h == nullptr) { fail(MF_ERR_INVALID_ARGUMENT, "handle is NULL"); }
+}
+
+void requireInit(const mf_handle *h)
+{
+    requireHandle(h);
+    if (!h->initialized) { fail(MF_ERR_BAD_STATE, "Class is not initialized"); }
+}
+
+void dropDevice(mf_handle *h)
+{
+    if (h->stream) { (void)hipStreamSynchronize(h->stream); }
+    h->ch.clear();
+    h->rows.clear();
+    h->lanes.clear();
+    h->groups.clear();
+    h->dRho.release();
+    h->dSpec.release();
+    h->dTmpl.release();
+    h->dTmplRaw.release();
+    h->dHmax.release();
+    h->dSegs.release();
+    h->dRowTab.release();
+    h->dItems.release();
+    h->dStackRows.release();
+    h->dStackOut.release();
+    h->tw.clear();
+    h->launches.clear();
+    h->segHost.clear();
+    h->rowTabIndex.clear();
+    h->initialized = false;
+    h->applied = false;
+}
+
+// Cost model of one lane for FFT size 2^(logM+1): blocks x (forward FFT +
+// one inverse FFT, product and epilogue per row), weighted by the LDS
+// occupancy of the kernel variant.  The measured autotuner (bench)
+// refines the choice through mf_set_fft_length.
+double laneCost(int logM, int nb, long long nOut, int nRows, int precision)
+{
+    const long long N = 2LL << logM;
+    const long long L = N - nb + 1;
+    if (L < 1) { return -1; }
+    const double blocks = std::ceil(static_cast<double>(nOut) / static_cast<double>(L));
+    const double M = static_cast<double>(N / 2);
+    static const double occF[] = {1.00, 1.00, 1.05, 1.20};  // logM 9..12, FP32
+    static const double occD[] = {1.00, 1.05, 1.20, 9.99};
+    const double occ = (precision == MF_PRECISION_FLOAT ? occF : occD)[logM - mfk::kMinLogM];
+    return blocks * (1.0 + 1.15 * nRows) * M * (logM + 1.5) * occ;
+}
+
+int chooseLogM(const mf_handle *h, int nb, long long nOut, int nRows)
+{
+    const int hiLog = maxLogM(h->precision);
+    if (h->userNfft > 0) {
+        int lg = 0;
+        while ((1 << (lg + 1)) < h->userNfft) { ++lg; }
+        if ((2 << lg) != h->userNfft || lg < mfk::kMinLogM || lg > hiLog) {
+            fail(MF_ERR_UNSUPPORTED, "FFT length " + std::to_string(h->userNfft) +
+                                         " is not a supported power of two in [" +
+                                         std::to_string(2 << mfk::kMinLogM) + ", " +
+                                         std::to_string(2 << hiLog) + "]");
+        }
+        if ((2LL << lg) - nb + 1 < 1) {
+            fail(MF_ERR_INVALID_ARGUMENT, "FFT length " + std::to_string(h->userNfft) +
+                                              " is shorter than the template length " +
+                                              std::to_string(nb));
+        }
+        return lg;
+    }
+    int best = -1;
+    double bestCost = 0;
+    for (int lg = mfk::kMinLogM; lg <= hiLog; ++lg) {
+        const long long N = 2LL << lg;
+        if (N - nb + 1 < N / 4 && lg < hiLog) { continue; }  // keep >= 25% useful outputs
+        const double c = laneCost(lg, nb, nOut, nRows, h->precision);
+        if (c < 0) { continue; }
+        if (best < 0 || c < bestCost) { best = lg; bestCost = c; }
+    }
+    if (best < 0) {
+        fail(MF_ERR_UNSUPPORTED, "template length " + std::to_string(nb) +
+                                     " exceeds the largest supported FFT block (" +
+                                     std::to_string(2 << hiLog) + " samples)");
+    }
+    return best;
+}
+
+void deriveLayout(mf_handle *h, std::vector<int> &rowCh, std::vector<int> &rowGroup,
+                  std::vector<long long> &chLen, int &nGroups)
+{
+    const int nT = static_cast<int>(h->tmpl.size());
+    if (h->mode == MF_MODE_BATCHED) {
+        if (!h->haveLayout) { fail(MF_ERR_INVALID_ARGUMENT, "batch layout not set"); }
+        if (static_cast<int>(h->rowChIn.size()) != nT) {
+            fail(MF_ERR_INVALID_ARGUMENT, "batch layout does not match the number of templates");
+        }
+        rowCh = h->rowChIn;
+        rowGroup = h->rowGroupIn;
+        chLen.assign(h->chLenIn.begin(), h->chLenIn.end());
+        nGroups = h->nGroupsIn;
+        return;
+    }
+    if (h->signalSize < 1) { fail(MF_ERR_INVALID_ARGUMENT, "Parameters are invalid: signal size not set"); }
+    rowGroup.assign(nT, 0);
+    nGroups = 1;
+    if (h->mode == MF_MODE_MULTI_CHANNEL) {
+        rowCh.resize(nT);
+        std::iota(rowCh.begin(), rowCh.end(), 0);
+        chLen.assign(nT, h->signalSize);
+    } else {
+        rowCh.assign(nT, 0);
+        chLen.assign(1, h->signalSize);
+    }
+}
+
+void initialize(mf_handle *h)
+{
+    dropDevice(h);
+    const int nT = static_cast<int>(h->tmpl.size());
+    if (nT < 1) { fail(MF_ERR_INVALID_ARGUMENT, "Parameters are invalid: no templates"); }
+    std::vector<int> rowCh, rowGroup;
+    std::vector<long long> chLen;
+    int nGroups = 0;
+    deriveLayout(h, rowCh, rowGroup, chLen, nGroups);
+    const int nC = static_cast<int>(chLen.size());
+
+    // groups: nb = max template length, nOut = min channel length - nb + 1
+    h->groups.assign(nGroups, {});
+    h->rows.assign(nT, {});
+    for (int r = 0; r < nT; ++r) {
+        auto &rp = h->rows[r];
+        rp.ch = rowCh[r];
+        rp.group = rowGroup[r];
+        rp.Lt = static_cast<int>(h->tmpl[r].size());
+        h->groups[rp.group].rows.push_back(r);
+        h->groups[rp.group].nb = std::max(h->groups[rp.group].nb, rp.Lt);
+    }
+    long long maxOut = 0;
+    for (int g = 0; g < nGroups; ++g) {
+        auto &gr = h->groups[g];
+        if (gr.rows.empty()) { fail(MF_ERR_INVALID_ARGUMENT, "group " + std::to_string(g) + " has no templates"); }
+        long long nmin = -1;
+        for (int r : gr.rows) { nmin = nmin < 0 ? chLen[rowCh[r]] : std::min(nmin, chLen[rowCh[r]]); }
+        // computeOptimalFFTAndBlockLength(nb, n + nb - 1) rejects n <= 1
+        if (nmin + gr.nb - 1 <= gr.nb) {
+            fail(MF_ERR_INVALID_ARGUMENT, "Signal length = " + std::to_string(nmin + gr.nb - 1) +
+                                              " must exceed filter length = " + std::to_string(gr.nb));
+        }
+        if (nmin < gr.nb) {
+            fail(MF_ERR_INVALID_ARGUMENT, "signal length " + std::to_string(nmin) +
+                                              " is shorter than the template length " +
+                                              std::to_string(gr.nb));
+        }
+        gr.nOut = static_cast<int>(nmin - gr.nb + 1);
+        maxOut = std::max<long long>(maxOut, gr.nOut);
+        for (int r : gr.rows) { h->rows[r].nOut = gr.nOut; }
+    }
+    // lanes: one per channel carrying rows
+    std::vector<std::vector<int>> chRows(nC);
+    for (int r = 0; r < nT; ++r) { chRows[rowCh[r]].push_back(r); }
+    for (int c = 0; c < nC; ++c) {
+        if (chRows[c].empty()) { continue; }
+        mf_handle::Lane ln;
+        ln.ch = c;
+        int nb = 0;
+        for (int r : chRows[c]) {
+            nb = std::max(nb, h->rows[r].Lt);
+            ln.nOut = std::max(ln.nOut, h->rows[r].nOut);
+        }
+        ln.logM = chooseLogM(h, nb, ln.nOut, static_cast<int>(chRows[c].size()));
+        ln.L = static_cast<int>((2LL << ln.logM) - nb + 1);
+        ln.rows = chRows[c];
+        std::stable_sort(ln.rows.begin(), ln.rows.end(),
+                         [&](int a, int b) { return h->rows[a].Lt < h->rows[b].Lt; });
+        for (int r : ln.rows) {
+            h->rows[r].lane = static_cast<int>(h->lanes.size());
+            h->rows[r].logM = ln.logM;
+        }
+        h->lanes.push_back(std::move(ln));
+    }
+
+    // device buffers
+    h->ldOut = roundUp(static_cast<size_t>(maxOut), 64);
+    h->dRho.alloc(h->ldOut * nT * h->tsize);
+    size_t specTotal = 0, tmplTotal = 0;
+    for (int r = 0; r < nT; ++r) {
+        auto &rp = h->rows[r];
+        rp.specOff = specTotal;
+        specTotal += roundUp((1u << rp.logM) + 1, 32);
+        rp.tmplOff = tmplTotal;
+        tmplTotal += roundUp(rp.Lt, 32);
+    }
+    h->dSpec.alloc(specTotal * 2 * h->tsize);
+    h->dTmpl.alloc(tmplTotal * sizeof(double));
+    h->dTmplRaw.alloc(tmplTotal * sizeof(double));
+    h->dHmax.alloc(nT * sizeof(float));
+    h->ch.clear();
+    h->ch.resize(nC);
+    for (int c = 0; c < nC; ++c) {
+        h->ch[c].n = chLen[c];
+        h->ch[c].buf.alloc(static_cast<size_t>(chLen[c]) * h->tsize);
+        MF_HIP(hipMemsetAsync(h->ch[c].buf.p, 0, h->ch[c].buf.bytes, h->stream));
+        h->ch[c].dead = true;
+        h->ch[c].set = false;
+    }
+    // twiddle tables
+    for (const auto &ln : h->lanes) {
+        if (h->tw.count(ln.logM)) { continue; }
+        TwTables &t = h->tw[ln.logM];
+        auto lo64 = twiddleTable<double>(ln.logM, false);
+        auto hi64 = twiddleTable<double>(ln.logM, true);
+        t.lo64.alloc(lo64.size() * sizeof(double));
+        t.hi64.alloc(hi64.size() * sizeof(double));
+        MF_HIP(hipMemcpy(t.lo64.p, lo64.data(), t.lo64.bytes, hipMemcpyHostToDevice));
+        MF_HIP(hipMemcpy(t.hi64.p, hi64.data(), t.hi64.bytes, hipMemcpyHostToDevice));
+        if (h->precision == MF_PRECISION_FLOAT) {
+            auto lo = twiddleTable<float>(ln.logM, false);
+            auto hi = twiddleTable<float>(ln.logM, true);
+            t.lo.alloc(lo.size() * sizeof(float));
+            t.hi.alloc(hi.size() * sizeof(float));
+            MF_HIP(hipMemcpy(t.lo.p, lo.data(), t.lo.bytes, hipMemcpyHostToDevice));
+            MF_HIP(hipMemcpy(t.hi.p, hi.data(), t.hi.bytes, hipMemcpyHostToDevice));
+        }
+    }
+    // templates: upload raw FP64, prepare + transform on the GPU
+    {
+        std::vector<double> raw(tmplTotal, 0.0);
+        for (int r = 0; r < nT; ++r) {
+            std::copy(h->tmpl[r].begin(), h->tmpl[r].end(), raw.begin() + h->rows[r].tmplOff);
+        }
+        MF_HIP(hipMemcpy(h->dTmplRaw.p, raw.data(), raw.size() * sizeof(double), hipMemcpyHostToDevice));
+        std::map<int, std::vector<mfk::TmplJob>> jobs;
+        for (int r = 0; r < nT; ++r) {
+            const auto &rp = h->rows[r];
+            mfk::TmplJob j;
+            j.t = h->dTmplRaw.as<double>() + rp.tmplOff;
+            j.Lt = rp.Lt;
+            j.tOut = h->dTmpl.as<double>() + rp.tmplOff;
+            j.H = static_cast<char *>(h->dSpec.p) + rp.specOff * 2 * h->tsize;
+            j.hmax = h->dHmax.as<float>() + r;
+            jobs[rp.logM].push_back(j);
+        }
+        for (auto &kv : jobs) {
+            DevBuf dj;
+            dj.alloc(kv.second.size() * sizeof(mfk::TmplJob));
+            MF_HIP(hipMemcpy(dj.p, kv.second.data(), dj.bytes, hipMemcpyHostToDevice));
+            const TwTables &t = h->tw.at(kv.first);
+            MF_HIP(mfk::launch_tmpl(h->precision, kv.first, dj.as<mfk::TmplJob>(),
+                                    static_cast<int>(kv.second.size()), t.lo64.p, t.hi64.p, h->stream));
+            MF_HIP(hipStreamSynchronize(h->stream));
+        }
+        std::vector<float> hm(nT);
+        MF_HIP(hipMemcpy(hm.data(), h->dHmax.p, nT * sizeof(float), hipMemcpyDeviceToHost));
+        for (int r = 0; r < nT; ++r) { h->rows[r].hmax = hm[r]; }
+    }
+    // row table (lane order), work items (XCD-aware order)
+    std::vector<mfk::Row> rowTab;
+    h->rowTabIndex.assign(nT, 0);
+    h->segHost.clear();
+    for (size_t li = 0; li < h->lanes.size(); ++li) {
+        const auto &ln = h->lanes[li];
+        mfk::Seg s{};
+        s.n = h->ch[ln.ch].n;
+        s.L = ln.L;
+        s.nOut = ln.nOut;
+        s.rowBegin = static_cast<int>(rowTab.size());
+        float hmax = 0;
+        for (int r : ln.rows) {
+            const auto &rp = h->rows[r];
+            mfk::Row row;
+            row.H = static_cast<char *>(h->dSpec.p) + rp.specOff * 2 * h->tsize;
+            row.tmpl = h->dTmpl.as<double>() + rp.tmplOff;
+            row.out = static_cast<char *>(h->dRho.p) + static_cast<size_t>(r) * h->ldOut * h->tsize;
+            row.nOut = rp.nOut;
+            row.Lt = rp.Lt;
+            h->rowTabIndex[r] = static_cast<int>(rowTab.size());
+            rowTab.push_back(row);
+            hmax = std::max(hmax, rp.hmax);
+        }
+        s.rowEnd = static_cast<int>(rowTab.size());
+        s.hmax = hmax;
+        s.x = h->ch[ln.ch].buf.p;
+        s.dead = 1;
+        h->segHost.push_back(s);
+    }
+    h->dRowTab.alloc(rowTab.size() * sizeof(mfk::Row));
+    MF_HIP(hipMemcpy(h->dRowTab.p, rowTab.data(), h->dRowTab.bytes, hipMemcpyHostToDevice));
+    h->dSegs.alloc(h->segHost.size() * sizeof(mfk::Seg));
+    std::vector<mfk::Item> items;
+    std::map<int, std::vector<int>> lanesByLog;
+    for (size_t li = 0; li < h->lanes.size(); ++li) { lanesByLog[h->lanes[li].logM].push_back(static_cast<int>(li)); }
+    constexpr int kXcd = 8;
+    for (auto &kv : lanesByLog) {
+        // assign lanes to XCD bins, largest first
+        std::vector<int> order = kv.second;
+        auto blocksOf = [&](int li) {
+            const auto &ln = h->lanes[li];
+            return static_cast<long long>((ln.nOut + ln.L - 1) / ln.L);
+        };
+        std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return blocksOf(a) > blocksOf(b); });
+        std::vector<std::vector<mfk::Item>> bins(kXcd);
+        std::vector<long long> load(kXcd, 0);
+        for (int li : order) {
+            int b = static_cast<int>(std::min_element(load.begin(), load.end()) - load.begin());
+            const long long nb = blocksOf(li);
+            for (long long k = 0; k < nb; ++k) { bins[b].push_back({li, static_cast<int>(k)}); }
+            load[b] += nb;
+        }
+        mf_handle::Launch la;
+        la.logM = kv.first;
+        la.first = static_cast<int>(items.size());
+        std::vector<size_t> pos(kXcd, 0);
+        bool any = true;
+        while (any) {
+            any = false;
+            for (int b = 0; b < kXcd; ++b) {
+                if (pos[b] < bins[b].size()) {
+                    items.push_back(bins[b][pos[b]++]);
+                    any = true;
+                }
+            }
+        }
+        la.count = static_cast<int>(items.size()) - la.first;
+        h->launches.push_back(la);
+    }
+    h->dItems.alloc(std::max<size_t>(items.size(), 1) * sizeof(mfk::Item));
+    if (!items.empty()) {
+        MF_HIP(hipMemcpy(h->dItems.p, items.data(), items.size() * sizeof(mfk::Item), hipMemcpyHostToDevice));
+    }
+    h->dStackRows.alloc(std::max<size_t>(nT, 1) * sizeof(mfk::StackRow));
+    h->dStackOut.alloc(std::max<size_t>(h->ldOut, 64) * h->tsize);
+    MF_HIP(hipStreamSynchronize(h->stream));
+    h->tablesDirty = true;
+    h->initialized = true;
+    h->applied = false;
+}
+
+template <typename T>
+bool isDead(const T *x, long long n)
+{
+    auto mm = std::minmax_element(x, x + n);
+    return std::abs(*mm.first) == std::abs(*mm.second);
+}
+
+void checkChannel(const mf_handle *h, int c)
+{
+    const int nC = static_cast<int>(h->ch.size());
+    if (c < 0 || c >= nC) {
+        fail(MF_ERR_INVALID_ARGUMENT, "it = " + std::to_string(c) + " must be in the range [0," +
+                                          std::to_string(nC - 1) + "]");
+    }
+}
+
+void checkRow(const mf_handle *h, int r)
+{
+    const int nT = static_cast<int>(h->rows.size());
+    if (r < 0 || r >= nT) {
+        fail(MF_ERR_INVALID_ARGUMENT, "it = " + std::to_string(r) + " must be in the range [0," +
+                                          std::to_string(nT - 1) + "]");
+    }
+}
+
+void checkSignalLength(const mf_handle *h, int c, int n)
+{
+    if (n != h->ch[c].n) {
+        fail(MF_ERR_INVALID_ARGUMENT, "Number of samples = " + std::to_string(n) + " must equal " +
+                                          std::to_string(h->ch[c].n));
+    }
+}
+
+void apply(mf_handle *h)
+{
+    requireInit(h);
+    h->applied = false;
+    if (h->tablesDirty) {
+        for (size_t li = 0; li < h->lanes.size(); ++li) {
+            h->segHost[li].dead = h->ch[h->lanes[li].ch].dead ? 1 : 0;
+        }
+        MF_HIP(hipMemcpyAsync(h->dSegs.p, h->segHost.data(), h->dSegs.bytes, hipMemcpyHostToDevice,
+                              h->stream));
+        MF_HIP(hipStreamSynchronize(h->stream));
+        h->tablesDirty = false;
+    }
+    MF_HIP(hipEventRecord(h->evA, h->stream));
+    for (const auto &la : h->launches) {
+        const TwTables &t = h->tw.at(la.logM);
+        const void *lo = h->precision == MF_PRECISION_FLOAT ? t.lo.p : t.lo64.p;
+        const void *hi = h->precision == MF_PRECISION_FLOAT ? t.hi.p : t.hi64.p;
+        MF_HIP(mfk::launch_corr(h->precision, la.logM, h->dItems.as<mfk::Item>() + la.first, la.count,
+                                h->dSegs.as<mfk::Seg>(), h->dRowTab.as<mfk::Row>(), lo, hi, h->stream));
+    }
+    MF_HIP(hipEventRecord(h->evB, h->stream));
+    h->applied = true;
+}
+
+template <typename T>
+void stackWeights(const mf_handle *h, const std::vector<int> &rows, std::vector<mfk::StackRow> &out,
+                  int &smax)
+{
+    T xnorm = 0;
+    smax = h->shifts[rows[0]];
+    for (int r : rows) {
+        smax = std::max(smax, h->shifts[r]);
+        xnorm = xnorm + static_cast<T>(h->weights[r]);
+    }
+    if (xnorm > 0) { xnorm = 1 / xnorm; }
+    out.clear();
+    for (int r : rows) {
+        mfk::StackRow sr{};
+        sr.rho = static_cast<const char *>(h->dRho.p) + static_cast<size_t>(r) * h->ldOut * h->tsize;
+        sr.shift = h->shifts[r];
+        sr.weight = static_cast<double>(static_cast<T>(xnorm * static_cast<T>(h->weights[r])));
+        out.push_back(sr);
+    }
+}
+
+void stack(mf_handle *h, int group, void *dOut, int cap, int *len)
+{
+    requireInit(h);
+    if (!h->applied) { fail(MF_ERR_BAD_STATE, "Matched filtered signals not yet computed"); }
+    if (!h->haveShifts) { fail(MF_ERR_BAD_STATE, "Templates missing info - can't shift"); }
+    if (group < 0 || group >= static_cast<int>(h->groups.size())) {
+        fail(MF_ERR_INVALID_ARGUMENT, "group " + std::to_string(group) + " out of range");
+    }
+    const auto &g = h->groups[group];
+    std::vector<mfk::StackRow> srows;
+    int smax = 0;
+    if (h->precision == MF_PRECISION_FLOAT) {
+        stackWeights<float>(h, g.rows, srows, smax);
+    } else {
+        stackWeights<double>(h, g.rows, srows, smax);
+    }
+    for (int r : g.rows) {
+        if (h->shifts[r] < 0) { fail(MF_ERR_INVALID_ARGUMENT, "negative moveout shifts are not supported"); }
+    }
+    const long long n = static_cast<long long>(g.nOut) - smax;
+    if (n < 0) {
+        fail(MF_ERR_INVALID_ARGUMENT, "maximum shift " + std::to_string(smax) +
+                                          " exceeds the filtered signal length " + std::to_string(g.nOut));
+    }
+    if (len) { *len = static_cast<int>(n); }
+    if (cap < n) { fail(MF_ERR_INVALID_ARGUMENT, "output holds " + std::to_string(cap) + " < " + std::to_string(n) + " samples"); }
+    MF_HIP(hipMemcpyAsync(h->dStackRows.p, srows.data(), srows.size() * sizeof(mfk::StackRow),
+                          hipMemcpyHostToDevice, h->stream));
+    MF_HIP(hipEventRecord(h->evC, h->stream));
+    MF_HIP(mfk::launch_stack(h->precision, h->dStackRows.as<mfk::StackRow>(),
+                             static_cast<int>(srows.size()), n, h->absval, dOut, h->stream));
+    MF_HIP(hipEventRecord(h->evD, h->stream));
+    MF_HIP(hipStreamSynchronize(h->stream));
+    float ms = 0;
+    if (hipEventElapsedTime(&ms, h->evC, h->evD) == hipSuccess) { h->stackMs = ms; }
+}
+
+// Peak finding: candidates on the GPU, pruning on the host.
+std::mutex g_peakMutex;
+
+template <typename T>
+void pruneCandidates(std::vector<std::pair<T, int>> &cand, int minDist, int *idx, int cap, int *nPeaks)
+{
+    std::sort(cand.begin(), cand.end(), [](const auto &a, const auto &b) { return a.second < b.second; });
+    std::vector<int> keep;
+    if (minDist > 0 && !cand.empty()) {
+        std::stable_sort(cand.begin(), cand.end(), [](const auto &a, const auto &b) { return a.first > b.first; });
+        keep.push_back(cand[0].second);
+        for (const auto &p : cand) {
+            int dmin = std::abs(keep[0] - p.second);
+            for (size_t i = 1; i < keep.size(); ++i) { dmin = std::min(dmin, std::abs(keep[i] - p.second)); }
+            if (dmin > minDist) { keep.push_back(p.second); }
+        }
+        std::sort(keep.begin(), keep.end());
+    } else {
+        for (const auto &p : cand) { keep.push_back(p.second); }
+    }
+    *nPeaks = static_cast<int>(keep.size());
+    if (static_cast<int>(keep.size()) > cap) {
+        fail(MF_ERR_INVALID_ARGUMENT, "maxx = " + std::to_string(cap) + " must be at least = " +
+                                          std::to_string(keep.size()));
+    }
+    std::copy(keep.begin(), keep.end(), idx);
+}
+
+template <typename T>
+void peaksDevice(const void *dx, long long n, double threshold, int minDist, int *idx, int cap, int *nPeaks,
+                 hipStream_t stream)
+{
+    const int candCap = static_cast<int>(std::min<long long>(n / 2 + 2, 0x7fffffffLL));
+    DevBuf dCount, dIdx, dVal;
+    dCount.alloc(sizeof(int));
+    dIdx.alloc(static_cast<size_t>(candCap) * sizeof(int));
+    dVal.alloc(static_cast<size_t>(candCap) * sizeof(T));
+    MF_HIP(hipMemsetAsync(dCount.p, 0, sizeof(int), stream));
+    MF_HIP(mfk::launch_peaks(sizeof(T) == 4 ? 0 : 1, dx, n, threshold, dCount.as<int>(), dIdx.as<int>(), dVal.p,
+                             candCap, stream));
+    int count = 0;
+    MF_HIP(hipMemcpyAsync(&count, dCount.p, sizeof(int), hipMemcpyDeviceToHost, stream));
+    MF_HIP(hipStreamSynchronize(stream));
+    if (count > candCap) { fail(MF_ERR_DEVICE, "peak candidate buffer overflow"); }
+    std::vector<int> ci(count);
+    std::vector<T> cv(count);
+    if (count > 0) {
+        MF_HIP(hipMemcpy(ci.data(), dIdx.p, count * sizeof(int), hipMemcpyDeviceToHost));
+        MF_HIP(hipMemcpy(cv.data(), dVal.p, count * sizeof(T), hipMemcpyDeviceToHost));
+    }
+    std::vector<std::pair<T, int>> cand(count);
+    for (int i = 0; i < count; ++i) { cand[i] = {cv[i], ci[i]}; }
+    pruneCandidates<T>(cand, minDist, idx, cap, nPeaks);
+}
+
+void peaks(const void *x, bool onDevice, int n, int precision, double threshold, int minDist, int *idx, int cap,
+           int *nPeaks)
+{
+    if (x == nullptr) { fail(MF_ERR_INVALID_ARGUMENT, "x is NULL"); }
+    if (n < 1) { fail(MF_ERR_INVALID_ARGUMENT, "No points in x"); }
+    if (nPeaks == nullptr || (idx == nullptr && cap > 0)) { fail(MF_ERR_INVALID_ARGUMENT, "peaks is NULL"); }
+    if (precision != MF_PRECISION_FLOAT && precision != MF_PRECISION_DOUBLE) {
+        fail(MF_ERR_INVALID_ARGUMENT, "unknown precision");
+    }
+    if (threshold < 0 || threshold > 1) {
+        fail(MF_ERR_INVALID_ARGUMENT, "Threshold = " + std::to_string(threshold) + " must be in the range [0,1]");
+    }
+    const size_t ts = precision == MF_PRECISION_FLOAT ? 4 : 8;
+    std::lock_guard<std::mutex> lock(g_peakMutex);
+    static hipStream_t stream = nullptr;
+    if (stream == nullptr) { MF_HIP(hipStreamCreate(&stream)); }
+    DevBuf staged;
+    const void *dx = x;
+    if (!onDevice) {
+        staged.alloc(static_cast<size_t>(n) * ts);
+        MF_HIP(hipMemcpyAsync(staged.p, x, static_cast<size_t>(n) * ts, hipMemcpyHostToDevice, stream));
+        dx = staged.p;
+    }
+    if (precision == MF_PRECISION_FLOAT) {
+        peaksDevice<float>(dx, n, static_cast<float>(threshold), minDist, idx, cap, nPeaks, stream);
+    } else {
+        peaksDevice<double>(dx, n, threshold, minDist, idx, cap, nPeaks, stream);
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *mf_last_error(void) { return g_lastError.c_str(); }
+
+int mf_version(int *major, int *minor, int *patch)
+{
+    return guard([&] {
+        if (major) { *major = 0; }
+        if (minor) { *minor = 2; }
+        if (patch) { *patch = 0; }
+    });
+}
+
+int mf_create(mf_handle **h, int precision, int mode)
+{
+    return guard([&] {
+        if (h == nullptr) { fail(MF_ERR_INVALID_ARGUMENT, "handle pointer is NULL"); }
+        *h = nullptr;
+        if (precision != MF_PRECISION_FLOAT && precision != MF_PRECISION_DOUBLE) {
+            fail(MF_ERR_INVALID_ARGUMENT, "unknown precision " + std::to_string(precision));
+        }
+        if (mode != MF_MODE_SINGLE_CHANNEL && mode != MF_MODE_MULTI_CHANNEL && mode != MF_MODE_BATCHED) {
+            fail(MF_ERR_INVALID_ARGUMENT, "unknown detection mode " + std::to_string(mode));
+        }
+        int ndev = 0;
+        if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) {
+            fail(MF_ERR_DEVICE, "no HIP device is available (the engine has no CPU fallback)");
+        }
+        auto *p = new mf_handle();
+        p->precision = precision;
+        p->mode = mode;
+        p->tsize = precision == MF_PRECISION_FLOAT ? 4 : 8;
+        hipError_t e = hipStreamCreate(&p->stream);
+        if (e == hipSuccess) { e = hipEventCreate(&p->evA); }
+        if (e == hipSuccess) { e = hipEventCreate(&p->evB); }
+        if (e == hipSuccess) { e = hipEventCreate(&p->evC); }
+        if (e == hipSuccess) { e = hipEventCreate(&p->evD); }
+        if (e != hipSuccess) {
+            delete p;
+            fail(MF_ERR_DEVICE, std::string("stream/event creation failed: ") + hipGetErrorString(e));
+        }
+        *h = p;
+    });
+}
+
+int mf_destroy(mf_handle *h)
+{
+    return guard([&] {
+        if (h == nullptr) { return; }
+        dropDevice(h);
+        delete h;
+    });
+}
+
+int mf_clear(mf_handle *h)
+{
+    return guard([&] {
+        requireHandle(h);
+        dropDevice(h);
+        h->tmpl.clear();
+        h->shifts.clear();
+        h->weights.clear();
+        h->haveShifts = false;
+        h->signalSize = 0;
+        h->haveLayout = false;
+    });
+}
+
+int mf_set_signal_size(mf_handle *h, int n)
+{
+    return guard([&] {
+        requireHandle(h);
+        if (n < 1) { fail(MF_ERR_INVALID_ARGUMENT, "Signal size must be positive"); }
+        h->signalSize = n;
+    });
+}
+
+int mf_set_templates(mf_handle *h, int nTemplates, const int *lengths, const double *const *samples,
+                     const int *shifts, const double *weights)
+{
+    return guard([&] {
+        requireHandle(h);
+        if (nTemplates < 1 || lengths == nullptr || samples == nullptr) {
+            fail(MF_ERR_INVALID_ARGUMENT, "Waveform not yet set on template");
+        }
+        std::vector<std::vector<double>> t(nTemplates);
+        for (int i = 0; i < nTemplates; ++i) {
+            if (lengths[i] < 1 || samples[i] == nullptr) { fail(MF_ERR_INVALID_ARGUMENT, "Waveform not yet set on template"); }
+            t[i].assign(samples[i], samples[i] + lengths[i]);
+        }
+        std::vector<double> w(nTemplates, 1.0);
+        if (weights) {
+            for (int i = 0; i < nTemplates; ++i) {
+                if (weights[i] < 0 || weights[i] > 1) {
+                    fail(MF_ERR_INVALID_ARGUMENT, "weight = " + std::to_string(weights[i]) + " must be in range [0,1]");
+                }
+                w[i] = weights[i];
+            }
+        }
+        h->tmpl = std::move(t);
+        h->weights = std::move(w);
+        h->haveShifts = shifts != nullptr;
+        h->shifts.assign(nTemplates, 0);
+        if (shifts) { std::copy(shifts, shifts + nTemplates, h->shifts.begin()); }
+    });
+}
+
+int mf_set_stack_absolute_values(mf_handle *h, int absval)
+{
+    return guard([&] {
+        requireHandle(h);
+        h->absval = absval ? 1 : 0;
+    });
+}
+
+int mf_set_batch_layout(mf_handle *h, int nChannels, const int *channelLengths, const int *templateChannel,
+                        const int *templateGroup, int nGroups)
+{
+    return guard([&] {
+        requireHandle(h);
+        if (h->mode != MF_MODE_BATCHED) { fail(MF_ERR_BAD_STATE, "batch layout needs MF_MODE_BATCHED"); }
+        const int nT = static_cast<int>(h->tmpl.size());
+        if (nT < 1) { fail(MF_ERR_BAD_STATE, "set the templates before the batch layout"); }
+        if (nChannels < 1 || nGroups < 1 || channelLengths == nullptr || templateChannel == nullptr ||
+            templateGroup == nullptr) {
+            fail(MF_ERR_INVALID_ARGUMENT, "incomplete batch layout");
+        }
+        for (int c = 0; c < nChannels; ++c) {
+            if (channelLengths[c] < 1) { fail(MF_ERR_INVALID_ARGUMENT, "channel lengths must be positive"); }
+        }
+        for (int r = 0; r < nT; ++r) {
+            if (templateChannel[r] < 0 || templateChannel[r] >= nChannels) {
+                fail(MF_ERR_INVALID_ARGUMENT, "template " + std::to_string(r) + " names channel " +
+                                                  std::to_string(templateChannel[r]) + " out of range");
+            }
+            if (templateGroup[r] < 0 || templateGroup[r] >= nGroups) {
+                fail(MF_ERR_INVALID_ARGUMENT, "template " + std::to_string(r) + " names group " +
+                                                  std::to_string(templateGroup[r]) + " out of range");
+            }
+        }
+        h->nChannels = nChannels;
+        h->nGroupsIn = nGroups;
+        h->chLenIn.assign(channelLengths, channelLengths + nChannels);
+        h->rowChIn.assign(templateChannel, templateChannel + nT);
+        h->rowGroupIn.assign(templateGroup, templateGroup + nT);
+        h->haveLayout = true;
+    });
+}
+
+int mf_set_fft_length(mf_handle *h, int nfft)
+{
+    return guard([&] {
+        requireHandle(h);
+        if (nfft < 0) { fail(MF_ERR_INVALID_ARGUMENT, "fftLength must be non-negative"); }
+        h->userNfft = nfft;
+    });
+}
+
+int mf_initialize(mf_handle *h)
+{
+    return guard([&] {
+        requireHandle(h);
+        initialize(h);
+    });
+}
+
+int mf_is_initialized(const mf_handle *h, int *yes)
+{
+    return guard([&] {
+        requireHandle(h);
+        if (yes) { *yes = h->initialized ? 1 : 0; }
+    });
+}
+
+int mf_get_number_of_templates(const mf_handle *h, int *n)
+{
+    return guard([&] {
+        requireInit(h);
+        if (n) { *n = static_cast<int>(h->rows.size()); }
+    });
+}
+
+int mf_get_signal_length(const mf_handle *h, int *n)
+{
+    return guard([&] {
+        requireInit(h);
+        if (n) { *n = static_cast<int>(h->ch[0].n); }
+    });
+}
+
+int mf_get_filtered_length(const mf_handle *h, int row, int *n)
+{
+    return guard([&] {
+        requireInit(h);
+        checkRow(h, row);
+        if (n) { *n = h->rows[row].nOut; }
+    });
+}
+
+int mf_get_plan(const mf_handle *h, int row, int *nfft, int *blockLength)
+{
+    return guard([&] {
+        requireInit(h);
+        checkRow(h, row);
+        const auto &ln = h->lanes[h->rows[row].lane];
+        if (nfft) { *nfft = 2 << ln.logM; }
+        if (blockLength) { *blockLength = ln.L; }
+    });
+}
+
+int mf_set_signal(mf_handle *h, int channel, int n, const void *x)
+{
+    return guard([&] {
+        requireInit(h);
+        h->applied = false;
+        checkChannel(h, channel);
+        checkSignalLength(h, channel, n);
+        if (x == nullptr) { fail(MF_ERR_INVALID_ARGUMENT, "signal is NULL"); }
+        auto &c = h->ch[channel];
+        const bool dead = h->precision == MF_PRECISION_FLOAT ? isDead(static_cast<const float *>(x), n)
+                                                             : isDead(static_cast<const double *>(x), n);
+        MF_HIP(hipMemcpyAsync(c.buf.p, x, static_cast<size_t>(n) * h->tsize, hipMemcpyHostToDevice, h->stream));
+        MF_HIP(hipStreamSynchronize(h->stream));
+        c.dead = dead;
+        c.set = true;
+        h->tablesDirty = true;
+    });
+}
+
+int mf_set_signal_device(mf_handle *h, int channel, int n, const void *dx)
+{
+    return guard([&] {
+        requireInit(h);
+        h->applied = false;
+        checkChannel(h, channel);
+        checkSignalLength(h, channel, n);
+        if (dx == nullptr) { fail(MF_ERR_INVALID_ARGUMENT, "signal is NULL"); }
+        auto &c = h->ch[channel];
+        MF_HIP(hipMemcpyAsync(c.buf.p, dx, static_cast<size_t>(n) * h->tsize, hipMemcpyDeviceToDevice, h->stream));
+        const int nb = static_cast<int>(std::min<long long>((n + 255) / 256, 1024));
+        DevBuf pmin, pmax;
+        pmin.alloc(nb * sizeof(double));
+        pmax.alloc(nb * sizeof(double));
+        MF_HIP(mfk::launch_minmax(h->precision, c.buf.p, n, pmin.as<double>(), pmax.as<double>(), nb, h->stream));
+        std::vector<double> lo(nb), hi(nb);
+        MF_HIP(hipMemcpyAsync(lo.data(), pmin.p, nb * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+        MF_HIP(hipMemcpyAsync(hi.data(), pmax.p, nb * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+        MF_HIP(hipStreamSynchronize(h->stream));
+        const double mn = *std::min_element(lo.begin(), lo.end());
+        const double mx = *std::max_element(hi.begin(), hi.end());
+        c.dead = std::abs(mn) == std::abs(mx);
+        c.set = true;
+        h->tablesDirty = true;
+    });
+}
+
+int mf_zero_signal(mf_handle *h, int channel)
+{
+    return guard([&] {
+        requireInit(h);
+        h->applied = false;
+        checkChannel(h, channel);
+        auto &c = h->ch[channel];
+        MF_HIP(hipMemsetAsync(c.buf.p, 0, c.buf.bytes, h->stream));
+        MF_HIP(hipStreamSynchronize(h->stream));
+        c.dead = true;
+        c.set = true;
+        h->tablesDirty = true;
+    });
+}
+
+int mf_get_signal(mf_handle *h, int channel, void *out, int cap)
+{
+    return guard([&] {
+        requireInit(h);
+        checkChannel(h, channel);
+        const auto &c = h->ch[channel];
+        if (out == nullptr || cap < c.n) { fail(MF_ERR_INVALID_ARGUMENT, "output buffer too small"); }
+        MF_HIP(hipMemcpyAsync(out, c.buf.p, static_cast<size_t>(c.n) * h->tsize, hipMemcpyDeviceToHost, h->stream));
+        MF_HIP(hipStreamSynchronize(h->stream));
+    });
+}
+
+int mf_apply(mf_handle *h)
+{
+    return guard([&] { apply(h); });
+}
+
+int mf_synchronize(mf_handle *h)
+{
+    return guard([&] {
+        requireHandle(h);
+        MF_HIP(hipStreamSynchronize(h->stream));
+        if (h->applied) {
+            float ms = 0;
+            if (hipEventElapsedTime(&ms, h->evA, h->evB) == hipSuccess) { h->corrMs = ms; }
+        }
+    });
+}
+
+int mf_have_matched_filtered_signals(const mf_handle *h, int *yes)
+{
+    return guard([&] {
+        requireHandle(h);
+        if (yes) { *yes = h->applied ? 1 : 0; }
+    });
+}
+
+int mf_get_rho(mf_handle *h, int row, void *out, int cap)
+{
+    return guard([&] {
+        requireInit(h);
+        checkRow(h, row);
+        if (!h->applied) { fail(MF_ERR_BAD_STATE, "Matched filtering not yet applied"); }
+        const int nOut = h->rows[row].nOut;
+        if (out == nullptr || cap < nOut) { fail(MF_ERR_INVALID_ARGUMENT, "output buffer too small"); }
+        const char *src = static_cast<const char *>(h->dRho.p) + static_cast<size_t>(row) * h->ldOut * h->tsize;
+        MF_HIP(hipMemcpyAsync(out, src, static_cast<size_t>(nOut) * h->tsize, hipMemcpyDeviceToHost, h->stream));
+        MF_HIP(hipStreamSynchronize(h->stream));
+    });
+}
+
+int mf_get_rho_device(mf_handle *h, int row, const void **dptr)
+{
+    return guard([&] {
+        requireInit(h);
+        checkRow(h, row);
+        if (dptr == nullptr) { fail(MF_ERR_INVALID_ARGUMENT, "dptr is NULL"); }
+        *dptr = static_cast<const char *>(h->dRho.p) + static_cast<size_t>(row) * h->ldOut * h->tsize;
+    });
+}
+
+int mf_shift_and_stack(mf_handle *h, int group, void *out, int cap, int *len)
+{
+    return guard([&] {
+        requireInit(h);
+        int n = 0;
+        stack(h, group, h->dStackOut.p, static_cast<int>(h->dStackOut.bytes / h->tsize), &n);
+        if (len) { *len = n; }
+        if (out == nullptr || cap < n) { fail(MF_ERR_INVALID_ARGUMENT, "output buffer too small"); }
+        MF_HIP(hipMemcpyAsync(out, h->dStackOut.p, static_cast<size_t>(n) * h->tsize, hipMemcpyDeviceToHost,
+                              h->stream));
+        MF_HIP(hipStreamSynchronize(h->stream));
+    });
+}
+
+int mf_shift_and_stack_device(mf_handle *h, int group, void *dout, int cap, int *len)
+{
+    return guard([&] {
+        requireInit(h);
+        if (dout == nullptr) { fail(MF_ERR_INVALID_ARGUMENT, "output is NULL"); }
+        stack(h, group, dout, cap, len);
+    });
+}
+
+int mf_template_spectrum(mf_handle *h, int row, void *outInterleaved, int cap, int *len)
+{
+    return guard([&] {
+        requireInit(h);
+        checkRow(h, row);
+        const auto &rp = h->rows[row];
+        const int M = 1 << rp.logM;
+        const int N = 2 * M;
+        if (len) { *len = M + 1; }
+        if (outInterleaved == nullptr || cap < M + 1) { fail(MF_ERR_INVALID_ARGUMENT, "output buffer too small"); }
+        std::vector<char> raw((M + 1) * 2 * h->tsize);
+        MF_HIP(hipMemcpy(raw.data(), static_cast<const char *>(h->dSpec.p) + rp.specOff * 2 * h->tsize, raw.size(),
+                         hipMemcpyDeviceToHost));
+        // B[k] = exp(-2 pi i k (Lt-1)/N) * 2N * H[k]  (spectrum of the reversed template)
+        for (int k = 0; k <= M; ++k) {
+            double hr, hi;
+            if (h->precision == MF_PRECISION_FLOAT) {
+                hr = reinterpret_cast<const float *>(raw.data())[2 * k];
+                hi = reinterpret_cast<const float *>(raw.data())[2 * k + 1];
+            } else {
+                hr = reinterpret_cast<const double *>(raw.data())[2 * k];
+                hi = reinterpret_cast<const double *>(raw.data())[2 * k + 1];
+            }
+            const double ang = -2.0 * M_PI * static_cast<double>((static_cast<long long>(k) * (rp.Lt - 1)) % N) / N;
+            const double c = std::cos(ang), s = std::sin(ang);
+            const double br = 2.0 * N * (hr * c - hi * s);
+            const double bi = 2.0 * N * (hr * s + hi * c);
+            if (h->precision == MF_PRECISION_FLOAT) {
+                static_cast<float *>(outInterleaved)[2 * k] = static_cast<float>(br);
+                static_cast<float *>(outInterleaved)[2 * k + 1] = static_cast<float>(bi);
+            } else {
+                static_cast<double *>(outInterleaved)[2 * k] = br;
+                static_cast<double *>(outInterleaved)[2 * k + 1] = bi;
+            }
+        }
+    });
+}
+
+int mf_peaks(const void *x, int n, int precision, double threshold, int minDist, int *idx, int cap, int *nPeaks)
+{
+    return guard([&] { peaks(x, false, n, precision, threshold, minDist, idx, cap, nPeaks); });
+}
+
+int mf_peaks_device(const void *dx, int n, int precision, double threshold, int minDist, int *idx, int cap,
+                    int *nPeaks)
+{
+    return guard([&] { peaks(dx, true, n, precision, threshold, minDist, idx, cap, nPeaks); });
+}
+
+int mf_compute_optimal_fft_and_block_length(int nb, int nx, int *nfft, int *blockLength)
+{
+    return guard([&] {
+        if (nfft == nullptr || blockLength == nullptr) { fail(MF_ERR_INVALID_ARGUMENT, "output is NULL"); }
+        referenceWindowLength(nb, nx, nfft, blockLength);
+    });
+}
+
+int mf_get_timings(mf_handle *h, double *corrMs, double *stackMs)
+{
+    return guard([&] {
+        requireHandle(h);
+        if (corrMs) { *corrMs = h->corrMs; }
+        if (stackMs) { *stackMs = h->stackMs; }
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,530 @@
+// Matched-filter kernels for gfx950 (MI355X, CDNA4, wave64).
+//
+// corr_kernel  -- the hot path.  One block = one FFT block of one channel
+//   lane; it serves every template row on that channel:
+//     1. load nfft samples (coalesced), FP64 block anchor (window mean),
+//        FP64 16-sample tile sums of (x-a) and (x-a)^2 via wave shuffles;
+//     2. forward real FFT of the anchored block (LDS Stockham), unpacked
+//        into X' = 2 X (kept in LDS for all rows);
+//     3. per template length: FP64 sliding window sums -> per-output
+//        scale sqrt(Lt)/sqrt(Lt*S2 - S1^2), with ill-conditioned windows
+//        flagged;
+//     4. per row: spectral product with the resident template spectrum
+//        (read from L2), inverse packing, inverse FFT, normalisation and
+//        the tooBig rule in the epilogue, one coalesced store of rho;
+//        flagged windows are recomputed by an exact FP64 two-pass Pearson
+//        from the LDS copy of the samples.
+//   Blocks are overlap-save: block b covers outputs [bL, bL+L) and reads
+//   samples [bL, bL+nfft), so every output is complete inside one block
+//   and no cross-block accumulation is needed.  Subtracting the anchor is
+//   exact because the template is zero-mean.
+//
+// tmpl_kernel  -- template demean/normalise and spectrum, FP64.
+// stack_kernel -- moveout-shifted weighted stack of one group.
+// peak_kernel  -- threshold + local-maximum candidates.
+// minmax_kernel -- per-block min/max for the dead-trace test.
+#include "mf_device.hpp"
+#include "mf_kernels.hpp"
+#include <cfloat>
+
+using namespace mfdev;
+
+namespace mfk {
+
+namespace {
+
+constexpr int NT = kThreads;
+
+template <typename T>
+struct Eps;
+template <>
+struct Eps<float> {
+    static constexpr double value = 5.9604644775390625e-08;  // 2^-24
+};
+template <>
+struct Eps<double> {
+    static constexpr double value = 1.1102230246251565e-16;  // 2^-53
+};
+
+// Error model of the FFT numerator: |du| <= kNoiseK * eps * sqrt(log2 nfft)
+// * max|T^| * rms(block).  A window is recomputed exactly when that bound,
+// divided by the window's own norm, could exceed kRhoTol.
+constexpr double kNoiseK = 4.0;
+constexpr double kRhoTol = 1.0e-6;
+
+template <typename T, int M>
+__global__ __launch_bounds__(NT) void corr_kernel(const Item *__restrict__ items,
+                                                  const Seg *__restrict__ segs,
+                                                  const Row *__restrict__ rows,
+                                                  const Cx<T> *__restrict__ twloG,
+                                                  const Cx<T> *__restrict__ twhiG)
+{
+    constexpr int N = 2 * M;
+    constexpr int NTILE = N / 16;
+    constexpr int SPT = N / NT;
+    constexpr int LOGN = Log2<N>::value;
+    static_assert(N % NT == 0, "block size");
+    __shared__ T raw[N];
+    __shared__ Cx<T> Xs[M + 1];
+    __shared__ Cx<T> Ws[M];
+    __shared__ T sc[N];
+    __shared__ double ts1[NTILE];
+    __shared__ double ts2[NTILE];
+    __shared__ Cx<T> twlo[64];
+    __shared__ Cx<T> twhi[N / 64];
+    __shared__ double red[NT / 64];
+
+    const Item item = items[blockIdx.x];
+    const Seg sg = segs[item.seg];
+    const int tid = threadIdx.x;
+    const long long base = static_cast<long long>(item.block) * sg.L;
+    const int nValidSeg = static_cast<int>(min(static_cast<long long>(sg.L), sg.nOut - base));
+
+    if (sg.dead) {
+        for (int r = sg.rowBegin; r < sg.rowEnd; ++r) {
+            const Row row = rows[r];
+            const int nv = static_cast<int>(min(static_cast<long long>(sg.L), row.nOut - base));
+            T *out = static_cast<T *>(row.out) + base;
+            for (int i = tid; i < nv; i += NT) { out[i] = T(0); }
+        }
+        return;
+    }
+
+    for (int i = tid; i < 64; i += NT) { twlo[i] = twloG[i]; }
+    for (int i = tid; i < N / 64; i += NT) { twhi[i] = twhiG[i]; }
+    const Twiddle<T> tw{twlo, twhi};
+
+    // 1. load the block, anchor, tile sums, energy
+    const T *x = static_cast<const T *>(sg.x);
+    const int nSamp = static_cast<int>(min(static_cast<long long>(N), sg.n - base));
+    T v[SPT];
+    double s = 0;
+#pragma unroll
+    for (int q = 0; q < SPT; ++q) {
+        const int j = tid + q * NT;
+        const long long g = base + j;
+        const T val = (j < nSamp) ? x[g] : T(0);
+        v[q] = val;
+        raw[j] = val;
+        s += static_cast<double>(val);
+    }
+    const double a = block_sum<NT>(s, red) / nSamp;
+    const T aT = static_cast<T>(a);
+    double e = 0;
+    T *wf = reinterpret_cast<T *>(Ws);
+#pragma unroll
+    for (int q = 0; q < SPT; ++q) {
+        const int j = tid + q * NT;
+        const bool valid = j < nSamp;
+        const double d = valid ? static_cast<double>(v[q]) - a : 0.0;
+        double t1 = d, t2 = d * d;
+        e += t2;
+#pragma unroll
+        for (int o = 8; o > 0; o >>= 1) {
+            t1 += __shfl_xor(t1, o, 64);
+            t2 += __shfl_xor(t2, o, 64);
+        }
+        if ((tid & 15) == 0) {
+            ts1[j >> 4] = t1;
+            ts2[j >> 4] = t2;
+        }
+        wf[j] = valid ? static_cast<T>(v[q] - aT) : T(0);
+    }
+    const double energy = block_sum<NT>(e, red);  // also publishes raw/ts/Ws
+
+    // 2. forward FFT of the packed block, unpack to X' = 2X
+    fft_lds<T, M, NT, false>(Ws, tw);
+    for (int k = tid; k <= M / 2; k += NT) {
+        const Cx<T> zk = Ws[k];
+        const Cx<T> zm = cconj(Ws[(M - k) & (M - 1)]);
+        const Cx<T> E = cadd(zk, zm);
+        const Cx<T> O = cmulmi(csub(zk, zm));
+        const Cx<T> WO = cmul(tw(k), O);
+        Xs[k] = cadd(E, WO);
+        Xs[M - k] = cconj(csub(E, WO));
+    }
+    __syncthreads();
+
+    const double rmsBlk = sqrt(energy / nSamp);
+    const double noise = kNoiseK / kRhoTol * Eps<T>::value * sqrt(static_cast<double>(LOGN)) *
+                         static_cast<double>(sg.hmax) * rmsBlk;
+    const int wave = tid >> 6;
+    const int lane = tid & 63;
+    int curLt = -1;
+    for (int r = sg.rowBegin; r < sg.rowEnd; ++r) {
+        const Row row = rows[r];
+        const int nvRow = static_cast<int>(min(static_cast<long long>(sg.L), row.nOut - base));
+        if (nvRow <= 0) { continue; }
+        const int Lt = row.Lt;
+        if (Lt != curLt) {
+            // 3. per-output scale for this template length
+            curLt = Lt;
+            const double sqrtLt = sqrt(static_cast<double>(Lt));
+            const double denThr = sqrtLt * noise;
+            const int C = (nValidSeg + NT - 1) / NT;
+            const double canc = 2.0e-9 * (Lt + C) * Lt;
+            const int i0 = tid * C;
+            const int i1 = min(i0 + C, nValidSeg);
+            if (i0 < i1) {
+                double S1 = 0, S2 = 0;
+                const int lo = i0, hi = i0 + Lt;
+                const int q0 = (lo + 15) >> 4, q1 = hi >> 4;
+                if (q0 < q1) {
+                    for (int j = lo; j < 16 * q0; ++j) {
+                        const double d = static_cast<double>(raw[j]) - a;
+                        S1 += d;
+                        S2 += d * d;
+                    }
+                    for (int q = q0; q < q1; ++q) {
+                        S1 += ts1[q];
+                        S2 += ts2[q];
+                    }
+                    for (int j = 16 * q1; j < hi; ++j) {
+                        const double d = static_cast<double>(raw[j]) - a;
+                        S1 += d;
+                        S2 += d * d;
+                    }
+                } else {
+                    for (int j = lo; j < hi; ++j) {
+                        const double d = static_cast<double>(raw[j]) - a;
+                        S1 += d;
+                        S2 += d * d;
+                    }
+                }
+                double S2max = S2;
+                for (int i = i0; i < i1; ++i) {
+                    const double den2 = Lt * S2 - S1 * S1;
+                    const double den = sqrt(fmax(den2, 0.0));
+                    const bool flag = !(den2 > canc * S2max) || !(den > denThr);
+                    sc[i] = flag ? T(-1) : static_cast<T>(sqrtLt / den);
+                    if (i + 1 < i1) {
+                        const double dn = static_cast<double>(raw[i + Lt]) - a;
+                        const double dd = static_cast<double>(raw[i]) - a;
+                        S1 += dn - dd;
+                        S2 += dn * dn - dd * dd;
+                        S2max = fmax(S2max, S2);
+                    }
+                }
+            }
+            __syncthreads();
+        }
+        // 4a. spectral product + inverse packing into Ws
+        const Cx<T> *H = static_cast<const Cx<T> *>(row.H);
+        for (int k = tid; k <= M / 2; k += NT) {
+            const Cx<T> yk = cmul(Xs[k], H[k]);
+            const Cx<T> ym = cmul(Xs[M - k], H[M - k]);
+            const Cx<T> S = cadd(yk, cconj(ym));
+            const Cx<T> D = csub(yk, cconj(ym));
+            const Cx<T> P = cmul(cconj(tw(k)), D);
+            Ws[k] = cadd(S, cmuli(P));
+            if (k > 0) { Ws[M - k] = cadd(cconj(S), cmuli(cconj(P))); }
+        }
+        __syncthreads();
+        // 4b. inverse FFT: Ws holds the correlation, interleaved = natural order
+        fft_lds<T, M, NT, true>(Ws, tw);
+        // 4c. normalisation epilogue
+        T *out = static_cast<T *>(row.out) + base;
+        const T *c = reinterpret_cast<const T *>(Ws);
+        for (int i = tid; i < nvRow; i += NT) {
+            const T f = sc[i];
+            if (f >= T(0)) {
+                T rho = c[i] * f;
+                if (fabs(rho) > T(1.1)) { rho = T(0); }
+                out[i] = rho;
+            }
+        }
+        // 4d. exact FP64 two-pass Pearson for flagged windows
+        {
+            constexpr int NW = NT / 64;
+            const int slice = (nvRow + NW - 1) / NW;
+            const int s0 = wave * slice;
+            const int s1 = min(s0 + slice, nvRow);
+            for (int w0 = s0; w0 < s1; w0 += 64) {
+                const int i = w0 + lane;
+                const bool fl = (i < s1) && (sc[i] < T(0));
+                unsigned long long m = __ballot(fl);
+                while (m != 0ull) {
+                    const int b = __ffsll(static_cast<long long>(m)) - 1;
+                    m &= m - 1ull;
+                    const int wi = w0 + b;
+                    double sm = 0;
+                    for (int mm = lane; mm < Lt; mm += 64) { sm += static_cast<double>(raw[wi + mm]); }
+                    sm = wave_sum(sm);
+                    const double mean = sm / Lt;
+                    double e2 = 0, dn = 0;
+                    for (int mm = lane; mm < Lt; mm += 64) {
+                        const double d = static_cast<double>(raw[wi + mm]) - mean;
+                        e2 += d * d;
+                        dn += row.tmpl[mm] * d;
+                    }
+                    e2 = wave_sum(e2);
+                    dn = wave_sum(dn);
+                    double rho = dn / fmax(DBL_EPSILON, sqrt(e2));
+                    if (fabs(rho) > 1.1) { rho = 0; }
+                    if (lane == 0) { out[wi] = static_cast<T>(rho); }
+                }
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// Template demean / unit-norm (FP64) and spectrum conj(T^)/(2 nfft).
+template <int M>
+__global__ __launch_bounds__(NT) void tmpl_kernel(const TmplJob *__restrict__ jobs, int precision,
+                                                  const Cx<double> *__restrict__ twloG,
+                                                  const Cx<double> *__restrict__ twhiG)
+{
+    constexpr int N = 2 * M;
+    __shared__ Cx<double> Z[M];
+    __shared__ Cx<double> twlo[64];
+    __shared__ Cx<double> twhi[N / 64];
+    __shared__ double red[NT / 64];
+    const TmplJob job = jobs[blockIdx.x];
+    const int tid = threadIdx.x;
+    for (int i = tid; i < 64; i += NT) { twlo[i] = twloG[i]; }
+    for (int i = tid; i < N / 64; i += NT) { twhi[i] = twhiG[i]; }
+    const Twiddle<double> tw{twlo, twhi};
+    double s = 0;
+    for (int m = tid; m < job.Lt; m += NT) { s += job.t[m]; }
+    const double mean = block_sum<NT>(s, red) / job.Lt;
+    double e = 0;
+    for (int m = tid; m < job.Lt; m += NT) {
+        const double d = job.t[m] - mean;
+        e += d * d;
+    }
+    const double norm = sqrt(block_sum<NT>(e, red));
+    const double scale = norm > 0 ? 1.0 / norm : 0.0;
+    double *zf = reinterpret_cast<double *>(Z);
+    for (int m = tid; m < N; m += NT) {
+        double tv = 0;
+        if (m < job.Lt) {
+            tv = (job.t[m] - mean) * scale;
+            job.tOut[m] = tv;
+        }
+        zf[m] = tv;
+    }
+    __syncthreads();
+    fft_lds<double, M, NT, false>(Z, tw);
+    const double hs = 1.0 / (2.0 * N);
+    double hmax = 0;
+    for (int k = tid; k <= M / 2; k += NT) {
+        const Cx<double> zk = Z[k];
+        const Cx<double> zm = cconj(Z[(M - k) & (M - 1)]);
+        const Cx<double> E = cadd(zk, zm);
+        const Cx<double> O = cmulmi(csub(zk, zm));
+        const Cx<double> WO = cmul(tw(k), O);
+        const Cx<double> tk = cadd(E, WO);             // 2 T^[k]
+        const Cx<double> tm = cconj(csub(E, WO));      // 2 T^[M-k]
+        hmax = fmax(hmax, 0.5 * sqrt(tk.x * tk.x + tk.y * tk.y));
+        hmax = fmax(hmax, 0.5 * sqrt(tm.x * tm.x + tm.y * tm.y));
+        // conj(T^)/(2N) = conj(2 T^) / (4N)
+        const Cx<double> hk{tk.x * hs * 0.5, -tk.y * hs * 0.5};
+        const Cx<double> hm{tm.x * hs * 0.5, -tm.y * hs * 0.5};
+        if (precision == 0) {
+            Cx<float> *H = static_cast<Cx<float> *>(job.H);
+            H[k] = Cx<float>{static_cast<float>(hk.x), static_cast<float>(hk.y)};
+            H[M - k] = Cx<float>{static_cast<float>(hm.x), static_cast<float>(hm.y)};
+        } else {
+            Cx<double> *H = static_cast<Cx<double> *>(job.H);
+            H[k] = hk;
+            H[M - k] = hm;
+        }
+    }
+    hmax = block_max<NT>(hmax, red);
+    if (tid == 0) { *job.hmax = static_cast<float>(hmax); }
+}
+
+// out[k] = sum_r w_r * |rho_r[k + s_r]| in row order, no contraction,
+// as the reference's shiftAndStack loop (src/matchedFilter.cpp:1341-1361).
+template <typename T>
+__global__ __launch_bounds__(NT) void stack_kernel(const StackRow *__restrict__ srows, int nRows,
+                                                   long long len, int absval, T *__restrict__ out)
+{
+    // hipcc contracts a*b+c into an FMA by default; the stack keeps the
+    // separately rounded product and sum of the loop it restates.
+#pragma clang fp contract(off)
+    const long long stride = static_cast<long long>(gridDim.x) * NT;
+    for (long long k = static_cast<long long>(blockIdx.x) * NT + threadIdx.x; k < len; k += stride) {
+        T acc = T(0);
+        for (int r = 0; r < nRows; ++r) {
+            const StackRow sr = srows[r];
+            T v = static_cast<const T *>(sr.rho)[k + sr.shift];
+            if (absval) { v = fabs(v); }
+            const T w = static_cast<T>(sr.weight);
+            acc = acc + w * v;
+        }
+        out[k] = acc;
+    }
+}
+
+// Candidates of src/peakFinder.cpp:500-521: strictly above threshold and
+// a local maximum (plateaus keep their last sample; end-point rules).
+template <typename T>
+__global__ __launch_bounds__(NT) void peak_kernel(const T *__restrict__ x, long long n, T thr,
+                                                  int *__restrict__ count, int *__restrict__ idx,
+                                                  T *__restrict__ val, int cap)
+{
+    const long long stride = static_cast<long long>(gridDim.x) * NT;
+    const long long start = static_cast<long long>(blockIdx.x) * NT + threadIdx.x;
+    // every lane of a wave runs the same number of iterations (ballot below)
+    const long long nIter = (n + stride - 1) / stride;
+    for (long long it = 0; it < nIter; ++it) {
+        const long long i = start + it * stride;
+        bool cand = false;
+        T xi = T(0);
+        if (i < n) {
+            xi = x[i];
+            if (xi > thr) {
+                if (n == 1) {
+                    cand = true;
+                } else if (i == 0) {
+                    cand = x[1] < xi;
+                } else if (i == n - 1) {
+                    cand = xi > x[n - 2];
+                } else {
+                    cand = (xi >= x[i - 1]) && (x[i + 1] < xi);
+                }
+            }
+        }
+        const unsigned long long mask = __ballot(cand);
+        if (mask != 0ull) {
+            const int lane = threadIdx.x & 63;
+            const int leader = __ffsll(static_cast<long long>(mask)) - 1;
+            int basePos = 0;
+            if (lane == leader) { basePos = atomicAdd(count, __popcll(mask)); }
+            basePos = __shfl(basePos, leader, 64);
+            if (cand) {
+                const int pos = basePos + __popcll(mask & ((1ull << lane) - 1ull));
+                if (pos < cap) {
+                    idx[pos] = static_cast<int>(i);
+                    val[pos] = xi;
+                }
+            }
+        }
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(NT) void minmax_kernel(const T *__restrict__ x, long long n,
+                                                    double *__restrict__ pmin,
+                                                    double *__restrict__ pmax)
+{
+    __shared__ double red[NT / 64];
+    const long long stride = static_cast<long long>(gridDim.x) * NT;
+    double lo = DBL_MAX, hi = -DBL_MAX;
+    for (long long i = static_cast<long long>(blockIdx.x) * NT + threadIdx.x; i < n; i += stride) {
+        const double v = static_cast<double>(x[i]);
+        lo = fmin(lo, v);
+        hi = fmax(hi, v);
+    }
+    const double bmax = block_max<NT>(hi, red);
+    const double bmin = -block_max<NT>(-lo, red);
+    if (threadIdx.x == 0) {
+        pmin[blockIdx.x] = bmin;
+        pmax[blockIdx.x] = bmax;
+    }
+}
+
+template <typename T, int M>
+hipError_t corr_launch(const Item *dItems, int nItems, const Seg *dSegs, const Row *dRows,
+                       const void *lo, const void *hi, hipStream_t stream)
+{
+    hipLaunchKernelGGL((corr_kernel<T, M>), dim3(nItems), dim3(NT), 0, stream, dItems, dSegs,
+                       dRows, static_cast<const Cx<T> *>(lo), static_cast<const Cx<T> *>(hi));
+    return hipGetLastError();
+}
+
+template <int M>
+hipError_t tmpl_launch(int precision, const TmplJob *dJobs, int nJobs, const void *lo,
+                       const void *hi, hipStream_t stream)
+{
+    hipLaunchKernelGGL((tmpl_kernel<M>), dim3(nJobs), dim3(NT), 0, stream, dJobs, precision,
+                       static_cast<const Cx<double> *>(lo), static_cast<const Cx<double> *>(hi));
+    return hipGetLastError();
+}
+
+}  // namespace
+
+hipError_t launch_corr(int precision, int logM, const Item *dItems, int nItems, const Seg *dSegs,
+                       const Row *dRows, const void *dTwLo, const void *dTwHi, hipStream_t stream)
+{
+    if (nItems <= 0) { return hipSuccess; }
+    if (precision == 0) {
+        switch (logM) {
+        case 9: return corr_launch<float, 512>(dItems, nItems, dSegs, dRows, dTwLo, dTwHi, stream);
+        case 10: return corr_launch<float, 1024>(dItems, nItems, dSegs, dRows, dTwLo, dTwHi, stream);
+        case 11: return corr_launch<float, 2048>(dItems, nItems, dSegs, dRows, dTwLo, dTwHi, stream);
+        case 12: return corr_launch<float, 4096>(dItems, nItems, dSegs, dRows, dTwLo, dTwHi, stream);
+        default: return hipErrorInvalidValue;
+        }
+    }
+    switch (logM) {
+    case 9: return corr_launch<double, 512>(dItems, nItems, dSegs, dRows, dTwLo, dTwHi, stream);
+    case 10: return corr_launch<double, 1024>(dItems, nItems, dSegs, dRows, dTwLo, dTwHi, stream);
+    case 11: return corr_launch<double, 2048>(dItems, nItems, dSegs, dRows, dTwLo, dTwHi, stream);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t launch_tmpl(int precision, int logM, const TmplJob *dJobs, int nJobs,
+                       const void *dTwLo64, const void *dTwHi64, hipStream_t stream)
+{
+    if (nJobs <= 0) { return hipSuccess; }
+    switch (logM) {
+    case 9: return tmpl_launch<512>(precision, dJobs, nJobs, dTwLo64, dTwHi64, stream);
+    case 10: return tmpl_launch<1024>(precision, dJobs, nJobs, dTwLo64, dTwHi64, stream);
+    case 11: return tmpl_launch<2048>(precision, dJobs, nJobs, dTwLo64, dTwHi64, stream);
+    case 12: return tmpl_launch<4096>(precision, dJobs, nJobs, dTwLo64, dTwHi64, stream);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t launch_stack(int precision, const StackRow *dRows, int nRows, long long len,
+                        int absval, void *dOut, hipStream_t stream)
+{
+    if (len <= 0) { return hipSuccess; }
+    const long long want = (len + NT - 1) / NT;
+    const int grid = static_cast<int>(want < 65536 ? want : 65536);
+    if (precision == 0) {
+        hipLaunchKernelGGL(stack_kernel<float>, dim3(grid), dim3(NT), 0, stream, dRows, nRows, len,
+                           absval, static_cast<float *>(dOut));
+    } else {
+        hipLaunchKernelGGL(stack_kernel<double>, dim3(grid), dim3(NT), 0, stream, dRows, nRows,
+                           len, absval, static_cast<double *>(dOut));
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_peaks(int precision, const void *dx, long long n, double threshold, int *dCount,
+                        int *dIdx, void *dVal, int cap, hipStream_t stream)
+{
+    if (n <= 0) { return hipSuccess; }
+    const long long want = (n + NT - 1) / NT;
+    const int grid = static_cast<int>(want < 65536 ? want : 65536);
+    if (precision == 0) {
+        hipLaunchKernelGGL(peak_kernel<float>, dim3(grid), dim3(NT), 0, stream,
+                           static_cast<const float *>(dx), n, static_cast<float>(threshold), dCount,
+                           dIdx, static_cast<float *>(dVal), cap);
+    } else {
+        hipLaunchKernelGGL(peak_kernel<double>, dim3(grid), dim3(NT), 0, stream,
+                           static_cast<const double *>(dx), n, threshold, dCount, dIdx,
+                           static_cast<double *>(dVal), cap);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_minmax(int precision, const void *dx, long long n, double *dPartMin,
+                         double *dPartMax, int nBlocks, hipStream_t stream)
+{
+    if (precision == 0) {
+        hipLaunchKernelGGL(minmax_kernel<float>, dim3(nBlocks), dim3(NT), 0, stream,
+                           static_cast<const float *>(dx), n, dPartMin, dPartMax);
+    } else {
+        hipLaunchKernelGGL(minmax_kernel<double>, dim3(nBlocks), dim3(NT), 0, stream,
+                           static_cast<const double *>(dx), n, dPartMin, dPartMax);
+    }
+    return hipGetLastError();
+}
+
+}  // namespace mfk

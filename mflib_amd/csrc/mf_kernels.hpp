@@ -1,0 +1,75 @@
+// Host-visible descriptors and launchers of the matched-filter kernels.
+// No device code here: the engine (mf_engine.hip) fills these plain
+// structs, copies them to device memory and calls the launchers.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+namespace mfk {
+
+constexpr int kThreads = 256;        // block size of every kernel here
+constexpr int kMinLogM = 9;          // smallest complex FFT size (M = 512)
+constexpr int kMaxLogMFloat = 12;    // M = 4096 (nfft 8192) in FP32
+constexpr int kMaxLogMDouble = 11;   // M = 2048 (nfft 4096) in FP64
+
+// One (channel, FFT plan) lane of the correlation work.
+struct Seg {
+    const void *x;      // channel samples (handle precision), length n
+    long long n;        // channel length
+    int L;              // correlation outputs per block (nfft - nb + 1)
+    int nOut;           // outputs needed on this lane (max over its rows)
+    int rowBegin;       // rows [rowBegin, rowEnd) of the row table,
+    int rowEnd;         //   sorted by template length
+    int dead;           // dead trace: every row of the lane is zero
+    float hmax;         // max |template spectrum| over the lane's rows
+};
+
+// One template row.
+struct Row {
+    const void *H;        // conj(spectrum)/(2 nfft), nfft/2+1 complex
+    const double *tmpl;   // demeaned unit-norm template (FP64), Lt samples
+    void *out;            // correlation output row
+    int nOut;             // outputs of this row
+    int Lt;               // template length
+};
+
+// One work item: block `block` of lane `seg`.
+struct Item {
+    int seg;
+    int block;
+};
+
+// Template preparation job (one per row).
+struct TmplJob {
+    const double *t;  // raw template samples
+    int Lt;
+    double *tOut;     // demeaned unit-norm template (FP64)
+    void *H;          // spectrum row (handle precision)
+    float *hmax;      // max |spectrum| of the demeaned unit-norm template
+};
+
+struct StackRow {
+    const void *rho;  // correlation row
+    int shift;        // moveout in samples
+    int pad;
+    double weight;    // normalised weight (already rounded to T)
+};
+
+// Correlation + normalisation: one launch per FFT size.
+hipError_t launch_corr(int precision, int logM, const Item *dItems, int nItems,
+                       const Seg *dSegs, const Row *dRows, const void *dTwLo,
+                       const void *dTwHi, hipStream_t stream);
+// Template demean/normalise + spectra (FP64 arithmetic): one block per job.
+hipError_t launch_tmpl(int precision, int logM, const TmplJob *dJobs, int nJobs,
+                       const void *dTwLo64, const void *dTwHi64, hipStream_t stream);
+// Shift and stack of one group.
+hipError_t launch_stack(int precision, const StackRow *dRows, int nRows, long long len,
+                        int absval, void *dOut, hipStream_t stream);
+// Threshold + local-maximum candidates (unordered, deterministic set).
+hipError_t launch_peaks(int precision, const void *dx, long long n, double threshold,
+                        int *dCount, int *dIdx, void *dVal, int cap, hipStream_t stream);
+// Per-block min and max of a signal (dead-trace test on device input).
+hipError_t launch_minmax(int precision, const void *dx, long long n, double *dPartMin,
+                         double *dPartMax, int nBlocks, hipStream_t stream);
+
+}  // namespace mfk

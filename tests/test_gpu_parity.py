@@ -1,0 +1,209 @@
+"""GPU parity of the HIP engine, called through the C ABI (libmflib_gpu.so).
+
+Each test restates a reference test (file:line in the docstring) or checks
+the engine against the CPU oracle on seeded inputs.  Tolerances are the
+reference's own where it states one, and 1e-5 (the engine's contract for
+FP32 correlation against the FP64 oracle) otherwise.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+import refcases as R
+from mflib_amd import capi
+
+pytestmark = pytest.mark.gpu
+
+
+def _multi(templates, signals, dtype, nfft=0, shifts=None, weights=None):
+    h = capi.Handle(dtype, "multi")
+    h.set_templates(templates, shifts=shifts, weights=weights)
+    h.set_signal_size(len(signals[0]))
+    if nfft:
+        h.set_fft_length(nfft)
+    h.initialize()
+    for i, s in enumerate(signals):
+        h.set_signal(i, s)
+    h.apply()
+    return h
+
+
+def test_basic_double():
+    """testing/matchedFilter.cpp:58-168: |rho - naive| < 1e-13, 1 then 4 templates."""
+    x, x2, b, b2 = R.basic_test_inputs()
+    ref = O.naive_pearson(b, x)
+    ref2 = O.naive_pearson(b2, x2)
+    h = _multi([b], [x], np.float64)
+    r = h.rho(0)
+    assert len(r) == 1000 - len(b) + 1
+    assert np.abs(r - ref).max() < 1e-13
+    h = _multi([b, b2, b, b2], [x, x2, x, x2], np.float64)
+    for i in range(4):
+        assert np.abs(h.rho(i) - (ref if i % 2 == 0 else ref2)).max() < 1e-13
+
+
+def test_basic_float():
+    """testing/matchedFilter.cpp:172-271: float path within 2e-6."""
+    x, x2, b, b2 = R.basic_test_inputs()
+    xf, x2f = x.astype(np.float32), x2.astype(np.float32)
+    bf = b.astype(np.float32).astype(np.float64)    # the reference test's float template
+    b2f = b2.astype(np.float32).astype(np.float64)
+    ref = O.pearson(bf, xf)
+    ref2 = O.pearson(b2f, x2f)
+    h = _multi([bf, b2f, bf, b2f], [xf, x2f, xf, x2f], np.float32)
+    for i in range(4):
+        assert np.abs(h.rho(i) - (ref if i % 2 == 0 else ref2)).max() < 2e-6
+
+
+def test_single_channel_uneven_lengths():
+    """testing/singleChannel.cpp:104-160: three uneven templates, < 1e-12."""
+    rng = np.random.default_rng(20240601)
+    wave, tpls = R.single_channel_inputs(rng)
+    h = capi.Handle(np.float64, "single")
+    h.set_templates(tpls)
+    h.set_signal_size(len(wave))
+    h.initialize()
+    h.set_signal(0, wave)
+    h.apply()
+    nout = len(wave) - 140 + 1
+    for k in range(3):
+        r = h.rho(k)
+        assert len(r) == nout
+        assert np.abs(r - O.naive_pearson(tpls[k], wave)[:nout]).max() < 1e-12
+
+
+def test_shift_and_stack_reference_case():
+    """testing/matchedFilter.cpp:275-465: rho within 1e-11, stack max 1 +/- 1e-3
+    at the origin time +/- dt/2."""
+    tp, sg, tt, toff, df, origin = R.shift_and_stack_inputs()
+    shifts = [R.moveout_shift(tt[k], toff[k], df) for k in range(3)]
+    h = _multi(tp, sg, np.float64, shifts=shifts, weights=[1, 1, 1])
+    for k in range(3):
+        r = h.rho(k)
+        assert np.abs(r - O.naive_pearson(tp[k], sg[k])[:len(r)]).max() <= 1e-11
+    st = h.shift_and_stack()
+    i = int(np.argmax(st))
+    assert abs(st[i] - 1.0) < 1e-3
+    assert abs(i / df - origin) < 1 / df / 2
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_stack_bit_exact_vs_oracle(dtype):
+    """shiftAndStack (src/matchedFilter.cpp:1318-1364) on identical rho:
+    bit-identical to the oracle's sequential accumulation."""
+    rng = np.random.default_rng(7)
+    nT, n, nb = 5, 20000, 300
+    tpls = [rng.standard_normal(nb) for _ in range(nT)]
+    sigs = [rng.standard_normal(n).astype(dtype) for _ in range(nT)]
+    shifts = rng.integers(0, 150, nT)
+    weights = rng.uniform(0.1, 1.0, nT)
+    h = _multi(tpls, sigs, dtype, shifts=shifts, weights=weights)
+    rho = np.stack([h.rho(i) for i in range(nT)])
+    for absval in (True, False):
+        h.set_stack_absolute_values(absval)
+        st = h.shift_and_stack()
+        ref = O.shift_and_stack(rho, shifts, weights, absval=absval)
+        assert st.dtype == ref.dtype and len(st) == len(ref)
+        assert np.array_equal(st, ref)
+
+
+def test_peak_finder_kats():
+    """testing/peakFinder.cpp:16-143: exact 13 and 21 peak indices."""
+    sig, thr, md, ref13, ref21 = R.peak_finder_signal()
+    assert list(capi.peaks(sig, thr, md)) == ref13
+    assert list(capi.peaks(sig, thr, 0)) == ref21
+    assert list(capi.peaks(sig.astype(np.float32), thr, md)) == ref13
+    assert len(capi.peaks(np.full(4000, 0.4), thr, md)) == 0
+
+
+@pytest.mark.parametrize("seed", [1, 2, 3])
+def test_peak_finder_random_vs_oracle(seed):
+    rng = np.random.default_rng(seed)
+    x = np.convolve(rng.standard_normal(50000), np.hanning(31), mode="same")
+    x /= np.abs(x).max()
+    x = np.round(x, 3)  # plateaus and ties
+    for md in (0, 1, 10, 200):
+        assert np.array_equal(capi.peaks(x, 0.3, md), O.peak_finder(x, 0.3, md))
+        xf = x.astype(np.float32)
+        assert np.array_equal(capi.peaks(xf, 0.3, md), O.peak_finder(xf, 0.3, md))
+
+
+@pytest.mark.parametrize("nfft", [1024, 2048, 4096, 8192])
+def test_multiblock_float_vs_fp64_oracle(nfft):
+    """Many blocks per channel, DC offsets, uneven template lengths."""
+    rng = np.random.default_rng(nfft)
+    n = 60000
+    lens = [257, 300, 211]
+    tpls = [rng.standard_normal(L) * np.hanning(L) for L in lens]
+    sigs = [(rng.standard_normal(n) + rng.uniform(-100, 100)).astype(np.float32) for _ in lens]
+    h = _multi(tpls, sigs, np.float32, nfft=nfft)
+    assert h.plan(0)[0] == nfft
+    nout = n - max(lens) + 1
+    for k in range(3):
+        r = h.rho(k)
+        ref = O.pearson(tpls[k], sigs[k])[:nout]
+        assert np.abs(r - ref).max() < 1e-5
+
+
+@pytest.mark.parametrize("nfft", [1024, 4096])
+def test_multiblock_double_vs_fp64_oracle(nfft):
+    rng = np.random.default_rng(nfft + 1)
+    n = 30000
+    lens = [400, 333]
+    tpls = [rng.standard_normal(L) for L in lens]
+    sigs = [rng.standard_normal(n) * 3 + 1e3 for _ in lens]
+    h = _multi(tpls, sigs, np.float64, nfft=nfft)
+    nout = n - max(lens) + 1
+    for k in range(2):
+        ref = O.pearson(tpls[k], sigs[k])[:nout]
+        assert np.abs(h.rho(k) - ref).max() < 1e-11
+
+
+def test_dead_and_zeroed_channels():
+    """Dead trace (|min| == |max|, src/matchedFilter.cpp:890-894) and
+    zeroSignal (823-845) give rho == 0."""
+    rng = np.random.default_rng(5)
+    n = 5000
+    tpls = [rng.standard_normal(100) for _ in range(3)]
+    sigs = [np.full(n, 3.25, np.float32), rng.standard_normal(n).astype(np.float32),
+            rng.standard_normal(n).astype(np.float32)]
+    h = _multi(tpls, sigs, np.float32)
+    assert np.all(h.rho(0) == 0)
+    assert np.abs(h.rho(1)).max() > 0.1
+    h.zero_signal(2)
+    h.apply()
+    assert np.all(h.rho(2) == 0)
+
+
+def test_gap_windows_recomputed_exactly():
+    """Constant and near-constant stretches inside ordinary blocks: the FFT
+    numerator is unreliable there, the engine recomputes those windows
+    exactly (FP64 two-pass); must match the FP64 oracle to 1e-5."""
+    rng = np.random.default_rng(11)
+    n = 40000
+    x = rng.standard_normal(n) * 0.4 + 25.0
+    x[5000:8000] = 25.0                                 # exact constant
+    x[15000:17000] = 3.0 + 1e-4 * rng.standard_normal(2000)  # quiet, offset level
+    x = x.astype(np.float32)
+    tpls = [rng.standard_normal(200)]
+    h = _multi(tpls, [x], np.float32, nfft=4096)
+    r = h.rho(0)
+    ref = O.pearson(tpls[0], x)[:len(r)]
+    assert np.abs(r - ref).max() < 1e-5
+
+
+def test_errors_match_reference_conventions():
+    h = capi.Handle(np.float32, "multi")
+    with pytest.raises(RuntimeError):
+        h.apply()                       # not initialised: runtime_error
+    h.set_templates([np.arange(10.0)])
+    h.set_signal_size(100)
+    h.initialize()
+    with pytest.raises(ValueError):
+        h.set_signal(1, np.zeros(100))  # index out of range: invalid_argument
+    with pytest.raises(ValueError):
+        h.set_signal(0, np.zeros(99))   # wrong length: invalid_argument
+    with pytest.raises(RuntimeError):
+        h.shift_and_stack()             # not applied: runtime_error
+    with pytest.raises(ValueError):
+        capi.peaks(np.zeros(10), 1.5, 0)  # threshold outside [0,1]
