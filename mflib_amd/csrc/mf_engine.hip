@@ -1207,6 +1207,10 @@ int mf_get_timings(mf_handle *h, double *corrMs, double *stackMs)
 {
     return guard([&] {
         requireHandle(h);
+        if (h->applied && hipEventQuery(h->evB) == hipSuccess) {
+            float ms = 0;
+            if (hipEventElapsedTime(&ms, h->evA, h->evB) == hipSuccess) { h->corrMs = ms; }
+        }
         if (corrMs) { *corrMs = h->corrMs; }
         if (stackMs) { *stackMs = h->stackMs; }
     });

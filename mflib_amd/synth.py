@@ -1,0 +1,155 @@
+"""Seeded synthetic seismic workloads for tests and benchmarks.
+
+Band-passed Gaussian noise channels with DC offsets, Hann-tapered
+band-limited template bursts, moveouts and injections.  Everything is
+reproducible from one integer seed: Gaussian samples come from numpy's
+PCG64 generator (stable across machines for a given numpy release) and
+the Butterworth band-pass is designed here by the bilinear transform and
+applied as a causal cascade of biquads.
+
+"4-pole band-pass" means four poles in total: a second-order low-pass
+prototype mapped to a band-pass (two biquads).
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass, field
+
+import numpy as np
+from scipy.signal import sosfilt
+
+SEED_BASE = 0x4D464C00
+
+
+def butter_bandpass_sos(f_lo: float, f_hi: float, fs: float) -> np.ndarray:
+    """Second-order Butterworth low-pass prototype -> band-pass (4 poles),
+    bilinear transform with pre-warping.  Returns a (2, 6) SOS array."""
+    w1 = 2 * fs * math.tan(math.pi * f_lo / fs)
+    w2 = 2 * fs * math.tan(math.pi * f_hi / fs)
+    bw = w2 - w1
+    w0 = math.sqrt(w1 * w2)
+    # prototype poles of a 2nd-order Butterworth: exp(i*pi*(2k+n+1)/(2n)), n=2
+    proto = [complex(math.cos(math.pi * (2 * k + 3) / 4), math.sin(math.pi * (2 * k + 3) / 4))
+             for k in range(2)]
+    # low-pass -> band-pass: each prototype pole p gives roots of
+    # s^2 - p*bw*s + w0^2 = 0
+    poles = []
+    for p in proto:
+        a = p * bw / 2
+        d = np.sqrt(a * a - w0 * w0 + 0j)
+        poles += [a + d, a - d]
+    # bilinear transform z = (2fs + s)/(2fs - s)
+    fs2 = 2 * fs
+    zp = [(fs2 + s) / (fs2 - s) for s in poles]
+    # band-pass zeros: two at s=0 (z=1) and two at s=inf (z=-1)
+    # pair poles into conjugate biquads
+    zp = sorted(zp, key=lambda z: (round(abs(z.imag), 12), z.real))
+    upper = [z for z in zp if z.imag > 0]
+    if len(upper) != 2:
+        raise ValueError("unexpected pole layout")
+    sos = np.zeros((2, 6))
+    for i, z in enumerate(upper):
+        sos[i, 3:] = [1.0, -2 * z.real, abs(z) ** 2]
+    sos[0, :3] = [1.0, 2.0, 1.0]     # zeros at z = -1
+    sos[1, :3] = [1.0, -2.0, 1.0]    # zeros at z = +1
+    # unit gain at the geometric centre frequency
+    fc = math.sqrt(f_lo * f_hi)
+    zc = np.exp(2j * math.pi * fc / fs)
+    g = 1.0
+    for s in sos:
+        g *= abs((s[0] * zc * zc + s[1] * zc + s[2]) / (s[3] * zc * zc + s[4] * zc + s[5]))
+    sos[0, :3] /= g
+    return sos
+
+
+def hann(n: int) -> np.ndarray:
+    if n == 1:
+        return np.ones(1)
+    k = np.arange(n)
+    return 0.5 * (1 - np.cos(2 * math.pi * k / (n - 1)))
+
+
+def bandpassed_noise(rng, n: int, fs: float, f_lo: float, f_hi: float, dtype=np.float64):
+    sos = butter_bandpass_sos(f_lo, f_hi, fs)
+    return sosfilt(sos, rng.standard_normal(n)).astype(dtype, copy=False)
+
+
+def template_burst(rng, n: int, fs: float, f_lo: float, f_hi: float) -> np.ndarray:
+    """Hann-tapered band-limited noise burst with unit energy (sum t^2 = 1)."""
+    pad = 4 * n
+    t = bandpassed_noise(rng, n + pad, fs, f_lo, f_hi)[pad:] * hann(n)
+    return t / np.linalg.norm(t)
+
+
+@dataclass
+class MultiChannelWorkload:
+    """Templates x channels detection workload (the multi-channel configs)."""
+    fs: float
+    signals: list                     # C arrays (float32)
+    templates: list                   # [event][channel] FP64 arrays
+    moveouts: np.ndarray              # [event, channel] integer samples
+    origins: list = field(default_factory=list)   # [event] injection origin samples
+    noise_rms: np.ndarray = None
+
+    @property
+    def n_events(self):
+        return len(self.templates)
+
+    @property
+    def n_channels(self):
+        return len(self.signals)
+
+
+def multichannel_day(seed: int = SEED_BASE + 2, n_events: int = 16, n_channels: int = 24,
+                     n_samples: int = 8_640_000, tlen: int = 512, fs: float = 100.0,
+                     n_inject: int = 50, max_moveout: int = 200, dc: float = 100.0,
+                     amp=(0.3, 1.5)) -> MultiChannelWorkload:
+    """16 templates x 24 channels, one day at 100 Hz, FP32 signals.
+    Channels: band-passed (1-10 Hz) N(0,1) noise + DC offset U(-dc, dc).
+    Templates: per (event, channel) unit-energy Hann-tapered bursts.
+    Moveouts: integers U[0, max_moveout].  Each event is injected n_inject
+    times; per-channel amplitude U(amp) times the channel noise RMS times
+    sqrt(tlen) (so the amplitude is a per-sample RMS ratio)."""
+    rng = np.random.default_rng(seed)
+    noise = []
+    for c in range(n_channels):
+        crng = np.random.default_rng([seed, 1, c])
+        noise.append(bandpassed_noise(crng, n_samples, fs, 1.0, 10.0))
+    rms = np.array([float(np.sqrt(np.mean(x[:100000] ** 2))) for x in noise])
+    templates = []
+    for e in range(n_events):
+        templates.append([template_burst(np.random.default_rng([seed, 2, e, c]), tlen, fs, 1.0, 10.0)
+                          for c in range(n_channels)])
+    moveouts = rng.integers(0, max_moveout + 1, size=(n_events, n_channels))
+    origins = []
+    span = n_samples - tlen - max_moveout - 1
+    for e in range(n_events):
+        orig = np.sort(rng.integers(0, span, size=n_inject))
+        origins.append(orig)
+        amps = rng.uniform(amp[0], amp[1], size=(n_inject, n_channels))
+        for c in range(n_channels):
+            t = templates[e][c] * math.sqrt(tlen) * rms[c]
+            for k, o in enumerate(orig):
+                i0 = int(o) + int(moveouts[e, c])
+                noise[c][i0:i0 + tlen] += amps[k, c] * t
+    offsets = rng.uniform(-dc, dc, size=n_channels)
+    signals = [(noise[c] + offsets[c]).astype(np.float32) for c in range(n_channels)]
+    return MultiChannelWorkload(fs=fs, signals=signals, templates=templates, moveouts=moveouts,
+                                origins=origins, noise_rms=rms)
+
+
+def single_channel_hour(seed: int = SEED_BASE + 1, n: int = 360_000, tlen: int = 400,
+                        fs: float = 100.0, n_inject: int = 20):
+    """One hour at 100 Hz, FP64 band-passed N(0,1) noise, one 400-sample
+    unit-energy template injected 20 times with amplitude U(0.5, 2) and SNR
+    U(3, 10) (SNR = injected peak amplitude / noise RMS)."""
+    rng = np.random.default_rng(seed)
+    x = bandpassed_noise(np.random.default_rng([seed, 1]), n, fs, 1.0, 10.0)
+    t = template_burst(np.random.default_rng([seed, 2]), tlen, fs, 1.0, 10.0)
+    rms = float(np.sqrt(np.mean(x ** 2)))
+    pos = np.sort(rng.integers(0, n - tlen, size=n_inject))
+    amp = rng.uniform(0.5, 2.0, size=n_inject)
+    snr = rng.uniform(3.0, 10.0, size=n_inject)
+    for p, a, s in zip(pos, amp, snr):
+        x[p:p + tlen] += (s * rms / np.abs(t).max()) * (a / 1.25) * t
+    return x, t, pos
