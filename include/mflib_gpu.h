@@ -80,6 +80,10 @@ int mf_set_batch_layout(mf_handle *h, int nChannels, const int *channelLengths,
  * forces it.  The reference's MatchedFilter ignores setFFTLength and uses
  * computeOptimalFFTAndBlockLength (src/matchedFilter.cpp:519-525). */
 int mf_set_fft_length(mf_handle *h, int nfft);
+/* No reference counterpart (tuning): correlation kernel variant, 0 = best
+ * available (default), 1 = block-wide LDS FFTs, 2 = wave-level register
+ * FFTs.  A variant that does not exist for a block size falls back to 0. */
+int mf_set_kernel_variant(mf_handle *h, int variant);
 
 /* MatchedFilter<T>::initialize (src/matchedFilter.cpp:506-821): template
  * demean/normalize (FP64), plans, template spectra, device buffers. */
@@ -124,8 +128,19 @@ int mf_get_rho_device(mf_handle *h, int row, const void **dptr);
  * `group` (0 for single/multi-channel).  *len receives the stack length
  * n - nb + 1 - max(shift); out must hold that many values. */
 int mf_shift_and_stack(mf_handle *h, int group, void *out, int cap, int *len);
-/* No reference counterpart: stack into device memory. */
+/* No reference counterpart: stack into device memory (returns when the
+ * stack is complete). */
 int mf_shift_and_stack_device(mf_handle *h, int group, void *dout, int cap, int *len);
+
+/* No reference counterpart: the batched detection step.  shiftAndStack of
+ * every stacking group, then PeakFinder(threshold, minDist) on every stack
+ * (src/matchedFilter.cpp:1318-1364, src/peakFinder.cpp:468-586), all on
+ * the handle's stream with one launch per stage.  The peaks of group g are
+ * idx[offsets[g] .. offsets[g+1]) in ascending order; offsets holds
+ * nGroups+1 entries.  The stacks stay on the device (mf_get_stack_device). */
+int mf_detect(mf_handle *h, double threshold, int minDist, int *idx, int cap, int *offsets);
+int mf_get_stack_device(mf_handle *h, int group, const void **dptr, int *len);
+int mf_get_number_of_groups(const mf_handle *h, int *n);
 
 /* getSpectraOfTemplate(it) (src/matchedFilter.cpp:1411-1431).  Writes
  * nfft/2+1 complex values (re,im interleaved, handle precision) of the

@@ -15,7 +15,7 @@ import re
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "lib", "libmflib_gpu.so")
+LIB_PATH = os.environ.get("MFLIB_GPU_LIB") or os.path.join(_HERE, "lib", "libmflib_gpu.so")
 HEADER_PATH = os.path.join(os.path.dirname(_HERE), "include", "mflib_gpu.h")
 
 MF_OK = 0
@@ -43,6 +43,7 @@ _SIGNATURES = {
     "mf_set_stack_absolute_values": ([_vp, ctypes.c_int], ctypes.c_int),
     "mf_set_batch_layout": ([_vp, ctypes.c_int, _ip, _ip, _ip, ctypes.c_int], ctypes.c_int),
     "mf_set_fft_length": ([_vp, ctypes.c_int], ctypes.c_int),
+    "mf_set_kernel_variant": ([_vp, ctypes.c_int], ctypes.c_int),
     "mf_initialize": ([_vp], ctypes.c_int),
     "mf_is_initialized": ([_vp, _ip], ctypes.c_int),
     "mf_get_number_of_templates": ([_vp, _ip], ctypes.c_int),
@@ -60,6 +61,9 @@ _SIGNATURES = {
     "mf_get_rho_device": ([_vp, ctypes.c_int, ctypes.POINTER(_vp)], ctypes.c_int),
     "mf_shift_and_stack": ([_vp, ctypes.c_int, _vp, ctypes.c_int, _ip], ctypes.c_int),
     "mf_shift_and_stack_device": ([_vp, ctypes.c_int, _vp, ctypes.c_int, _ip], ctypes.c_int),
+    "mf_detect": ([_vp, ctypes.c_double, ctypes.c_int, _ip, ctypes.c_int, _ip], ctypes.c_int),
+    "mf_get_stack_device": ([_vp, ctypes.c_int, ctypes.POINTER(_vp), _ip], ctypes.c_int),
+    "mf_get_number_of_groups": ([_vp, _ip], ctypes.c_int),
     "mf_template_spectrum": ([_vp, ctypes.c_int, _vp, ctypes.c_int, _ip], ctypes.c_int),
     "mf_peaks": ([_vp, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_int, _ip,
                   ctypes.c_int, _ip], ctypes.c_int),
@@ -186,6 +190,9 @@ class Handle:
     def set_fft_length(self, nfft: int):
         check(lib().mf_set_fft_length(self._h, int(nfft)), "set_fft_length")
 
+    def set_kernel_variant(self, variant: int):
+        check(lib().mf_set_kernel_variant(self._h, int(variant)), "set_kernel_variant")
+
     def initialize(self):
         check(lib().mf_initialize(self._h), "initialize")
 
@@ -267,6 +274,25 @@ class Handle:
         check(lib().mf_shift_and_stack_device(self._h, int(group), _vp(dptr), int(cap), ctypes.byref(ln)),
               "shift_and_stack_device")
         return ln.value
+
+    def number_of_groups(self) -> int:
+        return self._int(lib().mf_get_number_of_groups)
+
+    def detect(self, threshold: float, min_dist: int, cap: int | None = None):
+        """Stack every group and run PeakFinder on every stack; returns a
+        list of index arrays, one per group."""
+        ng = self.number_of_groups()
+        cap = cap or max(1, 1 << 20)
+        idx = np.empty(cap, dtype=np.int32)
+        off = np.empty(ng + 1, dtype=np.int32)
+        check(lib().mf_detect(self._h, float(threshold), int(min_dist), idx.ctypes.data_as(_ip), cap,
+                              off.ctypes.data_as(_ip)), "detect")
+        return [idx[off[g]:off[g + 1]].copy() for g in range(ng)]
+
+    def stack_device(self, group: int):
+        p, n = _vp(), ctypes.c_int()
+        check(lib().mf_get_stack_device(self._h, int(group), ctypes.byref(p), ctypes.byref(n)))
+        return p.value, n.value
 
     def template_spectrum(self, row: int):
         nfft, _ = self.plan(row)

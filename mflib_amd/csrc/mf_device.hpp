@@ -15,12 +15,36 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include "mf_codelets.hpp"
+
 namespace mfdev {
 
 template <typename T>
-struct Cx {
-    T x, y;
-};
+using Cx = mfcl::C2<T>;
+
+// Pointers read from descriptor tables are generic; accessing global
+// memory through them would compile to FLAT instructions, which also count
+// against the LDS wait counter.  These casts make them global.
+template <typename T>
+using gptr = __attribute__((address_space(1))) T *;
+template <typename T>
+__device__ __forceinline__ gptr<T> to_global(T *p)
+{
+    return (gptr<T>)p;
+}
+template <typename T>
+__device__ __forceinline__ gptr<const T> to_global(const T *p)
+{
+    return (gptr<const T>)p;
+}
+// Complex element i of a global array (component loads merge into one
+// 2-wide access).
+template <typename T>
+__device__ __forceinline__ Cx<T> load_cx(gptr<const Cx<T>> p, int i)
+{
+    const gptr<const T> s = (gptr<const T>)p;
+    return Cx<T>{s[2 * i], s[2 * i + 1]};
+}
 
 template <typename T>
 __device__ __forceinline__ Cx<T> cadd(Cx<T> a, Cx<T> b) { return {a.x + b.x, a.y + b.y}; }
@@ -174,6 +198,181 @@ __device__ __forceinline__ double block_max(double v, double *scratch)
     for (int i = 1; i < NT / 64; ++i) { s = fmax(s, scratch[i]); }
     __syncthreads();
     return s;
+}
+
+// ---- wave-level FFTs ----------------------------------------------------
+// One wave (64 lanes) transforms M = 64 Q points held as Q complex values
+// per lane.  Three register passes (radices R1, R2, R3, R1 R2 R3 = M) with
+// two exchanges through a wave-private, padded LDS buffer; no block
+// barrier is involved.  Element r of butterfly b of a pass lives in
+// v[b R + r]; butterfly j = lane + 64 b reads points j + r M/R.
+template <int M>
+struct WavePlan;
+template <>
+struct WavePlan<512> {
+    static constexpr int R1 = 8, R2 = 8, R3 = 8;
+};
+template <>
+struct WavePlan<1024> {
+    static constexpr int R1 = 16, R2 = 16, R3 = 4;
+};
+template <>
+struct WavePlan<2048> {
+    static constexpr int R1 = 16, R2 = 16, R3 = 8;
+};
+
+// Padded exchange index: one spare slot every 16 points breaks the
+// power-of-two strides of the Stockham stores across LDS banks.
+__device__ __forceinline__ int xpad(int i) { return i + (i >> 4); }
+template <int M>
+struct XBuf {
+    static constexpr int size = M + M / 16;
+};
+
+// Orders a wave's LDS stores before its later loads (LDS operations of one
+// wave complete in issue order; this stops the compiler from moving them).
+__device__ __forceinline__ void wave_lds_sync()
+{
+    asm volatile("" ::: "memory");
+    __builtin_amdgcn_wave_barrier();
+    asm volatile("" ::: "memory");
+}
+
+// Exchange addresses split into a per-lane part and a compile-time part:
+// xpad(lanePart + constPart) == xpad(lanePart) + xpad(constPart) for every
+// access below (one of the two parts is a multiple of 16 or their low four
+// bits never carry), so each access is one base register plus an
+// immediate offset.
+constexpr int cpad(int i) { return i + (i >> 4); }
+
+template <typename T, int M, int R>
+__device__ __forceinline__ void wave_load(Cx<T> *v, const Cx<T> *buf, int lane)
+{
+    constexpr int B = M / (R * 64);
+    static_assert((M / R) % 16 == 0, "load offsets must be 16-aligned");
+    const Cx<T> *p = buf + xpad(lane);
+#pragma unroll
+    for (int b = 0; b < B; ++b) {
+#pragma unroll
+        for (int r = 0; r < R; ++r) { v[b * R + r] = p[cpad(64 * b + r * (M / R))]; }
+    }
+}
+
+// Inter-pass twiddles of one lane.  They depend on the lane only (not on
+// the data or the template row), so a block computes them once and every
+// transform reuses them (conjugated for inverse transforms).
+//   pass 2: butterfly j has k = j mod R1 = lane mod R1 (64 is a multiple
+//           of R1), the same for all of the lane's butterflies;
+//   pass 3: k = j mod R1 R2, which differs per butterfly when R1 R2 > 64.
+template <typename T, int M>
+struct WaveTwiddles {
+    using P = WavePlan<M>;
+    static constexpr int NS2 = P::R1;
+    static constexpr int NS3 = P::R1 * P::R2;
+    static constexpr int B3 = M / (P::R3 * 64);
+#ifndef MF_W3REG
+#define MF_W3REG 1
+#endif
+    static constexpr int W3 = MF_W3REG ? B3 : 1;
+    Cx<T> w2[P::R2 - 1];  // pass 2, resident in registers
+    Cx<T> w3[W3][P::R3 - 1];
+    Twiddle<T> tab;       // pass 3, looked up per transform
+    __device__ __forceinline__ void init(int lane, const Twiddle<T> &tw)
+    {
+        tab = tw;
+        const int k2 = lane & (NS2 - 1);
+        const int s2 = (2 * M / (NS2 * P::R2)) * k2;
+#pragma unroll
+        for (int r = 1; r < P::R2; ++r) { w2[r - 1] = tw(r * s2); }
+        if constexpr (MF_W3REG) {
+#pragma unroll
+            for (int b = 0; b < B3; ++b) { lookup3(lane, b, w3[b]); }
+        }
+    }
+    __device__ __forceinline__ void lookup3(int lane, int b, Cx<T> *w) const
+    {
+        const int k3 = (lane + 64 * b) & (NS3 - 1);
+        const int s3 = (2 * M / (NS3 * P::R3)) * k3;
+#pragma unroll
+        for (int r = 1; r < P::R3; ++r) { w[r - 1] = tab(r * s3); }
+    }
+    // pass-3 twiddle row of butterfly b of this lane
+    __device__ __forceinline__ void pass3(int lane, int b, Cx<T> *w) const
+    {
+        if constexpr (MF_W3REG) {
+#pragma unroll
+            for (int r = 1; r < P::R3; ++r) { w[r - 1] = w3[b][r - 1]; }
+        } else {
+            lookup3(lane, b, w);
+        }
+    }
+};
+
+// Butterflies of one pass with per-butterfly twiddle rows w[b][r-1]
+// (wStride 0 when all butterflies share one row).
+template <typename T, int M, int R, bool INV, bool TWIDDLE>
+__device__ __forceinline__ void wave_butterflies(Cx<T> *v, const Cx<T> *w, int wStride)
+{
+    constexpr int B = M / (R * 64);
+#pragma unroll
+    for (int b = 0; b < B; ++b) {
+        if constexpr (TWIDDLE) {
+#pragma unroll
+            for (int r = 1; r < R; ++r) {
+                Cx<T> t = w[b * wStride + r - 1];
+                if (INV) { t.y = -t.y; }
+                v[b * R + r] = cmul(v[b * R + r], t);
+            }
+        }
+        mfcl::dftR<R, INV>(&v[b * R]);
+    }
+}
+
+template <typename T, int M, int R, int NS>
+__device__ __forceinline__ void wave_store(const Cx<T> *v, Cx<T> *buf, int lane)
+{
+    constexpr int B = M / (R * 64);
+    static_assert(NS <= 64, "butterfly phase must depend on the lane only");
+    // j = lane + 64 b, k = j mod NS = lane mod NS:
+    // dst = (lane - k) R + k  +  64 b R + r NS
+    const int k = lane & (NS - 1);
+    Cx<T> *p = buf + xpad((lane - k) * R + k);
+#pragma unroll
+    for (int b = 0; b < B; ++b) {
+#pragma unroll
+        for (int r = 0; r < R; ++r) { p[cpad(64 * b * R + r * NS)] = v[b * R + r]; }
+    }
+}
+
+// Passes 1..3 on data already in registers as pass-1 input.  On return the
+// lane holds output points j + r M/R3 (j = lane + 64 b) in v[b R3 + r].
+template <typename T, int M, bool INV>
+__device__ __forceinline__ void wave_fft_regs(Cx<T> *v, Cx<T> *buf, int lane, const WaveTwiddles<T, M> &tw)
+{
+    using P = WavePlan<M>;
+    using W = WaveTwiddles<T, M>;
+    wave_butterflies<T, M, P::R1, INV, false>(v, nullptr, 0);
+    wave_store<T, M, P::R1, 1>(v, buf, lane);
+    wave_lds_sync();
+    wave_load<T, M, P::R2>(v, buf, lane);
+    wave_butterflies<T, M, P::R2, INV, true>(v, tw.w2, 0);
+    wave_lds_sync();
+    wave_store<T, M, P::R2, P::R1>(v, buf, lane);
+    wave_lds_sync();
+    wave_load<T, M, P::R3>(v, buf, lane);
+#pragma unroll
+    for (int b = 0; b < W::B3; ++b) {
+        Cx<T> w3[P::R3 - 1];
+        tw.pass3(lane, b, w3);
+#pragma unroll
+        for (int r = 1; r < P::R3; ++r) {
+            Cx<T> t = w3[r - 1];
+            if (INV) { t.y = -t.y; }
+            v[b * P::R3 + r] = cmul(v[b * P::R3 + r], t);
+        }
+        mfcl::dftR<P::R3, INV>(&v[b * P::R3]);
+    }
+    wave_lds_sync();
 }
 
 }  // namespace mfdev

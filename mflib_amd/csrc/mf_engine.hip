@@ -177,6 +177,7 @@ struct mf_handle {
     int signalSize = 0;
     int absval = 1;
     int userNfft = 0;
+    int userVariant = 0;  // 0: best available correlation kernel
     // templates
     std::vector<std::vector<double>> tmpl;
     bool haveShifts = false;
@@ -227,6 +228,16 @@ struct mf_handle {
     DevBuf dRho, dSpec, dTmpl, dTmplRaw, dHmax;
     DevBuf dSegs, dRowTab, dItems;
     DevBuf dStackRows, dStackOut;
+    // stacking groups (built on first stack/detect)
+    struct StackInfo {
+        int first = 0, count = 0, smax = 0, minShift = 0;
+        long long len = 0;
+    };
+    bool stacksReady = false;
+    std::vector<StackInfo> stackInfo;
+    size_t ldStack = 0;
+    size_t candCap = 0;
+    DevBuf dStacks, dCounts, dCandIdx, dCandVal, dGroupTab;
     std::map<int, TwTables> tw;
     struct Launch {
         int logM, first, count;
@@ -276,6 +287,13 @@ void dropDevice(mf_handle *h)
     h->dRowTab.release();
     h->dItems.release();
     h->dStackRows.release();
+    h->stacksReady = false;
+    h->stackInfo.clear();
+    h->dStacks.release();
+    h->dCounts.release();
+    h->dCandIdx.release();
+    h->dCandVal.release();
+    h->dGroupTab.release();
     h->dStackOut.release();
     h->tw.clear();
     h->launches.clear();
@@ -286,9 +304,10 @@ void dropDevice(mf_handle *h)
 }
 
 // Cost model of one lane for FFT size 2^(logM+1): blocks x (forward FFT +
-// one inverse FFT, product and epilogue per row), weighted by the LDS
-// occupancy of the kernel variant.  The measured autotuner (bench)
-// refines the choice through mf_set_fft_length.
+// one inverse FFT, product and epilogue per row), weighted per block size
+// by the measured efficiency of the fastest kernel variant there (the
+// weights are time / (blocks M (log2 M + 1.5)) on MI355X for 16 rows of
+// 512 samples, relative to M = 1024).  mf_set_fft_length overrides it.
 double laneCost(int logM, int nb, long long nOut, int nRows, int precision)
 {
     const long long N = 2LL << logM;
@@ -296,8 +315,8 @@ double laneCost(int logM, int nb, long long nOut, int nRows, int precision)
     if (L < 1) { return -1; }
     const double blocks = std::ceil(static_cast<double>(nOut) / static_cast<double>(L));
     const double M = static_cast<double>(N / 2);
-    static const double occF[] = {1.00, 1.00, 1.05, 1.20};  // logM 9..12, FP32
-    static const double occD[] = {1.00, 1.05, 1.20, 9.99};
+    static const double occF[] = {0.97, 1.00, 1.97, 3.30};  // logM 9..12, FP32
+    static const double occD[] = {1.00, 1.60, 2.40, 9.99};
     const double occ = (precision == MF_PRECISION_FLOAT ? occF : occD)[logM - mfk::kMinLogM];
     return blocks * (1.0 + 1.15 * nRows) * M * (logM + 1.5) * occ;
 }
@@ -421,7 +440,10 @@ void initialize(mf_handle *h)
             ln.nOut = std::max(ln.nOut, h->rows[r].nOut);
         }
         ln.logM = chooseLogM(h, nb, ln.nOut, static_cast<int>(chRows[c].size()));
+        // even block lengths keep every block start 8-byte aligned for the
+        // paired stores of the epilogue
         ln.L = static_cast<int>((2LL << ln.logM) - nb + 1);
+        if (ln.L > 1 && (ln.L & 1)) { ln.L -= 1; }
         ln.rows = chRows[c];
         std::stable_sort(ln.rows.begin(), ln.rows.end(),
                          [&](int a, int b) { return h->rows[a].Lt < h->rows[b].Lt; });
@@ -580,7 +602,6 @@ void initialize(mf_handle *h)
     if (!items.empty()) {
         MF_HIP(hipMemcpy(h->dItems.p, items.data(), items.size() * sizeof(mfk::Item), hipMemcpyHostToDevice));
     }
-    h->dStackRows.alloc(std::max<size_t>(nT, 1) * sizeof(mfk::StackRow));
     h->dStackOut.alloc(std::max<size_t>(h->ldOut, 64) * h->tsize);
     MF_HIP(hipStreamSynchronize(h->stream));
     h->tablesDirty = true;
@@ -639,13 +660,24 @@ void apply(mf_handle *h)
         const TwTables &t = h->tw.at(la.logM);
         const void *lo = h->precision == MF_PRECISION_FLOAT ? t.lo.p : t.lo64.p;
         const void *hi = h->precision == MF_PRECISION_FLOAT ? t.hi.p : t.hi64.p;
-        MF_HIP(mfk::launch_corr(h->precision, la.logM, h->dItems.as<mfk::Item>() + la.first, la.count,
-                                h->dSegs.as<mfk::Seg>(), h->dRowTab.as<mfk::Row>(), lo, hi, h->stream));
+        int variant = h->userVariant;
+        if (variant == 0 || !mfk::corr_variant_available(h->precision, la.logM, variant)) {
+            // wave-level kernel where it keeps >= 2 waves per SIMD (measured
+            // faster there); block-wide kernel for the larger blocks
+            const bool waveFaster = h->precision == MF_PRECISION_FLOAT ? la.logM <= 10 : la.logM <= 9;
+            variant = waveFaster && mfk::corr_variant_available(h->precision, la.logM, 2) ? 2 : 1;
+        }
+        MF_HIP(mfk::launch_corr(h->precision, la.logM, variant, h->dItems.as<mfk::Item>() + la.first,
+                                la.count, h->dSegs.as<mfk::Seg>(), h->dRowTab.as<mfk::Row>(), lo, hi,
+                                h->stream));
     }
     MF_HIP(hipEventRecord(h->evB, h->stream));
     h->applied = true;
 }
 
+// Stack weights exactly as shiftAndStack forms them
+// (src/matchedFilter.cpp:1330-1343): xnorm = 1/sum(w) accumulated in T,
+// weight = xnorm * w in T.
 template <typename T>
 void stackWeights(const mf_handle *h, const std::vector<int> &rows, std::vector<mfk::StackRow> &out,
                   int &smax)
@@ -657,7 +689,6 @@ void stackWeights(const mf_handle *h, const std::vector<int> &rows, std::vector<
         xnorm = xnorm + static_cast<T>(h->weights[r]);
     }
     if (xnorm > 0) { xnorm = 1 / xnorm; }
-    out.clear();
     for (int r : rows) {
         mfk::StackRow sr{};
         sr.rho = static_cast<const char *>(h->dRho.p) + static_cast<size_t>(r) * h->ldOut * h->tsize;
@@ -667,38 +698,110 @@ void stackWeights(const mf_handle *h, const std::vector<int> &rows, std::vector<
     }
 }
 
-void stack(mf_handle *h, int group, void *dOut, int cap, int *len)
+// Stack tables and per-group output/candidate buffers (built on first use
+// after initialize; they depend only on the template rows).
+void prepareStacks(mf_handle *h)
+{
+    if (h->stacksReady) { return; }
+    const int nG = static_cast<int>(h->groups.size());
+    std::vector<mfk::StackRow> all;
+    h->stackInfo.assign(nG, {});
+    size_t ldStack = 64;
+    for (int g = 0; g < nG; ++g) {
+        auto &gi = h->stackInfo[g];
+        gi.first = static_cast<int>(all.size());
+        int smax = 0;
+        if (h->precision == MF_PRECISION_FLOAT) {
+            stackWeights<float>(h, h->groups[g].rows, all, smax);
+        } else {
+            stackWeights<double>(h, h->groups[g].rows, all, smax);
+        }
+        gi.count = static_cast<int>(all.size()) - gi.first;
+        gi.smax = smax;
+        gi.minShift = smax;
+        for (int r : h->groups[g].rows) { gi.minShift = std::min(gi.minShift, h->shifts[r]); }
+        gi.len = static_cast<long long>(h->groups[g].nOut) - smax;
+        ldStack = std::max(ldStack, roundUp(static_cast<size_t>(std::max<long long>(gi.len, 1)), 64));
+    }
+    h->ldStack = ldStack;
+    h->dStackRows.alloc(std::max<size_t>(all.size(), 1) * sizeof(mfk::StackRow));
+    MF_HIP(hipMemcpy(h->dStackRows.p, all.data(), all.size() * sizeof(mfk::StackRow), hipMemcpyHostToDevice));
+    h->dStacks.alloc(ldStack * nG * h->tsize);
+    h->dCounts.alloc(nG * sizeof(int));
+    const size_t candCap = ldStack / 2 + 2;
+    h->candCap = candCap;
+    h->dCandIdx.alloc(candCap * nG * sizeof(int));
+    h->dCandVal.alloc(candCap * nG * h->tsize);
+    std::vector<mfk::StackGroup> tab(nG);
+    for (int g = 0; g < nG; ++g) {
+        const auto &gi = h->stackInfo[g];
+        tab[g].rows = h->dStackRows.as<mfk::StackRow>() + gi.first;
+        tab[g].nRows = gi.count;
+        tab[g].candCap = static_cast<int>(candCap);
+        tab[g].len = std::max<long long>(gi.len, 0);
+        tab[g].out = static_cast<char *>(h->dStacks.p) + static_cast<size_t>(g) * ldStack * h->tsize;
+        tab[g].count = h->dCounts.as<int>() + g;
+        tab[g].candIdx = h->dCandIdx.as<int>() + static_cast<size_t>(g) * candCap;
+        tab[g].candVal = static_cast<char *>(h->dCandVal.p) + static_cast<size_t>(g) * candCap * h->tsize;
+    }
+    h->dGroupTab.alloc(nG * sizeof(mfk::StackGroup));
+    MF_HIP(hipMemcpy(h->dGroupTab.p, tab.data(), tab.size() * sizeof(mfk::StackGroup), hipMemcpyHostToDevice));
+    h->stacksReady = true;
+}
+
+void checkStackable(mf_handle *h, int group)
 {
     requireInit(h);
     if (!h->applied) { fail(MF_ERR_BAD_STATE, "Matched filtered signals not yet computed"); }
     if (!h->haveShifts) { fail(MF_ERR_BAD_STATE, "Templates missing info - can't shift"); }
-    if (group < 0 || group >= static_cast<int>(h->groups.size())) {
-        fail(MF_ERR_INVALID_ARGUMENT, "group " + std::to_string(group) + " out of range");
+    const int nG = static_cast<int>(h->groups.size());
+    if (group < -1 || group >= nG) { fail(MF_ERR_INVALID_ARGUMENT, "group " + std::to_string(group) + " out of range"); }
+    prepareStacks(h);
+    for (int g = (group < 0 ? 0 : group); g < (group < 0 ? nG : group + 1); ++g) {
+        const auto &gi = h->stackInfo[g];
+        if (gi.minShift < 0) { fail(MF_ERR_INVALID_ARGUMENT, "negative moveout shifts are not supported"); }
+        if (gi.len < 0) {
+            fail(MF_ERR_INVALID_ARGUMENT, "maximum shift " + std::to_string(gi.smax) +
+                                              " exceeds the filtered signal length " +
+                                              std::to_string(h->groups[g].nOut));
+        }
     }
-    const auto &g = h->groups[group];
-    std::vector<mfk::StackRow> srows;
-    int smax = 0;
-    if (h->precision == MF_PRECISION_FLOAT) {
-        stackWeights<float>(h, g.rows, srows, smax);
-    } else {
-        stackWeights<double>(h, g.rows, srows, smax);
-    }
-    for (int r : g.rows) {
-        if (h->shifts[r] < 0) { fail(MF_ERR_INVALID_ARGUMENT, "negative moveout shifts are not supported"); }
-    }
-    const long long n = static_cast<long long>(g.nOut) - smax;
-    if (n < 0) {
-        fail(MF_ERR_INVALID_ARGUMENT, "maximum shift " + std::to_string(smax) +
-                                          " exceeds the filtered signal length " + std::to_string(g.nOut));
-    }
-    if (len) { *len = static_cast<int>(n); }
-    if (cap < n) { fail(MF_ERR_INVALID_ARGUMENT, "output holds " + std::to_string(cap) + " < " + std::to_string(n) + " samples"); }
-    MF_HIP(hipMemcpyAsync(h->dStackRows.p, srows.data(), srows.size() * sizeof(mfk::StackRow),
-                          hipMemcpyHostToDevice, h->stream));
+}
+
+// Stack of one group (group >= 0) or of every group (-1) into the
+// handle's stack buffers; asynchronous on the handle's stream.
+void stackGroups(mf_handle *h, int group)
+{
+    const int nG = static_cast<int>(h->groups.size());
+    const int g0 = group < 0 ? 0 : group;
+    const int ng = group < 0 ? nG : 1;
+    long long maxLen = 0;
+    for (int g = g0; g < g0 + ng; ++g) { maxLen = std::max(maxLen, h->stackInfo[g].len); }
     MF_HIP(hipEventRecord(h->evC, h->stream));
-    MF_HIP(mfk::launch_stack(h->precision, h->dStackRows.as<mfk::StackRow>(),
-                             static_cast<int>(srows.size()), n, h->absval, dOut, h->stream));
+    MF_HIP(mfk::launch_stack_groups(h->precision, h->dGroupTab.as<mfk::StackGroup>() + g0, ng, maxLen,
+                                    h->absval, h->stream));
     MF_HIP(hipEventRecord(h->evD, h->stream));
+}
+
+const void *stackPtr(const mf_handle *h, int g)
+{
+    return static_cast<const char *>(h->dStacks.p) + static_cast<size_t>(g) * h->ldStack * h->tsize;
+}
+
+void stack(mf_handle *h, int group, void *dOut, int cap, int *len)
+{
+    checkStackable(h, group);
+    if (group < 0) { fail(MF_ERR_INVALID_ARGUMENT, "group must be non-negative"); }
+    const long long n = h->stackInfo[group].len;
+    if (len) { *len = static_cast<int>(n); }
+    if (cap < n) {
+        fail(MF_ERR_INVALID_ARGUMENT, "output holds " + std::to_string(cap) + " < " + std::to_string(n) + " samples");
+    }
+    stackGroups(h, group);
+    if (dOut != nullptr && n > 0) {
+        MF_HIP(hipMemcpyAsync(dOut, stackPtr(h, group), static_cast<size_t>(n) * h->tsize,
+                              hipMemcpyDeviceToDevice, h->stream));
+    }
     MF_HIP(hipStreamSynchronize(h->stream));
     float ms = 0;
     if (hipEventElapsedTime(&ms, h->evC, h->evD) == hipSuccess) { h->stackMs = ms; }
@@ -732,31 +835,88 @@ void pruneCandidates(std::vector<std::pair<T, int>> &cand, int minDist, int *idx
     std::copy(keep.begin(), keep.end(), idx);
 }
 
-template <typename T>
-void peaksDevice(const void *dx, long long n, double threshold, int minDist, int *idx, int cap, int *nPeaks,
-                 hipStream_t stream)
+// Device workspace of the standalone peak finder, grown on demand and kept
+// for the life of the process (guarded by g_peakMutex).
+struct PeakWork {
+    hipStream_t stream = nullptr;
+    DevBuf count, idx, val, staged;
+};
+PeakWork &peakWork()
 {
-    const int candCap = static_cast<int>(std::min<long long>(n / 2 + 2, 0x7fffffffLL));
-    DevBuf dCount, dIdx, dVal;
-    dCount.alloc(sizeof(int));
-    dIdx.alloc(static_cast<size_t>(candCap) * sizeof(int));
-    dVal.alloc(static_cast<size_t>(candCap) * sizeof(T));
-    MF_HIP(hipMemsetAsync(dCount.p, 0, sizeof(int), stream));
-    MF_HIP(mfk::launch_peaks(sizeof(T) == 4 ? 0 : 1, dx, n, threshold, dCount.as<int>(), dIdx.as<int>(), dVal.p,
-                             candCap, stream));
-    int count = 0;
-    MF_HIP(hipMemcpyAsync(&count, dCount.p, sizeof(int), hipMemcpyDeviceToHost, stream));
-    MF_HIP(hipStreamSynchronize(stream));
-    if (count > candCap) { fail(MF_ERR_DEVICE, "peak candidate buffer overflow"); }
+    static PeakWork *w = new PeakWork();  // never destroyed: outlives static teardown order
+    return *w;
+}
+
+void growTo(DevBuf &b, size_t bytes)
+{
+    if (b.bytes < bytes) { b.alloc(bytes); }
+}
+
+// Copy `count` candidates to the host and apply the reference pruning.
+template <typename T>
+void collectCandidates(const int *dIdx, const void *dVal, int count, int minDist, int *idx, int cap,
+                       int *nPeaks, hipStream_t stream)
+{
     std::vector<int> ci(count);
     std::vector<T> cv(count);
     if (count > 0) {
-        MF_HIP(hipMemcpy(ci.data(), dIdx.p, count * sizeof(int), hipMemcpyDeviceToHost));
-        MF_HIP(hipMemcpy(cv.data(), dVal.p, count * sizeof(T), hipMemcpyDeviceToHost));
+        MF_HIP(hipMemcpyAsync(ci.data(), dIdx, count * sizeof(int), hipMemcpyDeviceToHost, stream));
+        MF_HIP(hipMemcpyAsync(cv.data(), dVal, count * sizeof(T), hipMemcpyDeviceToHost, stream));
+        MF_HIP(hipStreamSynchronize(stream));
     }
     std::vector<std::pair<T, int>> cand(count);
     for (int i = 0; i < count; ++i) { cand[i] = {cv[i], ci[i]}; }
     pruneCandidates<T>(cand, minDist, idx, cap, nPeaks);
+}
+
+template <typename T>
+void peaksDevice(const void *dx, long long n, double threshold, int minDist, int *idx, int cap, int *nPeaks,
+                 hipStream_t stream)
+{
+    PeakWork &w = peakWork();
+    const int candCap = static_cast<int>(std::min<long long>(n / 2 + 2, 0x7fffffffLL));
+    growTo(w.count, sizeof(int));
+    growTo(w.idx, static_cast<size_t>(candCap) * sizeof(int));
+    growTo(w.val, static_cast<size_t>(candCap) * sizeof(T));
+    MF_HIP(hipMemsetAsync(w.count.p, 0, sizeof(int), stream));
+    MF_HIP(mfk::launch_peaks(sizeof(T) == 4 ? 0 : 1, dx, n, threshold, w.count.as<int>(), w.idx.as<int>(), w.val.p,
+                             candCap, stream));
+    int count = 0;
+    MF_HIP(hipMemcpyAsync(&count, w.count.p, sizeof(int), hipMemcpyDeviceToHost, stream));
+    MF_HIP(hipStreamSynchronize(stream));
+    if (count > candCap) { fail(MF_ERR_DEVICE, "peak candidate buffer overflow"); }
+    collectCandidates<T>(w.idx.as<int>(), w.val.p, count, minDist, idx, cap, nPeaks, stream);
+}
+
+// Stack every group and find the peaks of every stack (one launch each),
+// then prune per group on the host.
+template <typename T>
+void detectT(mf_handle *h, double threshold, int minDist, int *idx, int cap, int *offsets)
+{
+    const int nG = static_cast<int>(h->groups.size());
+    long long maxLen = 0;
+    for (const auto &gi : h->stackInfo) { maxLen = std::max(maxLen, gi.len); }
+    MF_HIP(hipMemsetAsync(h->dCounts.p, 0, nG * sizeof(int), h->stream));
+    stackGroups(h, -1);
+    MF_HIP(mfk::launch_peak_groups(h->precision, h->dGroupTab.as<mfk::StackGroup>(), nG, maxLen, threshold,
+                                   h->stream));
+    std::vector<int> counts(nG);
+    MF_HIP(hipMemcpyAsync(counts.data(), h->dCounts.p, nG * sizeof(int), hipMemcpyDeviceToHost, h->stream));
+    MF_HIP(hipStreamSynchronize(h->stream));
+    float ms = 0;
+    if (hipEventElapsedTime(&ms, h->evC, h->evD) == hipSuccess) { h->stackMs = ms; }
+    int off = 0;
+    for (int g = 0; g < nG; ++g) {
+        if (counts[g] > static_cast<int>(h->candCap)) { fail(MF_ERR_DEVICE, "peak candidate buffer overflow"); }
+        if (h->stackInfo[g].len <= 0) { counts[g] = 0; }
+        offsets[g] = off;
+        int np = 0;
+        collectCandidates<T>(h->dCandIdx.as<int>() + static_cast<size_t>(g) * h->candCap,
+                             static_cast<const char *>(h->dCandVal.p) + static_cast<size_t>(g) * h->candCap * h->tsize,
+                             counts[g], minDist, idx + off, cap - off, &np, h->stream);
+        off += np;
+    }
+    offsets[nG] = off;
 }
 
 void peaks(const void *x, bool onDevice, int n, int precision, double threshold, int minDist, int *idx, int cap,
@@ -773,14 +933,14 @@ void peaks(const void *x, bool onDevice, int n, int precision, double threshold,
     }
     const size_t ts = precision == MF_PRECISION_FLOAT ? 4 : 8;
     std::lock_guard<std::mutex> lock(g_peakMutex);
-    static hipStream_t stream = nullptr;
-    if (stream == nullptr) { MF_HIP(hipStreamCreate(&stream)); }
-    DevBuf staged;
+    PeakWork &w = peakWork();
+    if (w.stream == nullptr) { MF_HIP(hipStreamCreate(&w.stream)); }
+    hipStream_t stream = w.stream;
     const void *dx = x;
     if (!onDevice) {
-        staged.alloc(static_cast<size_t>(n) * ts);
-        MF_HIP(hipMemcpyAsync(staged.p, x, static_cast<size_t>(n) * ts, hipMemcpyHostToDevice, stream));
-        dx = staged.p;
+        growTo(w.staged, static_cast<size_t>(n) * ts);
+        MF_HIP(hipMemcpyAsync(w.staged.p, x, static_cast<size_t>(n) * ts, hipMemcpyHostToDevice, stream));
+        dx = w.staged.p;
     }
     if (precision == MF_PRECISION_FLOAT) {
         peaksDevice<float>(dx, n, static_cast<float>(threshold), minDist, idx, cap, nPeaks, stream);
@@ -937,6 +1097,15 @@ int mf_set_batch_layout(mf_handle *h, int nChannels, const int *channelLengths, 
         h->rowChIn.assign(templateChannel, templateChannel + nT);
         h->rowGroupIn.assign(templateGroup, templateGroup + nT);
         h->haveLayout = true;
+    });
+}
+
+int mf_set_kernel_variant(mf_handle *h, int variant)
+{
+    return guard([&] {
+        requireHandle(h);
+        if (variant < 0 || variant > 2) { fail(MF_ERR_INVALID_ARGUMENT, "kernel variant must be 0, 1 or 2"); }
+        h->userVariant = variant;
     });
 }
 
@@ -1128,10 +1297,13 @@ int mf_shift_and_stack(mf_handle *h, int group, void *out, int cap, int *len)
     return guard([&] {
         requireInit(h);
         int n = 0;
-        stack(h, group, h->dStackOut.p, static_cast<int>(h->dStackOut.bytes / h->tsize), &n);
+        checkStackable(h, group);
+        if (group < 0) { fail(MF_ERR_INVALID_ARGUMENT, "group must be non-negative"); }
+        n = static_cast<int>(h->stackInfo[group].len);
         if (len) { *len = n; }
         if (out == nullptr || cap < n) { fail(MF_ERR_INVALID_ARGUMENT, "output buffer too small"); }
-        MF_HIP(hipMemcpyAsync(out, h->dStackOut.p, static_cast<size_t>(n) * h->tsize, hipMemcpyDeviceToHost,
+        stack(h, group, nullptr, n, &n);
+        MF_HIP(hipMemcpyAsync(out, stackPtr(h, group), static_cast<size_t>(n) * h->tsize, hipMemcpyDeviceToHost,
                               h->stream));
         MF_HIP(hipStreamSynchronize(h->stream));
     });
@@ -1143,6 +1315,43 @@ int mf_shift_and_stack_device(mf_handle *h, int group, void *dout, int cap, int 
         requireInit(h);
         if (dout == nullptr) { fail(MF_ERR_INVALID_ARGUMENT, "output is NULL"); }
         stack(h, group, dout, cap, len);
+    });
+}
+
+int mf_detect(mf_handle *h, double threshold, int minDist, int *idx, int cap, int *offsets)
+{
+    return guard([&] {
+        checkStackable(h, -1);
+        if (threshold < 0 || threshold > 1) {
+            fail(MF_ERR_INVALID_ARGUMENT, "Threshold = " + std::to_string(threshold) + " must be in the range [0,1]");
+        }
+        if (offsets == nullptr || (idx == nullptr && cap > 0)) { fail(MF_ERR_INVALID_ARGUMENT, "output is NULL"); }
+        if (h->precision == MF_PRECISION_FLOAT) {
+            detectT<float>(h, threshold, minDist, idx, cap, offsets);
+        } else {
+            detectT<double>(h, threshold, minDist, idx, cap, offsets);
+        }
+    });
+}
+
+int mf_get_stack_device(mf_handle *h, int group, const void **dptr, int *len)
+{
+    return guard([&] {
+        requireInit(h);
+        if (!h->stacksReady) { fail(MF_ERR_BAD_STATE, "no stack computed yet"); }
+        if (group < 0 || group >= static_cast<int>(h->groups.size())) {
+            fail(MF_ERR_INVALID_ARGUMENT, "group " + std::to_string(group) + " out of range");
+        }
+        if (dptr) { *dptr = stackPtr(h, group); }
+        if (len) { *len = static_cast<int>(h->stackInfo[group].len); }
+    });
+}
+
+int mf_get_number_of_groups(const mf_handle *h, int *n)
+{
+    return guard([&] {
+        requireInit(h);
+        if (n) { *n = static_cast<int>(h->groups.size()); }
     });
 }
 

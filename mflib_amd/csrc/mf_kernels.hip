@@ -35,22 +35,23 @@ namespace {
 
 constexpr int NT = kThreads;
 
+// Error model of the FFT numerator: |du| <= kNoiseK * eps * sqrt(log2 nfft)
+// * max|T^| * rms(block).  A window is recomputed exactly when that bound,
+// divided by the window's own norm, could exceed the precision's rho
+// tolerance (1e-6 for FP32 correlation, 1e-12 for FP64).
 template <typename T>
 struct Eps;
 template <>
 struct Eps<float> {
     static constexpr double value = 5.9604644775390625e-08;  // 2^-24
+    static constexpr double rhoTol = 1.0e-6;
 };
 template <>
 struct Eps<double> {
     static constexpr double value = 1.1102230246251565e-16;  // 2^-53
+    static constexpr double rhoTol = 1.0e-12;
 };
-
-// Error model of the FFT numerator: |du| <= kNoiseK * eps * sqrt(log2 nfft)
-// * max|T^| * rms(block).  A window is recomputed exactly when that bound,
-// divided by the window's own norm, could exceed kRhoTol.
 constexpr double kNoiseK = 4.0;
-constexpr double kRhoTol = 1.0e-6;
 
 template <typename T, int M>
 __global__ __launch_bounds__(NT) void corr_kernel(const Item *__restrict__ items,
@@ -146,7 +147,7 @@ __global__ __launch_bounds__(NT) void corr_kernel(const Item *__restrict__ items
     __syncthreads();
 
     const double rmsBlk = sqrt(energy / nSamp);
-    const double noise = kNoiseK / kRhoTol * Eps<T>::value * sqrt(static_cast<double>(LOGN)) *
+    const double noise = kNoiseK / Eps<T>::rhoTol * Eps<T>::value * sqrt(static_cast<double>(LOGN)) *
                          static_cast<double>(sg.hmax) * rmsBlk;
     const int wave = tid >> 6;
     const int lane = tid & 63;
@@ -269,6 +270,328 @@ __global__ __launch_bounds__(NT) void corr_kernel(const Item *__restrict__ items
     }
 }
 
+// ---------------------------------------------------------------------------
+// corr2_kernel: same contract as corr_kernel, built from wave-level FFTs.
+// Wave 0 computes the forward FFT while waves 1-3 build the normalisation
+// scale; then every wave takes whole template rows (product with the
+// template spectrum straight into pass-1 registers, three register passes,
+// epilogue straight from pass-3 registers), so the row loop has no block
+// barrier.  Rows are processed in groups of equal template length; the
+// scale of a group is shared by its rows.
+// ---------------------------------------------------------------------------
+
+// Per-output scale sqrt(Lt)/||x - mean|| for outputs [0, nValid) of this
+// block, FP64 sliding sums restarted every C outputs; flagged outputs get
+// -1.  Threads [tid0, tid0 + nthr) take part.  Returns this thread's number
+// of flagged outputs.
+template <typename T, int N>
+__device__ __forceinline__ int compute_scale(const T *raw, const double *ts1, const double *ts2, T *sc,
+                                             double a, int nValid, int Lt, double noise, int t, int nthr)
+{
+    const double sqrtLt = sqrt(static_cast<double>(Lt));
+    const double denThr = sqrtLt * noise;
+    const int C = (nValid + nthr - 1) / nthr;
+    const double canc = 2.0e-9 * (Lt + C) * Lt;
+    const int i0 = t * C;
+    const int i1 = min(i0 + C, nValid);
+    int nflag = 0;
+    if (i0 >= i1) { return 0; }
+    double S1 = 0, S2 = 0;
+    const int lo = i0, hi = i0 + Lt;
+    const int q0 = (lo + 15) >> 4, q1 = hi >> 4;
+    if (q0 < q1) {
+        for (int j = lo; j < 16 * q0; ++j) {
+            const double d = static_cast<double>(raw[j]) - a;
+            S1 += d;
+            S2 += d * d;
+        }
+        for (int q = q0; q < q1; ++q) {
+            S1 += ts1[q];
+            S2 += ts2[q];
+        }
+        for (int j = 16 * q1; j < hi; ++j) {
+            const double d = static_cast<double>(raw[j]) - a;
+            S1 += d;
+            S2 += d * d;
+        }
+    } else {
+        for (int j = lo; j < hi; ++j) {
+            const double d = static_cast<double>(raw[j]) - a;
+            S1 += d;
+            S2 += d * d;
+        }
+    }
+    double S2max = S2;
+    for (int i = i0; i < i1; ++i) {
+        const double den2 = Lt * S2 - S1 * S1;
+        const double den = sqrt(fmax(den2, 0.0));
+        const bool flag = !(den2 > canc * S2max) || !(den > denThr);
+        nflag += flag ? 1 : 0;
+        sc[i] = flag ? T(-1) : static_cast<T>(sqrtLt / den);
+        if (i + 1 < i1) {
+            const double dn = static_cast<double>(raw[i + Lt]) - a;
+            const double dd = static_cast<double>(raw[i]) - a;
+            S1 += dn - dd;
+            S2 += dn * dn - dd * dd;
+            S2max = fmax(S2max, S2);
+        }
+    }
+    return nflag;
+}
+
+// Exact FP64 two-pass Pearson for the flagged outputs of one row (one wave).
+template <typename T>
+__device__ __forceinline__ void fixup_row_wave(const T *raw, const T *sc, gptr<const double> tmpl, int Lt,
+                                               gptr<T> out, int nv, int lane)
+{
+    for (int w0 = 0; w0 < nv; w0 += 64) {
+        const int i = w0 + lane;
+        const bool fl = (i < nv) && (sc[i] < T(0));
+        unsigned long long m = __ballot(fl);
+        while (m != 0ull) {
+            const int b = __ffsll(static_cast<long long>(m)) - 1;
+            m &= m - 1ull;
+            const int wi = w0 + b;
+            double sm = 0;
+            for (int mm = lane; mm < Lt; mm += 64) { sm += static_cast<double>(raw[wi + mm]); }
+            sm = wave_sum(sm);
+            const double mean = sm / Lt;
+            double e2 = 0, dn = 0;
+            for (int mm = lane; mm < Lt; mm += 64) {
+                const double d = static_cast<double>(raw[wi + mm]) - mean;
+                e2 += d * d;
+                dn += tmpl[mm] * d;
+            }
+            e2 = wave_sum(e2);
+            dn = wave_sum(dn);
+            double rho = dn / fmax(DBL_EPSILON, sqrt(e2));
+            if (fabs(rho) > 1.1) { rho = 0; }
+            if (lane == 0) { out[wi] = static_cast<T>(rho); }
+        }
+    }
+}
+
+template <typename T>
+struct Vec2;
+template <>
+struct Vec2<float> {
+    using type = float2;
+};
+template <>
+struct Vec2<double> {
+    using type = double2;
+};
+
+// Waves per SIMD that the LDS footprint of corr2_kernel<T, M> allows; the
+// register budget is capped to match so registers never lower occupancy.
+#ifndef MF_CAP
+#define MF_CAP 1
+#endif
+template <typename T, int M>
+struct Corr2Waves {
+    static constexpr int value =
+        !MF_CAP ? 1 : (sizeof(T) * M <= 2048) ? 3 : (sizeof(T) * M <= 4096) ? 2 : 1;
+};
+
+template <typename T, int M>
+__global__ __launch_bounds__(NT, (Corr2Waves<T, M>::value)) void corr2_kernel(const Item *__restrict__ items,
+                                                   const Seg *__restrict__ segs,
+                                                   const Row *__restrict__ rows,
+                                                   const Cx<T> *__restrict__ twloG,
+                                                   const Cx<T> *__restrict__ twhiG)
+{
+    constexpr int N = 2 * M;
+    constexpr int NTILE = N / 16;
+    constexpr int SPT = N / NT;
+    constexpr int LOGN = Log2<N>::value;
+    constexpr int Q = M / 64;
+    constexpr int KP = (M / 2 + 1 + 63) / 64;  // (k, M-k) pair steps per lane
+    constexpr int NWAVE = NT / 64;
+    using P = WavePlan<M>;
+    static_assert(N % NT == 0 && NWAVE == 4, "block shape");
+    __shared__ T raw[N];
+    __shared__ Cx<T> Xs[M + 1];
+    __shared__ __align__(16) T sc[N];
+    __shared__ Cx<T> xbuf[NWAVE][XBuf<M>::size];
+    __shared__ double ts1[NTILE];
+    __shared__ double ts2[NTILE];
+    __shared__ Cx<T> twlo[64];
+    __shared__ Cx<T> twhi[N / 64];
+    __shared__ double red[NWAVE];
+
+    const Item item = items[blockIdx.x];
+    const Seg sg = segs[item.seg];
+    const int tid = threadIdx.x;
+    const int wave = tid >> 6;
+    const int lane = tid & 63;
+    const long long base = static_cast<long long>(item.block) * sg.L;
+    const int nValidSeg = static_cast<int>(min(static_cast<long long>(sg.L), sg.nOut - base));
+
+    if (sg.dead) {
+        for (int r = sg.rowBegin; r < sg.rowEnd; ++r) {
+            const Row row = rows[r];
+            const int nv = static_cast<int>(min(static_cast<long long>(sg.L), row.nOut - base));
+            gptr<T> out = to_global(static_cast<T *>(row.out)) + base;
+            for (int i = tid; i < nv; i += NT) { out[i] = T(0); }
+        }
+        return;
+    }
+    for (int i = tid; i < 64; i += NT) { twlo[i] = twloG[i]; }
+    for (int i = tid; i < N / 64; i += NT) { twhi[i] = twhiG[i]; }
+    const Twiddle<T> tw{twlo, twhi};
+
+    // A. load, anchor, tile sums, energy; packed anchored input -> xbuf[0]
+    const gptr<const T> x = to_global(static_cast<const T *>(sg.x));
+    const int nSamp = static_cast<int>(min(static_cast<long long>(N), sg.n - base));
+    T v0[SPT];
+    double s = 0;
+#pragma unroll
+    for (int q = 0; q < SPT; ++q) {
+        const int j = tid + q * NT;
+        const T val = (j < nSamp) ? x[base + j] : T(0);
+        v0[q] = val;
+        raw[j] = val;
+        s += static_cast<double>(val);
+    }
+    const double a = block_sum<NT>(s, red) / nSamp;
+    const T aT = static_cast<T>(a);
+    double e = 0;
+    T *xin = reinterpret_cast<T *>(xbuf[0]);
+#pragma unroll
+    for (int q = 0; q < SPT; ++q) {
+        const int j = tid + q * NT;
+        const bool valid = j < nSamp;
+        const double d = valid ? static_cast<double>(v0[q]) - a : 0.0;
+        double t1 = d, t2 = d * d;
+        e += t2;
+#pragma unroll
+        for (int o = 8; o > 0; o >>= 1) {
+            t1 += __shfl_xor(t1, o, 64);
+            t2 += __shfl_xor(t2, o, 64);
+        }
+        if ((tid & 15) == 0) {
+            ts1[j >> 4] = t1;
+            ts2[j >> 4] = t2;
+        }
+        xin[2 * xpad(j >> 1) + (j & 1)] = valid ? static_cast<T>(v0[q] - aT) : T(0);
+    }
+    const double energy = block_sum<NT>(e, red);
+    const double rmsBlk = sqrt(energy / nSamp);
+    const double noise = kNoiseK / Eps<T>::rhoTol * Eps<T>::value * sqrt(static_cast<double>(LOGN)) *
+                         static_cast<double>(sg.hmax) * rmsBlk;
+    WaveTwiddles<T, M> wt;
+    wt.init(lane, tw);
+
+    Cx<T> v[Q];
+    int g0 = sg.rowBegin;
+    bool first = true;
+    while (g0 < sg.rowEnd) {
+        const int Lt = rows[g0].Lt;
+        int g1 = g0 + 1;
+        while (g1 < sg.rowEnd && rows[g1].Lt == Lt) { ++g1; }
+        int myFlags = 0;
+        if (first) {
+            // B. wave 0: forward FFT + unpack into Xs; waves 1..3: scale
+            if (wave == 0) {
+                Cx<T> *buf = xbuf[0];
+                wave_load<T, M, P::R1>(v, buf, lane);
+                wave_lds_sync();
+                wave_fft_regs<T, M, false>(v, buf, lane, wt);
+                // natural-order store of pass 3 (point j + r M/R3, j = lane + 64 b)
+                constexpr int B3 = M / (P::R3 * 64);
+                Cx<T> *po = buf + xpad(lane);
+#pragma unroll
+                for (int b = 0; b < B3; ++b) {
+#pragma unroll
+                    for (int r = 0; r < P::R3; ++r) { po[cpad(64 * b + r * (M / P::R3))] = v[b * P::R3 + r]; }
+                }
+                wave_lds_sync();
+                // unpack pairs (k, M-k), k = lane + 64 t
+                const Cx<T> *pm = buf + xpad(M - lane);
+                const Cx<T> wlo = twlo[lane];
+#pragma unroll
+                for (int t = 0; t < KP; ++t) {
+                    const int k = lane + 64 * t;
+                    if (k <= M / 2) {
+                        const Cx<T> zk = po[68 * t];
+                        const Cx<T> zm = cconj(k == 0 ? po[0] : pm[-68 * t]);
+                        const Cx<T> E = cadd(zk, zm);
+                        const Cx<T> O = cmulmi(csub(zk, zm));
+                        const Cx<T> WO = cmul(cmul(wlo, twhi[t]), O);
+                        Xs[k] = cadd(E, WO);
+                        Xs[M - k] = cconj(csub(E, WO));
+                    }
+                }
+                wave_lds_sync();
+            } else {
+                myFlags = compute_scale<T, N>(raw, ts1, ts2, sc, a, nValidSeg, Lt, noise, tid - 64, NT - 64);
+            }
+            first = false;
+        } else {
+            myFlags = compute_scale<T, N>(raw, ts1, ts2, sc, a, nValidSeg, Lt, noise, tid, NT);
+        }
+        const bool anyFlag = block_sum<NT>(static_cast<double>(myFlags), red) > 0.0;
+
+        // C. rows of this length, one wave per row
+        for (int r = g0 + wave; r < g1; r += NWAVE) {
+            const Row row = rows[r];
+            const int nvRow = static_cast<int>(min(static_cast<long long>(sg.L), row.nOut - base));
+            if (nvRow <= 0) { continue; }
+            Cx<T> *buf = xbuf[wave];
+            // spectral product + inverse packing, one (k, M-k) pair per lane
+            // step (k = lane + 64 t; tw(k) = lo[lane] hi[t])
+            const gptr<const Cx<T>> H = to_global(static_cast<const Cx<T> *>(row.H));
+            const Cx<T> wlo = twlo[lane];
+            Cx<T> *pk = buf + xpad(lane);
+            Cx<T> *pm = buf + xpad(M - lane);
+#pragma unroll
+            for (int t = 0; t < KP; ++t) {
+                const int k = lane + 64 * t;
+                if (k <= M / 2) {
+                    const Cx<T> yk = cmul(Xs[k], load_cx(H, k));
+                    const Cx<T> ym = cmul(Xs[M - k], load_cx(H, M - k));
+                    const Cx<T> S = cadd(yk, cconj(ym));
+                    const Cx<T> D = csub(yk, cconj(ym));
+                    const Cx<T> Pp = cmul(cconj(cmul(wlo, twhi[t])), D);
+                    pk[68 * t] = cadd(S, cmuli(Pp));
+                    if (k > 0) { pm[-68 * t] = cadd(cconj(S), cmuli(cconj(Pp))); }
+                }
+            }
+            wave_lds_sync();
+            wave_load<T, M, P::R1>(v, buf, lane);
+            wave_lds_sync();
+            wave_fft_regs<T, M, true>(v, buf, lane, wt);
+            // normalisation epilogue: rho = c * scale, paired stores.  Flagged
+            // windows (scale -1) are overwritten below by the exact fix-up;
+            // for the others |rho| <= 1 + O(eps), so the reference's tooBig
+            // rule (|rho| > 1.1 -> 0) cannot fire and is not evaluated.
+            gptr<T> outRow = to_global(static_cast<T *>(row.out)) + base;
+            gptr<T> outl = outRow + 2 * lane;
+            const T *scl = sc + 2 * lane;
+            constexpr int B3 = M / (P::R3 * 64);
+#pragma unroll
+            for (int b = 0; b < B3; ++b) {
+#pragma unroll
+                for (int rr = 0; rr < P::R3; ++rr) {
+                    const int off = 2 * (64 * b + rr * (M / P::R3));
+                    const int i0 = 2 * lane + off;
+                    const Cx<T> c = v[b * P::R3 + rr];
+                    if (i0 + 1 < nvRow) {
+                        const typename Vec2<T>::type f = *reinterpret_cast<const typename Vec2<T>::type *>(scl + off);
+                        outl[off] = c.x * f.x;
+                        outl[off + 1] = c.y * f.y;
+                    } else if (i0 < nvRow) {
+                        outl[off] = c.x * scl[off];
+                    }
+                }
+            }
+            if (anyFlag) { fixup_row_wave<T>(raw, sc, to_global(row.tmpl), row.Lt, outRow, nvRow, lane); }
+        }
+        g0 = g1;
+        if (g0 < sg.rowEnd) { __syncthreads(); }
+    }
+}
+
 // Template demean / unit-norm (FP64) and spectrum conj(T^)/(2 nfft).
 template <int M>
 __global__ __launch_bounds__(NT) void tmpl_kernel(const TmplJob *__restrict__ jobs, int precision,
@@ -338,32 +661,57 @@ __global__ __launch_bounds__(NT) void tmpl_kernel(const TmplJob *__restrict__ jo
 // out[k] = sum_r w_r * |rho_r[k + s_r]| in row order, no contraction,
 // as the reference's shiftAndStack loop (src/matchedFilter.cpp:1341-1361).
 template <typename T>
-__global__ __launch_bounds__(NT) void stack_kernel(const StackRow *__restrict__ srows, int nRows,
-                                                   long long len, int absval, T *__restrict__ out)
+__global__ __launch_bounds__(NT) void stack_kernel(const StackGroup *__restrict__ groups, int absval)
 {
     // hipcc contracts a*b+c into an FMA by default; the stack keeps the
     // separately rounded product and sum of the loop it restates.
 #pragma clang fp contract(off)
+    const StackGroup grp = groups[blockIdx.y];
+    const StackRow *__restrict__ srows = grp.rows;
+    const int nRows = grp.nRows;
+    const long long len = grp.len;
+    T *__restrict__ out = static_cast<T *>(grp.out);
+    // Each thread owns U outputs a grid stride apart (coalesced across the
+    // wave) and walks the rows in order; the U loads of a row are
+    // independent, and the row loop is unrolled so several rows' loads are
+    // in flight while the per-output sums keep their sequential order.
+    constexpr int U = 4;
     const long long stride = static_cast<long long>(gridDim.x) * NT;
-    for (long long k = static_cast<long long>(blockIdx.x) * NT + threadIdx.x; k < len; k += stride) {
-        T acc = T(0);
+    gptr<T> o = to_global(out);
+    for (long long k0 = static_cast<long long>(blockIdx.x) * NT + threadIdx.x; k0 < len; k0 += stride * U) {
+        T acc[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) { acc[u] = T(0); }
+#pragma unroll 4
         for (int r = 0; r < nRows; ++r) {
             const StackRow sr = srows[r];
-            T v = static_cast<const T *>(sr.rho)[k + sr.shift];
-            if (absval) { v = fabs(v); }
+            const gptr<const T> p = to_global(static_cast<const T *>(sr.rho)) + sr.shift;
             const T w = static_cast<T>(sr.weight);
-            acc = acc + w * v;
+            T v[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const long long k = k0 + u * stride;
+                v[u] = k < len ? p[k] : T(0);
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const T a = absval ? fabs(v[u]) : v[u];
+                acc[u] = acc[u] + w * a;
+            }
         }
-        out[k] = acc;
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const long long k = k0 + u * stride;
+            if (k < len) { o[k] = acc[u]; }
+        }
     }
 }
 
 // Candidates of src/peakFinder.cpp:500-521: strictly above threshold and
 // a local maximum (plateaus keep their last sample; end-point rules).
 template <typename T>
-__global__ __launch_bounds__(NT) void peak_kernel(const T *__restrict__ x, long long n, T thr,
-                                                  int *__restrict__ count, int *__restrict__ idx,
-                                                  T *__restrict__ val, int cap)
+__device__ __forceinline__ void peak_scan(gptr<const T> x, long long n, T thr, int *count, gptr<int> idx,
+                                          gptr<T> val, int cap)
 {
     const long long stride = static_cast<long long>(gridDim.x) * NT;
     const long long start = static_cast<long long>(blockIdx.x) * NT + threadIdx.x;
@@ -406,6 +754,23 @@ __global__ __launch_bounds__(NT) void peak_kernel(const T *__restrict__ x, long 
 }
 
 template <typename T>
+__global__ __launch_bounds__(NT) void peak_kernel(const T *__restrict__ x, long long n, T thr,
+                                                  int *__restrict__ count, int *__restrict__ idx,
+                                                  T *__restrict__ val, int cap)
+{
+    peak_scan<T>(to_global(x), n, thr, count, to_global(idx), to_global(val), cap);
+}
+
+// The candidates of every group's stack (grid.y = group).
+template <typename T>
+__global__ __launch_bounds__(NT) void peak_groups_kernel(const StackGroup *__restrict__ groups, T thr)
+{
+    const StackGroup g = groups[blockIdx.y];
+    peak_scan<T>(to_global(static_cast<const T *>(g.out)), g.len, thr, g.count, to_global(g.candIdx),
+                 to_global(static_cast<T *>(g.candVal)), g.candCap);
+}
+
+template <typename T>
 __global__ __launch_bounds__(NT) void minmax_kernel(const T *__restrict__ x, long long n,
                                                     double *__restrict__ pmin,
                                                     double *__restrict__ pmax)
@@ -435,6 +800,15 @@ hipError_t corr_launch(const Item *dItems, int nItems, const Seg *dSegs, const R
     return hipGetLastError();
 }
 
+template <typename T, int M>
+hipError_t corr2_launch(const Item *dItems, int nItems, const Seg *dSegs, const Row *dRows,
+                        const void *lo, const void *hi, hipStream_t stream)
+{
+    hipLaunchKernelGGL((corr2_kernel<T, M>), dim3(nItems), dim3(NT), 0, stream, dItems, dSegs,
+                       dRows, static_cast<const Cx<T> *>(lo), static_cast<const Cx<T> *>(hi));
+    return hipGetLastError();
+}
+
 template <int M>
 hipError_t tmpl_launch(int precision, const TmplJob *dJobs, int nJobs, const void *lo,
                        const void *hi, hipStream_t stream)
@@ -446,10 +820,33 @@ hipError_t tmpl_launch(int precision, const TmplJob *dJobs, int nJobs, const voi
 
 }  // namespace
 
-hipError_t launch_corr(int precision, int logM, const Item *dItems, int nItems, const Seg *dSegs,
-                       const Row *dRows, const void *dTwLo, const void *dTwHi, hipStream_t stream)
+bool corr_variant_available(int precision, int logM, int variant)
+{
+    if (variant == 2) { return precision == 0 ? (logM >= 9 && logM <= 11) : (logM >= 9 && logM <= 10); }
+    if (variant == 1) { return precision == 0 ? (logM >= 9 && logM <= 12) : (logM >= 9 && logM <= 11); }
+    return false;
+}
+
+hipError_t launch_corr(int precision, int logM, int variant, const Item *dItems, int nItems,
+                       const Seg *dSegs, const Row *dRows, const void *dTwLo, const void *dTwHi,
+                       hipStream_t stream)
 {
     if (nItems <= 0) { return hipSuccess; }
+    if (variant == 2) {
+        if (precision == 0) {
+            switch (logM) {
+            case 9: return corr2_launch<float, 512>(dItems, nItems, dSegs, dRows, dTwLo, dTwHi, stream);
+            case 10: return corr2_launch<float, 1024>(dItems, nItems, dSegs, dRows, dTwLo, dTwHi, stream);
+            case 11: return corr2_launch<float, 2048>(dItems, nItems, dSegs, dRows, dTwLo, dTwHi, stream);
+            default: return hipErrorInvalidValue;
+            }
+        }
+        switch (logM) {
+        case 9: return corr2_launch<double, 512>(dItems, nItems, dSegs, dRows, dTwLo, dTwHi, stream);
+        case 10: return corr2_launch<double, 1024>(dItems, nItems, dSegs, dRows, dTwLo, dTwHi, stream);
+        default: return hipErrorInvalidValue;
+        }
+    }
     if (precision == 0) {
         switch (logM) {
         case 9: return corr_launch<float, 512>(dItems, nItems, dSegs, dRows, dTwLo, dTwHi, stream);
@@ -480,18 +877,32 @@ hipError_t launch_tmpl(int precision, int logM, const TmplJob *dJobs, int nJobs,
     }
 }
 
-hipError_t launch_stack(int precision, const StackRow *dRows, int nRows, long long len,
-                        int absval, void *dOut, hipStream_t stream)
+hipError_t launch_stack_groups(int precision, const StackGroup *dGroups, int nGroups, long long maxLen,
+                               int absval, hipStream_t stream)
 {
-    if (len <= 0) { return hipSuccess; }
-    const long long want = (len + NT - 1) / NT;
-    const int grid = static_cast<int>(want < 65536 ? want : 65536);
+    if (maxLen <= 0 || nGroups <= 0) { return hipSuccess; }
+    const long long want = (maxLen + 4LL * NT - 1) / (4LL * NT);  // 4 outputs per thread
+    const int grid = static_cast<int>(want < 16384 ? want : 16384);
     if (precision == 0) {
-        hipLaunchKernelGGL(stack_kernel<float>, dim3(grid), dim3(NT), 0, stream, dRows, nRows, len,
-                           absval, static_cast<float *>(dOut));
+        hipLaunchKernelGGL(stack_kernel<float>, dim3(grid, nGroups), dim3(NT), 0, stream, dGroups, absval);
     } else {
-        hipLaunchKernelGGL(stack_kernel<double>, dim3(grid), dim3(NT), 0, stream, dRows, nRows,
-                           len, absval, static_cast<double *>(dOut));
+        hipLaunchKernelGGL(stack_kernel<double>, dim3(grid, nGroups), dim3(NT), 0, stream, dGroups, absval);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_peak_groups(int precision, const StackGroup *dGroups, int nGroups, long long maxLen,
+                              double threshold, hipStream_t stream)
+{
+    if (maxLen <= 0 || nGroups <= 0) { return hipSuccess; }
+    const long long want = (maxLen + NT - 1) / NT;
+    const int grid = static_cast<int>(want < 8192 ? want : 8192);
+    if (precision == 0) {
+        hipLaunchKernelGGL(peak_groups_kernel<float>, dim3(grid, nGroups), dim3(NT), 0, stream, dGroups,
+                           static_cast<float>(threshold));
+    } else {
+        hipLaunchKernelGGL(peak_groups_kernel<double>, dim3(grid, nGroups), dim3(NT), 0, stream, dGroups,
+                           threshold);
     }
     return hipGetLastError();
 }

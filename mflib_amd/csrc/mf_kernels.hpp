@@ -55,16 +55,37 @@ struct StackRow {
     double weight;    // normalised weight (already rounded to T)
 };
 
-// Correlation + normalisation: one launch per FFT size.
-hipError_t launch_corr(int precision, int logM, const Item *dItems, int nItems,
+// One stacking group: rows[0..nRows) in template order, stack length and
+// destination; peak candidates of the group go to candIdx/candVal with
+// capacity candCap and the counter *count.
+struct StackGroup {
+    const StackRow *rows;
+    int nRows;
+    int candCap;
+    long long len;
+    void *out;
+    int *count;
+    int *candIdx;
+    void *candVal;
+};
+
+// Correlation + normalisation: one launch per FFT size.  Variant 1 runs
+// block-wide LDS Stockham FFTs; variant 2 runs wave-level register FFTs
+// (one wave per template row).
+bool corr_variant_available(int precision, int logM, int variant);
+hipError_t launch_corr(int precision, int logM, int variant, const Item *dItems, int nItems,
                        const Seg *dSegs, const Row *dRows, const void *dTwLo,
                        const void *dTwHi, hipStream_t stream);
 // Template demean/normalise + spectra (FP64 arithmetic): one block per job.
 hipError_t launch_tmpl(int precision, int logM, const TmplJob *dJobs, int nJobs,
                        const void *dTwLo64, const void *dTwHi64, hipStream_t stream);
-// Shift and stack of one group.
-hipError_t launch_stack(int precision, const StackRow *dRows, int nRows, long long len,
-                        int absval, void *dOut, hipStream_t stream);
+// Shift and stack of nGroups groups (grid.y = group); maxLen bounds the
+// grid.  With peaks != 0 the same launch sequence also finds the
+// threshold/local-maximum candidates of every stack.
+hipError_t launch_stack_groups(int precision, const StackGroup *dGroups, int nGroups, long long maxLen,
+                               int absval, hipStream_t stream);
+hipError_t launch_peak_groups(int precision, const StackGroup *dGroups, int nGroups, long long maxLen,
+                              double threshold, hipStream_t stream);
 // Threshold + local-maximum candidates (unordered, deterministic set).
 hipError_t launch_peaks(int precision, const void *dx, long long n, double threshold,
                         int *dCount, int *dIdx, void *dVal, int cap, hipStream_t stream);

@@ -159,6 +159,81 @@ def test_multiblock_double_vs_fp64_oracle(nfft):
         assert np.abs(h.rho(k) - ref).max() < 1e-11
 
 
+@pytest.mark.parametrize("dtype,nfft", [(np.float32, 1024), (np.float32, 2048), (np.float32, 4096),
+                                        (np.float64, 1024), (np.float64, 2048)])
+def test_kernel_variants_match_oracle(dtype, nfft):
+    """Block-wide (1) and wave-level (2) correlation kernels: single channel,
+    several template-length groups (one wave per row, more rows than waves
+    in a group, a group smaller than the wave count), a DC offset and a
+    quiet stretch that forces the exact fix-up."""
+    rng = np.random.default_rng(nfft + (7 if dtype == np.float64 else 0))
+    n = 50000
+    x = rng.standard_normal(n) * 0.5 + 40.0
+    x[20000:21500] = 40.0 + 1e-5 * rng.standard_normal(1500)
+    x = x.astype(dtype)
+    lens = [200, 200, 173, 173, 173, 173, 173, 150, 211]
+    tpls = [rng.standard_normal(L) * np.hanning(L) for L in lens]
+    nout = n - max(lens) + 1
+    refs = [O.pearson(t, x)[:nout] for t in tpls]
+    tol = 1e-5 if dtype == np.float32 else 1e-11
+    out = {}
+    for variant in (1, 2):
+        h = capi.Handle(dtype, "single")
+        h.set_templates(tpls)
+        h.set_signal_size(n)
+        h.set_fft_length(nfft)
+        h.set_kernel_variant(variant)
+        h.initialize()
+        h.set_signal(0, x)
+        h.apply()
+        out[variant] = [h.rho(k) for k in range(len(tpls))]
+        for k in range(len(tpls)):
+            assert np.abs(out[variant][k] - refs[k]).max() < tol, (variant, k)
+        h.close()
+
+
+def test_batched_layout_detect_matches_oracle():
+    """Batched layout (templates of two groups share three channels): each
+    row equals the single-instance result, mf_detect's stacks equal the
+    oracle's shiftAndStack of the GPU rows bit for bit, and its peaks equal
+    the oracle PeakFinder's."""
+    rng = np.random.default_rng(99)
+    C, n = 3, 40000
+    sigs = [(np.convolve(rng.standard_normal(n), np.hanning(9), "same") + rng.uniform(-20, 20))
+            .astype(np.float32) for _ in range(C)]
+    G = 2
+    tpls, chans, groups, shifts, weights = [], [], [], [], []
+    for g in range(G):
+        for c in range(C):
+            L = 150 + 10 * c + 5 * g
+            i0 = 1000 + 7000 * g + 300 * c
+            tpls.append(sigs[c][i0:i0 + L].astype(np.float64) + 0.01 * rng.standard_normal(L))
+            chans.append(c)
+            groups.append(g)
+            shifts.append(300 * c)
+            weights.append(0.5 + 0.25 * c)
+    h = capi.Handle(np.float32, "batched")
+    h.set_templates(tpls, shifts=shifts, weights=weights)
+    h.set_batch_layout([n] * C, chans, groups, G)
+    h.initialize()
+    for c in range(C):
+        h.set_signal(c, sigs[c])
+    h.apply()
+    rows = [h.rho(r) for r in range(len(tpls))]
+    for r, t in enumerate(tpls):
+        ref = O.pearson(t, sigs[chans[r]])[:len(rows[r])]
+        assert np.abs(rows[r] - ref).max() < 1e-5
+    peaks = h.detect(0.3, 25)
+    for g in range(G):
+        members = [r for r in range(len(tpls)) if groups[r] == g]
+        st_ref = O.shift_and_stack(np.stack([rows[r] for r in members]), [shifts[r] for r in members],
+                                   [weights[r] for r in members])
+        st = h.shift_and_stack(g)
+        assert np.array_equal(st, st_ref)
+        assert np.array_equal(peaks[g], O.peak_finder(st_ref, 0.3, 25))
+        assert len(peaks[g]) >= 1
+
+
 def test_dead_and_zeroed_channels():
     """Dead trace (|min| == |max|, src/matchedFilter.cpp:890-894) and
     zeroSignal (823-845) give rho == 0."""
