@@ -30,6 +30,24 @@ def log(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
+def measured_traffic(workload: str, nfft: int, kernel: str):
+    """HBM bytes per launch of the correlation kernel for this workload and
+    plan, from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE summaries under
+    profiles/ (counter values scaled by a copy-kernel calibration).  None
+    when no summary matches."""
+    import glob
+    import re
+    pat = re.compile(r"traffic:\s+workload=(\S+)\s+dtype=(\S+)\s+nfft=(\d+)\s+kernel=(\S+)\s+bytes=(\d+)")
+    here = os.path.dirname(os.path.abspath(__file__))
+    for path in sorted(glob.glob(os.path.join(here, "profiles", "*.md"))):
+        with open(path, encoding="utf-8") as f:
+            for m in pat.finditer(f.read()):
+                if m.group(1) == workload and m.group(2) == "f32" and int(m.group(3)) == nfft \
+                        and m.group(4) == kernel:
+                    return int(m.group(5)), os.path.relpath(path, here)
+    return None, None
+
+
 def cpu_baseline(wl, threads: int, accuracy_rows, gpu_rho_rows):
     """Reference-faithful CPU restatement (oracle) on a bounded sample: one
     template event (24 channels x full day) in the reference's
@@ -75,11 +93,23 @@ def main() -> int:
     ap.add_argument("--variant", type=int, default=0, help="correlation kernel variant (0 = best)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--copy-probe", type=int, default=0,
+                    help="only copy BYTES device-to-device 3 times (calibrates rocprof HBM counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); the "
               "multi-GPU sharding of the path is described in DESIGN.md", file=sys.stderr)
         return 2
+    if args.copy_probe:
+        import torch
+        a = torch.ones(args.copy_probe // 4, dtype=torch.float32, device="cuda:0")
+        b = torch.empty_like(a)
+        torch.cuda.synchronize()
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes": a.numel() * 4, "copies": 3}))
+        return 0
 
     from mflib_amd import capi, synth
     import torch
@@ -137,6 +167,12 @@ def main() -> int:
     corr_avg = float(np.mean(corr_ms))
     bytes_per_launch = n_corr * (4.0 / E + 4.0)
     achieved = bytes_per_launch / (corr_avg * 1e-3)
+    nfft = h.plan(0)[0]
+    # FP32 kernel choice: the wave-level kernel exists for M = nfft/2 <= 2048
+    # and is the default for M <= 1024; the block-level kernel otherwise
+    wave = (args.variant == 2 and nfft <= 4096) or (args.variant == 0 and nfft <= 2048)
+    kernel = "corr2_kernel" if wave else "corr_kernel"
+    traffic, traffic_src = measured_traffic(f"{E}x{C}x{n}", nfft, kernel)
     hits = 0
     for e in range(E):
         det = np.asarray(peaks[e])
@@ -166,8 +202,10 @@ def main() -> int:
         },
         "roofline": {
             "bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-            "frac": achieved / HBM_PEAK, "traffic": None,
-            "kernel": "corr_kernel (FFT correlation + fused normalization)",
+            "frac": achieved / HBM_PEAK, "traffic": traffic,
+            "traffic_unit": "bytes per launch (rocprofv3 FETCH_SIZE x2.0 + WRITE_SIZE x1.0)",
+            "traffic_source": traffic_src,
+            "kernel": f"{kernel}<float,{nfft // 2}> (FFT correlation + fused normalization)",
             "algorithmic_bytes_per_corr_sample": 4.0 / E + 4.0,
             "avg_launch_ms": corr_avg,
         },
