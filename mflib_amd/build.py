@@ -62,7 +62,8 @@ def build_gpu_lib() -> str:
 
 
 FACADE_SRCS = ["waveformTemplate.cpp", "networkStationPhase.cpp", "matchedFilterParameters.cpp",
-               "matchedFilter.cpp", "singleChannelMatchedFilter.cpp", "peakFinder.cpp"]
+               "matchedFilter.cpp", "singleChannelMatchedFilterParameters.cpp",
+               "singleChannelMatchedFilter.cpp", "peakFinder.cpp"]
 
 
 def build_facade() -> str:
@@ -87,12 +88,24 @@ def build_pymflib() -> str:
     return out
 
 
+TESTS_CPP = os.path.join(ROOT, "tests", "cpp")
+
+
+def build_cpp_tests() -> str:
+    """The C++ API test program (tests/cpp/test_facade.cpp), linked against
+    libmflib.so; written to tests/cpp/bin/ (git-ignored)."""
+    outdir = os.path.join(TESTS_CPP, "bin")
+    os.makedirs(outdir, exist_ok=True)
+    out = os.path.join(outdir, "test_facade")
+    _run(["g++", *CXX_FLAGS, "-I", INC, "-o", out + ".tmp", os.path.join(TESTS_CPP, "test_facade.cpp"),
+          "-L", LIB, "-lmflib", "-lmflib_gpu",
+          "-Wl,-rpath,$ORIGIN/../../../mflib_amd/lib"])
+    os.replace(out + ".tmp", out)
+    return out
+
+
 def build_all(verbose: bool = True) -> list:
-    built = [build_gpu_lib()]
-    if os.path.isdir(os.path.join(CSRC, "facade")):
-        built.append(build_facade())
-        if os.path.exists(os.path.join(CSRC, "pymflib.cpp")):
-            built.append(build_pymflib())
+    built = [build_gpu_lib(), build_facade(), build_pymflib(), build_cpp_tests()]
     if verbose:
         for b in built:
             print("built", os.path.relpath(b, ROOT))

@@ -1,0 +1,31 @@
+// Shared by the C++ facade: turns C-ABI status codes into the exception
+// types MFLib's API throws (src/matchedFilter.cpp:513-516, 829-834,
+// 876-881, 977-980, 1322-1329 in the reference).
+#ifndef MFLIB_FACADE_ABI_HPP
+#define MFLIB_FACADE_ABI_HPP
+#include <new>
+#include <stdexcept>
+#include <string>
+#include "mflib_gpu.h"
+
+namespace MFLib
+{
+namespace Facade
+{
+inline void check(int rc, const char *where)
+{
+    if (rc == MF_OK) { return; }
+    std::string msg = std::string(where) + ": " + mf_last_error();
+    switch (rc)
+    {
+    case MF_ERR_INVALID_ARGUMENT: throw std::invalid_argument(msg);
+    case MF_ERR_OUT_OF_MEMORY: throw std::bad_alloc();
+    default: throw std::runtime_error(msg);
+    }
+}
+template<class T> constexpr int precisionOf();
+template<> constexpr int precisionOf<float>() { return MF_PRECISION_FLOAT; }
+template<> constexpr int precisionOf<double>() { return MF_PRECISION_DOUBLE; }
+}
+}
+#endif

@@ -1,0 +1,321 @@
+// pymflib: the Python module of MFLib's detection path on the GPU engine.
+// Class, method and enum names follow the reference's bindings
+// (python/pymflib.cpp, python/waveformTemplate.cpp,
+// python/networkStationPhase.cpp, python/multiChannelMatchedFilter*.cpp,
+// python/singleChannel/matchedFilter*.cpp), so scripts written against
+// them run unchanged.  Like the reference, the matched-filter classes are
+// the FP64 instantiations and inputs are cast to float64.
+//
+// Additions with no reference counterpart:
+//  - MultiChannel.MatchedFilter.shift_and_stack;
+//  - get_signal_length / get_filtered_signal_length;
+//  - MultiChannel.MatchedFilterParameters.set/get_stack_absolute_values;
+//  - PeakFinder;
+//  - apply() releases the GIL while the GPU works.
+#include <pybind11/numpy.h>
+#include <pybind11/pybind11.h>
+#include <pybind11/stl.h>
+
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "mflib/enums.hpp"
+#include "mflib/matchedFilter.hpp"
+#include "mflib/matchedFilterParameters.hpp"
+#include "mflib/networkStationPhase.hpp"
+#include "mflib/singleChannel/matchedFilter.hpp"
+#include "mflib/singleChannel/matchedFilterParameters.hpp"
+#include "mflib/version.hpp"
+#include "mflib/waveformTemplate.hpp"
+#include "private/peakFinder.hpp"
+
+namespace py = pybind11;
+using DArray = py::array_t<double, py::array::c_style | py::array::forcecast>;
+
+namespace
+{
+
+py::array_t<double> toArray(const std::vector<double> &v)
+{
+    py::array_t<double> out(static_cast<py::ssize_t>(v.size()));
+    std::copy(v.begin(), v.end(), out.mutable_data());
+    return out;
+}
+
+py::array_t<double> toArray(const double *p, int n)
+{
+    py::array_t<double> out(n);
+    std::copy(p, p + n, out.mutable_data());
+    return out;
+}
+
+int arrayLength(const DArray &x)
+{
+    if (x.ndim() != 1) { throw std::invalid_argument("signal must be one-dimensional"); }
+    if (x.size() > 0x7fffffff) { throw std::invalid_argument("signal is too long"); }
+    return static_cast<int>(x.size());
+}
+
+// ---- NetworkStationPhase (python/networkStationPhase.cpp) ---------------
+void bindNetworkStationPhase(py::module_ &m)
+{
+    using N = MFLib::NetworkStationPhase;
+    py::class_<N> c(m, "NetworkStationPhase");
+    c.def(py::init<>());
+    c.doc() = "Network, station, channel, location code and phase of a waveform template.";
+    c.def("set_network", &N::setNetwork);
+    c.def("get_network", &N::getNetwork);
+    c.def("set_station", &N::setStation);
+    c.def("get_station", &N::getStation);
+    c.def("set_channel", &N::setChannel);
+    c.def("get_channel", &N::getChannel);
+    c.def("set_location_code", &N::setLocationCode);
+    c.def("get_location_code", &N::getLocationCode);
+    c.def("set_phase", &N::setPhase);
+    c.def("get_phase", &N::getPhase);
+    c.def("clear", &N::clear);
+    c.def("__eq__", [](const N &a, const N &b) { return a == b; });
+    c.def("__hash__", &N::getHash);
+    c.def(py::pickle(
+        [](const N &p) {
+            return py::make_tuple(p.getNetwork(), p.getStation(), p.getChannel(), p.getLocationCode(),
+                                  p.getPhase());
+        },
+        [](const py::tuple &t) {
+            if (t.size() != 5) { throw std::runtime_error("invalid NetworkStationPhase state"); }
+            N p;
+            p.setNetwork(t[0].cast<std::string>());
+            p.setStation(t[1].cast<std::string>());
+            p.setChannel(t[2].cast<std::string>());
+            p.setLocationCode(t[3].cast<std::string>());
+            p.setPhase(t[4].cast<std::string>());
+            return p;
+        }));
+}
+
+// ---- WaveformTemplate (python/waveformTemplate.cpp) ---------------------
+std::vector<double> templateSignal(const MFLib::WaveformTemplate &w)
+{
+    const int n = w.getSignalLength();
+    std::vector<double> v(n);
+    double *p = v.data();
+    w.getSignal(n, &p);
+    return v;
+}
+
+void bindWaveformTemplate(py::module_ &m)
+{
+    using W = MFLib::WaveformTemplate;
+    py::class_<W> c(m, "WaveformTemplate");
+    c.def(py::init<>());
+    c.doc() = "A template waveform with its sampling rate, onset time, travel time and stack weight.";
+    c.def("set_signal", [](W &w, DArray x) { w.setSignal(arrayLength(x), x.data()); });
+    c.def("get_signal", [](const W &w) { return toArray(templateSignal(w)); });
+    c.def("set_sampling_rate", &W::setSamplingRate);
+    c.def("get_sampling_rate", &W::getSamplingRate);
+    c.def("set_shift_and_stack_weight", &W::setShiftAndStackWeight);
+    c.def("get_shift_and_stack_weight", &W::getShiftAndStackWeight);
+    c.def("set_phase_onset_time_in_signal", &W::setPhaseOnsetTime);
+    c.def("get_phase_onset_time_in_signal", &W::getPhaseOnsetTime);
+    c.def("set_phase_travel_time", [](W &w, double t) {
+        // the reference's binding rejects this as a ValueError
+        if (t < 0) { throw std::invalid_argument("travel time = " + std::to_string(t) + " must be positive"); }
+        w.setTravelTime(t);
+    });
+    c.def("get_phase_travel_time", &W::getTravelTime);
+    c.def("set_magnitude", &W::setMagnitude);
+    c.def("get_magnitude", &W::getMagnitude);
+    c.def("set_polarity", &W::setPolarity);
+    c.def("get_polarity", &W::getPolarity);
+    c.def("set_identifier", [](W &w, const py::sequence &id) {
+        if (py::len(id) != 2) { throw std::invalid_argument("identifier must be [NetworkStationPhase, int]"); }
+        w.setIdentifier({id[0].cast<MFLib::NetworkStationPhase>(), id[1].cast<uint64_t>()});
+    });
+    c.def("get_identifier", [](const W &w) {
+        auto id = w.getIdentifier();
+        return py::make_tuple(id.first, id.second);
+    });
+    c.def("clear", &W::clear);
+    c.def(py::pickle(
+        [](const W &w) {
+            py::object sig = w.haveSignal() ? py::object(toArray(templateSignal(w))) : py::object(py::none());
+            py::object fs = w.haveSamplingRate() ? py::object(py::float_(w.getSamplingRate())) : py::none();
+            py::object on = w.havePhaseOnsetTime() ? py::object(py::float_(w.getPhaseOnsetTime())) : py::none();
+            py::object tt = w.haveTravelTime() ? py::object(py::float_(w.getTravelTime())) : py::none();
+            py::object mag = w.haveMagnitude() ? py::object(py::float_(w.getMagnitude())) : py::none();
+            py::object id = py::none();
+            if (w.haveIdentifier())
+            {
+                auto p = w.getIdentifier();
+                id = py::make_tuple(p.first, p.second);
+            }
+            return py::make_tuple(sig, fs, w.getShiftAndStackWeight(), on, tt, mag,
+                                  static_cast<int>(w.getPolarity()), id);
+        },
+        [](const py::tuple &t) {
+            if (t.size() != 8) { throw std::runtime_error("invalid WaveformTemplate state"); }
+            W w;
+            if (!t[0].is_none())
+            {
+                DArray x = t[0].cast<DArray>();
+                w.setSignal(arrayLength(x), x.data());
+            }
+            if (!t[1].is_none()) { w.setSamplingRate(t[1].cast<double>()); }
+            w.setShiftAndStackWeight(t[2].cast<double>());
+            if (!t[3].is_none()) { w.setPhaseOnsetTime(t[3].cast<double>()); }
+            if (!t[4].is_none()) { w.setTravelTime(t[4].cast<double>()); }
+            if (!t[5].is_none()) { w.setMagnitude(t[5].cast<double>()); }
+            w.setPolarity(static_cast<MFLib::Polarity>(t[6].cast<int>()));
+            if (!t[7].is_none())
+            {
+                py::tuple id = t[7].cast<py::tuple>();
+                w.setIdentifier({id[0].cast<MFLib::NetworkStationPhase>(), id[1].cast<uint64_t>()});
+            }
+            return w;
+        }));
+}
+
+// ---- parameters (python/multiChannelMatchedFilterParameters.cpp,
+//      python/singleChannel/matchedFilterParameters.cpp) -----------------
+template<class P>
+void bindParameterCommon(py::class_<P> &c)
+{
+    c.def(py::init<>());
+    c.def("add_template", &P::addTemplate);
+    c.def("get_template", &P::getTemplate);
+    c.def("get_signal_size", &P::getSignalSize);
+    c.def("set_signal_size", &P::setSignalSize);
+    c.def("get_number_of_templates", &P::getNumberOfTemplates);
+    c.def("get_fft_length", &P::getFFTLength);
+    c.def("set_fft_length", &P::setFFTLength);
+    c.def("get_block_length", &P::getBlockLength);
+    c.def("clear_templates", &P::clearTemplates);
+    c.def("clear", &P::clear);
+}
+
+// ---- matched filters (python/multiChannelMatchedFilter.cpp,
+//      python/singleChannel/matchedFilter.cpp) ---------------------------
+void bindMultiChannel(py::module_ &mc)
+{
+    using P = MFLib::MatchedFilterParameters;
+    py::class_<P> p(mc, "MatchedFilterParameters");
+    p.doc() = "Templates and signal size of a multi-channel matched filter (template i matches signal i).";
+    bindParameterCommon(p);
+    // the reference binds block_length to getFFTLength; this returns the block length
+    p.def("block_length", &P::getBlockLength);
+    p.def("set_stack_absolute_values", &P::setStackAbsoluteValues);
+    p.def("get_stack_absolute_values", &P::getStackAbsoluteValues);
+
+    using M = MFLib::MatchedFilter<double>;
+    py::class_<M> c(mc, "MatchedFilter");
+    c.def(py::init<>());
+    c.doc() = "Applies matched filtering to multiple channels where each channel has a specific template.";
+    c.def("initialize", [](M &m, const P &parms) {
+        if (!parms.isValid()) { throw std::invalid_argument("Parameters are invalid"); }
+        m.initialize(parms);
+    });
+    c.def("set_signal", [](M &m, int it, DArray x) {
+        if (!m.isInitialized()) { throw std::runtime_error("Class is not initialized"); }
+        m.setSignal(it, arrayLength(x), x.data());
+    });
+    c.def("zero_signal", &M::zeroSignal);
+    c.def("get_number_of_templates", &M::getNumberOfTemplates);
+    c.def("get_signal_length", &M::getSignalLength);
+    c.def("get_filtered_signal_length", &M::getFilteredSignalLength);
+    c.def("apply", &M::apply, py::call_guard<py::gil_scoped_release>());
+    c.def("have_matched_filtered_signals", &M::haveMatchedFilteredSignals);
+    c.def("get_matched_filtered_signal", [](const M &m, int it) {
+        return toArray(m.getMatchedFilterSignalPointer(it), m.getFilteredSignalLength());
+    });
+    c.def("shift_and_stack", [](M &m) { return toArray(m.shiftAndStack()); });
+}
+
+void bindSingleChannel(py::module_ &sc)
+{
+    using P = MFLib::SingleChannel::MatchedFilterParameters;
+    py::class_<P> p(sc, "MatchedFilterParameters");
+    p.doc() = "Templates and signal size of a single-channel matched filter (every template matches one signal).";
+    bindParameterCommon(p);
+
+    using M = MFLib::SingleChannel::MatchedFilter<double>;
+    py::class_<M> c(sc, "MatchedFilter");
+    c.def(py::init<>());
+    c.doc() = "Applies many templates to one signal.";
+    c.def("initialize", [](M &m, const P &parms) {
+        if (!parms.isValid()) { throw std::invalid_argument("Parameters are invalid"); }
+        m.initialize(parms);
+    });
+    c.def("set_signal", [](M &m, DArray x) {
+        if (!m.isInitialized()) { throw std::runtime_error("Class is not initialized"); }
+        m.setSignal(arrayLength(x), x.data());
+    });
+    c.def("zero_signal", &M::zeroSignal);
+    c.def("get_number_of_templates", &M::getNumberOfTemplates);
+    c.def("get_signal_length", &M::getSignalLength);
+    c.def("get_filtered_signal_length", &M::getFilteredSignalLength);
+    c.def("apply", &M::apply, py::call_guard<py::gil_scoped_release>());
+    c.def("have_matched_filtered_signals", &M::haveMatchedFilteredSignals);
+    c.def("get_matched_filtered_signal", [](const M &m, int it) {
+        return toArray(m.getMatchedFilterSignalPointer(it), m.getFilteredSignalLength());
+    });
+}
+
+void bindPeakFinder(py::module_ &m)
+{
+    using F = MFLib::PeakFinder<double>;
+    py::class_<F> c(m, "PeakFinder");
+    c.def(py::init<>());
+    c.doc() = "Thresholded local maxima with greedy minimum-distance pruning.";
+    c.def("set_threshold", &F::setThreshold);
+    c.def("get_threshold", &F::getThreshold);
+    c.def("set_minimum_peak_distance", &F::setMinimumPeakDistance);
+    c.def("get_minimum_peak_distance", &F::getMinimumPeakDistance);
+    c.def("set_signal", [](F &f, DArray x) { f.setSignal(arrayLength(x), x.data()); });
+    c.def("apply", &F::apply, py::call_guard<py::gil_scoped_release>());
+    c.def("get_number_of_peaks", &F::getNumberOfPeaks);
+    c.def("get_peak_indices", [](const F &f) {
+        const int n = f.getNumberOfPeaks();
+        py::array_t<int> out(n);
+        if (n > 0)
+        {
+            int *p = out.mutable_data();
+            f.getPeakIndices(n, &p);
+        }
+        return out;
+    });
+    c.def("clear", &F::clear);
+}
+
+}  // namespace
+
+PYBIND11_MODULE(pymflib, m)
+{
+    m.attr("__version__") = MFLIB_VERSION;
+    bindWaveformTemplate(m);
+    bindNetworkStationPhase(m);
+    py::enum_<MFLib::RelativeMagnitudeType>(m, "RelativeMagnitudeType")
+        .value("gibbons_ringdal_2006", MFLib::RelativeMagnitudeType::GIBBONS_RINGDAL_2006)
+        .value("schaff_richards_2014", MFLib::RelativeMagnitudeType::SCHAFF_RICHARDS_2014);
+    // the reference spells this enum "MaxmimumMatchedFilterPolicy"; both names are kept
+    auto maxPolicy = py::enum_<MFLib::MaximumMatchedFilterPolicy>(m, "MaxmimumMatchedFilterPolicy")
+        .value("maximum", MFLib::MaximumMatchedFilterPolicy::MAXIMUM)
+        .value("absolute_maximum", MFLib::MaximumMatchedFilterPolicy::ABSOLUTE_MAXIMUM);
+    m.attr("MaximumMatchedFilterPolicy") = maxPolicy;
+    py::enum_<MFLib::MatchedFilteredSignalDetectorPolicy>(m, "MatchedFilteredSignalDetectorPolicy")
+        .value("reduce", MFLib::MatchedFilteredSignalDetectorPolicy::REDUCED)
+        .value("single", MFLib::MatchedFilteredSignalDetectorPolicy::SINGLE);
+    py::enum_<MFLib::Polarity>(m, "Polarity")
+        .value("unknown", MFLib::Polarity::UNKNOWN)
+        .value("compression", MFLib::Polarity::COMPRESSION)
+        .value("dilatation", MFLib::Polarity::DILATATION);
+    py::enum_<MFLib::DetectionTimeInterpolationType>(m, "DetectionTimeInterpolationType")
+        .value("quadratic", MFLib::DetectionTimeInterpolationType::QUADRATIC)
+        .value("lanczos", MFLib::DetectionTimeInterpolationType::LANCZOS)
+        .value("none", MFLib::DetectionTimeInterpolationType::NONE);
+    py::module_ sc = m.def_submodule("SingleChannel");
+    bindSingleChannel(sc);
+    py::module_ mc = m.def_submodule("MultiChannel");
+    bindMultiChannel(mc);
+    bindPeakFinder(m);
+}

@@ -1,0 +1,160 @@
+"""pymflib host-side classes on the CPU: the reference's own Python cases
+for NetworkStationPhase, WaveformTemplate and MatchedFilterParameters
+(python/unit_tests.py:8-178), pickling and error conventions.  No GPU
+call is made."""
+import pickle
+
+import numpy as np
+import pytest
+
+pymflib = pytest.importorskip("mflib_amd.pymflib", reason="pymflib not built (python -m mflib_amd.build)")
+
+
+def test_network_station_phase():
+    nsp = pymflib.NetworkStationPhase()
+    nsp.set_network("UU")
+    nsp.set_station("ALT")
+    nsp.set_channel("EHN")
+    nsp.set_location_code("01")
+    nsp.set_phase("P")
+    assert (nsp.get_network(), nsp.get_station(), nsp.get_channel(), nsp.get_location_code(),
+            nsp.get_phase()) == ("UU", "ALT", "EHN", "01", "P")
+    p = pickle.loads(pickle.dumps(nsp, 2))
+    assert p == nsp and hash(p) == hash(nsp)
+    p.set_phase("S")
+    assert p != nsp
+
+
+def test_waveform_template():
+    template_signal = np.arange(1, 400, 1)
+    nsp = pymflib.NetworkStationPhase()
+    nsp.set_network("UU")
+    nsp.set_station("FORU")
+    nsp.set_channel("HHZ")
+    nsp.set_location_code("01")
+    nsp.set_phase("P")
+    wt = pymflib.WaveformTemplate()
+    wt.set_signal(template_signal)
+    assert np.max(np.abs(wt.get_signal() - template_signal)) == 0
+    wt.set_sampling_rate(100)
+    assert wt.get_sampling_rate() == 100
+    wt.set_shift_and_stack_weight(0.8)
+    assert wt.get_shift_and_stack_weight() == 0.8
+    wt.set_phase_travel_time(9)
+    assert wt.get_phase_travel_time() == 9
+    wt.set_phase_onset_time_in_signal(0.5)
+    assert wt.get_phase_onset_time_in_signal() == 0.5
+    wt.set_magnitude(6)
+    assert wt.get_magnitude() == 6
+    wt.set_polarity(pymflib.Polarity.compression)
+    assert wt.get_polarity() == pymflib.Polarity.compression
+    wt.set_identifier([nsp, 4])
+    nid, num = wt.get_identifier()
+    assert nid.get_station() == "FORU" and num == 4
+    for w in (wt, pickle.loads(pickle.dumps(wt, 2))):
+        assert np.max(np.abs(w.get_signal() - template_signal)) == 0
+        assert w.get_sampling_rate() == 100 and w.get_shift_and_stack_weight() == 0.8
+        assert w.get_phase_travel_time() == 9 and w.get_phase_onset_time_in_signal() == 0.5
+        assert w.get_magnitude() == 6 and w.get_polarity() == pymflib.Polarity.compression
+        assert w.get_identifier()[0] == nsp and w.get_identifier()[1] == 4
+    wt.clear()
+    assert wt.get_shift_and_stack_weight() == 1
+
+
+def test_waveform_template_errors():
+    wt = pymflib.WaveformTemplate()
+    with pytest.raises(RuntimeError):
+        wt.get_signal()
+    with pytest.raises(ValueError):
+        wt.set_sampling_rate(0)
+    with pytest.raises(ValueError):
+        wt.set_shift_and_stack_weight(1.2)
+    with pytest.raises(ValueError):
+        wt.set_phase_travel_time(-1)
+    wt.set_signal(np.ones(10))
+    with pytest.raises(RuntimeError):
+        wt.set_phase_onset_time_in_signal(0.1)  # no sampling rate yet
+    wt.set_sampling_rate(100)
+    # src/waveformTemplate.cpp:287-303 bounds the onset by (npts-1)*fs
+    wt.set_phase_onset_time_in_signal(9 * 100)
+    with pytest.raises(ValueError):
+        wt.set_phase_onset_time_in_signal(9 * 100 + 1)
+    # setting the signal or rate forgets the onset time (157, 206)
+    wt.set_sampling_rate(50)
+    with pytest.raises(RuntimeError):
+        wt.get_phase_onset_time_in_signal()
+
+
+def test_matched_filter_parameters():
+    wf1, wf2 = pymflib.WaveformTemplate(), pymflib.WaveformTemplate()
+    params = pymflib.MultiChannel.MatchedFilterParameters()
+    t1, t2 = np.arange(1, 100, 1), np.arange(1, 95, 1)
+    for wf, t in ((wf1, t1), (wf2, t2)):
+        wf.set_signal(t)
+        wf.set_sampling_rate(100)
+        wf.set_shift_and_stack_weight(0.8)
+        wf.set_phase_onset_time_in_signal(0.05)
+        wf.set_phase_travel_time(4)
+        params.add_template(wf)
+    params.set_signal_size(1024)
+    assert params.get_number_of_templates() == 2
+    assert params.get_fft_length() == 512
+    assert params.get_block_length() == 512 - 99 + 1
+    params.set_fft_length(550)
+    assert params.get_fft_length() == 550
+    for i, t in enumerate((t1, t2)):
+        back = params.get_template(i)
+        assert back.get_sampling_rate() == 100 and back.get_shift_and_stack_weight() == 0.8
+        assert back.get_phase_onset_time_in_signal() == 0.05 and back.get_phase_travel_time() == 4
+        assert np.max(np.abs(back.get_signal() - t)) == 0
+    with pytest.raises(ValueError):
+        params.get_template(2)
+    with pytest.raises(ValueError):
+        params.set_fft_length(50)  # shorter than the longest template
+    params.clear_templates()
+    assert params.get_number_of_templates() == 0
+    with pytest.raises(RuntimeError):
+        params.get_fft_length()
+
+
+def test_parameters_sampling_rate_mismatch():
+    params = pymflib.SingleChannel.MatchedFilterParameters()
+    a, b = pymflib.WaveformTemplate(), pymflib.WaveformTemplate()
+    a.set_signal(np.ones(20))
+    a.set_sampling_rate(100)
+    b.set_signal(np.ones(20))
+    b.set_sampling_rate(100.01)
+    params.add_template(a)
+    with pytest.raises(ValueError):
+        params.add_template(b)
+    no_rate = pymflib.WaveformTemplate()
+    no_rate.set_signal(np.ones(5))
+    with pytest.raises(RuntimeError):
+        params.add_template(no_rate)
+
+
+def test_matched_filter_requires_initialize():
+    mf = pymflib.MultiChannel.MatchedFilter()
+    with pytest.raises(RuntimeError):
+        mf.set_signal(0, np.zeros(10))
+    with pytest.raises(RuntimeError):
+        mf.apply()
+    with pytest.raises(ValueError):
+        mf.initialize(pymflib.MultiChannel.MatchedFilterParameters())
+
+
+def test_peak_finder_setters():
+    pf = pymflib.PeakFinder()
+    assert abs(pf.get_threshold() - 0.7) < 1e-15  # src/peakFinder.cpp:24
+    with pytest.raises(ValueError):
+        pf.set_threshold(1.01)
+    pf.set_minimum_peak_distance(7)
+    assert pf.get_minimum_peak_distance() == 7
+    with pytest.raises(ValueError):
+        pf.set_signal(np.zeros(0))
+
+
+def test_cpp_test_program_built():
+    import os
+    exe = os.path.join(os.path.dirname(__file__), "cpp", "bin", "test_facade")
+    assert os.access(exe, os.X_OK), "tests/cpp/bin/test_facade missing (python -m mflib_amd.build)"
