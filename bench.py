@@ -48,6 +48,81 @@ def measured_traffic(workload: str, nfft: int, kernel: str):
     return None, None
 
 
+def serve_section(args, torch):
+    """Served stream of detection requests: resident library of 1000
+    templates (200-2000 samples), 48 resident one-hour channel streams,
+    requests of 60 s - 1 h windows on 1-48 channels x 1-100 templates.
+    Measures the batching server at capacity and under Poisson arrivals
+    (p50/p99 latency), against the one-request-at-a-time forms."""
+    from mflib_amd import capi, loadgen, synth
+    log("serving: generating library and channel streams")
+    wl = synth.serving_stream()
+
+    class _Dev:
+        def __init__(self, a):
+            self.t = torch.from_numpy(a).to("cuda:0")
+            self.ptr = self.t.data_ptr()
+
+    res = loadgen.ResidentStreams(wl.streams, _Dev)
+    torch.cuda.synchronize()
+    free, _ = torch.cuda.mem_get_info()
+    budget = min(200e9, free - 24e9)
+    srv = capi.Server(np.float32, budget_bytes=budget, max_batch=args.serve_batch, max_inflight=2)
+    lib_ids = srv.add_templates(wl.library)
+    log(f"serving: budget {budget / 1e9:.0f} GB, library {len(lib_ids)} templates")
+    loadgen.run_closed_loop(srv, wl, res, lib_ids, 16, 8, seed=1)  # warm-up
+    cap = loadgen.run_closed_loop(srv, wl, res, lib_ids, args.serve_requests, 64, seed=2)
+    log(f"serving: batched capacity {cap['corr_samples_per_s'] / 1e9:.1f} G corr samples/s")
+    runs, audit = [], []
+    for lam in args.serve_lambda:
+        n_req = max(200, int(lam * args.serve_seconds))
+        r, a = loadgen.run_poisson(srv, wl, res, lib_ids, lam, n_req, seed=3 + int(lam))
+        runs.append(r)
+        audit += a
+        log(f"serving: lambda {lam:g}/s: p50 {r['p50_ms']:.2f} ms, p99 {r['p99_ms']:.2f} ms, "
+            f"{r['corr_samples_per_s'] / 1e9:.1f} G corr samples/s")
+    b, n, _ = srv.stats()
+    srv.close()
+    one = capi.Server(np.float32, budget_bytes=min(budget, 64e9), max_batch=1, max_inflight=1)
+    lib1 = one.add_templates(wl.library)
+    loadgen.run_closed_loop(one, wl, res, lib1, 8, 1, seed=1)
+    single = loadgen.run_closed_loop(one, wl, res, lib1, max(20, args.serve_requests // 4), 1, seed=2)
+    one.close()
+    log(f"serving: one request at a time {single['corr_samples_per_s'] / 1e9:.1f} G corr samples/s")
+    ref = loadgen.run_reference_form(wl, res, args.serve_ref_requests, seed=2)
+    log(f"serving: reference form {ref['corr_samples_per_s'] / 1e9:.2f} G corr samples/s")
+    out = {
+        "workload": "served stream: library 1000 templates (200-2000 samples, resident spectra), "
+                    "48 resident 1 h channels at 100 Hz, requests = 60 s-1 h window (log-uniform) "
+                    "x 1-48 channels x 1-100 templates, FP32, Poisson arrivals",
+        "budget_bytes": budget,
+        "batched_capacity_corr_samples_per_s": cap["corr_samples_per_s"],
+        "one_at_a_time_server_corr_samples_per_s": single["corr_samples_per_s"],
+        "reference_form_corr_samples_per_s": ref["corr_samples_per_s"],
+        "batched_vs_one_at_a_time": cap["corr_samples_per_s"] / single["corr_samples_per_s"],
+        "batched_vs_reference_form": cap["corr_samples_per_s"] / ref["corr_samples_per_s"],
+        "poisson": runs,
+        "batches": b,
+        "requests_completed": n,
+    }
+    return out, (wl, audit)
+
+
+def serve_audit(wl, audit):
+    """FP64 oracle check of the audited served rows (1 in 100 requests, one
+    (template, channel) row each, 2000 seeded lags per row)."""
+    import oracle as O
+    rng = np.random.default_rng(0x5E7)
+    worst, n = 0.0, 0
+    for (chans, start, nwin, ids), ti, ci, row in audit:
+        x = wl.streams[chans[ci]][start:start + nwin]
+        lags = rng.integers(0, row.size, 2000)
+        ref = O.pearson(wl.library[ids[ti]], x, lags)
+        worst = max(worst, float(np.abs(row[lags].astype(np.float64) - ref).max()))
+        n += lags.size
+    return {"max_abs_drho_vs_fp64": worst, "audited_rows": len(audit), "audited_samples": n}
+
+
 def cpu_baseline(wl, threads: int, accuracy_rows, gpu_rho_rows):
     """Reference-faithful CPU restatement (oracle) on a bounded sample: one
     template event (24 channels x full day) in the reference's
@@ -93,6 +168,13 @@ def main() -> int:
     ap.add_argument("--variant", type=int, default=0, help="correlation kernel variant (0 = best)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--no-serve", action="store_true", help="skip the served-stream measurement")
+    ap.add_argument("--serve-only", action="store_true", help="only the served-stream measurement")
+    ap.add_argument("--serve-lambda", type=float, nargs="+", default=[500.0, 1000.0])
+    ap.add_argument("--serve-seconds", type=float, default=2.0)
+    ap.add_argument("--serve-requests", type=int, default=400, help="requests of the capacity run")
+    ap.add_argument("--serve-ref-requests", type=int, default=6)
+    ap.add_argument("--serve-batch", type=int, default=256)
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="only copy BYTES device-to-device 3 times (calibrates rocprof HBM counters)")
     args = ap.parse_args()
@@ -105,14 +187,26 @@ def main() -> int:
         a = torch.ones(args.copy_probe // 4, dtype=torch.float32, device="cuda:0")
         b = torch.empty_like(a)
         torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
         for _ in range(3):
             b.copy_(a)
+        e1.record()
         torch.cuda.synchronize()
-        print(json.dumps({"copy_probe_bytes": a.numel() * 4, "copies": 3}))
+        ms = e0.elapsed_time(e1) / 3
+        print(json.dumps({"copy_probe_bytes": a.numel() * 4, "copies": 3, "ms_per_copy": ms,
+                          "copy_GBps_read_plus_write": 2 * a.numel() * 4 / (ms * 1e-3) / 1e9}))
         return 0
 
+    import torch  # before the engine library: one HIP runtime per process
     from mflib_amd import capi, synth
-    import torch
+
+    if args.serve_only:
+        serving, (swl, audit) = serve_section(args, torch)
+        if not args.no_cpu_baseline:
+            serving["accuracy"] = serve_audit(swl, audit)
+        print(json.dumps({"serving": serving}), flush=True)
+        return 0
 
     log("generating the synthetic day (16 templates x 24 channels)")
     t0 = time.perf_counter()
@@ -168,9 +262,9 @@ def main() -> int:
     bytes_per_launch = n_corr * (4.0 / E + 4.0)
     achieved = bytes_per_launch / (corr_avg * 1e-3)
     nfft = h.plan(0)[0]
-    # FP32 kernel choice: the wave-level kernel exists for M = nfft/2 <= 2048
-    # and is the default for M <= 1024; the block-level kernel otherwise
-    wave = (args.variant == 2 and nfft <= 4096) or (args.variant == 0 and nfft <= 2048)
+    # FP32 kernel choice: the wave-level kernel exists and is the default for
+    # M = nfft/2 <= 2048; the block-level kernel otherwise
+    wave = args.variant in (0, 2) and nfft <= 4096
     kernel = "corr2_kernel" if wave else "corr_kernel"
     traffic, traffic_src = measured_traffic(f"{E}x{C}x{n}", nfft, kernel)
     hits = 0
@@ -213,14 +307,24 @@ def main() -> int:
         "detections_recovered": hits,
         "detections_injected": E * len(wl.origins[0]),
     }
+    audit_rows = [h.rho(e * C + c) for e, c in [(e, c) for e in (0, E - 1) for c in (0, C // 2, C - 1)]]
+    serve_state = None
+    if not args.no_serve:
+        h.close()
+        serving, serve_state = serve_section(args, torch)
+        out["serving"] = serving
+        lat = min(serving["poisson"], key=lambda r: abs(r["lambda_per_s"] - 1000.0))
+        out["latency"] = {"p50_ms": lat["p50_ms"], "p99_ms": lat["p99_ms"],
+                          "lambda_per_s": lat["lambda_per_s"], "workload": "served stream (see serving)"}
     if not args.no_cpu_baseline:
         log("CPU baseline (bounded sample) and accuracy audit")
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
         rows = [(e, c) for e in (0, E - 1) for c in (0, C // 2, C - 1)]
-        gpu_rows = [h.rho(e * C + c) for e, c in rows]
-        cb, acc = cpu_baseline(wl, threads, rows, gpu_rows)
+        cb, acc = cpu_baseline(wl, threads, rows, audit_rows)
         out["cpu_baseline"] = cb
         out["accuracy"] = acc
+        if serve_state is not None:
+            out["serving"]["accuracy"] = serve_audit(*serve_state)
     print(json.dumps(out), flush=True)
     h.close()
     return 0

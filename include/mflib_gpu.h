@@ -167,6 +167,55 @@ int mf_compute_optimal_fft_and_block_length(int nb, int nx, int *nfft, int *bloc
  * kernel of the last mf_shift_and_stack*. */
 int mf_get_timings(mf_handle *h, double *corrMs, double *stackMs);
 
+/* ---- request server: no reference counterpart ---------------------------
+ * Serving form of the path: a template library resident in HBM (FP64
+ * templates and their spectra at every supported FFT size) and a request
+ * loop with continuous batching.  A request is a window of nChannels
+ * channels matched against nTemplates library templates: every template is
+ * correlated with every channel, as one reference MatchedFilter instance
+ * per (request, template) would do (src/matchedFilter.cpp:971-1316).  Row
+ * (template i, channel c) has n_c - nb + 1 outputs, nb being the longest
+ * template of the request (src/matchedFilter.cpp:1395-1402).
+ *
+ * A scheduler thread admits pending requests in arrival order while their
+ * inputs and outputs fit in the HBM budget, launches every admitted request
+ * as one batch (at most maxBatch requests) on one of maxInflight streams,
+ * and keeps going while earlier batches still run, so that requests that
+ * arrive meanwhile overlap on the device.  A request's results depend only
+ * on that request: the same alone and in any mix.  maxBatch = 1 with
+ * maxInflight = 1 is the one-request-at-a-time form. */
+typedef struct mf_server mf_server;
+int mf_server_create(mf_server **s, int precision, double budgetBytes, int maxBatch, int maxInflight);
+/* Waits for launched batches, cancels pending requests, frees everything. */
+int mf_server_destroy(mf_server *s);
+/* Adds n templates (FP64 samples) to the library; their ids are
+ * *firstId .. *firstId + n - 1.  Demean, unit norm and spectra are
+ * computed on the GPU once (src/matchedFilter.cpp:168-190, 594-649). */
+int mf_server_add_templates(mf_server *s, int n, const int *lengths, const double *const *samples,
+                            int *firstId);
+/* Queues a request.  channels[c] points to lengths[c] samples of the
+ * server precision in device memory (onDevice != 0) or host memory; the
+ * memory must stay valid until the request completes.  Errors in the
+ * request (bad ids, channels shorter than a template, a request larger
+ * than the whole budget) are reported here. */
+int mf_server_submit(mf_server *s, int nChannels, const int *lengths, const void *const *channels,
+                     int onDevice, int nTemplates, const int *templateIds, long long *requestId);
+/* Blocks until the request completes; *status = MF_OK or the failure code. */
+int mf_server_wait(mf_server *s, long long requestId, int *status);
+int mf_server_poll(mf_server *s, long long requestId, int *done);
+/* Correlation row (template index i of the request, channel c): copy to
+ * host, or its device address (valid until mf_server_release). */
+int mf_server_get_rho(mf_server *s, long long requestId, int templateIndex, int channel, void *out,
+                      int cap, int *len);
+int mf_server_get_rho_device(mf_server *s, long long requestId, int templateIndex, int channel,
+                             const void **dptr, int *len);
+/* Milliseconds from submission to launch and to completion. */
+int mf_server_latency(mf_server *s, long long requestId, double *queuedMs, double *totalMs);
+/* Frees the request's HBM; waiting requests may then be admitted. */
+int mf_server_release(mf_server *s, long long requestId);
+/* Batches launched, requests completed, and correlation samples completed. */
+int mf_server_stats(mf_server *s, long long *batches, long long *requests, double *corrSamples);
+
 #ifdef __cplusplus
 }
 #endif

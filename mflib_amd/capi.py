@@ -71,6 +71,22 @@ _SIGNATURES = {
                          ctypes.c_int, _ip], ctypes.c_int),
     "mf_compute_optimal_fft_and_block_length": ([ctypes.c_int, ctypes.c_int, _ip, _ip], ctypes.c_int),
     "mf_get_timings": ([_vp, _dp, _dp], ctypes.c_int),
+    "mf_server_create": ([ctypes.POINTER(_vp), ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_int],
+                         ctypes.c_int),
+    "mf_server_destroy": ([_vp], ctypes.c_int),
+    "mf_server_add_templates": ([_vp, ctypes.c_int, _ip, ctypes.POINTER(_dp), _ip], ctypes.c_int),
+    "mf_server_submit": ([_vp, ctypes.c_int, _ip, ctypes.POINTER(_vp), ctypes.c_int, ctypes.c_int, _ip,
+                          ctypes.POINTER(ctypes.c_longlong)], ctypes.c_int),
+    "mf_server_wait": ([_vp, ctypes.c_longlong, _ip], ctypes.c_int),
+    "mf_server_poll": ([_vp, ctypes.c_longlong, _ip], ctypes.c_int),
+    "mf_server_get_rho": ([_vp, ctypes.c_longlong, ctypes.c_int, ctypes.c_int, _vp, ctypes.c_int, _ip],
+                          ctypes.c_int),
+    "mf_server_get_rho_device": ([_vp, ctypes.c_longlong, ctypes.c_int, ctypes.c_int, ctypes.POINTER(_vp), _ip],
+                                 ctypes.c_int),
+    "mf_server_latency": ([_vp, ctypes.c_longlong, _dp, _dp], ctypes.c_int),
+    "mf_server_release": ([_vp, ctypes.c_longlong], ctypes.c_int),
+    "mf_server_stats": ([_vp, ctypes.POINTER(ctypes.c_longlong), ctypes.POINTER(ctypes.c_longlong), _dp],
+                        ctypes.c_int),
 }
 
 
@@ -306,3 +322,98 @@ class Handle:
         a, b = ctypes.c_double(), ctypes.c_double()
         check(lib().mf_get_timings(self._h, ctypes.byref(a), ctypes.byref(b)))
         return a.value, b.value
+
+
+class Server:
+    """Request server (mf_server_*): resident template library, continuous
+    batching of matched-filter requests, per-request latency."""
+
+    def __init__(self, dtype=np.float32, budget_bytes: float = 8 << 30, max_batch: int = 256,
+                 max_inflight: int = 2):
+        self.dtype = np.dtype(dtype)
+        self._h = _vp()
+        check(lib().mf_server_create(ctypes.byref(self._h), PRECISION[self.dtype], float(budget_bytes),
+                                     int(max_batch), int(max_inflight)), "mf_server_create")
+        self._keep = {}
+
+    def close(self):
+        if self._h:
+            lib().mf_server_destroy(self._h)
+            self._h = _vp()
+            self._keep.clear()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def add_templates(self, templates) -> list:
+        arrs = [np.ascontiguousarray(t, dtype=np.float64) for t in templates]
+        lens = (ctypes.c_int * len(arrs))(*[a.size for a in arrs])
+        ptrs = (_dp * len(arrs))(*[a.ctypes.data_as(_dp) for a in arrs])
+        first = ctypes.c_int()
+        check(lib().mf_server_add_templates(self._h, len(arrs), lens, ptrs, ctypes.byref(first)),
+              "mf_server_add_templates")
+        return list(range(first.value, first.value + len(arrs)))
+
+    def submit(self, channels, template_ids, on_device: bool = False, lengths=None) -> int:
+        """channels: host numpy arrays of the server dtype, or (with
+        on_device) device addresses with `lengths`."""
+        n = len(channels)
+        if on_device:
+            lens = list(lengths)
+            ptrs = (_vp * n)(*[int(p) for p in channels])
+            keep = None
+        else:
+            arrs = [np.ascontiguousarray(c, dtype=self.dtype) for c in channels]
+            lens = [a.size for a in arrs]
+            ptrs = (_vp * n)(*[a.ctypes.data for a in arrs])
+            keep = arrs
+        ids = (ctypes.c_int * len(template_ids))(*[int(t) for t in template_ids])
+        rid = ctypes.c_longlong()
+        check(lib().mf_server_submit(self._h, n, (ctypes.c_int * n)(*lens), ptrs, int(bool(on_device)),
+                                     len(template_ids), ids, ctypes.byref(rid)), "mf_server_submit")
+        if keep is not None:
+            self._keep[rid.value] = keep  # host inputs must outlive the request
+        return rid.value
+
+    def wait(self, rid: int) -> None:
+        st = ctypes.c_int()
+        check(lib().mf_server_wait(self._h, rid, ctypes.byref(st)), "mf_server_wait")
+        check(st.value, f"request {rid}")
+
+    def done(self, rid: int) -> bool:
+        d = ctypes.c_int()
+        check(lib().mf_server_poll(self._h, rid, ctypes.byref(d)), "mf_server_poll")
+        return bool(d.value)
+
+    def rho(self, rid: int, template_index: int, channel: int) -> np.ndarray:
+        p, n = _vp(), ctypes.c_int()
+        check(lib().mf_server_get_rho_device(self._h, rid, template_index, channel, ctypes.byref(p),
+                                             ctypes.byref(n)), "mf_server_get_rho_device")
+        out = np.empty(n.value, dtype=self.dtype)
+        check(lib().mf_server_get_rho(self._h, rid, template_index, channel, out.ctypes.data, out.size,
+                                      ctypes.byref(n)), "mf_server_get_rho")
+        return out
+
+    def rho_device(self, rid: int, template_index: int, channel: int):
+        p, n = _vp(), ctypes.c_int()
+        check(lib().mf_server_get_rho_device(self._h, rid, template_index, channel, ctypes.byref(p),
+                                             ctypes.byref(n)), "mf_server_get_rho_device")
+        return p.value, n.value
+
+    def latency(self, rid: int):
+        q, t = ctypes.c_double(), ctypes.c_double()
+        check(lib().mf_server_latency(self._h, rid, ctypes.byref(q), ctypes.byref(t)), "mf_server_latency")
+        return q.value, t.value
+
+    def release(self, rid: int) -> None:
+        check(lib().mf_server_release(self._h, rid), "mf_server_release")
+        self._keep.pop(rid, None)
+
+    def stats(self):
+        b, r, c = ctypes.c_longlong(), ctypes.c_longlong(), ctypes.c_double()
+        check(lib().mf_server_stats(self._h, ctypes.byref(b), ctypes.byref(r), ctypes.byref(c)),
+              "mf_server_stats")
+        return b.value, r.value, c.value

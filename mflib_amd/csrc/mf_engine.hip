@@ -14,6 +14,7 @@
 // (workgroup i runs on XCD i mod 8), which keeps that channel's template
 // spectra resident in that XCD's L2.
 #include "mflib_gpu.h"
+#include "mf_host.hpp"
 #include "mf_kernels.hpp"
 
 #include <algorithm>
@@ -28,147 +29,7 @@
 #include <utility>
 #include <vector>
 
-namespace {
-
-thread_local std::string g_lastError;
-
-struct MfError {
-    int code;
-    std::string msg;
-};
-
-[[noreturn]] void fail(int code, const std::string &msg) { throw MfError{code, msg}; }
-
-#define MF_HIP(expr)                                                                        \
-    do {                                                                                    \
-        hipError_t e_ = (expr);                                                             \
-        if (e_ != hipSuccess) {                                                             \
-            fail(e_ == hipErrorOutOfMemory ? MF_ERR_OUT_OF_MEMORY : MF_ERR_DEVICE,          \
-                 std::string(#expr) + " failed: " + hipGetErrorString(e_));                 \
-        }                                                                                   \
-    } while (0)
-
-template <class F>
-int guard(F &&f)
-{
-    try {
-        f();
-        return MF_OK;
-    } catch (const MfError &e) {
-        g_lastError = e.msg;
-        return e.code;
-    } catch (const std::bad_alloc &) {
-        g_lastError = "out of host memory";
-        return MF_ERR_OUT_OF_MEMORY;
-    } catch (const std::exception &e) {
-        g_lastError = e.what();
-        return MF_ERR_DEVICE;
-    } catch (...) {
-        g_lastError = "unknown failure";
-        return MF_ERR_DEVICE;
-    }
-}
-
-// RAII device allocation.
-struct DevBuf {
-    void *p = nullptr;
-    size_t bytes = 0;
-    DevBuf() = default;
-    DevBuf(const DevBuf &) = delete;
-    DevBuf &operator=(const DevBuf &) = delete;
-    DevBuf(DevBuf &&o) noexcept : p(o.p), bytes(o.bytes) { o.p = nullptr; o.bytes = 0; }
-    DevBuf &operator=(DevBuf &&o) noexcept
-    {
-        if (this != &o) { release(); p = o.p; bytes = o.bytes; o.p = nullptr; o.bytes = 0; }
-        return *this;
-    }
-    ~DevBuf() { release(); }
-    void release()
-    {
-        if (p) { (void)hipFree(p); }
-        p = nullptr;
-        bytes = 0;
-    }
-    void alloc(size_t nb)
-    {
-        release();
-        if (nb == 0) { return; }
-        hipError_t e = hipMalloc(&p, nb);
-        if (e != hipSuccess) {
-            p = nullptr;
-            fail(e == hipErrorOutOfMemory ? MF_ERR_OUT_OF_MEMORY : MF_ERR_DEVICE,
-                 "hipMalloc of " + std::to_string(nb) + " bytes failed: " + hipGetErrorString(e));
-        }
-        bytes = nb;
-    }
-    template <typename T>
-    T *as() const { return static_cast<T *>(p); }
-};
-
-size_t roundUp(size_t v, size_t m) { return (v + m - 1) / m * m; }
-
-// Same table and rule as computeWindowLength
-// (src/matchedFilterParameters.cpp:47-104, 371-385).
-const double kFftCost[21] = {18,       59,       138,      303,      660,     1441,    3150,
-                             6875,     14952,    32373,    69762,    149647,  319644,  680105,
-                             1441974,  3047619,  6422736,  13500637, 28311786, 59244791,
-                             59244791 * 2.09};
-
-int referenceWindowLength(int nb, int nx, int *nfft, int *L)
-{
-    if (nb < 1) { fail(MF_ERR_INVALID_ARGUMENT, "Filter length must be positive"); }
-    if (nx <= nb) {
-        fail(MF_ERR_INVALID_ARGUMENT, "Signal length = " + std::to_string(nx) +
-                                          " must exceed filter length = " + std::to_string(nb));
-    }
-    int first = -1;
-    for (int i = 0; i < 21; ++i) {
-        if ((2LL << i) > nb - 1) { first = i; break; }
-    }
-    if (first < 0) {
-        long long p = 1;
-        while (p < nx) { p <<= 1; }
-        if (p > 0x7fffffffLL) { fail(MF_ERR_INVALID_ARGUMENT, "Overflow error"); }
-        *nfft = static_cast<int>(p);
-        *L = static_cast<int>(p) - (nb - 1);
-        return 0;
-    }
-    double best = 1e308;
-    for (int i = first; i < 21; ++i) {
-        long long len = 2LL << i;
-        long long lt = len - (nb - 1);
-        double obj = std::ceil(nx / static_cast<double>(lt)) * kFftCost[i];
-        if (obj < best) {
-            best = obj;
-            *nfft = static_cast<int>(len);
-            *L = static_cast<int>(lt);
-        }
-    }
-    return 0;
-}
-
-// Two-level twiddle tables for W_{2M}: lo[i] = W^i (i < 64),
-// hi[j] = W^(64 j) (j < 2M/64), FP64 from cos/sin, rounded to T.
-template <typename T>
-std::vector<T> twiddleTable(int logM, bool high)
-{
-    const long long N = 2LL << logM;
-    const int count = high ? static_cast<int>(N / 64) : 64;
-    std::vector<T> v(2 * count);
-    for (int i = 0; i < count; ++i) {
-        const long long e = high ? 64LL * i : i;
-        const double ang = -2.0 * M_PI * static_cast<double>(e) / static_cast<double>(N);
-        v[2 * i] = static_cast<T>(std::cos(ang));
-        v[2 * i + 1] = static_cast<T>(std::sin(ang));
-    }
-    return v;
-}
-
-struct TwTables {
-    DevBuf lo, hi, lo64, hi64;
-};
-
-}  // namespace
+using namespace mfh;
 
 struct mf_handle {
     int precision = MF_PRECISION_FLOAT;
@@ -258,8 +119,6 @@ struct mf_handle {
 
 namespace {
 
-int maxLogM(int precision) { return precision == MF_PRECISION_FLOAT ? mfk::kMaxLogMFloat : mfk::kMaxLogMDouble; }
-
 void requireHandle(const mf_handle *h)
 {
     if (h == nullptr) { fail(MF_ERR_INVALID_ARGUMENT, "handle is NULL"); }
@@ -303,58 +162,9 @@ void dropDevice(mf_handle *h)
     h->applied = false;
 }
 
-// Cost model of one lane for FFT size 2^(logM+1): blocks x (forward FFT +
-// one inverse FFT, product and epilogue per row), weighted per block size
-// by the measured efficiency of the fastest kernel variant there (the
-// weights are time / (blocks M (log2 M + 1.5)) on MI355X for 16 rows of
-// 512 samples, relative to M = 1024).  mf_set_fft_length overrides it.
-double laneCost(int logM, int nb, long long nOut, int nRows, int precision)
-{
-    const long long N = 2LL << logM;
-    const long long L = N - nb + 1;
-    if (L < 1) { return -1; }
-    const double blocks = std::ceil(static_cast<double>(nOut) / static_cast<double>(L));
-    const double M = static_cast<double>(N / 2);
-    static const double occF[] = {0.97, 1.00, 1.97, 3.30};  // logM 9..12, FP32
-    static const double occD[] = {1.00, 1.60, 2.40, 9.99};
-    const double occ = (precision == MF_PRECISION_FLOAT ? occF : occD)[logM - mfk::kMinLogM];
-    return blocks * (1.0 + 1.15 * nRows) * M * (logM + 1.5) * occ;
-}
-
 int chooseLogM(const mf_handle *h, int nb, long long nOut, int nRows)
 {
-    const int hiLog = maxLogM(h->precision);
-    if (h->userNfft > 0) {
-        int lg = 0;
-        while ((1 << (lg + 1)) < h->userNfft) { ++lg; }
-        if ((2 << lg) != h->userNfft || lg < mfk::kMinLogM || lg > hiLog) {
-            fail(MF_ERR_UNSUPPORTED, "FFT length " + std::to_string(h->userNfft) +
-                                         " is not a supported power of two in [" +
-                                         std::to_string(2 << mfk::kMinLogM) + ", " +
-                                         std::to_string(2 << hiLog) + "]");
-        }
-        if ((2LL << lg) - nb + 1 < 1) {
-            fail(MF_ERR_INVALID_ARGUMENT, "FFT length " + std::to_string(h->userNfft) +
-                                              " is shorter than the template length " +
-                                              std::to_string(nb));
-        }
-        return lg;
-    }
-    int best = -1;
-    double bestCost = 0;
-    for (int lg = mfk::kMinLogM; lg <= hiLog; ++lg) {
-        const long long N = 2LL << lg;
-        if (N - nb + 1 < N / 4 && lg < hiLog) { continue; }  // keep >= 25% useful outputs
-        const double c = laneCost(lg, nb, nOut, nRows, h->precision);
-        if (c < 0) { continue; }
-        if (best < 0 || c < bestCost) { best = lg; bestCost = c; }
-    }
-    if (best < 0) {
-        fail(MF_ERR_UNSUPPORTED, "template length " + std::to_string(nb) +
-                                     " exceeds the largest supported FFT block (" +
-                                     std::to_string(2 << hiLog) + " samples)");
-    }
-    return best;
+    return mfh::chooseLogM(h->precision, h->userNfft, nb, nOut, nRows);
 }
 
 void deriveLayout(mf_handle *h, std::vector<int> &rowCh, std::vector<int> &rowGroup,
@@ -440,10 +250,7 @@ void initialize(mf_handle *h)
             ln.nOut = std::max(ln.nOut, h->rows[r].nOut);
         }
         ln.logM = chooseLogM(h, nb, ln.nOut, static_cast<int>(chRows[c].size()));
-        // even block lengths keep every block start 8-byte aligned for the
-        // paired stores of the epilogue
-        ln.L = static_cast<int>((2LL << ln.logM) - nb + 1);
-        if (ln.L > 1 && (ln.L & 1)) { ln.L -= 1; }
+        ln.L = blockLength(ln.logM, nb);
         ln.rows = chRows[c];
         std::stable_sort(ln.rows.begin(), ln.rows.end(),
                          [&](int a, int b) { return h->rows[a].Lt < h->rows[b].Lt; });
@@ -481,21 +288,7 @@ void initialize(mf_handle *h)
     // twiddle tables
     for (const auto &ln : h->lanes) {
         if (h->tw.count(ln.logM)) { continue; }
-        TwTables &t = h->tw[ln.logM];
-        auto lo64 = twiddleTable<double>(ln.logM, false);
-        auto hi64 = twiddleTable<double>(ln.logM, true);
-        t.lo64.alloc(lo64.size() * sizeof(double));
-        t.hi64.alloc(hi64.size() * sizeof(double));
-        MF_HIP(hipMemcpy(t.lo64.p, lo64.data(), t.lo64.bytes, hipMemcpyHostToDevice));
-        MF_HIP(hipMemcpy(t.hi64.p, hi64.data(), t.hi64.bytes, hipMemcpyHostToDevice));
-        if (h->precision == MF_PRECISION_FLOAT) {
-            auto lo = twiddleTable<float>(ln.logM, false);
-            auto hi = twiddleTable<float>(ln.logM, true);
-            t.lo.alloc(lo.size() * sizeof(float));
-            t.hi.alloc(hi.size() * sizeof(float));
-            MF_HIP(hipMemcpy(t.lo.p, lo.data(), t.lo.bytes, hipMemcpyHostToDevice));
-            MF_HIP(hipMemcpy(t.hi.p, hi.data(), t.hi.bytes, hipMemcpyHostToDevice));
-        }
+        makeTwiddles(h->tw[ln.logM], ln.logM, h->precision == MF_PRECISION_FLOAT);
     }
     // templates: upload raw FP64, prepare + transform on the GPU
     {
@@ -564,37 +357,16 @@ void initialize(mf_handle *h)
     std::vector<mfk::Item> items;
     std::map<int, std::vector<int>> lanesByLog;
     for (size_t li = 0; li < h->lanes.size(); ++li) { lanesByLog[h->lanes[li].logM].push_back(static_cast<int>(li)); }
-    constexpr int kXcd = 8;
     for (auto &kv : lanesByLog) {
-        // assign lanes to XCD bins, largest first
-        std::vector<int> order = kv.second;
-        auto blocksOf = [&](int li) {
+        std::vector<long long> blocks;
+        for (int li : kv.second) {
             const auto &ln = h->lanes[li];
-            return static_cast<long long>((ln.nOut + ln.L - 1) / ln.L);
-        };
-        std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return blocksOf(a) > blocksOf(b); });
-        std::vector<std::vector<mfk::Item>> bins(kXcd);
-        std::vector<long long> load(kXcd, 0);
-        for (int li : order) {
-            int b = static_cast<int>(std::min_element(load.begin(), load.end()) - load.begin());
-            const long long nb = blocksOf(li);
-            for (long long k = 0; k < nb; ++k) { bins[b].push_back({li, static_cast<int>(k)}); }
-            load[b] += nb;
+            blocks.push_back((ln.nOut + ln.L - 1) / ln.L);
         }
         mf_handle::Launch la;
         la.logM = kv.first;
         la.first = static_cast<int>(items.size());
-        std::vector<size_t> pos(kXcd, 0);
-        bool any = true;
-        while (any) {
-            any = false;
-            for (int b = 0; b < kXcd; ++b) {
-                if (pos[b] < bins[b].size()) {
-                    items.push_back(bins[b][pos[b]++]);
-                    any = true;
-                }
-            }
-        }
+        appendXcdItems(kv.second, blocks, items);
         la.count = static_cast<int>(items.size()) - la.first;
         h->launches.push_back(la);
     }
@@ -660,13 +432,7 @@ void apply(mf_handle *h)
         const TwTables &t = h->tw.at(la.logM);
         const void *lo = h->precision == MF_PRECISION_FLOAT ? t.lo.p : t.lo64.p;
         const void *hi = h->precision == MF_PRECISION_FLOAT ? t.hi.p : t.hi64.p;
-        int variant = h->userVariant;
-        if (variant == 0 || !mfk::corr_variant_available(h->precision, la.logM, variant)) {
-            // wave-level kernel where it keeps >= 2 waves per SIMD (measured
-            // faster there); block-wide kernel for the larger blocks
-            const bool waveFaster = h->precision == MF_PRECISION_FLOAT ? la.logM <= 10 : la.logM <= 9;
-            variant = waveFaster && mfk::corr_variant_available(h->precision, la.logM, 2) ? 2 : 1;
-        }
+        const int variant = corrVariant(h->precision, la.logM, h->userVariant);
         MF_HIP(mfk::launch_corr(h->precision, la.logM, variant, h->dItems.as<mfk::Item>() + la.first,
                                 la.count, h->dSegs.as<mfk::Seg>(), h->dRowTab.as<mfk::Row>(), lo, hi,
                                 h->stream));
@@ -975,10 +741,7 @@ int mf_create(mf_handle **h, int precision, int mode)
         if (mode != MF_MODE_SINGLE_CHANNEL && mode != MF_MODE_MULTI_CHANNEL && mode != MF_MODE_BATCHED) {
             fail(MF_ERR_INVALID_ARGUMENT, "unknown detection mode " + std::to_string(mode));
         }
-        int ndev = 0;
-        if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) {
-            fail(MF_ERR_DEVICE, "no HIP device is available (the engine has no CPU fallback)");
-        }
+        requireDevice();
         auto *p = new mf_handle();
         p->precision = precision;
         p->mode = mode;

@@ -1,0 +1,331 @@
+// Host-side helpers shared by the engine (mf_engine.hip) and the request
+// server (mf_server.hip): error plumbing of the C ABI, device and pinned
+// buffers, FFT planning and the XCD-aware ordering of work items.
+#ifndef MF_HOST_HPP
+#define MF_HOST_HPP
+
+#include "mflib_gpu.h"
+#include "mf_kernels.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <new>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+namespace mfh {
+
+
+inline thread_local std::string g_lastError;
+
+struct MfError {
+    int code;
+    std::string msg;
+};
+
+[[noreturn]] inline void fail(int code, const std::string &msg) { throw MfError{code, msg}; }
+
+#define MF_HIP(expr)                                                                        \
+    do {                                                                                    \
+        hipError_t e_ = (expr);                                                             \
+        if (e_ != hipSuccess) {                                                             \
+            fail(e_ == hipErrorOutOfMemory ? MF_ERR_OUT_OF_MEMORY : MF_ERR_DEVICE,          \
+                 std::string(#expr) + " failed: " + hipGetErrorString(e_));                 \
+        }                                                                                   \
+    } while (0)
+
+template <class F>
+int guard(F &&f)
+{
+    try {
+        f();
+        return MF_OK;
+    } catch (const MfError &e) {
+        g_lastError = e.msg;
+        return e.code;
+    } catch (const std::bad_alloc &) {
+        g_lastError = "out of host memory";
+        return MF_ERR_OUT_OF_MEMORY;
+    } catch (const std::exception &e) {
+        g_lastError = e.what();
+        return MF_ERR_DEVICE;
+    } catch (...) {
+        g_lastError = "unknown failure";
+        return MF_ERR_DEVICE;
+    }
+}
+
+// RAII device allocation.
+struct DevBuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf &) = delete;
+    DevBuf &operator=(const DevBuf &) = delete;
+    DevBuf(DevBuf &&o) noexcept : p(o.p), bytes(o.bytes) { o.p = nullptr; o.bytes = 0; }
+    DevBuf &operator=(DevBuf &&o) noexcept
+    {
+        if (this != &o) { release(); p = o.p; bytes = o.bytes; o.p = nullptr; o.bytes = 0; }
+        return *this;
+    }
+    ~DevBuf() { release(); }
+    void release()
+    {
+        if (p) { (void)hipFree(p); }
+        p = nullptr;
+        bytes = 0;
+    }
+    void alloc(size_t nb)
+    {
+        release();
+        if (nb == 0) { return; }
+        hipError_t e = hipMalloc(&p, nb);
+        if (e != hipSuccess) {
+            p = nullptr;
+            fail(e == hipErrorOutOfMemory ? MF_ERR_OUT_OF_MEMORY : MF_ERR_DEVICE,
+                 "hipMalloc of " + std::to_string(nb) + " bytes failed: " + hipGetErrorString(e));
+        }
+        bytes = nb;
+    }
+    template <typename T>
+    T *as() const { return static_cast<T *>(p); }
+};
+
+inline size_t roundUp(size_t v, size_t m) { return (v + m - 1) / m * m; }
+
+// Same table and rule as computeWindowLength
+// (src/matchedFilterParameters.cpp:47-104, 371-385).
+inline const double kFftCost[21] = {18,       59,       138,      303,      660,     1441,    3150,
+                             6875,     14952,    32373,    69762,    149647,  319644,  680105,
+                             1441974,  3047619,  6422736,  13500637, 28311786, 59244791,
+                             59244791 * 2.09};
+
+inline int referenceWindowLength(int nb, int nx, int *nfft, int *L)
+{
+    if (nb < 1) { fail(MF_ERR_INVALID_ARGUMENT, "Filter length must be positive"); }
+    if (nx <= nb) {
+        fail(MF_ERR_INVALID_ARGUMENT, "Signal length = " + std::to_string(nx) +
+                                          " must exceed filter length = " + std::to_string(nb));
+    }
+    int first = -1;
+    for (int i = 0; i < 21; ++i) {
+        if ((2LL << i) > nb - 1) { first = i; break; }
+    }
+    if (first < 0) {
+        long long p = 1;
+        while (p < nx) { p <<= 1; }
+        if (p > 0x7fffffffLL) { fail(MF_ERR_INVALID_ARGUMENT, "Overflow error"); }
+        *nfft = static_cast<int>(p);
+        *L = static_cast<int>(p) - (nb - 1);
+        return 0;
+    }
+    double best = 1e308;
+    for (int i = first; i < 21; ++i) {
+        long long len = 2LL << i;
+        long long lt = len - (nb - 1);
+        double obj = std::ceil(nx / static_cast<double>(lt)) * kFftCost[i];
+        if (obj < best) {
+            best = obj;
+            *nfft = static_cast<int>(len);
+            *L = static_cast<int>(lt);
+        }
+    }
+    return 0;
+}
+
+// Two-level twiddle tables for W_{2M}: lo[i] = W^i (i < 64),
+// hi[j] = W^(64 j) (j < 2M/64), FP64 from cos/sin, rounded to T.
+template <typename T>
+std::vector<T> twiddleTable(int logM, bool high)
+{
+    const long long N = 2LL << logM;
+    const int count = high ? static_cast<int>(N / 64) : 64;
+    std::vector<T> v(2 * count);
+    for (int i = 0; i < count; ++i) {
+        const long long e = high ? 64LL * i : i;
+        const double ang = -2.0 * M_PI * static_cast<double>(e) / static_cast<double>(N);
+        v[2 * i] = static_cast<T>(std::cos(ang));
+        v[2 * i + 1] = static_cast<T>(std::sin(ang));
+    }
+    return v;
+}
+
+struct TwTables {
+    DevBuf lo, hi, lo64, hi64;
+};
+
+// Uploads the FP64 twiddle tables of FFT size 2^(logM+1) and, for FP32
+// work, their FP32 roundings.
+inline void makeTwiddles(TwTables &t, int logM, bool wantFloat)
+{
+    auto lo64 = twiddleTable<double>(logM, false);
+    auto hi64 = twiddleTable<double>(logM, true);
+    t.lo64.alloc(lo64.size() * sizeof(double));
+    t.hi64.alloc(hi64.size() * sizeof(double));
+    MF_HIP(hipMemcpy(t.lo64.p, lo64.data(), t.lo64.bytes, hipMemcpyHostToDevice));
+    MF_HIP(hipMemcpy(t.hi64.p, hi64.data(), t.hi64.bytes, hipMemcpyHostToDevice));
+    if (wantFloat) {
+        auto lo = twiddleTable<float>(logM, false);
+        auto hi = twiddleTable<float>(logM, true);
+        t.lo.alloc(lo.size() * sizeof(float));
+        t.hi.alloc(hi.size() * sizeof(float));
+        MF_HIP(hipMemcpy(t.lo.p, lo.data(), t.lo.bytes, hipMemcpyHostToDevice));
+        MF_HIP(hipMemcpy(t.hi.p, hi.data(), t.hi.bytes, hipMemcpyHostToDevice));
+    }
+}
+
+// Correlation kernel variant for an FFT size: the wave-level kernel up to
+// nfft 4096 in FP32 and 1024 in FP64 (measured faster there: at nfft 4096
+// with 50 rows per lane it runs 1.2-1.3x the block-wide kernel), the
+// block-wide kernel for the larger blocks.  A requested variant that does
+// not exist for the size falls back to this choice.
+inline int corrVariant(int precision, int logM, int requested)
+{
+    if (requested != 0 && mfk::corr_variant_available(precision, logM, requested)) { return requested; }
+    const bool waveFaster = precision == MF_PRECISION_FLOAT ? logM <= 11 : logM <= 9;
+    return waveFaster && mfk::corr_variant_available(precision, logM, 2) ? 2 : 1;
+}
+
+// First device present, or a failure: the engine has no CPU fallback.
+inline void requireDevice()
+{
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) {
+        fail(MF_ERR_DEVICE, "no HIP device is available (the engine has no CPU fallback)");
+    }
+}
+
+
+// RAII pinned host allocation (hipHostMalloc).
+struct PinnedBuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+    PinnedBuf() = default;
+    PinnedBuf(const PinnedBuf &) = delete;
+    PinnedBuf &operator=(const PinnedBuf &) = delete;
+    ~PinnedBuf() { release(); }
+    void release()
+    {
+        if (p) { (void)hipHostFree(p); }
+        p = nullptr;
+        bytes = 0;
+    }
+    void reserve(size_t nb)
+    {
+        if (nb <= bytes) { return; }
+        release();
+        hipError_t e = hipHostMalloc(&p, nb, hipHostMallocDefault);
+        if (e != hipSuccess) {
+            p = nullptr;
+            fail(MF_ERR_OUT_OF_MEMORY, "hipHostMalloc of " + std::to_string(nb) + " bytes failed: " +
+                                           hipGetErrorString(e));
+        }
+        bytes = nb;
+    }
+};
+
+inline int maxLogM(int precision)
+{
+    return precision == MF_PRECISION_FLOAT ? mfk::kMaxLogMFloat : mfk::kMaxLogMDouble;
+}
+
+// Cost model of one lane for FFT size 2^(logM+1): blocks x (forward FFT +
+// one inverse FFT, product and epilogue per row), weighted per block size
+// by the measured efficiency of the fastest kernel variant there (the
+// weights are time / (blocks M (log2 M + 1.5)) on MI355X for 16 rows of
+// 512 samples, relative to M = 1024).
+inline double laneCost(int logM, int nb, long long nOut, int nRows, int precision)
+{
+    const long long N = 2LL << logM;
+    const long long L = N - nb + 1;
+    if (L < 2) { return -1; }  // even block starts need L >= 2
+    const double blocks = std::ceil(static_cast<double>(nOut) / static_cast<double>(L));
+    const double M = static_cast<double>(N / 2);
+    static const double occF[] = {0.97, 1.00, 1.97, 3.30};  // logM 9..12, FP32
+    static const double occD[] = {1.00, 1.60, 2.40, 9.99};
+    const double occ = (precision == MF_PRECISION_FLOAT ? occF : occD)[logM - mfk::kMinLogM];
+    return blocks * (1.0 + 1.15 * nRows) * M * (logM + 1.5) * occ;
+}
+
+// FFT size (as logM, nfft = 2^(logM+1)) of a lane whose longest template
+// has nb samples and which produces nOut outputs for nRows rows.  A
+// positive userNfft forces the size.
+inline int chooseLogM(int precision, int userNfft, int nb, long long nOut, int nRows)
+{
+    const int hiLog = maxLogM(precision);
+    if (userNfft > 0) {
+        int lg = 0;
+        while ((1 << (lg + 1)) < userNfft) { ++lg; }
+        if ((2 << lg) != userNfft || lg < mfk::kMinLogM || lg > hiLog) {
+            fail(MF_ERR_UNSUPPORTED, "FFT length " + std::to_string(userNfft) +
+                                         " is not a supported power of two in [" +
+                                         std::to_string(2 << mfk::kMinLogM) + ", " +
+                                         std::to_string(2 << hiLog) + "]");
+        }
+        if ((2LL << lg) - nb + 1 < 1) {
+            fail(MF_ERR_INVALID_ARGUMENT, "FFT length " + std::to_string(userNfft) +
+                                              " is shorter than the template length " + std::to_string(nb));
+        }
+        return lg;
+    }
+    int best = -1;
+    double bestCost = 0;
+    for (int lg = mfk::kMinLogM; lg <= hiLog; ++lg) {
+        const long long N = 2LL << lg;
+        if (N - nb + 1 < N / 4 && lg < hiLog) { continue; }  // keep >= 25% useful outputs
+        const double c = laneCost(lg, nb, nOut, nRows, precision);
+        if (c < 0) { continue; }
+        if (best < 0 || c < bestCost) { best = lg; bestCost = c; }
+    }
+    if (best < 0) {
+        fail(MF_ERR_UNSUPPORTED, "template length " + std::to_string(nb) +
+                                     " exceeds the largest supported FFT block (" + std::to_string(2 << hiLog) +
+                                     " samples)");
+    }
+    return best;
+}
+
+// Block length of a lane: nfft - nb + 1, made even so that every block
+// start stays 8-byte aligned for the paired stores of the epilogue.
+inline int blockLength(int logM, int nb)
+{
+    int L = static_cast<int>((2LL << logM) - nb + 1);
+    if (L > 1 && (L & 1)) { L -= 1; }
+    return L;
+}
+
+// Work items of a set of lanes sharing one FFT size, in XCD-aware order:
+// lanes are binned to the 8 XCDs (largest first, least-loaded bin), and
+// the bins are interleaved so that workgroup i, which runs on XCD i mod 8,
+// takes its blocks from one bin.  All blocks of a lane then share one L2.
+inline void appendXcdItems(const std::vector<int> &laneIds, const std::vector<long long> &laneBlocks,
+                           std::vector<mfk::Item> &items)
+{
+    constexpr int kXcd = 8;
+    std::vector<int> order(laneIds.size());
+    for (size_t i = 0; i < order.size(); ++i) { order[i] = static_cast<int>(i); }
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return laneBlocks[a] > laneBlocks[b]; });
+    std::vector<std::vector<mfk::Item>> bins(kXcd);
+    std::vector<long long> load(kXcd, 0);
+    for (int i : order) {
+        const int b = static_cast<int>(std::min_element(load.begin(), load.end()) - load.begin());
+        for (long long k = 0; k < laneBlocks[i]; ++k) { bins[b].push_back({laneIds[i], static_cast<int>(k)}); }
+        load[b] += laneBlocks[i];
+    }
+    std::vector<size_t> pos(kXcd, 0);
+    bool any = true;
+    while (any) {
+        any = false;
+        for (int b = 0; b < kXcd; ++b) {
+            if (pos[b] < bins[b].size()) {
+                items.push_back(bins[b][pos[b]++]);
+                any = true;
+            }
+        }
+    }
+}
+
+}  // namespace mfh
+
+#endif

@@ -38,13 +38,18 @@ constexpr int NT = kThreads;
 // Error model of the FFT numerator: |du| <= kNoiseK * eps * sqrt(log2 nfft)
 // * max|T^| * rms(block).  A window is recomputed exactly when that bound,
 // divided by the window's own norm, could exceed the precision's rho
-// tolerance (1e-6 for FP32 correlation, 1e-12 for FP64).
+// tolerance.  Measured against the FP64 oracle with the recomputation
+// switched off (band-limited noise with injected events 2-8x the noise RMS
+// times sqrt(Lt), templates of 200-2000 samples, nfft 2048/4096), the worst
+// error was 1.47x the bound and 4.5e-7 absolute.  The FP32 tolerance of
+// 2.5e-6 therefore keeps unflagged windows below ~4e-6, inside the 1e-5
+// contract; FP64 correlation keeps 1e-12.
 template <typename T>
 struct Eps;
 template <>
 struct Eps<float> {
     static constexpr double value = 5.9604644775390625e-08;  // 2^-24
-    static constexpr double rhoTol = 1.0e-6;
+    static constexpr double rhoTol = 2.5e-6;
 };
 template <>
 struct Eps<double> {
@@ -52,6 +57,139 @@ struct Eps<double> {
     static constexpr double rhoTol = 1.0e-12;
 };
 constexpr double kNoiseK = 4.0;
+
+// Exclusive prefix sums tp[q] = sum of ts[q'] for q' < q (q = 0..NTILE) of
+// the 16-sample tile sums, computed by wave 0 with a shuffle scan (fixed
+// order, deterministic).  The caller synchronises before reading tp.
+template <int NTILE>
+__device__ __forceinline__ void tile_prefix(const double *ts1, const double *ts2, double *tp1, double *tp2,
+                                            int tid)
+{
+    static_assert(NTILE >= 64 && NTILE % 64 == 0, "tile count");
+    constexpr int PER = NTILE / 64;
+    if (tid >= 64) { return; }
+    double l1[PER], l2[PER];
+    double s1 = 0, s2 = 0;
+#pragma unroll
+    for (int p = 0; p < PER; ++p) {
+        l1[p] = s1;
+        l2[p] = s2;
+        s1 += ts1[tid * PER + p];
+        s2 += ts2[tid * PER + p];
+    }
+    double x1 = s1, x2 = s2;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const double y1 = __shfl_up(x1, o, 64);
+        const double y2 = __shfl_up(x2, o, 64);
+        if (tid >= o) {
+            x1 += y1;
+            x2 += y2;
+        }
+    }
+    const double e1 = x1 - s1, e2 = x2 - s2;
+#pragma unroll
+    for (int p = 0; p < PER; ++p) {
+        tp1[tid * PER + p] = e1 + l1[p];
+        tp2[tid * PER + p] = e2 + l2[p];
+    }
+    if (tid == 63) {
+        tp1[NTILE] = x1;
+        tp2[NTILE] = x2;
+    }
+}
+
+// Sums of d = x - a and d^2 over the window [lo, lo + Lt) of the block's
+// samples: partial tiles sample by sample, whole tiles from the tile
+// prefix sums (FP32 correlation) or tile by tile (FP64 correlation, whose
+// tighter tolerance keeps the summation order of the tiles).
+template <typename T>
+__device__ __forceinline__ void window_sums(const T *raw, const double *ts1, const double *ts2, const double *tp1,
+                                            const double *tp2, double a, int lo, int Lt, double &S1, double &S2)
+{
+    const int hi = lo + Lt;
+    const int q0 = (lo + 15) >> 4, q1 = hi >> 4;
+    S1 = 0;
+    S2 = 0;
+    if (q0 < q1) {
+        for (int j = lo; j < 16 * q0; ++j) {
+            const double d = static_cast<double>(raw[j]) - a;
+            S1 += d;
+            S2 += d * d;
+        }
+        if constexpr (sizeof(T) == 4) {
+            S1 += tp1[q1] - tp1[q0];
+            S2 += tp2[q1] - tp2[q0];
+        } else {
+            for (int q = q0; q < q1; ++q) {
+                S1 += ts1[q];
+                S2 += ts2[q];
+            }
+        }
+        for (int j = 16 * q1; j < hi; ++j) {
+            const double d = static_cast<double>(raw[j]) - a;
+            S1 += d;
+            S2 += d * d;
+        }
+    } else {
+        for (int j = lo; j < hi; ++j) {
+            const double d = static_cast<double>(raw[j]) - a;
+            S1 += d;
+            S2 += d * d;
+        }
+    }
+}
+
+// Per-output scale sqrt(Lt)/||x - mean|| for outputs [0, nValid) of this
+// block, FP64 sliding sums restarted every C outputs; flagged outputs get
+// -1.  Threads t = 0..nthr-1 take part, each a contiguous run of C
+// outputs.  The FP32 scale is rsqrt(den^2) in FP32 plus one Newton step
+// (relative error ~2^-23, far inside the FP32 contract); the FP64 scale is
+// sqrt/division in FP64.  Returns this thread's number of flagged outputs.
+template <typename T>
+__device__ __forceinline__ int compute_scale(const T *raw, const double *ts1, const double *ts2,
+                                             const double *tp1, const double *tp2, T *sc, double a, int nValid,
+                                             int Lt, double noise, int t, int nthr)
+{
+    const double sqrtLt = sqrt(static_cast<double>(Lt));
+    const double denThr = sqrtLt * noise;
+    const double denThr2 = denThr * denThr;
+    const int C = (nValid + nthr - 1) / nthr;
+    const double canc = 2.0e-9 * (Lt + C) * Lt;
+    const int i0 = t * C;
+    const int i1 = min(i0 + C, nValid);
+    int nflag = 0;
+    if (i0 >= i1) { return 0; }
+    double S1, S2;
+    window_sums<T>(raw, ts1, ts2, tp1, tp2, a, i0, Lt, S1, S2);
+    double S2max = S2;
+    const float sqrtLtF = static_cast<float>(sqrtLt);
+    for (int i = i0; i < i1; ++i) {
+        const double den2 = Lt * S2 - S1 * S1;
+        const bool flag = !(den2 > canc * S2max) || !(den2 > denThr2);
+        nflag += flag ? 1 : 0;
+        if constexpr (sizeof(T) == 4) {
+            float f = -1.0f;
+            if (!flag) {
+                const float d2 = static_cast<float>(den2);
+                float y = __builtin_amdgcn_rsqf(d2);
+                y = y * (1.5f - 0.5f * d2 * y * y);
+                f = sqrtLtF * y;
+            }
+            sc[i] = f;
+        } else {
+            sc[i] = flag ? T(-1) : static_cast<T>(sqrtLt / sqrt(den2));
+        }
+        if (i + 1 < i1) {
+            const double dn = static_cast<double>(raw[i + Lt]) - a;
+            const double dd = static_cast<double>(raw[i]) - a;
+            S1 += dn - dd;
+            S2 += dn * dn - dd * dd;
+            S2max = fmax(S2max, S2);
+        }
+    }
+    return nflag;
+}
 
 template <typename T, int M>
 __global__ __launch_bounds__(NT) void corr_kernel(const Item *__restrict__ items,
@@ -71,6 +209,8 @@ __global__ __launch_bounds__(NT) void corr_kernel(const Item *__restrict__ items
     __shared__ T sc[N];
     __shared__ double ts1[NTILE];
     __shared__ double ts2[NTILE];
+    __shared__ double tp1[NTILE + 1];
+    __shared__ double tp2[NTILE + 1];
     __shared__ Cx<T> twlo[64];
     __shared__ Cx<T> twhi[N / 64];
     __shared__ double red[NT / 64];
@@ -132,6 +272,7 @@ __global__ __launch_bounds__(NT) void corr_kernel(const Item *__restrict__ items
         wf[j] = valid ? static_cast<T>(v[q] - aT) : T(0);
     }
     const double energy = block_sum<NT>(e, red);  // also publishes raw/ts/Ws
+    tile_prefix<NTILE>(ts1, ts2, tp1, tp2, tid);   // read after the FFT's barriers
 
     // 2. forward FFT of the packed block, unpack to X' = 2X
     fft_lds<T, M, NT, false>(Ws, tw);
@@ -160,53 +301,7 @@ __global__ __launch_bounds__(NT) void corr_kernel(const Item *__restrict__ items
         if (Lt != curLt) {
             // 3. per-output scale for this template length
             curLt = Lt;
-            const double sqrtLt = sqrt(static_cast<double>(Lt));
-            const double denThr = sqrtLt * noise;
-            const int C = (nValidSeg + NT - 1) / NT;
-            const double canc = 2.0e-9 * (Lt + C) * Lt;
-            const int i0 = tid * C;
-            const int i1 = min(i0 + C, nValidSeg);
-            if (i0 < i1) {
-                double S1 = 0, S2 = 0;
-                const int lo = i0, hi = i0 + Lt;
-                const int q0 = (lo + 15) >> 4, q1 = hi >> 4;
-                if (q0 < q1) {
-                    for (int j = lo; j < 16 * q0; ++j) {
-                        const double d = static_cast<double>(raw[j]) - a;
-                        S1 += d;
-                        S2 += d * d;
-                    }
-                    for (int q = q0; q < q1; ++q) {
-                        S1 += ts1[q];
-                        S2 += ts2[q];
-                    }
-                    for (int j = 16 * q1; j < hi; ++j) {
-                        const double d = static_cast<double>(raw[j]) - a;
-                        S1 += d;
-                        S2 += d * d;
-                    }
-                } else {
-                    for (int j = lo; j < hi; ++j) {
-                        const double d = static_cast<double>(raw[j]) - a;
-                        S1 += d;
-                        S2 += d * d;
-                    }
-                }
-                double S2max = S2;
-                for (int i = i0; i < i1; ++i) {
-                    const double den2 = Lt * S2 - S1 * S1;
-                    const double den = sqrt(fmax(den2, 0.0));
-                    const bool flag = !(den2 > canc * S2max) || !(den > denThr);
-                    sc[i] = flag ? T(-1) : static_cast<T>(sqrtLt / den);
-                    if (i + 1 < i1) {
-                        const double dn = static_cast<double>(raw[i + Lt]) - a;
-                        const double dd = static_cast<double>(raw[i]) - a;
-                        S1 += dn - dd;
-                        S2 += dn * dn - dd * dd;
-                        S2max = fmax(S2max, S2);
-                    }
-                }
-            }
+            compute_scale<T>(raw, ts1, ts2, tp1, tp2, sc, a, nValidSeg, Lt, noise, tid, NT);
             __syncthreads();
         }
         // 4a. spectral product + inverse packing into Ws
@@ -279,65 +374,6 @@ __global__ __launch_bounds__(NT) void corr_kernel(const Item *__restrict__ items
 // barrier.  Rows are processed in groups of equal template length; the
 // scale of a group is shared by its rows.
 // ---------------------------------------------------------------------------
-
-// Per-output scale sqrt(Lt)/||x - mean|| for outputs [0, nValid) of this
-// block, FP64 sliding sums restarted every C outputs; flagged outputs get
-// -1.  Threads [tid0, tid0 + nthr) take part.  Returns this thread's number
-// of flagged outputs.
-template <typename T, int N>
-__device__ __forceinline__ int compute_scale(const T *raw, const double *ts1, const double *ts2, T *sc,
-                                             double a, int nValid, int Lt, double noise, int t, int nthr)
-{
-    const double sqrtLt = sqrt(static_cast<double>(Lt));
-    const double denThr = sqrtLt * noise;
-    const int C = (nValid + nthr - 1) / nthr;
-    const double canc = 2.0e-9 * (Lt + C) * Lt;
-    const int i0 = t * C;
-    const int i1 = min(i0 + C, nValid);
-    int nflag = 0;
-    if (i0 >= i1) { return 0; }
-    double S1 = 0, S2 = 0;
-    const int lo = i0, hi = i0 + Lt;
-    const int q0 = (lo + 15) >> 4, q1 = hi >> 4;
-    if (q0 < q1) {
-        for (int j = lo; j < 16 * q0; ++j) {
-            const double d = static_cast<double>(raw[j]) - a;
-            S1 += d;
-            S2 += d * d;
-        }
-        for (int q = q0; q < q1; ++q) {
-            S1 += ts1[q];
-            S2 += ts2[q];
-        }
-        for (int j = 16 * q1; j < hi; ++j) {
-            const double d = static_cast<double>(raw[j]) - a;
-            S1 += d;
-            S2 += d * d;
-        }
-    } else {
-        for (int j = lo; j < hi; ++j) {
-            const double d = static_cast<double>(raw[j]) - a;
-            S1 += d;
-            S2 += d * d;
-        }
-    }
-    double S2max = S2;
-    for (int i = i0; i < i1; ++i) {
-        const double den2 = Lt * S2 - S1 * S1;
-        const double den = sqrt(fmax(den2, 0.0));
-        const bool flag = !(den2 > canc * S2max) || !(den > denThr);
-        nflag += flag ? 1 : 0;
-        sc[i] = flag ? T(-1) : static_cast<T>(sqrtLt / den);
-        if (i + 1 < i1) {
-            const double dn = static_cast<double>(raw[i + Lt]) - a;
-            const double dd = static_cast<double>(raw[i]) - a;
-            S1 += dn - dd;
-            S2 += dn * dn - dd * dd;
-            S2max = fmax(S2max, S2);
-        }
-    }
-    return nflag;
-}
 
 // Exact FP64 two-pass Pearson for the flagged outputs of one row (one wave).
 template <typename T>
@@ -415,6 +451,8 @@ __global__ __launch_bounds__(NT, (Corr2Waves<T, M>::value)) void corr2_kernel(co
     __shared__ Cx<T> xbuf[NWAVE][XBuf<M>::size];
     __shared__ double ts1[NTILE];
     __shared__ double ts2[NTILE];
+    __shared__ double tp1[NTILE + 1];
+    __shared__ double tp2[NTILE + 1];
     __shared__ Cx<T> twlo[64];
     __shared__ Cx<T> twhi[N / 64];
     __shared__ double red[NWAVE];
@@ -476,119 +514,162 @@ __global__ __launch_bounds__(NT, (Corr2Waves<T, M>::value)) void corr2_kernel(co
         xin[2 * xpad(j >> 1) + (j & 1)] = valid ? static_cast<T>(v0[q] - aT) : T(0);
     }
     const double energy = block_sum<NT>(e, red);
+    tile_prefix<NTILE>(ts1, ts2, tp1, tp2, tid);
+    __syncthreads();
     const double rmsBlk = sqrt(energy / nSamp);
     const double noise = kNoiseK / Eps<T>::rhoTol * Eps<T>::value * sqrt(static_cast<double>(LOGN)) *
                          static_cast<double>(sg.hmax) * rmsBlk;
     WaveTwiddles<T, M> wt;
     wt.init(lane, tw);
-
     Cx<T> v[Q];
-    int g0 = sg.rowBegin;
-    bool first = true;
-    while (g0 < sg.rowEnd) {
+
+    // B. forward FFT + unpack into Xs (wave 0)
+    auto forwardFft = [&]() {
+        Cx<T> *buf = xbuf[0];
+        wave_load<T, M, P::R1>(v, buf, lane);
+        wave_lds_sync();
+        wave_fft_regs<T, M, false>(v, buf, lane, wt);
+        // natural-order store of pass 3 (point j + r M/R3, j = lane + 64 b)
+        constexpr int B3 = M / (P::R3 * 64);
+        Cx<T> *po = buf + xpad(lane);
+#pragma unroll
+        for (int b = 0; b < B3; ++b) {
+#pragma unroll
+            for (int r = 0; r < P::R3; ++r) { po[cpad(64 * b + r * (M / P::R3))] = v[b * P::R3 + r]; }
+        }
+        wave_lds_sync();
+        // unpack pairs (k, M-k), k = lane + 64 t
+        const Cx<T> *pm = buf + xpad(M - lane);
+        const Cx<T> wlo = twlo[lane];
+#pragma unroll
+        for (int t = 0; t < KP; ++t) {
+            const int k = lane + 64 * t;
+            if (k <= M / 2) {
+                const Cx<T> zk = po[68 * t];
+                const Cx<T> zm = cconj(k == 0 ? po[0] : pm[-68 * t]);
+                const Cx<T> E = cadd(zk, zm);
+                const Cx<T> O = cmulmi(csub(zk, zm));
+                const Cx<T> WO = cmul(cmul(wlo, twhi[t]), O);
+                Xs[k] = cadd(E, WO);
+                Xs[M - k] = cconj(csub(E, WO));
+            }
+        }
+        wave_lds_sync();
+    };
+
+    // C. one template row on this wave: spectral product + inverse packing,
+    // three register passes, then the normalisation epilogue straight from
+    // the pass-3 registers.  scl is the row's scale (a block-shared buffer),
+    // or null: the wave then computes the row's scale itself, into its own
+    // exchange buffer, which is free once the inverse FFT is in registers.
+    auto processRow = [&](const Row &row, const T *sharedScale, bool anyFlagShared) {
+        const int nvRow = static_cast<int>(min(static_cast<long long>(sg.L), row.nOut - base));
+        if (nvRow <= 0) { return; }
+        Cx<T> *buf = xbuf[wave];
+        // (k, M-k) pair per lane step (k = lane + 64 t; tw(k) = lo[lane] hi[t])
+        const gptr<const Cx<T>> H = to_global(static_cast<const Cx<T> *>(row.H));
+        const Cx<T> wlo = twlo[lane];
+        Cx<T> *pk = buf + xpad(lane);
+        Cx<T> *pm = buf + xpad(M - lane);
+#pragma unroll
+        for (int t = 0; t < KP; ++t) {
+            const int k = lane + 64 * t;
+            if (k <= M / 2) {
+                const Cx<T> yk = cmul(Xs[k], load_cx(H, k));
+                const Cx<T> ym = cmul(Xs[M - k], load_cx(H, M - k));
+                const Cx<T> S = cadd(yk, cconj(ym));
+                const Cx<T> D = csub(yk, cconj(ym));
+                const Cx<T> Pp = cmul(cconj(cmul(wlo, twhi[t])), D);
+                pk[68 * t] = cadd(S, cmuli(Pp));
+                if (k > 0) { pm[-68 * t] = cadd(cconj(S), cmuli(cconj(Pp))); }
+            }
+        }
+        wave_lds_sync();
+        wave_load<T, M, P::R1>(v, buf, lane);
+        wave_lds_sync();
+        wave_fft_regs<T, M, true>(v, buf, lane, wt);
+        const T *scale = sharedScale;
+        bool anyFlag = anyFlagShared;
+        if (scale == nullptr) {
+            T *own = reinterpret_cast<T *>(buf);
+            wave_lds_sync();
+            const int nf = compute_scale<T>(raw, ts1, ts2, tp1, tp2, own, a, nvRow, row.Lt, noise, lane, 64);
+            anyFlag = __ballot(nf > 0) != 0ull;
+            wave_lds_sync();
+            scale = own;
+        }
+        // rho = c * scale, paired stores.  Flagged windows (scale -1) are
+        // overwritten below by the exact fix-up; for the others
+        // |rho| <= 1 + O(eps), so the reference's tooBig rule (|rho| > 1.1
+        // -> 0) cannot fire and is not evaluated.
+        gptr<T> outRow = to_global(static_cast<T *>(row.out)) + base;
+        gptr<T> outl = outRow + 2 * lane;
+        const T *scl = scale + 2 * lane;
+        constexpr int B3 = M / (P::R3 * 64);
+#pragma unroll
+        for (int b = 0; b < B3; ++b) {
+#pragma unroll
+            for (int rr = 0; rr < P::R3; ++rr) {
+                const int off = 2 * (64 * b + rr * (M / P::R3));
+                const int i0 = 2 * lane + off;
+                const Cx<T> c = v[b * P::R3 + rr];
+                if (i0 + 1 < nvRow) {
+                    const typename Vec2<T>::type f = *reinterpret_cast<const typename Vec2<T>::type *>(scl + off);
+                    outl[off] = c.x * f.x;
+                    outl[off + 1] = c.y * f.y;
+                } else if (i0 < nvRow) {
+                    outl[off] = c.x * scl[off];
+                }
+            }
+        }
+        if (anyFlag) { fixup_row_wave<T>(raw, scale, to_global(row.tmpl), row.Lt, outRow, nvRow, lane); }
+        if (sharedScale == nullptr) { wave_lds_sync(); }  // the next row reuses buf
+    };
+
+    // D. rows in groups of equal template length.  A group of NWAVE rows or
+    // more shares one scale computed by the block; rows of smaller groups
+    // are "solo": each wave takes every NWAVE-th of them and computes its
+    // row's scale itself, so no wave waits for a group it has no row in.
+    bool fftDone = false;
+    for (int g0 = sg.rowBegin; g0 < sg.rowEnd;) {
         const int Lt = rows[g0].Lt;
         int g1 = g0 + 1;
         while (g1 < sg.rowEnd && rows[g1].Lt == Lt) { ++g1; }
-        int myFlags = 0;
-        if (first) {
-            // B. wave 0: forward FFT + unpack into Xs; waves 1..3: scale
-            if (wave == 0) {
-                Cx<T> *buf = xbuf[0];
-                wave_load<T, M, P::R1>(v, buf, lane);
-                wave_lds_sync();
-                wave_fft_regs<T, M, false>(v, buf, lane, wt);
-                // natural-order store of pass 3 (point j + r M/R3, j = lane + 64 b)
-                constexpr int B3 = M / (P::R3 * 64);
-                Cx<T> *po = buf + xpad(lane);
-#pragma unroll
-                for (int b = 0; b < B3; ++b) {
-#pragma unroll
-                    for (int r = 0; r < P::R3; ++r) { po[cpad(64 * b + r * (M / P::R3))] = v[b * P::R3 + r]; }
+        if (g1 - g0 >= NWAVE) {
+            int myFlags = 0;
+            if (!fftDone) {
+                // wave 0 transforms while waves 1..3 build the scale
+                if (wave == 0) {
+                    forwardFft();
+                } else {
+                    myFlags = compute_scale<T>(raw, ts1, ts2, tp1, tp2, sc, a, nValidSeg, Lt, noise, tid - 64,
+                                               NT - 64);
                 }
-                wave_lds_sync();
-                // unpack pairs (k, M-k), k = lane + 64 t
-                const Cx<T> *pm = buf + xpad(M - lane);
-                const Cx<T> wlo = twlo[lane];
-#pragma unroll
-                for (int t = 0; t < KP; ++t) {
-                    const int k = lane + 64 * t;
-                    if (k <= M / 2) {
-                        const Cx<T> zk = po[68 * t];
-                        const Cx<T> zm = cconj(k == 0 ? po[0] : pm[-68 * t]);
-                        const Cx<T> E = cadd(zk, zm);
-                        const Cx<T> O = cmulmi(csub(zk, zm));
-                        const Cx<T> WO = cmul(cmul(wlo, twhi[t]), O);
-                        Xs[k] = cadd(E, WO);
-                        Xs[M - k] = cconj(csub(E, WO));
-                    }
-                }
-                wave_lds_sync();
+                fftDone = true;
             } else {
-                myFlags = compute_scale<T, N>(raw, ts1, ts2, sc, a, nValidSeg, Lt, noise, tid - 64, NT - 64);
+                myFlags = compute_scale<T>(raw, ts1, ts2, tp1, tp2, sc, a, nValidSeg, Lt, noise, tid, NT);
             }
-            first = false;
-        } else {
-            myFlags = compute_scale<T, N>(raw, ts1, ts2, sc, a, nValidSeg, Lt, noise, tid, NT);
-        }
-        const bool anyFlag = block_sum<NT>(static_cast<double>(myFlags), red) > 0.0;
-
-        // C. rows of this length, one wave per row
-        for (int r = g0 + wave; r < g1; r += NWAVE) {
-            const Row row = rows[r];
-            const int nvRow = static_cast<int>(min(static_cast<long long>(sg.L), row.nOut - base));
-            if (nvRow <= 0) { continue; }
-            Cx<T> *buf = xbuf[wave];
-            // spectral product + inverse packing, one (k, M-k) pair per lane
-            // step (k = lane + 64 t; tw(k) = lo[lane] hi[t])
-            const gptr<const Cx<T>> H = to_global(static_cast<const Cx<T> *>(row.H));
-            const Cx<T> wlo = twlo[lane];
-            Cx<T> *pk = buf + xpad(lane);
-            Cx<T> *pm = buf + xpad(M - lane);
-#pragma unroll
-            for (int t = 0; t < KP; ++t) {
-                const int k = lane + 64 * t;
-                if (k <= M / 2) {
-                    const Cx<T> yk = cmul(Xs[k], load_cx(H, k));
-                    const Cx<T> ym = cmul(Xs[M - k], load_cx(H, M - k));
-                    const Cx<T> S = cadd(yk, cconj(ym));
-                    const Cx<T> D = csub(yk, cconj(ym));
-                    const Cx<T> Pp = cmul(cconj(cmul(wlo, twhi[t])), D);
-                    pk[68 * t] = cadd(S, cmuli(Pp));
-                    if (k > 0) { pm[-68 * t] = cadd(cconj(S), cmuli(cconj(Pp))); }
-                }
-            }
-            wave_lds_sync();
-            wave_load<T, M, P::R1>(v, buf, lane);
-            wave_lds_sync();
-            wave_fft_regs<T, M, true>(v, buf, lane, wt);
-            // normalisation epilogue: rho = c * scale, paired stores.  Flagged
-            // windows (scale -1) are overwritten below by the exact fix-up;
-            // for the others |rho| <= 1 + O(eps), so the reference's tooBig
-            // rule (|rho| > 1.1 -> 0) cannot fire and is not evaluated.
-            gptr<T> outRow = to_global(static_cast<T *>(row.out)) + base;
-            gptr<T> outl = outRow + 2 * lane;
-            const T *scl = sc + 2 * lane;
-            constexpr int B3 = M / (P::R3 * 64);
-#pragma unroll
-            for (int b = 0; b < B3; ++b) {
-#pragma unroll
-                for (int rr = 0; rr < P::R3; ++rr) {
-                    const int off = 2 * (64 * b + rr * (M / P::R3));
-                    const int i0 = 2 * lane + off;
-                    const Cx<T> c = v[b * P::R3 + rr];
-                    if (i0 + 1 < nvRow) {
-                        const typename Vec2<T>::type f = *reinterpret_cast<const typename Vec2<T>::type *>(scl + off);
-                        outl[off] = c.x * f.x;
-                        outl[off + 1] = c.y * f.y;
-                    } else if (i0 < nvRow) {
-                        outl[off] = c.x * scl[off];
-                    }
-                }
-            }
-            if (anyFlag) { fixup_row_wave<T>(raw, sc, to_global(row.tmpl), row.Lt, outRow, nvRow, lane); }
+            const bool anyFlag = block_sum<NT>(static_cast<double>(myFlags), red) > 0.0;
+            for (int r = g0 + wave; r < g1; r += NWAVE) { processRow(rows[r], sc, anyFlag); }
+            __syncthreads();  // sc is rebuilt for the next group
         }
         g0 = g1;
-        if (g0 < sg.rowEnd) { __syncthreads(); }
+    }
+    if (!fftDone) {
+        if (wave == 0) { forwardFft(); }
+        __syncthreads();
+    }
+    int k = 0;
+    for (int g0 = sg.rowBegin; g0 < sg.rowEnd;) {
+        const int Lt = rows[g0].Lt;
+        int g1 = g0 + 1;
+        while (g1 < sg.rowEnd && rows[g1].Lt == Lt) { ++g1; }
+        if (g1 - g0 < NWAVE) {
+            for (int r = g0; r < g1; ++r, ++k) {
+                if ((k & (NWAVE - 1)) == wave) { processRow(rows[r], nullptr, false); }
+            }
+        }
+        g0 = g1;
     }
 }
 
@@ -671,14 +752,21 @@ __global__ __launch_bounds__(NT) void stack_kernel(const StackGroup *__restrict_
     const int nRows = grp.nRows;
     const long long len = grp.len;
     T *__restrict__ out = static_cast<T *>(grp.out);
-    // Each thread owns U outputs a grid stride apart (coalesced across the
-    // wave) and walks the rows in order; the U loads of a row are
-    // independent, and the row loop is unrolled so several rows' loads are
-    // in flight while the per-output sums keep their sequential order.
-    constexpr int U = 4;
-    const long long stride = static_cast<long long>(gridDim.x) * NT;
+    // Each block owns one contiguous chunk of kStackChunk outputs, so the
+    // cache lines a block reads are shared only with the chunks beside it
+    // (neighbouring blocks sit on other XCDs, whose L2s would each fetch a
+    // shared line).  In a pass a thread owns U outputs NT apart, so every
+    // load instruction of a wave reads 64 consecutive samples of a row.
+    // The row loop is unrolled so several rows' loads are in flight while
+    // the per-output sums keep their sequential order.
+    constexpr int U = 8;
+    static_assert(kStackChunk % (NT * U) == 0, "stack chunk");
+    const long long chunk0 = static_cast<long long>(blockIdx.x) * kStackChunk;
+    if (chunk0 >= len) { return; }
     gptr<T> o = to_global(out);
-    for (long long k0 = static_cast<long long>(blockIdx.x) * NT + threadIdx.x; k0 < len; k0 += stride * U) {
+    for (int pass = 0; pass < kStackChunk / (NT * U); ++pass) {
+        const long long k0 = chunk0 + static_cast<long long>(pass) * NT * U + threadIdx.x;
+        if (k0 >= len) { break; }
         T acc[U];
 #pragma unroll
         for (int u = 0; u < U; ++u) { acc[u] = T(0); }
@@ -690,7 +778,7 @@ __global__ __launch_bounds__(NT) void stack_kernel(const StackGroup *__restrict_
             T v[U];
 #pragma unroll
             for (int u = 0; u < U; ++u) {
-                const long long k = k0 + u * stride;
+                const long long k = k0 + u * NT;
                 v[u] = k < len ? p[k] : T(0);
             }
 #pragma unroll
@@ -701,7 +789,7 @@ __global__ __launch_bounds__(NT) void stack_kernel(const StackGroup *__restrict_
         }
 #pragma unroll
         for (int u = 0; u < U; ++u) {
-            const long long k = k0 + u * stride;
+            const long long k = k0 + u * NT;
             if (k < len) { o[k] = acc[u]; }
         }
     }
@@ -789,6 +877,52 @@ __global__ __launch_bounds__(NT) void minmax_kernel(const T *__restrict__ x, lon
         pmin[blockIdx.x] = bmin;
         pmax[blockIdx.x] = bmax;
     }
+}
+
+// Dead-trace test of src/matchedFilter.cpp:894 for a device-resident
+// segment table.  Pass 1: min and max of each kDeadChunk-sample chunk of
+// each segment's channel (grid.y = segment); pass 2: one thread per
+// segment folds its chunks and writes seg.dead.
+template <typename T>
+__global__ __launch_bounds__(NT) void seg_minmax_kernel(const Seg *__restrict__ segs, double *__restrict__ part,
+                                                        int chunksPerSeg)
+{
+    __shared__ double red[NT / 64];
+    const Seg sg = segs[blockIdx.y];
+    const long long i0 = static_cast<long long>(blockIdx.x) * kDeadChunk;
+    double lo = DBL_MAX, hi = -DBL_MAX;
+    if (i0 < sg.n) {
+        const gptr<const T> x = to_global(static_cast<const T *>(sg.x));
+        const long long i1 = min(sg.n, i0 + kDeadChunk);
+        for (long long i = i0 + threadIdx.x; i < i1; i += NT) {
+            const double v = static_cast<double>(x[i]);
+            lo = fmin(lo, v);
+            hi = fmax(hi, v);
+        }
+    }
+    const double bmax = block_max<NT>(hi, red);
+    const double bmin = -block_max<NT>(-lo, red);
+    if (threadIdx.x == 0) {
+        const size_t k = (static_cast<size_t>(blockIdx.y) * chunksPerSeg + blockIdx.x) * 2;
+        part[k] = bmin;
+        part[k + 1] = bmax;
+    }
+}
+
+__global__ __launch_bounds__(NT) void seg_dead_kernel(Seg *__restrict__ segs, int nSegs,
+                                                      const double *__restrict__ part, int chunksPerSeg)
+{
+    const int s = blockIdx.x * NT + threadIdx.x;
+    if (s >= nSegs) { return; }
+    const long long n = segs[s].n;
+    const int nch = static_cast<int>((n + kDeadChunk - 1) / kDeadChunk);
+    double lo = DBL_MAX, hi = -DBL_MAX;
+    for (int c = 0; c < nch; ++c) {
+        const size_t k = (static_cast<size_t>(s) * chunksPerSeg + c) * 2;
+        lo = fmin(lo, part[k]);
+        hi = fmax(hi, part[k + 1]);
+    }
+    segs[s].dead = (n < 1 || fabs(lo) == fabs(hi)) ? 1 : 0;
 }
 
 template <typename T, int M>
@@ -881,8 +1015,9 @@ hipError_t launch_stack_groups(int precision, const StackGroup *dGroups, int nGr
                                int absval, hipStream_t stream)
 {
     if (maxLen <= 0 || nGroups <= 0) { return hipSuccess; }
-    const long long want = (maxLen + 4LL * NT - 1) / (4LL * NT);  // 4 outputs per thread
-    const int grid = static_cast<int>(want < 16384 ? want : 16384);
+    const long long chunks = (maxLen + kStackChunk - 1) / kStackChunk;
+    if (chunks > 0x7fffffffLL) { return hipErrorInvalidValue; }
+    const int grid = static_cast<int>(chunks);
     if (precision == 0) {
         hipLaunchKernelGGL(stack_kernel<float>, dim3(grid, nGroups), dim3(NT), 0, stream, dGroups, absval);
     } else {
@@ -904,6 +1039,23 @@ hipError_t launch_peak_groups(int precision, const StackGroup *dGroups, int nGro
         hipLaunchKernelGGL(peak_groups_kernel<double>, dim3(grid, nGroups), dim3(NT), 0, stream, dGroups,
                            threshold);
     }
+    return hipGetLastError();
+}
+
+hipError_t launch_seg_dead(int precision, Seg *dSegs, int nSegs, long long maxN, double *dPart,
+                           hipStream_t stream)
+{
+    if (nSegs <= 0) { return hipSuccess; }
+    const long long chunks = maxN < 1 ? 1 : (maxN + kDeadChunk - 1) / kDeadChunk;
+    if (chunks > 65535 || nSegs > 65535) { return hipErrorInvalidValue; }
+    const int cps = static_cast<int>(chunks);
+    if (precision == 0) {
+        hipLaunchKernelGGL(seg_minmax_kernel<float>, dim3(cps, nSegs), dim3(NT), 0, stream, dSegs, dPart, cps);
+    } else {
+        hipLaunchKernelGGL(seg_minmax_kernel<double>, dim3(cps, nSegs), dim3(NT), 0, stream, dSegs, dPart, cps);
+    }
+    hipLaunchKernelGGL(seg_dead_kernel, dim3((nSegs + NT - 1) / NT), dim3(NT), 0, stream, dSegs, nSegs,
+                       static_cast<const double *>(dPart), cps);
     return hipGetLastError();
 }
 

@@ -8,6 +8,8 @@
 namespace mfk {
 
 constexpr int kThreads = 256;        // block size of every kernel here
+constexpr int kStackChunk = 2048;   // outputs per block of the stack kernel
+constexpr int kDeadChunk = 16384;   // samples per block of the segment dead-trace pass
 constexpr int kMinLogM = 9;          // smallest complex FFT size (M = 512)
 constexpr int kMaxLogMFloat = 12;    // M = 4096 (nfft 8192) in FP32
 constexpr int kMaxLogMDouble = 11;   // M = 2048 (nfft 4096) in FP64
@@ -89,6 +91,11 @@ hipError_t launch_peak_groups(int precision, const StackGroup *dGroups, int nGro
 // Threshold + local-maximum candidates (unordered, deterministic set).
 hipError_t launch_peaks(int precision, const void *dx, long long n, double threshold,
                         int *dCount, int *dIdx, void *dVal, int cap, hipStream_t stream);
+// Dead-trace flag of every segment in a device-resident table, computed
+// on the device: seg.dead = (|min x| == |max x|) over the segment's whole
+// channel.  dPart holds 2 * nSegs * ceil(maxN / kDeadChunk) doubles.
+hipError_t launch_seg_dead(int precision, Seg *dSegs, int nSegs, long long maxN, double *dPart,
+                           hipStream_t stream);
 // Per-block min and max of a signal (dead-trace test on device input).
 hipError_t launch_minmax(int precision, const void *dx, long long n, double *dPartMin,
                          double *dPartMax, int nBlocks, hipStream_t stream);

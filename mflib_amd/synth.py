@@ -153,3 +153,59 @@ def single_channel_hour(seed: int = SEED_BASE + 1, n: int = 360_000, tlen: int =
     for p, a, s in zip(pos, amp, snr):
         x[p:p + tlen] += (s * rms / np.abs(t).max()) * (a / 1.25) * t
     return x, t, pos
+
+
+@dataclass
+class ServingWorkload:
+    """Served stream of detection requests: a template library and a set of
+    channel streams from which requests cut their windows."""
+    fs: float
+    library: list          # FP64 unit-energy templates
+    streams: list          # C float32 channel streams
+    min_window: int
+    max_window: int
+    max_channels: int
+    max_templates: int
+
+    def sample_request(self, rng):
+        """One request: window length log-uniform in [min_window, max_window],
+        1..max_channels distinct channels, 1..max_templates distinct library
+        templates; returns (channels, start, length, template ids)."""
+        lo, hi = math.log(self.min_window), math.log(self.max_window)
+        n = int(round(math.exp(rng.uniform(lo, hi))))
+        n = max(n, max(len(t) for t in self.library) + 1)
+        n = min(n, self.streams[0].size)
+        nc = int(rng.integers(1, self.max_channels + 1))
+        chans = np.sort(rng.choice(len(self.streams), nc, replace=False))
+        start = int(rng.integers(0, self.streams[0].size - n + 1))
+        nt = int(rng.integers(1, self.max_templates + 1))
+        ids = np.sort(rng.choice(len(self.library), nt, replace=False))
+        return chans.tolist(), start, n, ids.tolist()
+
+
+def serving_stream(seed: int = SEED_BASE + 5, n_templates: int = 1000, n_channels: int = 48,
+                   stream_len: int = 360_000, fs: float = 100.0, lt=(200, 2000),
+                   min_window: int = 6_000, max_window: int = 360_000, max_channels: int = 48,
+                   max_templates: int = 100, n_inject: int = 40) -> ServingWorkload:
+    """Library: n_templates unit-energy Hann-tapered 1-10 Hz bursts with
+    lengths U[lt].  Streams: n_channels one-hour float32 band-passed noise
+    channels (DC offset U(-100, 100)), each with n_inject library templates
+    injected at amplitude U(2, 8) x noise RMS x sqrt(Lt).  Requests cut
+    windows of 60 s - 1 h (log-uniform) on 1-48 channels and match 1-100
+    library templates."""
+    rng = np.random.default_rng(seed)
+    lens = rng.integers(lt[0], lt[1] + 1, size=n_templates)
+    library = [template_burst(np.random.default_rng([seed, 2, i]), int(L), fs, 1.0, 10.0)
+               for i, L in enumerate(lens)]
+    streams = []
+    for c in range(n_channels):
+        x = bandpassed_noise(np.random.default_rng([seed, 1, c]), stream_len, fs, 1.0, 10.0)
+        rms = float(np.sqrt(np.mean(x ** 2)))
+        for _ in range(n_inject):
+            t = library[int(rng.integers(0, n_templates))]
+            p = int(rng.integers(0, stream_len - len(t)))
+            x[p:p + len(t)] += rng.uniform(2, 8) * rms * math.sqrt(len(t)) * t
+        streams.append((x + rng.uniform(-100, 100)).astype(np.float32))
+    return ServingWorkload(fs=fs, library=library, streams=streams, min_window=min_window,
+                           max_window=max_window, max_channels=max_channels,
+                           max_templates=max_templates)

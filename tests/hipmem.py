@@ -1,0 +1,45 @@
+"""Minimal device buffers for GPU tests through the HIP runtime (ctypes):
+hipMalloc / hipMemcpy / hipFree, so tests of device-pointer entry points
+do not depend on another framework's device management."""
+import ctypes
+import ctypes.util
+import os
+
+import numpy as np
+
+_hip = None
+
+
+def hip():
+    global _hip
+    if _hip is None:
+        path = "/opt/rocm/lib/libamdhip64.so"
+        if not os.path.exists(path):
+            path = ctypes.util.find_library("amdhip64") or "libamdhip64.so"
+        h = ctypes.CDLL(path)
+        h.hipMalloc.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_size_t]
+        h.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
+        h.hipFree.argtypes = [ctypes.c_void_p]
+        _hip = h
+    return _hip
+
+
+class DeviceArray:
+    """A device copy of a host numpy array."""
+
+    def __init__(self, a: np.ndarray):
+        a = np.ascontiguousarray(a)
+        self.dtype, self.size, self.nbytes = a.dtype, a.size, a.nbytes
+        p = ctypes.c_void_p()
+        rc = hip().hipMalloc(ctypes.byref(p), max(a.nbytes, 1))
+        if rc != 0:
+            raise RuntimeError(f"hipMalloc failed ({rc})")
+        self.ptr = p.value
+        rc = hip().hipMemcpy(self.ptr, a.ctypes.data, a.nbytes, 1)  # host to device
+        if rc != 0:
+            raise RuntimeError(f"hipMemcpy failed ({rc})")
+
+    def free(self):
+        if self.ptr:
+            hip().hipFree(self.ptr)
+            self.ptr = None

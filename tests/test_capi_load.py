@@ -31,3 +31,13 @@ def test_no_gpu_means_loud_failure():
     import numpy as np
     with pytest.raises(RuntimeError):
         capi.Handle(np.float32, "multi")
+
+
+def test_server_needs_a_device():
+    """The request server has no CPU fallback either."""
+    import pytest
+    if os.path.exists("/dev/kfd"):
+        pytest.skip("a GPU is present")
+    import numpy as np
+    with pytest.raises(RuntimeError):
+        capi.Server(np.float32, budget_bytes=1 << 26)
