@@ -108,6 +108,180 @@ def serve_section(args, torch):
     return out, (wl, audit)
 
 
+def single_channel_section(args, torch):
+    """One hour at 100 Hz, FP64, one 400-sample template injected 20 times:
+    SingleChannel MatchedFilter<double> apply + PeakFinder(0.7, 200), with
+    the signal resident in HBM.  Returns the GPU figures and the state the
+    CPU-baseline leg checks and times."""
+    from mflib_amd import capi, synth
+    x, t, pos = synth.single_channel_hour()
+    n = x.size
+    h = capi.Handle(np.float64, "single")
+    h.set_templates([t])
+    h.set_signal_size(n)
+    h.initialize()
+    xd = torch.from_numpy(x).to("cuda:0")
+    torch.cuda.synchronize()
+    h.set_signal_device(0, xd.data_ptr(), n)
+    nout = h.filtered_length(0)
+    for _ in range(3):
+        h.apply()
+        h.synchronize()
+    reps = max(20, args.steps * 5)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        h.apply()
+        pk = capi.peaks_device(h.rho_device_ptr(0), nout, np.float64, 0.7, 200)
+    dt = (time.perf_counter() - t0) / reps
+    corr_ms = h.timings()[0]
+    rho = h.rho(0)
+    hits = sum(1 for p in pos if pk.size and np.min(np.abs(pk - p)) <= 2)
+    out = {
+        "workload": "single channel: 360,000 FP64 samples (1 h at 100 Hz), one 400-sample template "
+                    "injected 20 times, apply + PeakFinder(0.7, 200)",
+        "corr_samples_per_apply": nout,
+        "plan_nfft_L": list(h.plan(0)),
+        "ms_per_request": dt * 1e3,
+        "corr_kernel_ms": corr_ms,
+        "corr_samples_per_s": nout / dt,
+        "peaks": int(pk.size),
+        "injections_recovered": hits,
+        "injections": len(pos),
+    }
+    h.close()
+    del xd
+    log(f"single channel hour: {dt * 1e3:.3f} ms per apply+peaks (corr kernel {corr_ms:.3f} ms), "
+        f"{nout / dt / 1e9:.2f} G corr samples/s, {hits}/{len(pos)} injections")
+    return out, (x, t, rho, pk)
+
+
+def single_channel_cpu(state, threads):
+    """The oracle's restated SingleChannel MatchedFilter<double> apply and
+    PeakFinder on the host (about 2 s of repeats), and |drho| of the GPU row
+    against the FP64 two-pass Pearson oracle."""
+    import oracle as O
+    x, t, rho, pk = state
+    reps, t0 = 0, time.perf_counter()
+    while True:
+        ref_rho, nfft, L = O.mf_apply([t], [x], mode="single", dtype=np.float64, threads=threads)
+        ref_pk = O.peak_finder(ref_rho[0], 0.7, 200)
+        reps += 1
+        if time.perf_counter() - t0 > 2.0:
+            break
+    dt = (time.perf_counter() - t0) / reps
+    lags = np.random.default_rng(0x51).integers(0, rho.size, 20000)
+    exact = O.pearson(t, x, lags)
+    return {
+        "value": rho.size / dt, "unit": "corr samples/s", "cores": threads, "kind": "port",
+        "sample": f"the whole hour ({rho.size} corr samples), {reps} repeats of apply + PeakFinder",
+        "ms_per_request": dt * 1e3,
+    }, {
+        "max_abs_drho_vs_fp64": float(np.abs(rho[lags] - exact).max()),
+        "audited_samples": int(lags.size),
+        "peaks_equal_to_oracle_pipeline": bool(np.array_equal(np.sort(pk), np.sort(ref_pk))),
+    }
+
+
+def uneven_section(args, torch):
+    """Uneven batched workload: 200 templates of 200-3000 samples, each on
+    1-50 of 50 channels lasting 1-24 h; moveouts 0-500; 0-30 injections per
+    template.  One batched engine instance (FFT plans from the measured
+    autotuner, length buckets, apply replayed from a HIP graph) against the
+    plain per-template loop (one instance per template, as the reference's
+    one-signal-size-per-instance API needs)."""
+    from collections import Counter
+    from mflib_amd import capi, synth
+    log("uneven batch: generating 200 templates x 1-50 channels x 1-24 h")
+    wl = synth.uneven_batch()
+    table = capi.autotune(np.float32)
+    rows = wl.rows()
+    ns = [s.size for s in wl.signals]
+    dev = [torch.from_numpy(s).to("cuda:0") for s in wl.signals]
+    torch.cuda.synchronize()
+    h = capi.Handle(np.float32, "batched")
+    h.set_templates([wl.templates[t][c] for t, c in rows], shifts=[wl.moveouts[t][c] for t, c in rows],
+                    weights=[1.0] * len(rows))
+    h.set_batch_layout(ns, [c for _, c in rows], [t for t, _ in rows], len(wl.templates))
+    h.set_graph_launch(True)
+    t0 = time.perf_counter()
+    h.initialize()
+    init_s = time.perf_counter() - t0
+    for c, d in enumerate(dev):
+        h.set_signal_device(c, d.data_ptr(), ns[c])
+    n_corr = sum(h.filtered_length(r) for r in range(len(rows)))
+    plans = Counter(h.plan(r)[0] for r in range(len(rows)))
+    for _ in range(2):
+        h.apply()
+        h.detect(0.4, 500)
+    steps = max(3, args.steps // 2)
+    corr = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        h.apply()
+        peaks = h.detect(0.4, 500)
+        corr.append(h.timings()[0])
+    dt = (time.perf_counter() - t0) / steps
+    corr_ms = float(np.median(corr))
+    hits = 0
+    for t in range(len(wl.templates)):
+        det = np.asarray(peaks[t])
+        hits += sum(1 for o in wl.origins[t] if det.size and np.min(np.abs(det - o)) <= 2)
+    n_inj = sum(len(o) for o in wl.origins)
+    # bytes the correlation launch must move: channel reads (once per lane, at
+    # least once per channel) and the rho writes
+    alg_bytes = 4.0 * n_corr + 4.0 * sum(ns)
+    sample_rows = list(range(0, len(rows), max(1, len(rows) // 12)))
+    batched_rows = {r: h.rho(r) for r in sample_rows}
+    h.close()
+    log(f"uneven batch: {len(rows)} rows, {n_corr / 1e9:.2f} G corr samples, {dt * 1e3:.1f} ms per step "
+        f"(corr {corr_ms:.1f} ms), {n_corr / dt / 1e9:.1f} G corr samples/s, plans {dict(plans)}")
+    # plain per-template loop
+    row_index = {rc: i for i, rc in enumerate(rows)}
+    plain_rows = {}
+    want = {rows[r]: r for r in sample_rows}
+    t0 = time.perf_counter()
+    total = 0
+    for t in range(len(wl.templates)):
+        chans = sorted(wl.templates[t])
+        n = min(ns[c] for c in chans)
+        p = capi.Handle(np.float32, "multi")
+        p.set_templates([wl.templates[t][c] for c in chans], shifts=[wl.moveouts[t][c] for c in chans],
+                        weights=[1.0] * len(chans))
+        p.set_signal_size(n)
+        p.initialize()
+        for i, c in enumerate(chans):
+            p.set_signal_device(i, dev[c].data_ptr(), n)
+        p.apply()
+        p.detect(0.4, 500)
+        total += len(chans) * p.filtered_length(0)
+        for i, c in enumerate(chans):
+            if (t, c) in want:
+                plain_rows[want[(t, c)]] = p.rho(i)
+        p.close()
+    plain_dt = time.perf_counter() - t0
+    diff = max(float(np.abs(batched_rows[r].astype(np.float64) - plain_rows[r]).max()) for r in sample_rows)
+    del row_index
+    log(f"uneven batch: plain per-template loop {plain_dt:.2f} s, {total / plain_dt / 1e9:.2f} G corr samples/s")
+    return {
+        "workload": "uneven batch: 200 templates (200-3000 samples) x 1-50 of 50 channels (1-24 h at 100 Hz), "
+                    "moveouts 0-500, 0-30 injections per template, FP32, stack + PeakFinder(0.4, 500)",
+        "rows": len(rows),
+        "corr_samples_per_step": n_corr,
+        "ms_per_step": dt * 1e3,
+        "corr_kernel_ms": corr_ms,
+        "corr_samples_per_s": n_corr / dt,
+        "corr_hbm_GBps_algorithmic": alg_bytes / (corr_ms * 1e-3) / 1e9,
+        "initialize_s": init_s,
+        "fft_sizes_rows": {str(k): v for k, v in sorted(plans.items())},
+        "autotune_table": table,
+        "plain_per_template_loop_corr_samples_per_s": total / plain_dt,
+        "batched_vs_plain": (n_corr / dt) / (total / plain_dt),
+        "max_abs_drho_batched_vs_plain": diff,
+        "injections_recovered": hits,
+        "injections": n_inj,
+    }
+
+
 def serve_audit(wl, audit):
     """FP64 oracle check of the audited served rows (1 in 100 requests, one
     (template, channel) row each, 2000 seeded lags per row)."""
@@ -169,6 +343,8 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-serve", action="store_true", help="skip the served-stream measurement")
+    ap.add_argument("--no-uneven", action="store_true", help="skip the uneven batched workload")
+    ap.add_argument("--uneven-only", action="store_true", help="only the uneven batched workload")
     ap.add_argument("--serve-only", action="store_true", help="only the served-stream measurement")
     ap.add_argument("--serve-lambda", type=float, nargs="+", default=[500.0, 1000.0])
     ap.add_argument("--serve-seconds", type=float, default=2.0)
@@ -201,6 +377,9 @@ def main() -> int:
     import torch  # before the engine library: one HIP runtime per process
     from mflib_amd import capi, synth
 
+    if args.uneven_only:
+        print(json.dumps({"uneven_batch": uneven_section(args, torch)}), flush=True)
+        return 0
     if args.serve_only:
         serving, (swl, audit) = serve_section(args, torch)
         if not args.no_cpu_baseline:
@@ -308,14 +487,18 @@ def main() -> int:
         "detections_injected": E * len(wl.origins[0]),
     }
     audit_rows = [h.rho(e * C + c) for e, c in [(e, c) for e in (0, E - 1) for c in (0, C // 2, C - 1)]]
+    h.close()
+    sc_out, sc_state = single_channel_section(args, torch)
+    out["single_channel"] = sc_out
     serve_state = None
     if not args.no_serve:
-        h.close()
         serving, serve_state = serve_section(args, torch)
         out["serving"] = serving
         lat = min(serving["poisson"], key=lambda r: abs(r["lambda_per_s"] - 1000.0))
         out["latency"] = {"p50_ms": lat["p50_ms"], "p99_ms": lat["p99_ms"],
                           "lambda_per_s": lat["lambda_per_s"], "workload": "served stream (see serving)"}
+    if not args.no_uneven:
+        out["uneven_batch"] = uneven_section(args, torch)
     if not args.no_cpu_baseline:
         log("CPU baseline (bounded sample) and accuracy audit")
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
@@ -325,6 +508,9 @@ def main() -> int:
         out["accuracy"] = acc
         if serve_state is not None:
             out["serving"]["accuracy"] = serve_audit(*serve_state)
+        sc_cpu, sc_acc = single_channel_cpu(sc_state, threads)
+        out["single_channel"]["cpu_baseline"] = sc_cpu
+        out["single_channel"]["accuracy"] = sc_acc
     print(json.dumps(out), flush=True)
     h.close()
     return 0

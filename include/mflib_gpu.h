@@ -85,6 +85,11 @@ int mf_set_fft_length(mf_handle *h, int nfft);
  * FFTs.  A variant that does not exist for a block size falls back to 0. */
 int mf_set_kernel_variant(mf_handle *h, int variant);
 
+/* No reference counterpart: with on != 0, mf_apply captures its
+ * correlation launches (one per FFT size) into a HIP graph on first use and
+ * replays that graph on every later apply, until the next mf_initialize. */
+int mf_set_graph_launch(mf_handle *h, int on);
+
 /* MatchedFilter<T>::initialize (src/matchedFilter.cpp:506-821): template
  * demean/normalize (FP64), plans, template spectra, device buffers. */
 int mf_initialize(mf_handle *h);
@@ -161,6 +166,16 @@ int mf_peaks_device(const void *dx, int n, int precision, double threshold,
 /* computeOptimalFFTAndBlockLength(nb, nx)
  * (include/mflib/matchedFilterParameters.hpp:208, src/matchedFilterParameters.cpp:371-385). */
 int mf_compute_optimal_fft_and_block_length(int nb, int nx, int *nfft, int *blockLength);
+
+/* No reference counterpart (autotuner): times every FFT size and
+ * correlation kernel variant on the running device (16 synthetic lanes x
+ * 128 blocks with 1 row, 8 rows of one template length, 8 rows of distinct
+ * lengths) and installs the fitted per-block costs for this process; FFT
+ * plans (mf_initialize, the request server) then choose the FFT size per
+ * lane and the variant per launch from the measured costs.  table
+ * receives up to cap rows of (logM, variant, a, bShared, bSolo), in
+ * microseconds per block; *n the number of rows.  Takes about a second. */
+int mf_autotune(int precision, double *table, int cap, int *n);
 
 /* No reference counterpart: per-apply kernel timings (ms) measured with
  * HIP events on the handle's stream: correlation kernels and the stack

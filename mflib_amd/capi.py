@@ -44,6 +44,7 @@ _SIGNATURES = {
     "mf_set_batch_layout": ([_vp, ctypes.c_int, _ip, _ip, _ip, ctypes.c_int], ctypes.c_int),
     "mf_set_fft_length": ([_vp, ctypes.c_int], ctypes.c_int),
     "mf_set_kernel_variant": ([_vp, ctypes.c_int], ctypes.c_int),
+    "mf_set_graph_launch": ([_vp, ctypes.c_int], ctypes.c_int),
     "mf_initialize": ([_vp], ctypes.c_int),
     "mf_is_initialized": ([_vp, _ip], ctypes.c_int),
     "mf_get_number_of_templates": ([_vp, _ip], ctypes.c_int),
@@ -71,6 +72,7 @@ _SIGNATURES = {
                          ctypes.c_int, _ip], ctypes.c_int),
     "mf_compute_optimal_fft_and_block_length": ([ctypes.c_int, ctypes.c_int, _ip, _ip], ctypes.c_int),
     "mf_get_timings": ([_vp, _dp, _dp], ctypes.c_int),
+    "mf_autotune": ([ctypes.c_int, _dp, ctypes.c_int, _ip], ctypes.c_int),
     "mf_server_create": ([ctypes.POINTER(_vp), ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_int],
                          ctypes.c_int),
     "mf_server_destroy": ([_vp], ctypes.c_int),
@@ -129,6 +131,17 @@ def compute_optimal_fft_and_block_length(nb: int, nx: int):
     check(lib().mf_compute_optimal_fft_and_block_length(nb, nx, ctypes.byref(a), ctypes.byref(b)),
           "computeOptimalFFTAndBlockLength")
     return a.value, b.value
+
+
+def autotune(dtype=np.float32) -> list:
+    """Measures the per-block cost of every FFT size and kernel variant on
+    the running GPU and installs it for this process's FFT plans."""
+    buf = np.zeros(64 * 5, dtype=np.float64)
+    n = ctypes.c_int()
+    check(lib().mf_autotune(PRECISION[np.dtype(dtype)], buf.ctypes.data_as(_dp), 64, ctypes.byref(n)),
+          "mf_autotune")
+    return [{"nfft": 2 << int(r[0]), "variant": int(r[1]), "a_us": r[2], "b_shared_us": r[3],
+             "b_solo_us": r[4]} for r in buf[:n.value * 5].reshape(-1, 5)]
 
 
 def peaks(x, threshold: float, min_dist: int):
@@ -205,6 +218,9 @@ class Handle:
 
     def set_fft_length(self, nfft: int):
         check(lib().mf_set_fft_length(self._h, int(nfft)), "set_fft_length")
+
+    def set_graph_launch(self, on: bool = True):
+        check(lib().mf_set_graph_launch(self._h, int(bool(on))), "set_graph_launch")
 
     def set_kernel_variant(self, variant: int):
         check(lib().mf_set_kernel_variant(self._h, int(variant)), "set_kernel_variant")

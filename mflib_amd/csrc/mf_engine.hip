@@ -5,12 +5,13 @@
 //   channels   [C][n_c]            samples in the handle precision
 //   templates  [rows][Lt]          FP64 demeaned unit-norm templates
 //   spectra    [rows][nfft/2+1]    conj(T^)/(2 nfft), handle precision
-//   rho        [rows][ldOut]       correlation rows (ldOut = 64-multiple)
+//   rho        rows back to back   each row padded to a 64-sample multiple
 // plus the small descriptor tables the kernels read (lanes, rows, work
 // items) and the two-level twiddle tables of every FFT size in use.
 //
-// Work layout: a lane is (channel, FFT size); its blocks are the work
-// items.  Items are ordered so that all blocks of a lane land on one XCD
+// Work layout: a lane is (channel, template-length bucket, FFT size); its
+// blocks are the work items.  Items are ordered so that all blocks of a
+// lane land on one XCD
 // (workgroup i runs on XCD i mod 8), which keeps that channel's template
 // spectra resident in that XCD's L2.
 #include "mflib_gpu.h"
@@ -21,6 +22,7 @@
 #include <cmath>
 #include <cstring>
 #include <map>
+#include <memory>
 #include <mutex>
 #include <new>
 #include <numeric>
@@ -39,6 +41,9 @@ struct mf_handle {
     int absval = 1;
     int userNfft = 0;
     int userVariant = 0;  // 0: best available correlation kernel
+    bool useGraph = false;            // replay apply's launches as one HIP graph
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graphExec = nullptr;
     // templates
     std::vector<std::vector<double>> tmpl;
     bool haveShifts = false;
@@ -62,11 +67,13 @@ struct mf_handle {
         int lane = 0;
         int logM = 0;
         size_t specOff = 0;  // elements of complex T
+        size_t rhoOff = 0;   // elements of T
         size_t tmplOff = 0;  // doubles
         float hmax = 0;
     };
     struct Lane {
         int ch = 0, logM = 0, L = 0, nOut = 0;
+        RowMix mix;
         std::vector<int> rows;  // sorted by Lt
     };
     struct Group {
@@ -101,14 +108,23 @@ struct mf_handle {
     DevBuf dStacks, dCounts, dCandIdx, dCandVal, dGroupTab;
     std::map<int, TwTables> tw;
     struct Launch {
-        int logM, first, count;
+        int logM, first, count, variant;
     };
     std::vector<Launch> launches;
     std::vector<mfk::Seg> segHost;
     std::vector<int> rowTabIndex;  // row table position of each row
 
+    void dropGraph()
+    {
+        if (graphExec) { (void)hipGraphExecDestroy(graphExec); }
+        if (graph) { (void)hipGraphDestroy(graph); }
+        graphExec = nullptr;
+        graph = nullptr;
+    }
+
     ~mf_handle()
     {
+        dropGraph();
         if (evA) { (void)hipEventDestroy(evA); }
         if (evB) { (void)hipEventDestroy(evB); }
         if (evC) { (void)hipEventDestroy(evC); }
@@ -133,6 +149,7 @@ void requireInit(const mf_handle *h)
 void dropDevice(mf_handle *h)
 {
     if (h->stream) { (void)hipStreamSynchronize(h->stream); }
+    h->dropGraph();
     h->ch.clear();
     h->rows.clear();
     h->lanes.clear();
@@ -160,11 +177,6 @@ void dropDevice(mf_handle *h)
     h->rowTabIndex.clear();
     h->initialized = false;
     h->applied = false;
-}
-
-int chooseLogM(const mf_handle *h, int nb, long long nOut, int nRows)
-{
-    return mfh::chooseLogM(h->precision, h->userNfft, nb, nOut, nRows);
 }
 
 void deriveLayout(mf_handle *h, std::vector<int> &rowCh, std::vector<int> &rowGroup,
@@ -237,33 +249,44 @@ void initialize(mf_handle *h)
         maxOut = std::max<long long>(maxOut, gr.nOut);
         for (int r : gr.rows) { h->rows[r].nOut = gr.nOut; }
     }
-    // lanes: one per channel carrying rows
+    // lanes: one per (channel, template-length bucket) carrying rows
     std::vector<std::vector<int>> chRows(nC);
     for (int r = 0; r < nT; ++r) { chRows[rowCh[r]].push_back(r); }
     for (int c = 0; c < nC; ++c) {
         if (chRows[c].empty()) { continue; }
-        mf_handle::Lane ln;
-        ln.ch = c;
-        int nb = 0;
-        for (int r : chRows[c]) {
-            nb = std::max(nb, h->rows[r].Lt);
-            ln.nOut = std::max(ln.nOut, h->rows[r].nOut);
-        }
-        ln.logM = chooseLogM(h, nb, ln.nOut, static_cast<int>(chRows[c].size()));
-        ln.L = blockLength(ln.logM, nb);
-        ln.rows = chRows[c];
-        std::stable_sort(ln.rows.begin(), ln.rows.end(),
+        std::vector<int> sorted = chRows[c];
+        std::stable_sort(sorted.begin(), sorted.end(),
                          [&](int a, int b) { return h->rows[a].Lt < h->rows[b].Lt; });
-        for (int r : ln.rows) {
-            h->rows[r].lane = static_cast<int>(h->lanes.size());
-            h->rows[r].logM = ln.logM;
+        long long chOut = 0;
+        std::vector<int> lts;
+        for (int r : sorted) {
+            chOut = std::max<long long>(chOut, h->rows[r].nOut);
+            lts.push_back(h->rows[r].Lt);
         }
-        h->lanes.push_back(std::move(ln));
+        for (const Bucket &bk : planBuckets(h->precision, h->userNfft, lts, chOut)) {
+            mf_handle::Lane ln;
+            ln.ch = c;
+            ln.mix = bk.mix;
+            ln.logM = bk.logM;
+            ln.L = blockLength(ln.logM, bk.nb);
+            ln.rows.assign(sorted.begin() + bk.begin, sorted.begin() + bk.end);
+            for (int r : ln.rows) {
+                ln.nOut = std::max(ln.nOut, h->rows[r].nOut);
+                h->rows[r].lane = static_cast<int>(h->lanes.size());
+                h->rows[r].logM = ln.logM;
+            }
+            h->lanes.push_back(std::move(ln));
+        }
     }
 
     // device buffers
     h->ldOut = roundUp(static_cast<size_t>(maxOut), 64);
-    h->dRho.alloc(h->ldOut * nT * h->tsize);
+    size_t rhoTotal = 0;
+    for (int r = 0; r < nT; ++r) {
+        h->rows[r].rhoOff = rhoTotal;
+        rhoTotal += roundUp(static_cast<size_t>(std::max(h->rows[r].nOut, 1)), 64);
+    }
+    h->dRho.alloc(rhoTotal * h->tsize);
     size_t specTotal = 0, tmplTotal = 0;
     for (int r = 0; r < nT; ++r) {
         auto &rp = h->rows[r];
@@ -338,7 +361,7 @@ void initialize(mf_handle *h)
             mfk::Row row;
             row.H = static_cast<char *>(h->dSpec.p) + rp.specOff * 2 * h->tsize;
             row.tmpl = h->dTmpl.as<double>() + rp.tmplOff;
-            row.out = static_cast<char *>(h->dRho.p) + static_cast<size_t>(r) * h->ldOut * h->tsize;
+            row.out = static_cast<char *>(h->dRho.p) + rp.rhoOff * h->tsize;
             row.nOut = rp.nOut;
             row.Lt = rp.Lt;
             h->rowTabIndex[r] = static_cast<int>(rowTab.size());
@@ -359,12 +382,15 @@ void initialize(mf_handle *h)
     for (size_t li = 0; li < h->lanes.size(); ++li) { lanesByLog[h->lanes[li].logM].push_back(static_cast<int>(li)); }
     for (auto &kv : lanesByLog) {
         std::vector<long long> blocks;
+        std::vector<RowMix> mixes;
         for (int li : kv.second) {
             const auto &ln = h->lanes[li];
             blocks.push_back((ln.nOut + ln.L - 1) / ln.L);
+            mixes.push_back(ln.mix);
         }
         mf_handle::Launch la;
         la.logM = kv.first;
+        la.variant = corrVariant(h->precision, kv.first, h->userVariant, mixes, blocks);
         la.first = static_cast<int>(items.size());
         appendXcdItems(kv.second, blocks, items);
         la.count = static_cast<int>(items.size()) - la.first;
@@ -427,15 +453,37 @@ void apply(mf_handle *h)
         MF_HIP(hipStreamSynchronize(h->stream));
         h->tablesDirty = false;
     }
+    auto launchAll = [&]() {
+        for (const auto &la : h->launches) {
+            const TwTables &t = h->tw.at(la.logM);
+            const void *lo = h->precision == MF_PRECISION_FLOAT ? t.lo.p : t.lo64.p;
+            const void *hi = h->precision == MF_PRECISION_FLOAT ? t.hi.p : t.hi64.p;
+            MF_HIP(mfk::launch_corr(h->precision, la.logM, la.variant, h->dItems.as<mfk::Item>() + la.first,
+                                    la.count, h->dSegs.as<mfk::Seg>(), h->dRowTab.as<mfk::Row>(), lo, hi,
+                                    h->stream));
+        }
+    };
+    if (h->useGraph && h->graphExec == nullptr && !h->launches.empty()) {
+        // capture the correlation launches once; their descriptor tables and
+        // buffers stay in place until the next initialize.  The graph holds
+        // kernel nodes only (no memset nodes).
+        MF_HIP(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+        try {
+            launchAll();
+        } catch (...) {
+            hipGraph_t g = nullptr;
+            (void)hipStreamEndCapture(h->stream, &g);
+            if (g) { (void)hipGraphDestroy(g); }
+            throw;
+        }
+        MF_HIP(hipStreamEndCapture(h->stream, &h->graph));
+        MF_HIP(hipGraphInstantiate(&h->graphExec, h->graph, nullptr, nullptr, 0));
+    }
     MF_HIP(hipEventRecord(h->evA, h->stream));
-    for (const auto &la : h->launches) {
-        const TwTables &t = h->tw.at(la.logM);
-        const void *lo = h->precision == MF_PRECISION_FLOAT ? t.lo.p : t.lo64.p;
-        const void *hi = h->precision == MF_PRECISION_FLOAT ? t.hi.p : t.hi64.p;
-        const int variant = corrVariant(h->precision, la.logM, h->userVariant);
-        MF_HIP(mfk::launch_corr(h->precision, la.logM, variant, h->dItems.as<mfk::Item>() + la.first,
-                                la.count, h->dSegs.as<mfk::Seg>(), h->dRowTab.as<mfk::Row>(), lo, hi,
-                                h->stream));
+    if (h->useGraph && h->graphExec != nullptr) {
+        MF_HIP(hipGraphLaunch(h->graphExec, h->stream));
+    } else {
+        launchAll();
     }
     MF_HIP(hipEventRecord(h->evB, h->stream));
     h->applied = true;
@@ -457,7 +505,7 @@ void stackWeights(const mf_handle *h, const std::vector<int> &rows, std::vector<
     if (xnorm > 0) { xnorm = 1 / xnorm; }
     for (int r : rows) {
         mfk::StackRow sr{};
-        sr.rho = static_cast<const char *>(h->dRho.p) + static_cast<size_t>(r) * h->ldOut * h->tsize;
+        sr.rho = static_cast<const char *>(h->dRho.p) + h->rows[r].rhoOff * h->tsize;
         sr.shift = h->shifts[r];
         sr.weight = static_cast<double>(static_cast<T>(xnorm * static_cast<T>(h->weights[r])));
         out.push_back(sr);
@@ -715,6 +763,108 @@ void peaks(const void *x, bool onDevice, int n, int precision, double threshold,
     }
 }
 
+// Measured cost table (mf_autotune): for every FFT size and kernel variant
+// the correlation launch is timed on 16 synthetic lanes x 128 blocks
+// (2048 work items: the whole GPU) with 1 row, 8 rows of one template
+// length and 8 rows of distinct lengths per lane; the per-block costs
+// a + bShared*shared + bSolo*solo follow from the three times.
+void calibrate(int precision, std::vector<double> &out)
+{
+    constexpr int C = 16, B = 128, R = 8;
+    const size_t ts = precision == MF_PRECISION_FLOAT ? 4 : 8;
+    CostTable t;
+    uint64_t state = 0x9E3779B97F4A7C15ull;
+    auto next = [&state]() {  // xorshift64*, deterministic
+        state ^= state >> 12;
+        state ^= state << 25;
+        state ^= state >> 27;
+        return static_cast<double>((state * 0x2545F4914F6CDD1Dull) >> 11) / 9007199254740992.0 - 0.5;
+    };
+    for (int lg = mfk::kMinLogM; lg <= maxLogM(precision); ++lg) {
+        const int N = 2 << lg;
+        const int nb = N / 4;
+        const int L = blockLength(lg, nb);
+        const int n = B * L + nb - 1;
+        std::vector<char> sig(static_cast<size_t>(n) * ts);
+        double times[3][3] = {};  // [config][variant]
+        for (int cfg = 0; cfg < 3; ++cfg) {
+            const int rowsPerLane = cfg == 0 ? 1 : R;
+            std::unique_ptr<mf_handle> h(new mf_handle());
+            h->precision = precision;
+            h->mode = MF_MODE_BATCHED;
+            h->tsize = ts;
+            MF_HIP(hipStreamCreate(&h->stream));
+            MF_HIP(hipEventCreate(&h->evA));
+            MF_HIP(hipEventCreate(&h->evB));
+            MF_HIP(hipEventCreate(&h->evC));
+            MF_HIP(hipEventCreate(&h->evD));
+            for (int c = 0; c < C; ++c) {
+                for (int k = 0; k < rowsPerLane; ++k) {
+                    const int Lt = cfg == 2 ? nb - 4 * k : nb;  // cfg 2: distinct lengths
+                    std::vector<double> tv(Lt);
+                    for (int m = 0; m < Lt; ++m) { tv[m] = next() * std::sin(M_PI * (m + 0.5) / Lt); }
+                    h->tmpl.push_back(std::move(tv));
+                    h->rowChIn.push_back(c);
+                    h->rowGroupIn.push_back(c);
+                }
+            }
+            h->weights.assign(h->tmpl.size(), 1.0);
+            h->shifts.assign(h->tmpl.size(), 0);
+            h->haveLayout = true;
+            h->nChannels = C;
+            h->nGroupsIn = C;
+            h->chLenIn.assign(C, n);
+            h->userNfft = N;
+            initialize(h.get());
+            for (int c = 0; c < C; ++c) {
+                if (precision == MF_PRECISION_FLOAT) {
+                    float *f = reinterpret_cast<float *>(sig.data());
+                    for (int i = 0; i < n; ++i) { f[i] = static_cast<float>(next()); }
+                } else {
+                    double *d = reinterpret_cast<double *>(sig.data());
+                    for (int i = 0; i < n; ++i) { d[i] = next(); }
+                }
+                MF_HIP(hipMemcpy(h->ch[c].buf.p, sig.data(), sig.size(), hipMemcpyHostToDevice));
+                h->ch[c].dead = false;
+                h->ch[c].set = true;
+            }
+            h->tablesDirty = true;
+            for (int v = 1; v <= 2; ++v) {
+                if (!variantAvailable(precision, lg, v)) { continue; }
+                for (auto &la : h->launches) { la.variant = v; }
+                std::vector<float> ms;
+                for (int rep = 0; rep < 4; ++rep) {
+                    apply(h.get());
+                    MF_HIP(hipStreamSynchronize(h->stream));
+                    float e = 0;
+                    MF_HIP(hipEventElapsedTime(&e, h->evA, h->evB));
+                    if (rep > 0) { ms.push_back(e); }
+                }
+                std::sort(ms.begin(), ms.end());
+                times[cfg][v] = ms[ms.size() / 2];
+            }
+            dropDevice(h.get());
+        }
+        const double blocks = static_cast<double>(C) * B;
+        for (int v = 1; v <= 2; ++v) {
+            if (times[0][v] <= 0) { continue; }
+            const double t1 = times[0][v] * 1e3 / blocks;  // us per block
+            const double tS = times[1][v] * 1e3 / blocks;
+            const double tD = times[2][v] * 1e3 / blocks;
+            const double bSolo = std::max((tD - t1) / (R - 1), 1e-6);
+            const double a = std::max(t1 - bSolo, 0.05 * t1);
+            const double bShared = std::max((tS - a) / R, 1e-6);
+            t.a[lg][v] = a;
+            t.bShared[lg][v] = bShared;
+            t.bSolo[lg][v] = bSolo;
+            out.insert(out.end(), {static_cast<double>(lg), static_cast<double>(v), a, bShared, bSolo});
+        }
+    }
+    t.measured = true;
+    std::lock_guard<std::mutex> lk(costMutex());
+    costTable(precision) = t;
+}
+
 }  // namespace
 
 extern "C" {
@@ -881,6 +1031,16 @@ int mf_set_fft_length(mf_handle *h, int nfft)
     });
 }
 
+int mf_set_graph_launch(mf_handle *h, int on)
+{
+    return guard([&] {
+        requireHandle(h);
+        if (h->stream) { MF_HIP(hipStreamSynchronize(h->stream)); }
+        h->dropGraph();
+        h->useGraph = on != 0;
+    });
+}
+
 int mf_initialize(mf_handle *h)
 {
     return guard([&] {
@@ -1039,7 +1199,7 @@ int mf_get_rho(mf_handle *h, int row, void *out, int cap)
         if (!h->applied) { fail(MF_ERR_BAD_STATE, "Matched filtering not yet applied"); }
         const int nOut = h->rows[row].nOut;
         if (out == nullptr || cap < nOut) { fail(MF_ERR_INVALID_ARGUMENT, "output buffer too small"); }
-        const char *src = static_cast<const char *>(h->dRho.p) + static_cast<size_t>(row) * h->ldOut * h->tsize;
+        const char *src = static_cast<const char *>(h->dRho.p) + h->rows[row].rhoOff * h->tsize;
         MF_HIP(hipMemcpyAsync(out, src, static_cast<size_t>(nOut) * h->tsize, hipMemcpyDeviceToHost, h->stream));
         MF_HIP(hipStreamSynchronize(h->stream));
     });
@@ -1051,7 +1211,7 @@ int mf_get_rho_device(mf_handle *h, int row, const void **dptr)
         requireInit(h);
         checkRow(h, row);
         if (dptr == nullptr) { fail(MF_ERR_INVALID_ARGUMENT, "dptr is NULL"); }
-        *dptr = static_cast<const char *>(h->dRho.p) + static_cast<size_t>(row) * h->ldOut * h->tsize;
+        *dptr = static_cast<const char *>(h->dRho.p) + h->rows[row].rhoOff * h->tsize;
     });
 }
 
@@ -1172,6 +1332,24 @@ int mf_compute_optimal_fft_and_block_length(int nb, int nx, int *nfft, int *bloc
     return guard([&] {
         if (nfft == nullptr || blockLength == nullptr) { fail(MF_ERR_INVALID_ARGUMENT, "output is NULL"); }
         referenceWindowLength(nb, nx, nfft, blockLength);
+    });
+}
+
+int mf_autotune(int precision, double *table, int cap, int *n)
+{
+    return guard([&] {
+        if (precision != MF_PRECISION_FLOAT && precision != MF_PRECISION_DOUBLE) {
+            fail(MF_ERR_INVALID_ARGUMENT, "unknown precision " + std::to_string(precision));
+        }
+        requireDevice();
+        std::vector<double> out;
+        calibrate(precision, out);
+        const int rows = static_cast<int>(out.size() / 5);
+        if (n) { *n = rows; }
+        if (table != nullptr) {
+            std::copy(out.begin(), out.begin() + std::min<size_t>(out.size(), static_cast<size_t>(std::max(cap, 0)) * 5),
+                      table);
+        }
     });
 }
 

@@ -9,6 +9,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <mutex>
 #include <new>
 #include <stdexcept>
 #include <string>
@@ -175,18 +176,6 @@ inline void makeTwiddles(TwTables &t, int logM, bool wantFloat)
     }
 }
 
-// Correlation kernel variant for an FFT size: the wave-level kernel up to
-// nfft 4096 in FP32 and 1024 in FP64 (measured faster there: at nfft 4096
-// with 50 rows per lane it runs 1.2-1.3x the block-wide kernel), the
-// block-wide kernel for the larger blocks.  A requested variant that does
-// not exist for the size falls back to this choice.
-inline int corrVariant(int precision, int logM, int requested)
-{
-    if (requested != 0 && mfk::corr_variant_available(precision, logM, requested)) { return requested; }
-    const bool waveFaster = precision == MF_PRECISION_FLOAT ? logM <= 11 : logM <= 9;
-    return waveFaster && mfk::corr_variant_available(precision, logM, 2) ? 2 : 1;
-}
-
 // First device present, or a failure: the engine has no CPU fallback.
 inline void requireDevice()
 {
@@ -230,28 +219,137 @@ inline int maxLogM(int precision)
     return precision == MF_PRECISION_FLOAT ? mfk::kMaxLogMFloat : mfk::kMaxLogMDouble;
 }
 
-// Cost model of one lane for FFT size 2^(logM+1): blocks x (forward FFT +
-// one inverse FFT, product and epilogue per row), weighted per block size
-// by the measured efficiency of the fastest kernel variant there (the
-// weights are time / (blocks M (log2 M + 1.5)) on MI355X for 16 rows of
-// 512 samples, relative to M = 1024).
-inline double laneCost(int logM, int nb, long long nOut, int nRows, int precision)
+// Row mix of a lane: rows whose template length is shared by at least
+// four rows of the lane share one normalisation scale; the others are
+// solo rows that each compute their own.
+struct RowMix {
+    int shared = 0;
+    int solo = 0;
+    int rows() const { return shared + solo; }
+};
+
+inline RowMix rowMix(std::vector<int> lengths)
+{
+    std::sort(lengths.begin(), lengths.end());
+    RowMix m;
+    for (size_t i = 0; i < lengths.size();) {
+        size_t j = i;
+        while (j < lengths.size() && lengths[j] == lengths[i]) { ++j; }
+        if (j - i >= 4) { m.shared += static_cast<int>(j - i); } else { m.solo += static_cast<int>(j - i); }
+        i = j;
+    }
+    return m;
+}
+
+// Per-block cost of the correlation kernels: a + bShared * shared rows +
+// bSolo * solo rows (microseconds per block for a full GPU), per FFT size
+// and kernel variant.  The static table comes from MI355X measurements;
+// mf_autotune replaces it with a measurement on the running device.
+struct CostTable {
+    bool measured = false;
+    double a[mfk::kMaxLogMFloat + 1][3] = {};
+    double bShared[mfk::kMaxLogMFloat + 1][3] = {};
+    double bSolo[mfk::kMaxLogMFloat + 1][3] = {};
+};
+
+inline std::mutex &costMutex()
+{
+    static std::mutex m;
+    return m;
+}
+
+inline CostTable &costTable(int precision)
+{
+    static CostTable t[2];
+    return t[precision == MF_PRECISION_FLOAT ? 0 : 1];
+}
+
+// Static per-block cost: forward FFT plus, per row, product + inverse FFT +
+// epilogue, proportional to M (log2 M + 1.5) and weighted per block size by
+// the measured efficiency of the default variant there (16 rows of 512
+// samples, relative to M = 1024); solo rows cost 1.6x a shared row
+// (measured: 16 distinct lengths ran at 216 against 339 G samples/s).
+inline double staticBlockCost(int logM, RowMix mix, int precision)
+{
+    const double M = static_cast<double>(1LL << logM);
+    static const double occF[] = {0.97, 1.00, 1.97, 3.30};  // logM 9..12, FP32
+    static const double occD[] = {1.00, 1.60, 2.40, 9.99};
+    const double occ = (precision == MF_PRECISION_FLOAT ? occF : occD)[logM - mfk::kMinLogM];
+    return (1.0 + 1.15 * mix.shared + 1.84 * mix.solo) * M * (logM + 1.5) * occ;
+}
+
+inline bool variantAvailable(int precision, int logM, int variant)
+{
+    return mfk::corr_variant_available(precision, logM, variant);
+}
+
+// Default correlation kernel variant for an FFT size: the wave-level kernel
+// up to nfft 4096 in FP32 and 1024 in FP64 (measured faster there: at nfft
+// 4096 with 50 rows per lane it runs 1.2-1.3x the block-wide kernel), the
+// block-wide kernel for the larger blocks.
+inline int staticVariant(int precision, int logM)
+{
+    const bool waveFaster = precision == MF_PRECISION_FLOAT ? logM <= 11 : logM <= 9;
+    return waveFaster && variantAvailable(precision, logM, 2) ? 2 : 1;
+}
+
+// Predicted cost of a lane (blocks x per-block cost) for an FFT size and a
+// variant (0 = the default variant), or -1 if the plan is impossible.
+inline double laneCost(int precision, int logM, int variant, int nb, long long nOut, RowMix mix)
 {
     const long long N = 2LL << logM;
     const long long L = N - nb + 1;
     if (L < 2) { return -1; }  // even block starts need L >= 2
     const double blocks = std::ceil(static_cast<double>(nOut) / static_cast<double>(L));
-    const double M = static_cast<double>(N / 2);
-    static const double occF[] = {0.97, 1.00, 1.97, 3.30};  // logM 9..12, FP32
-    static const double occD[] = {1.00, 1.60, 2.40, 9.99};
-    const double occ = (precision == MF_PRECISION_FLOAT ? occF : occD)[logM - mfk::kMinLogM];
-    return blocks * (1.0 + 1.15 * nRows) * M * (logM + 1.5) * occ;
+    std::lock_guard<std::mutex> lk(costMutex());
+    const CostTable &t = costTable(precision);
+    if (t.measured) {
+        int v = variant;
+        if (v == 0) {
+            double best = -1;
+            for (int cand = 1; cand <= 2; ++cand) {
+                if (t.a[logM][cand] <= 0) { continue; }
+                const double c = t.a[logM][cand] + t.bShared[logM][cand] * mix.shared + t.bSolo[logM][cand] * mix.solo;
+                if (best < 0 || c < best) { best = c; v = cand; }
+            }
+            if (best >= 0) { return blocks * best; }
+        } else if (t.a[logM][v] > 0) {
+            return blocks * (t.a[logM][v] + t.bShared[logM][v] * mix.shared + t.bSolo[logM][v] * mix.solo);
+        }
+    }
+    return blocks * staticBlockCost(logM, mix, precision);
+}
+
+// Correlation kernel variant of one launch: a requested variant if it
+// exists for the size; otherwise, with a measured cost table, the variant
+// of least predicted cost over the launch's lanes; otherwise the default.
+inline int corrVariant(int precision, int logM, int requested, const std::vector<RowMix> &mixes = {},
+                       const std::vector<long long> &blocks = {})
+{
+    if (requested != 0 && variantAvailable(precision, logM, requested)) { return requested; }
+    std::lock_guard<std::mutex> lk(costMutex());
+    const CostTable &t = costTable(precision);
+    if (t.measured && !mixes.empty()) {
+        int bestV = 0;
+        double best = -1;
+        for (int v = 1; v <= 2; ++v) {
+            if (!variantAvailable(precision, logM, v) || t.a[logM][v] <= 0) { continue; }
+            double c = 0;
+            for (size_t i = 0; i < mixes.size(); ++i) {
+                c += static_cast<double>(blocks[i]) *
+                     (t.a[logM][v] + t.bShared[logM][v] * mixes[i].shared + t.bSolo[logM][v] * mixes[i].solo);
+            }
+            if (best < 0 || c < best) { best = c; bestV = v; }
+        }
+        if (bestV != 0) { return bestV; }
+    }
+    return staticVariant(precision, logM);
 }
 
 // FFT size (as logM, nfft = 2^(logM+1)) of a lane whose longest template
-// has nb samples and which produces nOut outputs for nRows rows.  A
+// has nb samples and which produces nOut outputs for the rows in `mix`.  A
 // positive userNfft forces the size.
-inline int chooseLogM(int precision, int userNfft, int nb, long long nOut, int nRows)
+inline int chooseLogM(int precision, int userNfft, int nb, long long nOut, RowMix mix)
 {
     const int hiLog = maxLogM(precision);
     if (userNfft > 0) {
@@ -263,7 +361,7 @@ inline int chooseLogM(int precision, int userNfft, int nb, long long nOut, int n
                                          std::to_string(2 << mfk::kMinLogM) + ", " +
                                          std::to_string(2 << hiLog) + "]");
         }
-        if ((2LL << lg) - nb + 1 < 1) {
+        if ((2LL << lg) - nb + 1 < 2) {
             fail(MF_ERR_INVALID_ARGUMENT, "FFT length " + std::to_string(userNfft) +
                                               " is shorter than the template length " + std::to_string(nb));
         }
@@ -274,7 +372,7 @@ inline int chooseLogM(int precision, int userNfft, int nb, long long nOut, int n
     for (int lg = mfk::kMinLogM; lg <= hiLog; ++lg) {
         const long long N = 2LL << lg;
         if (N - nb + 1 < N / 4 && lg < hiLog) { continue; }  // keep >= 25% useful outputs
-        const double c = laneCost(lg, nb, nOut, nRows, precision);
+        const double c = laneCost(precision, lg, 0, nb, nOut, mix);
         if (c < 0) { continue; }
         if (best < 0 || c < bestCost) { best = lg; bestCost = c; }
     }
@@ -284,6 +382,66 @@ inline int chooseLogM(int precision, int userNfft, int nb, long long nOut, int n
                                      " samples)");
     }
     return best;
+}
+
+// Length buckets of one channel's rows.  Rows (sorted by template length)
+// are split into contiguous buckets; each bucket becomes a lane with its own
+// FFT size, so short templates are not forced onto the long templates'
+// blocks.  Start: one bucket per FFT size a row would pick alone; then the
+// adjacent pair whose merge saves the most predicted cost is merged while
+// any merge saves (a merge shares the forward FFT and block loads).
+struct Bucket {
+    int begin = 0, end = 0;  // [begin, end) of the sorted rows
+    int nb = 0;              // longest template of the bucket
+    int logM = 0;
+    RowMix mix;
+    double cost = 0;
+};
+
+inline Bucket makeBucket(int precision, int userNfft, const std::vector<int> &sortedLt, int begin, int end,
+                         long long nOut)
+{
+    Bucket b;
+    b.begin = begin;
+    b.end = end;
+    b.nb = sortedLt[end - 1];
+    b.mix = rowMix(std::vector<int>(sortedLt.begin() + begin, sortedLt.begin() + end));
+    b.logM = chooseLogM(precision, userNfft, b.nb, nOut, b.mix);
+    b.cost = laneCost(precision, b.logM, 0, b.nb, nOut, b.mix);
+    return b;
+}
+
+inline std::vector<Bucket> planBuckets(int precision, int userNfft, const std::vector<int> &sortedLt,
+                                       long long nOut)
+{
+    const int n = static_cast<int>(sortedLt.size());
+    if (userNfft > 0 || n <= 1) { return {makeBucket(precision, userNfft, sortedLt, 0, n, nOut)}; }
+    std::vector<Bucket> bk;
+    for (int i = 0; i < n;) {
+        const int lg = chooseLogM(precision, 0, sortedLt[i], nOut, RowMix{0, 1});
+        int j = i + 1;
+        while (j < n && chooseLogM(precision, 0, sortedLt[j], nOut, RowMix{0, 1}) == lg) { ++j; }
+        bk.push_back(makeBucket(precision, 0, sortedLt, i, j, nOut));
+        i = j;
+    }
+    while (bk.size() > 1) {
+        int bestK = -1;
+        double bestGain = 0;
+        Bucket bestM;
+        for (size_t k = 0; k + 1 < bk.size(); ++k) {
+            const Bucket m = makeBucket(precision, 0, sortedLt, bk[k].begin, bk[k + 1].end, nOut);
+            const double gain = bk[k].cost + bk[k + 1].cost - m.cost;
+            if (gain > bestGain) {
+                bestGain = gain;
+                bestK = static_cast<int>(k);
+                bestM = m;
+            }
+        }
+        if (bestK < 0) { break; }
+        bk[bestK] = bestM;
+        bk.erase(bk.begin() + bestK + 1);
+    }
+    return bk;
 }
 
 // Block length of a lane: nfft - nb + 1, made even so that every block

@@ -58,7 +58,9 @@ struct Request {
     std::vector<const void *> src;
     std::vector<int> tmplIds;
     std::vector<LibTemplate> tpl;  // library entries, copied at submission
-    std::vector<int> logM, L, nOut;
+    std::vector<int> order;        // template indices sorted by length
+    std::vector<std::vector<Bucket>> buckets;  // per channel: length buckets (lanes)
+    std::vector<int> nOut;
     size_t ldOut = 0;
     size_t inBytes = 0, bytes = 0, off = 0;
     double corrSamples = 0;
@@ -161,16 +163,13 @@ void launchBatch(mf_server *s, Slot &slot)
     std::vector<mfk::Row> rows;
     std::map<int, std::vector<int>> segsByLog;
     std::map<int, std::vector<long long>> blocksByLog;
+    std::map<int, std::vector<RowMix>> mixByLog;
     long long maxN = 1;
     char *arena = static_cast<char *>(s->arena.p);
     for (Request *r : slot.reqs) {
         char *base = arena + r->off;
         char *in = base;
         char *out = base + r->inBytes;
-        // rows of a lane in template-length order (the kernels walk them so)
-        std::vector<int> order(r->nT);
-        for (int i = 0; i < r->nT; ++i) { order[i] = i; }
-        std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return r->tpl[a].Lt < r->tpl[b].Lt; });
         for (int c = 0; c < r->nC; ++c) {
             const void *x = r->src[c];
             if (!r->onDevice) {
@@ -179,41 +178,49 @@ void launchBatch(mf_server *s, Slot &slot)
                 x = in;
                 in += roundUp(static_cast<size_t>(r->len[c]) * s->tsize, kAlign);
             }
-            const int lg = r->logM[c];
-            mfk::Seg sg{};
-            sg.x = x;
-            sg.n = r->len[c];
-            sg.L = r->L[c];
-            sg.nOut = r->nOut[c];
-            sg.rowBegin = static_cast<int>(rows.size());
-            sg.dead = 1;
-            float hmax = 0;
-            for (int i : order) {
-                const LibTemplate &t = r->tpl[i];
-                mfk::Row row;
-                row.H = t.spec[lg];
-                row.tmpl = t.tmpl;
-                row.out = out + (static_cast<size_t>(i) * r->nC + c) * r->ldOut * s->tsize;
-                row.nOut = r->nOut[c];
-                row.Lt = t.Lt;
-                rows.push_back(row);
-                hmax = std::max(hmax, t.hmax[lg]);
+            // one lane per length bucket; rows in template-length order (the
+            // kernels walk them so)
+            for (const Bucket &bk : r->buckets[c]) {
+                const int lg = bk.logM;
+                const int L = blockLength(lg, bk.nb);
+                mfk::Seg sg{};
+                sg.x = x;
+                sg.n = r->len[c];
+                sg.L = L;
+                sg.nOut = r->nOut[c];
+                sg.rowBegin = static_cast<int>(rows.size());
+                sg.dead = 1;
+                float hmax = 0;
+                for (int k = bk.begin; k < bk.end; ++k) {
+                    const int i = r->order[k];
+                    const LibTemplate &t = r->tpl[i];
+                    mfk::Row row;
+                    row.H = t.spec[lg];
+                    row.tmpl = t.tmpl;
+                    row.out = out + (static_cast<size_t>(i) * r->nC + c) * r->ldOut * s->tsize;
+                    row.nOut = r->nOut[c];
+                    row.Lt = t.Lt;
+                    rows.push_back(row);
+                    hmax = std::max(hmax, t.hmax[lg]);
+                }
+                sg.rowEnd = static_cast<int>(rows.size());
+                sg.hmax = hmax;
+                segsByLog[lg].push_back(static_cast<int>(segs.size()));
+                blocksByLog[lg].push_back((r->nOut[c] + L - 1) / L);
+                mixByLog[lg].push_back(bk.mix);
+                maxN = std::max<long long>(maxN, r->len[c]);
+                segs.push_back(sg);
             }
-            sg.rowEnd = static_cast<int>(rows.size());
-            sg.hmax = hmax;
-            segsByLog[lg].push_back(static_cast<int>(segs.size()));
-            blocksByLog[lg].push_back((r->nOut[c] + r->L[c] - 1) / r->L[c]);
-            maxN = std::max<long long>(maxN, r->len[c]);
-            segs.push_back(sg);
         }
     }
     struct Launch {
-        int logM, first, count;
+        int logM, first, count, variant;
     };
     std::vector<mfk::Item> items;
     std::vector<Launch> launches;
     for (auto &kv : segsByLog) {
-        Launch la{kv.first, static_cast<int>(items.size()), 0};
+        Launch la{kv.first, static_cast<int>(items.size()), 0,
+                  corrVariant(s->precision, kv.first, 0, mixByLog[kv.first], blocksByLog[kv.first])};
         appendXcdItems(kv.second, blocksByLog[kv.first], items);
         la.count = static_cast<int>(items.size()) - la.first;
         launches.push_back(la);
@@ -240,7 +247,7 @@ void launchBatch(mf_server *s, Slot &slot)
     for (const auto &la : launches) {
         const TwTables &t = s->tw.at(la.logM);
         const bool f = s->precision == MF_PRECISION_FLOAT;
-        MF_HIP(mfk::launch_corr(s->precision, la.logM, corrVariant(s->precision, la.logM, 0), dItems + la.first,
+        MF_HIP(mfk::launch_corr(s->precision, la.logM, la.variant, dItems + la.first,
                                 la.count, dSegs, dRows, f ? t.lo.p : t.lo64.p, f ? t.hi.p : t.hi64.p,
                                 slot.stream));
     }
@@ -433,10 +440,14 @@ std::unique_ptr<Request> planRequest(mf_server *s, int nC, const int *lengths, c
             r->nb = std::max(r->nb, s->lib[id].Lt);
         }
     }
+    r->order.resize(nT);
+    for (int i = 0; i < nT; ++i) { r->order[i] = i; }
+    std::stable_sort(r->order.begin(), r->order.end(), [&](int a, int b) { return r->tpl[a].Lt < r->tpl[b].Lt; });
+    std::vector<int> lts;
+    for (int i : r->order) { lts.push_back(r->tpl[i].Lt); }
     int maxOut = 0;
-    r->logM.resize(nC);
-    r->L.resize(nC);
     r->nOut.resize(nC);
+    r->buckets.resize(nC);
     for (int c = 0; c < nC; ++c) {
         if (channels[c] == nullptr) { fail(MF_ERR_INVALID_ARGUMENT, "channel " + std::to_string(c) + " is NULL"); }
         const long long no = static_cast<long long>(lengths[c]) - r->nb + 1;
@@ -445,8 +456,8 @@ std::unique_ptr<Request> planRequest(mf_server *s, int nC, const int *lengths, c
                                               " samples, fewer than the template length " + std::to_string(r->nb));
         }
         r->nOut[c] = static_cast<int>(no);
-        r->logM[c] = chooseLogM(s->precision, 0, r->nb, no, nT);
-        r->L[c] = blockLength(r->logM[c], r->nb);
+        r->buckets[c] = (c > 0 && lengths[c] == lengths[c - 1]) ? r->buckets[c - 1]
+                                                                 : planBuckets(s->precision, 0, lts, no);
         maxOut = std::max(maxOut, r->nOut[c]);
         r->corrSamples += static_cast<double>(no) * nT;
         if (!r->onDevice) { r->inBytes += roundUp(static_cast<size_t>(lengths[c]) * s->tsize, kAlign); }

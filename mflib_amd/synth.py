@@ -209,3 +209,58 @@ def serving_stream(seed: int = SEED_BASE + 5, n_templates: int = 1000, n_channel
     return ServingWorkload(fs=fs, library=library, streams=streams, min_window=min_window,
                            max_window=max_window, max_channels=max_channels,
                            max_templates=max_templates)
+
+
+@dataclass
+class UnevenBatch:
+    """Templates of uneven length over channels of uneven duration."""
+    fs: float
+    signals: list                 # C float32 channels of different lengths
+    templates: list               # [template] -> {channel: FP64 waveform}
+    lengths: np.ndarray           # template lengths
+    moveouts: list                # [template] -> {channel: shift}
+    origins: list                 # [template] -> injection origins
+
+    def rows(self):
+        """(template, channel) pairs in template order."""
+        return [(t, c) for t in range(len(self.templates)) for c in sorted(self.templates[t])]
+
+
+def uneven_batch(seed: int = SEED_BASE + 4, n_templates: int = 200, n_channels: int = 50,
+                 lt=(200, 3000), hours=(1.0, 24.0), fs: float = 100.0, max_moveout: int = 500,
+                 max_inject: int = 30, snr=(2.0, 8.0)) -> UnevenBatch:
+    """200 templates with lengths U[lt], each on 1..n_channels channels
+    (count uniform, channels a random subset); n_channels float32 channels
+    of band-passed (1-10 Hz) noise lasting U[hours] at fs; moveouts
+    U[0, max_moveout]; each template injected 0..max_inject times inside the
+    span all its channels cover, at per-sample amplitude U[snr] x noise RMS."""
+    rng = np.random.default_rng(seed)
+    ns = (rng.uniform(hours[0], hours[1], n_channels) * 3600 * fs).astype(np.int64)
+    signals = []
+    for c in range(n_channels):
+        x = bandpassed_noise(np.random.default_rng([seed, 1, c]), int(ns[c]), fs, 1.0, 10.0,
+                             dtype=np.float32)
+        signals.append(x)
+    rms = np.array([float(np.sqrt(np.mean(s[:200000].astype(np.float64) ** 2))) for s in signals])
+    lengths = rng.integers(lt[0], lt[1] + 1, n_templates)
+    templates, moveouts, origins = [], [], []
+    for t in range(n_templates):
+        k = int(rng.integers(1, n_channels + 1))
+        chans = np.sort(rng.choice(n_channels, k, replace=False))
+        L = int(lengths[t])
+        templates.append({int(c): template_burst(np.random.default_rng([seed, 2, t, int(c)]), L, fs, 1.0, 10.0)
+                          for c in chans})
+        moveouts.append({int(c): int(rng.integers(0, max_moveout + 1)) for c in chans})
+        span = int(min(ns[c] for c in chans)) - L - max_moveout - 1
+        n_inj = int(rng.integers(0, max_inject + 1)) if span > 0 else 0
+        orig = np.sort(rng.integers(0, span, n_inj)) if n_inj else np.zeros(0, dtype=np.int64)
+        origins.append(orig)
+        for o in orig:
+            for c in chans:
+                w = templates[t][int(c)] / np.abs(templates[t][int(c)]).max()
+                i0 = int(o) + moveouts[t][int(c)]
+                signals[c][i0:i0 + L] += (rng.uniform(snr[0], snr[1]) * rms[c] * w).astype(np.float32)
+    for c in range(n_channels):
+        signals[c] += np.float32(rng.uniform(-100, 100))
+    return UnevenBatch(fs=fs, signals=signals, templates=templates, lengths=lengths,
+                       moveouts=moveouts, origins=origins)

@@ -282,3 +282,91 @@ def test_errors_match_reference_conventions():
         h.shift_and_stack()             # not applied: runtime_error
     with pytest.raises(ValueError):
         capi.peaks(np.zeros(10), 1.5, 0)  # threshold outside [0,1]
+
+
+def test_length_buckets_get_their_own_fft_size():
+    """Rows of very different template lengths on one channel are split into
+    length buckets, each with its own FFT size; every row still matches the
+    FP64 oracle."""
+    rng = np.random.default_rng(606)
+    n = 200_000
+    x = (rng.standard_normal(n) + 30).astype(np.float32)
+    lens = [200, 210, 220, 230, 2900, 3000]
+    tpls = [rng.standard_normal(L) * np.hanning(L) for L in lens]
+    h = capi.Handle(np.float32, "batched")
+    h.set_templates(tpls, shifts=[0] * len(lens), weights=[1.0] * len(lens))
+    h.set_batch_layout([n], [0] * len(lens), [0] * len(lens), 1)
+    h.initialize()
+    h.set_signal(0, x)
+    h.apply()
+    short, long_ = h.plan(0)[0], h.plan(len(lens) - 1)[0]
+    assert short < long_, (short, long_)
+    nout = n - max(lens) + 1
+    lags = rng.integers(0, nout, 3000)
+    for i, t in enumerate(tpls):
+        r = h.rho(i)
+        assert r.size == nout
+        assert np.abs(r[lags].astype(np.float64) - O.pearson(t, x, lags)).max() < 1e-5
+
+
+def test_autotune_then_uneven_batch_matches_oracle():
+    """The measured cost table installs and plans built from it stay exact."""
+    table = capi.autotune(np.float32)
+    assert len(table) >= 6
+    assert all(r["a_us"] > 0 and r["b_shared_us"] > 0 and r["b_solo_us"] > 0 for r in table)
+    assert {r["nfft"] for r in table} == {1024, 2048, 4096, 8192}
+    rng = np.random.default_rng(607)
+    C, lens_c = 3, [150_000, 90_000, 120_000]
+    xs = [(rng.standard_normal(L) * 2 + rng.uniform(-50, 50)).astype(np.float32) for L in lens_c]
+    tl = [300, 1200, 2500]
+    tpls, rch, rgr = [], [], []
+    for g, L in enumerate(tl):
+        for c in range(C):
+            tpls.append(rng.standard_normal(L) * np.hanning(L))
+            rch.append(c)
+            rgr.append(g)
+    h = capi.Handle(np.float32, "batched")
+    h.set_templates(tpls, shifts=[0] * len(tpls), weights=[1.0] * len(tpls))
+    h.set_batch_layout(lens_c, rch, rgr, len(tl))
+    h.initialize()
+    for c in range(C):
+        h.set_signal(c, xs[c])
+    h.apply()
+    for r, (t, c, g) in enumerate(zip(tpls, rch, rgr)):
+        nout = min(lens_c) - tl[g] + 1
+        row = h.rho(r)
+        assert row.size == nout
+        lags = rng.integers(0, nout, 1500)
+        assert np.abs(row[lags].astype(np.float64) - O.pearson(t, xs[c], lags)).max() < 1e-5
+
+
+def test_graph_launch_replays_match_eager():
+    """mf_apply replayed from a HIP graph gives the eager result on every
+    replay, with new signals and host synchronisations between replays."""
+    rng = np.random.default_rng(608)
+    n, C = 60_000, 3
+    lens = [300, 700, 1500]
+    tpls, rch = [], []
+    for c in range(C):
+        for L in lens:
+            tpls.append(rng.standard_normal(L) * np.hanning(L))
+            rch.append(c)
+
+    def handle(graph):
+        h = capi.Handle(np.float32, "batched")
+        h.set_templates(tpls, shifts=[0] * len(tpls), weights=[1.0] * len(tpls))
+        h.set_batch_layout([n] * C, rch, [0] * len(tpls), 1)
+        h.set_graph_launch(graph)
+        h.initialize()
+        return h
+
+    eager, graphed = handle(False), handle(True)
+    for replay in range(3):
+        xs = [(rng.standard_normal(n) + rng.uniform(-20, 20)).astype(np.float32) for _ in range(C)]
+        for h in (eager, graphed):
+            for c in range(C):
+                h.set_signal(c, xs[c])
+            h.apply()
+            h.synchronize()
+        for r in range(len(tpls)):
+            assert np.array_equal(graphed.rho(r), eager.rho(r)), (replay, r)
