@@ -378,19 +378,22 @@ void initialize(mf_handle *h)
     MF_HIP(hipMemcpy(h->dRowTab.p, rowTab.data(), h->dRowTab.bytes, hipMemcpyHostToDevice));
     h->dSegs.alloc(h->segHost.size() * sizeof(mfk::Seg));
     std::vector<mfk::Item> items;
-    std::map<int, std::vector<int>> lanesByLog;
-    for (size_t li = 0; li < h->lanes.size(); ++li) { lanesByLog[h->lanes[li].logM].push_back(static_cast<int>(li)); }
-    for (auto &kv : lanesByLog) {
+    // one launch per (FFT size, kernel variant)
+    std::map<std::pair<int, int>, std::vector<int>> lanesByPlan;
+    for (size_t li = 0; li < h->lanes.size(); ++li) {
+        const auto &ln = h->lanes[li];
+        lanesByPlan[{ln.logM, laneVariant(h->precision, ln.logM, h->userVariant, ln.mix)}].push_back(
+            static_cast<int>(li));
+    }
+    for (auto &kv : lanesByPlan) {
         std::vector<long long> blocks;
-        std::vector<RowMix> mixes;
         for (int li : kv.second) {
             const auto &ln = h->lanes[li];
             blocks.push_back((ln.nOut + ln.L - 1) / ln.L);
-            mixes.push_back(ln.mix);
         }
         mf_handle::Launch la;
-        la.logM = kv.first;
-        la.variant = corrVariant(h->precision, kv.first, h->userVariant, mixes, blocks);
+        la.logM = kv.first.first;
+        la.variant = kv.first.second;
         la.first = static_cast<int>(items.size());
         appendXcdItems(kv.second, blocks, items);
         la.count = static_cast<int>(items.size()) - la.first;

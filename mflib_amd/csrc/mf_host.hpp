@@ -320,25 +320,22 @@ inline double laneCost(int precision, int logM, int variant, int nb, long long n
     return blocks * staticBlockCost(logM, mix, precision);
 }
 
-// Correlation kernel variant of one launch: a requested variant if it
-// exists for the size; otherwise, with a measured cost table, the variant
-// of least predicted cost over the launch's lanes; otherwise the default.
-inline int corrVariant(int precision, int logM, int requested, const std::vector<RowMix> &mixes = {},
-                       const std::vector<long long> &blocks = {})
+// Correlation kernel variant of one lane: a requested variant if it exists
+// for the size; otherwise, with a measured cost table, the variant of least
+// predicted per-block cost for the lane's rows; otherwise the default.  It
+// depends on the lane alone, so a lane computes the same numbers whatever
+// it is launched with.
+inline int laneVariant(int precision, int logM, int requested, RowMix mix)
 {
     if (requested != 0 && variantAvailable(precision, logM, requested)) { return requested; }
     std::lock_guard<std::mutex> lk(costMutex());
     const CostTable &t = costTable(precision);
-    if (t.measured && !mixes.empty()) {
+    if (t.measured) {
         int bestV = 0;
         double best = -1;
         for (int v = 1; v <= 2; ++v) {
             if (!variantAvailable(precision, logM, v) || t.a[logM][v] <= 0) { continue; }
-            double c = 0;
-            for (size_t i = 0; i < mixes.size(); ++i) {
-                c += static_cast<double>(blocks[i]) *
-                     (t.a[logM][v] + t.bShared[logM][v] * mixes[i].shared + t.bSolo[logM][v] * mixes[i].solo);
-            }
+            const double c = t.a[logM][v] + t.bShared[logM][v] * mix.shared + t.bSolo[logM][v] * mix.solo;
             if (best < 0 || c < best) { best = c; bestV = v; }
         }
         if (bestV != 0) { return bestV; }

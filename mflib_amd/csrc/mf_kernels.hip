@@ -142,7 +142,7 @@ __device__ __forceinline__ void window_sums(const T *raw, const double *ts1, con
 
 // Per-output scale sqrt(Lt)/||x - mean|| for outputs [0, nValid) of this
 // block, FP64 sliding sums restarted every C outputs; flagged outputs get
-// -1.  Threads t = 0..nthr-1 take part, each a contiguous run of C
+// -1.  Threads t = 0..nthr-1 take part, each a contiguous run of C (odd)
 // outputs.  The FP32 scale is rsqrt(den^2) in FP32 plus one Newton step
 // (relative error ~2^-23, far inside the FP32 contract); the FP64 scale is
 // sqrt/division in FP64.  Returns this thread's number of flagged outputs.
@@ -154,7 +154,9 @@ __device__ __forceinline__ int compute_scale(const T *raw, const double *ts1, co
     const double sqrtLt = sqrt(static_cast<double>(Lt));
     const double denThr = sqrtLt * noise;
     const double denThr2 = denThr * denThr;
-    const int C = (nValid + nthr - 1) / nthr;
+    // odd run length: thread t's accesses start t*C apart, and an odd stride
+    // spreads the threads of a wave over all 32 LDS banks
+    const int C = ((nValid + nthr - 1) / nthr) | 1;
     const double canc = 2.0e-9 * (Lt + C) * Lt;
     const int i0 = t * C;
     const int i1 = min(i0 + C, nValid);
@@ -562,12 +564,31 @@ __global__ __launch_bounds__(NT, (Corr2Waves<T, M>::value)) void corr2_kernel(co
     // the pass-3 registers.  scl is the row's scale (a block-shared buffer),
     // or null: the wave then computes the row's scale itself, into its own
     // exchange buffer, which is free once the inverse FFT is in registers.
-    auto processRow = [&](const Row &row, const T *sharedScale, bool anyFlagShared) {
+#ifndef MF_PREFETCH_H
+#define MF_PREFETCH_H 1
+#endif
+    // template spectrum pairs (H[k], H[M-k]) of the row about to be processed
+    Cx<T> hk[KP], hm[KP];
+    auto loadH = [&](const void *Hp) {
+        const gptr<const Cx<T>> H = to_global(static_cast<const Cx<T> *>(Hp));
+#pragma unroll
+        for (int t = 0; t < KP; ++t) {
+            const int k = lane + 64 * t;
+            if (k <= M / 2) {
+                hk[t] = load_cx(H, k);
+                hm[t] = load_cx(H, M - k);
+            }
+        }
+    };
+    auto processRow = [&](const Row &row, const T *sharedScale, bool anyFlagShared, const void *nextH) {
         const int nvRow = static_cast<int>(min(static_cast<long long>(sg.L), row.nOut - base));
-        if (nvRow <= 0) { return; }
+        if (nvRow <= 0) {
+            if (MF_PREFETCH_H && nextH != nullptr) { loadH(nextH); }
+            return;
+        }
         Cx<T> *buf = xbuf[wave];
         // (k, M-k) pair per lane step (k = lane + 64 t; tw(k) = lo[lane] hi[t])
-        const gptr<const Cx<T>> H = to_global(static_cast<const Cx<T> *>(row.H));
+        if (!MF_PREFETCH_H) { loadH(row.H); }
         const Cx<T> wlo = twlo[lane];
         Cx<T> *pk = buf + xpad(lane);
         Cx<T> *pm = buf + xpad(M - lane);
@@ -575,8 +596,8 @@ __global__ __launch_bounds__(NT, (Corr2Waves<T, M>::value)) void corr2_kernel(co
         for (int t = 0; t < KP; ++t) {
             const int k = lane + 64 * t;
             if (k <= M / 2) {
-                const Cx<T> yk = cmul(Xs[k], load_cx(H, k));
-                const Cx<T> ym = cmul(Xs[M - k], load_cx(H, M - k));
+                const Cx<T> yk = cmul(Xs[k], hk[t]);
+                const Cx<T> ym = cmul(Xs[M - k], hm[t]);
                 const Cx<T> S = cadd(yk, cconj(ym));
                 const Cx<T> D = csub(yk, cconj(ym));
                 const Cx<T> Pp = cmul(cconj(cmul(wlo, twhi[t])), D);
@@ -588,6 +609,9 @@ __global__ __launch_bounds__(NT, (Corr2Waves<T, M>::value)) void corr2_kernel(co
         wave_load<T, M, P::R1>(v, buf, lane);
         wave_lds_sync();
         wave_fft_regs<T, M, true>(v, buf, lane, wt);
+        // next row's spectrum loads overlap this row's epilogue (the loads
+        // are not live across the FFT passes, so they cost no registers there)
+        if (MF_PREFETCH_H && nextH != nullptr) { loadH(nextH); }
         const T *scale = sharedScale;
         bool anyFlag = anyFlagShared;
         if (scale == nullptr) {
@@ -650,7 +674,10 @@ __global__ __launch_bounds__(NT, (Corr2Waves<T, M>::value)) void corr2_kernel(co
                 myFlags = compute_scale<T>(raw, ts1, ts2, tp1, tp2, sc, a, nValidSeg, Lt, noise, tid, NT);
             }
             const bool anyFlag = block_sum<NT>(static_cast<double>(myFlags), red) > 0.0;
-            for (int r = g0 + wave; r < g1; r += NWAVE) { processRow(rows[r], sc, anyFlag); }
+            if (MF_PREFETCH_H && g0 + wave < g1) { loadH(rows[g0 + wave].H); }
+            for (int r = g0 + wave; r < g1; r += NWAVE) {
+                processRow(rows[r], sc, anyFlag, r + NWAVE < g1 ? rows[r + NWAVE].H : nullptr);
+            }
             __syncthreads();  // sc is rebuilt for the next group
         }
         g0 = g1;
@@ -666,7 +693,10 @@ __global__ __launch_bounds__(NT, (Corr2Waves<T, M>::value)) void corr2_kernel(co
         while (g1 < sg.rowEnd && rows[g1].Lt == Lt) { ++g1; }
         if (g1 - g0 < NWAVE) {
             for (int r = g0; r < g1; ++r, ++k) {
-                if ((k & (NWAVE - 1)) == wave) { processRow(rows[r], nullptr, false); }
+                if ((k & (NWAVE - 1)) == wave) {
+                    if (MF_PREFETCH_H) { loadH(rows[r].H); }
+                    processRow(rows[r], nullptr, false, nullptr);
+                }
             }
         }
         g0 = g1;

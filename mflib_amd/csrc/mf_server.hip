@@ -161,9 +161,10 @@ void launchBatch(mf_server *s, Slot &slot)
 {
     std::vector<mfk::Seg> segs;
     std::vector<mfk::Row> rows;
-    std::map<int, std::vector<int>> segsByLog;
-    std::map<int, std::vector<long long>> blocksByLog;
-    std::map<int, std::vector<RowMix>> mixByLog;
+    // one launch per (FFT size, kernel variant); each lane's variant depends
+    // on the lane alone
+    std::map<std::pair<int, int>, std::vector<int>> segsByPlan;
+    std::map<std::pair<int, int>, std::vector<long long>> blocksByPlan;
     long long maxN = 1;
     char *arena = static_cast<char *>(s->arena.p);
     for (Request *r : slot.reqs) {
@@ -205,9 +206,9 @@ void launchBatch(mf_server *s, Slot &slot)
                 }
                 sg.rowEnd = static_cast<int>(rows.size());
                 sg.hmax = hmax;
-                segsByLog[lg].push_back(static_cast<int>(segs.size()));
-                blocksByLog[lg].push_back((r->nOut[c] + L - 1) / L);
-                mixByLog[lg].push_back(bk.mix);
+                const std::pair<int, int> plan{lg, laneVariant(s->precision, lg, 0, bk.mix)};
+                segsByPlan[plan].push_back(static_cast<int>(segs.size()));
+                blocksByPlan[plan].push_back((r->nOut[c] + L - 1) / L);
                 maxN = std::max<long long>(maxN, r->len[c]);
                 segs.push_back(sg);
             }
@@ -218,10 +219,9 @@ void launchBatch(mf_server *s, Slot &slot)
     };
     std::vector<mfk::Item> items;
     std::vector<Launch> launches;
-    for (auto &kv : segsByLog) {
-        Launch la{kv.first, static_cast<int>(items.size()), 0,
-                  corrVariant(s->precision, kv.first, 0, mixByLog[kv.first], blocksByLog[kv.first])};
-        appendXcdItems(kv.second, blocksByLog[kv.first], items);
+    for (auto &kv : segsByPlan) {
+        Launch la{kv.first.first, static_cast<int>(items.size()), 0, kv.first.second};
+        appendXcdItems(kv.second, blocksByPlan[kv.first], items);
         la.count = static_cast<int>(items.size()) - la.first;
         launches.push_back(la);
     }
