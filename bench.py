@@ -282,6 +282,91 @@ def uneven_section(args, torch):
     }
 
 
+def precision_section(args, torch):
+    """Precision stress: 256 templates of 128-256 samples against 4 channels
+    of 2,000,000 FP32 samples with a 1e4 DC offset and constant gaps, on
+    three paths: FFT FP32 (wave-level kernel), and the time-domain MFMA
+    path with FP16 and with BF16 hi/lo operands.  Keeps rows and peaks for
+    the FP64-oracle audit in the CPU leg."""
+    from mflib_amd import capi, synth
+    log("precision stress: generating 4 channels x 2M samples, 256 templates")
+    wl = synth.precision_stress()
+    C, T = len(wl.signals), len(wl.templates)
+    n = wl.signals[0].size
+    dev = [torch.from_numpy(s).to("cuda:0") for s in wl.signals]
+    torch.cuda.synchronize()
+    rng = np.random.default_rng(0x93)
+    audit_rows = [(int(rng.integers(0, C)), int(rng.integers(0, T))) for _ in range(6)]
+    # lags: random, inside gaps and across gap edges
+    lag_sets = {}
+    for c, t in audit_rows:
+        g = wl.gaps[c]
+        lags = list(rng.integers(0, n - 256, 1500))
+        for s0, L in g[:8]:
+            lags += [s0 - 100, s0 - 3, s0, s0 + 5, s0 + L // 2, s0 + L - 130, s0 + L - 2, s0 + L + 10]
+        lag_sets[(c, t)] = np.clip(np.array(lags, dtype=np.int64), 0, n - 257)
+    paths = {}
+    for name, variant in (("fft_fp32", 0), ("mfma_fp16_hilo", 3), ("mfma_bf16_hilo", 4)):
+        h = capi.Handle(np.float32, "batched")
+        h.set_templates([wl.templates[t] for c in range(C) for t in range(T)])
+        h.set_batch_layout([n] * C, [c for c in range(C) for _ in range(T)], [c for c in range(C) for _ in range(T)], C)
+        h.set_kernel_variant(variant)
+        h.initialize()
+        for c in range(C):
+            h.set_signal_device(c, dev[c].data_ptr(), n)
+        h.apply()
+        h.synchronize()
+        ms = []
+        for _ in range(3):
+            h.apply()
+            h.synchronize()
+            ms.append(h.timings()[0])
+        n_corr = sum(h.filtered_length(r) for r in range(C * T))
+        corr_ms = float(np.median(ms))
+        rows = {}
+        for c, t in audit_rows:
+            r = h.rho(c * T + t)
+            pk = capi.peaks_device(h.rho_device_ptr(c * T + t), r.size, np.float32, 0.6, 200)
+            rows[(c, t)] = (r[lag_sets[(c, t)]].astype(np.float64), pk)
+        plan = h.plan(0)
+        h.close()
+        paths[name] = {"corr_kernel_ms": corr_ms, "corr_samples_per_s": n_corr / (corr_ms * 1e-3),
+                       "plan_row0_nfft": plan[0], "rows": rows}
+        log(f"precision stress {name}: corr {corr_ms:.2f} ms, {n_corr / (corr_ms * 1e-3) / 1e9:.1f} G corr samples/s")
+    out = {
+        "workload": "precision stress: 256 templates (128-256 samples, 2-20 Hz) x 4 channels x 2,000,000 FP32 "
+                    "samples, DC 1e4, 20 constant gaps of 500-5000 samples per channel",
+        "corr_samples_per_apply": n_corr,
+        "paths": {k: {kk: vv for kk, vv in v.items() if kk != "rows"} for k, v in paths.items()},
+    }
+    return out, (wl, audit_rows, lag_sets, paths)
+
+
+def precision_cpu(state):
+    """FP64 oracle audit of the precision-stress rows of every path, and
+    peak-index agreement against PeakFinder(0.6, 200) on the oracle rows."""
+    import oracle as O
+    wl, audit_rows, lag_sets, paths = state
+    res = {name: {"max_abs_drho_vs_fp64": 0.0, "peak_sets_equal": 0, "rows": len(audit_rows)} for name in paths}
+    for k, (c, t) in enumerate(audit_rows):
+        lags = lag_sets[(c, t)]
+        ref = O.pearson(wl.templates[t], wl.signals[c], lags)
+        full = None
+        if k < 2:
+            full = O.pearson(wl.templates[t], wl.signals[c])[: wl.signals[c].size - 256 + 1]
+            ref_pk = O.peak_finder(full, 0.6, 200)
+        for name, p in paths.items():
+            vals, pk = p["rows"][(c, t)]
+            res[name]["max_abs_drho_vs_fp64"] = max(res[name]["max_abs_drho_vs_fp64"],
+                                                    float(np.abs(vals - ref).max()))
+            if full is not None:
+                res[name]["peak_sets_equal"] += int(np.array_equal(np.sort(pk), np.sort(ref_pk)))
+    for name in res:
+        res[name]["peak_rows_checked"] = 2
+        res[name]["audited_samples"] = int(sum(lag_sets[r].size for r in audit_rows))
+    return res
+
+
 def serve_audit(wl, audit):
     """FP64 oracle check of the audited served rows (1 in 100 requests, one
     (template, channel) row each, 2000 seeded lags per row)."""
@@ -344,6 +429,8 @@ def main() -> int:
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-serve", action="store_true", help="skip the served-stream measurement")
     ap.add_argument("--no-uneven", action="store_true", help="skip the uneven batched workload")
+    ap.add_argument("--no-precision", action="store_true", help="skip the precision-stress workload")
+    ap.add_argument("--precision-only", action="store_true", help="only the precision-stress workload")
     ap.add_argument("--uneven-only", action="store_true", help="only the uneven batched workload")
     ap.add_argument("--serve-only", action="store_true", help="only the served-stream measurement")
     ap.add_argument("--serve-lambda", type=float, nargs="+", default=[500.0, 1000.0])
@@ -377,6 +464,12 @@ def main() -> int:
     import torch  # before the engine library: one HIP runtime per process
     from mflib_amd import capi, synth
 
+    if args.precision_only:
+        out, state = precision_section(args, torch)
+        if not args.no_cpu_baseline:
+            out["accuracy"] = precision_cpu(state)
+        print(json.dumps({"precision_stress": out}), flush=True)
+        return 0
     if args.uneven_only:
         print(json.dumps({"uneven_batch": uneven_section(args, torch)}), flush=True)
         return 0
@@ -499,6 +592,9 @@ def main() -> int:
                           "lambda_per_s": lat["lambda_per_s"], "workload": "served stream (see serving)"}
     if not args.no_uneven:
         out["uneven_batch"] = uneven_section(args, torch)
+    prec_state = None
+    if not args.no_precision:
+        out["precision_stress"], prec_state = precision_section(args, torch)
     if not args.no_cpu_baseline:
         log("CPU baseline (bounded sample) and accuracy audit")
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
@@ -508,6 +604,8 @@ def main() -> int:
         out["accuracy"] = acc
         if serve_state is not None:
             out["serving"]["accuracy"] = serve_audit(*serve_state)
+        if prec_state is not None:
+            out["precision_stress"]["accuracy"] = precision_cpu(prec_state)
         sc_cpu, sc_acc = single_channel_cpu(sc_state, threads)
         out["single_channel"]["cpu_baseline"] = sc_cpu
         out["single_channel"]["accuracy"] = sc_acc

@@ -82,7 +82,10 @@ int mf_set_batch_layout(mf_handle *h, int nChannels, const int *channelLengths,
 int mf_set_fft_length(mf_handle *h, int nfft);
 /* No reference counterpart (tuning): correlation kernel variant, 0 = best
  * available (default), 1 = block-wide LDS FFTs, 2 = wave-level register
- * FFTs.  A variant that does not exist for a block size falls back to 0. */
+ * FFTs, 3 / 4 = the short-template time-domain path on MFMA with FP16 /
+ * BF16 hi/lo operands (FP32 instances; rows of at most 256 taps, longer
+ * rows stay on the FFT path).  A variant that does not exist for a block
+ * size falls back to 0.  mf_get_plan reports nfft 0 for time-domain rows. */
 int mf_set_kernel_variant(mf_handle *h, int variant);
 
 /* No reference counterpart: with on != 0, mf_apply captures its

@@ -73,6 +73,7 @@ struct mf_handle {
     };
     struct Lane {
         int ch = 0, logM = 0, L = 0, nOut = 0;
+        int td = 0, KT = 0;  // time-domain lane: kernel kind (3/4) and padded taps
         RowMix mix;
         std::vector<int> rows;  // sorted by Lt
     };
@@ -95,6 +96,7 @@ struct mf_handle {
     // device state
     DevBuf dRho, dSpec, dTmpl, dTmplRaw, dHmax;
     DevBuf dSegs, dRowTab, dItems;
+    DevBuf dFrag, dTdRowTab, dTdItems;  // time-domain lanes
     DevBuf dStackRows, dStackOut;
     // stacking groups (built on first stack/detect)
     struct StackInfo {
@@ -109,6 +111,7 @@ struct mf_handle {
     std::map<int, TwTables> tw;
     struct Launch {
         int logM, first, count, variant;
+        int KT = 0;  // time-domain launches (variant 3/4): padded taps; items in dTdItems
     };
     std::vector<Launch> launches;
     std::vector<mfk::Seg> segHost;
@@ -162,6 +165,9 @@ void dropDevice(mf_handle *h)
     h->dSegs.release();
     h->dRowTab.release();
     h->dItems.release();
+    h->dFrag.release();
+    h->dTdRowTab.release();
+    h->dTdItems.release();
     h->dStackRows.release();
     h->stacksReady = false;
     h->stackInfo.clear();
@@ -263,7 +269,33 @@ void initialize(mf_handle *h)
             chOut = std::max<long long>(chOut, h->rows[r].nOut);
             lts.push_back(h->rows[r].Lt);
         }
-        for (const Bucket &bk : planBuckets(h->precision, h->userNfft, lts, chOut)) {
+        // time-domain path (variant 3/4, FP32): the rows of at most kTdMaxTaps
+        // taps form one lane; longer rows keep FFT lanes
+        size_t nTd = 0;
+        if ((h->userVariant == mfk::kTdFp16 || h->userVariant == mfk::kTdBf16) &&
+            h->precision == MF_PRECISION_FLOAT) {
+            while (nTd < lts.size() && mfk::td_supported(lts[nTd])) { ++nTd; }
+        }
+        if (nTd > 0) {
+            mf_handle::Lane ln;
+            ln.ch = c;
+            ln.td = h->userVariant;
+            ln.KT = mfk::td_taps(lts[nTd - 1]);
+            ln.logM = mfk::kMinLogM;  // template spectra for mf_template_spectrum only
+            ln.L = mfk::kTdOutputsPerItem;
+            ln.mix = rowMix(std::vector<int>(lts.begin(), lts.begin() + nTd));
+            ln.rows.assign(sorted.begin(), sorted.begin() + nTd);
+            for (int r : ln.rows) {
+                ln.nOut = std::max(ln.nOut, h->rows[r].nOut);
+                h->rows[r].lane = static_cast<int>(h->lanes.size());
+                h->rows[r].logM = ln.logM;
+            }
+            h->lanes.push_back(std::move(ln));
+            sorted.erase(sorted.begin(), sorted.begin() + nTd);
+            lts.erase(lts.begin(), lts.begin() + nTd);
+            if (sorted.empty()) { continue; }
+        }
+        for (const Bucket &bk : planBuckets(h->precision, h->userNfft == 0 ? 0 : h->userNfft, lts, chOut)) {
             mf_handle::Lane ln;
             ln.ch = c;
             ln.mix = bk.mix;
@@ -356,6 +388,15 @@ void initialize(mf_handle *h)
         s.nOut = ln.nOut;
         s.rowBegin = static_cast<int>(rowTab.size());
         float hmax = 0;
+        if (ln.td) {
+            // time-domain lane: its rows go to the TD row table (built below)
+            s.rowEnd = s.rowBegin;
+            s.hmax = 0;
+            s.x = h->ch[ln.ch].buf.p;
+            s.dead = 1;
+            h->segHost.push_back(s);
+            continue;
+        }
         for (int r : ln.rows) {
             const auto &rp = h->rows[r];
             mfk::Row row;
@@ -374,14 +415,79 @@ void initialize(mf_handle *h)
         s.dead = 1;
         h->segHost.push_back(s);
     }
-    h->dRowTab.alloc(rowTab.size() * sizeof(mfk::Row));
-    MF_HIP(hipMemcpy(h->dRowTab.p, rowTab.data(), h->dRowTab.bytes, hipMemcpyHostToDevice));
+    h->dRowTab.alloc(std::max<size_t>(rowTab.size(), 1) * sizeof(mfk::Row));
+    if (!rowTab.empty()) {
+        MF_HIP(hipMemcpy(h->dRowTab.p, rowTab.data(), rowTab.size() * sizeof(mfk::Row), hipMemcpyHostToDevice));
+    }
+    // time-domain lanes: split taps, row table, work items
+    std::vector<mfk::TdRow> tdRows;
+    std::vector<mfk::TdItem> tdItems;
+    std::map<std::pair<int, int>, std::pair<int, int>> tdLaunch;  // (kind, KT) -> (first, count)
+    {
+        size_t fragBytes = 0;
+        for (const auto &ln : h->lanes) {
+            if (ln.td) { fragBytes += ln.rows.size() * 4 * static_cast<size_t>(ln.KT); }
+        }
+        if (fragBytes > 0) {
+            h->dFrag.alloc(fragBytes);
+            std::map<std::pair<int, int>, std::vector<mfk::TdPrepJob>> jobs;
+            std::map<std::pair<int, int>, std::vector<mfk::TdItem>> itemsByKind;
+            size_t off = 0;
+            for (size_t li = 0; li < h->lanes.size(); ++li) {
+                const auto &ln = h->lanes[li];
+                if (!ln.td) { continue; }
+                const int rowBegin = static_cast<int>(tdRows.size());
+                for (int r : ln.rows) {
+                    const auto &rp = h->rows[r];
+                    mfk::TdRow tr{};
+                    tr.frag = static_cast<char *>(h->dFrag.p) + off;
+                    tr.tmpl = h->dTmpl.as<double>() + rp.tmplOff;
+                    tr.out = static_cast<char *>(h->dRho.p) + rp.rhoOff * h->tsize;
+                    tr.nOut = rp.nOut;
+                    tr.Lt = rp.Lt;
+                    tdRows.push_back(tr);
+                    jobs[{ln.td, ln.KT}].push_back({tr.tmpl, rp.Lt, 0, const_cast<void *>(tr.frag)});
+                    off += 4 * static_cast<size_t>(ln.KT);
+                }
+                const int nRows = static_cast<int>(ln.rows.size());
+                for (long long s0 = 0; s0 < ln.nOut; s0 += mfk::kTdOutputsPerItem) {
+                    for (int g = 0; g < nRows; g += 128) {
+                        mfk::TdItem it{};
+                        it.seg = static_cast<int>(li);
+                        it.rowBegin = rowBegin + g;
+                        it.seg0 = s0;
+                        it.nRows = std::min(128, nRows - g);
+                        itemsByKind[{ln.td, ln.KT}].push_back(it);
+                    }
+                }
+            }
+            for (auto &kv : jobs) {
+                DevBuf dj;
+                dj.alloc(kv.second.size() * sizeof(mfk::TdPrepJob));
+                MF_HIP(hipMemcpy(dj.p, kv.second.data(), dj.bytes, hipMemcpyHostToDevice));
+                MF_HIP(mfk::launch_td_prep(kv.first.first, dj.as<mfk::TdPrepJob>(), static_cast<int>(kv.second.size()),
+                                           kv.first.second, h->stream));
+                MF_HIP(hipStreamSynchronize(h->stream));
+            }
+            for (auto &kv : itemsByKind) {
+                tdLaunch[kv.first] = {static_cast<int>(tdItems.size()), static_cast<int>(kv.second.size())};
+                tdItems.insert(tdItems.end(), kv.second.begin(), kv.second.end());
+            }
+        }
+    }
+    h->dTdRowTab.alloc(std::max<size_t>(tdRows.size(), 1) * sizeof(mfk::TdRow));
+    h->dTdItems.alloc(std::max<size_t>(tdItems.size(), 1) * sizeof(mfk::TdItem));
+    if (!tdRows.empty()) {
+        MF_HIP(hipMemcpy(h->dTdRowTab.p, tdRows.data(), tdRows.size() * sizeof(mfk::TdRow), hipMemcpyHostToDevice));
+        MF_HIP(hipMemcpy(h->dTdItems.p, tdItems.data(), tdItems.size() * sizeof(mfk::TdItem), hipMemcpyHostToDevice));
+    }
     h->dSegs.alloc(h->segHost.size() * sizeof(mfk::Seg));
     std::vector<mfk::Item> items;
     // one launch per (FFT size, kernel variant)
     std::map<std::pair<int, int>, std::vector<int>> lanesByPlan;
     for (size_t li = 0; li < h->lanes.size(); ++li) {
         const auto &ln = h->lanes[li];
+        if (ln.td) { continue; }
         lanesByPlan[{ln.logM, laneVariant(h->precision, ln.logM, h->userVariant, ln.mix)}].push_back(
             static_cast<int>(li));
     }
@@ -397,6 +503,15 @@ void initialize(mf_handle *h)
         la.first = static_cast<int>(items.size());
         appendXcdItems(kv.second, blocks, items);
         la.count = static_cast<int>(items.size()) - la.first;
+        h->launches.push_back(la);
+    }
+    for (const auto &kv : tdLaunch) {
+        mf_handle::Launch la;
+        la.logM = 0;
+        la.variant = kv.first.first;
+        la.KT = kv.first.second;
+        la.first = kv.second.first;
+        la.count = kv.second.second;
         h->launches.push_back(la);
     }
     h->dItems.alloc(std::max<size_t>(items.size(), 1) * sizeof(mfk::Item));
@@ -458,6 +573,11 @@ void apply(mf_handle *h)
     }
     auto launchAll = [&]() {
         for (const auto &la : h->launches) {
+            if (la.KT > 0) {
+                MF_HIP(mfk::launch_td(la.variant, la.KT, h->dTdItems.as<mfk::TdItem>() + la.first, la.count,
+                                      h->dSegs.as<mfk::Seg>(), h->dTdRowTab.as<mfk::TdRow>(), h->stream));
+                continue;
+            }
             const TwTables &t = h->tw.at(la.logM);
             const void *lo = h->precision == MF_PRECISION_FLOAT ? t.lo.p : t.lo64.p;
             const void *hi = h->precision == MF_PRECISION_FLOAT ? t.hi.p : t.hi64.p;
@@ -1020,7 +1140,7 @@ int mf_set_kernel_variant(mf_handle *h, int variant)
 {
     return guard([&] {
         requireHandle(h);
-        if (variant < 0 || variant > 2) { fail(MF_ERR_INVALID_ARGUMENT, "kernel variant must be 0, 1 or 2"); }
+        if (variant < 0 || variant > 4) { fail(MF_ERR_INVALID_ARGUMENT, "kernel variant must be 0..4"); }
         h->userVariant = variant;
     });
 }
@@ -1091,7 +1211,7 @@ int mf_get_plan(const mf_handle *h, int row, int *nfft, int *blockLength)
         requireInit(h);
         checkRow(h, row);
         const auto &ln = h->lanes[h->rows[row].lane];
-        if (nfft) { *nfft = 2 << ln.logM; }
+        if (nfft) { *nfft = ln.td ? 0 : 2 << ln.logM; }  // 0: time-domain (no FFT)
         if (blockLength) { *blockLength = ln.L; }
     });
 }

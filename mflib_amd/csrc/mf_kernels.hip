@@ -193,6 +193,29 @@ __device__ __forceinline__ int compute_scale(const T *raw, const double *ts1, co
     return nflag;
 }
 
+// Flagged windows per 64 outputs from which a wave recomputes them one per
+// lane (serial FP64 loops, no cross-lane reductions) instead of one per
+// wave (strided sums and wave reductions, latency-bound when repeated).
+constexpr int kDenseFlags = 16;
+
+// Exact FP64 two-pass Pearson of the window at wi, by one lane.
+template <typename T>
+__device__ __forceinline__ double exact_window(const T *raw, int wi, gptr<const double> tmpl, int Lt)
+{
+    double sm = 0;
+    for (int mm = 0; mm < Lt; ++mm) { sm += static_cast<double>(raw[wi + mm]); }
+    const double mean = sm / Lt;
+    double e2 = 0, dn = 0;
+    for (int mm = 0; mm < Lt; ++mm) {
+        const double d = static_cast<double>(raw[wi + mm]) - mean;
+        e2 += d * d;
+        dn += tmpl[mm] * d;
+    }
+    double rho = dn / fmax(DBL_EPSILON, sqrt(e2));
+    if (fabs(rho) > 1.1) { rho = 0; }
+    return rho;
+}
+
 template <typename T, int M>
 __global__ __launch_bounds__(NT) void corr_kernel(const Item *__restrict__ items,
                                                   const Seg *__restrict__ segs,
@@ -341,6 +364,10 @@ __global__ __launch_bounds__(NT) void corr_kernel(const Item *__restrict__ items
                 const int i = w0 + lane;
                 const bool fl = (i < s1) && (sc[i] < T(0));
                 unsigned long long m = __ballot(fl);
+                if (__popcll(m) >= kDenseFlags) {
+                    if (fl) { out[i] = static_cast<T>(exact_window<T>(raw, i, to_global(row.tmpl), Lt)); }
+                    continue;
+                }
                 while (m != 0ull) {
                     const int b = __ffsll(static_cast<long long>(m)) - 1;
                     m &= m - 1ull;
@@ -386,6 +413,11 @@ __device__ __forceinline__ void fixup_row_wave(const T *raw, const T *sc, gptr<c
         const int i = w0 + lane;
         const bool fl = (i < nv) && (sc[i] < T(0));
         unsigned long long m = __ballot(fl);
+        if (__popcll(m) >= kDenseFlags) {
+            // many flagged windows (a gap, a dead stretch): one lane per window
+            if (fl) { out[i] = static_cast<T>(exact_window<T>(raw, i, tmpl, Lt)); }
+            continue;
+        }
         while (m != 0ull) {
             const int b = __ffsll(static_cast<long long>(m)) - 1;
             m &= m - 1ull;

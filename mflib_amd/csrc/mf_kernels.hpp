@@ -71,6 +71,46 @@ struct StackGroup {
     void *candVal;
 };
 
+// ---- time-domain correlation of short templates on MFMA -----------------
+constexpr int kTdFp16 = 3;              // kernel variant: FP16 hi/lo operands
+constexpr int kTdBf16 = 4;              // kernel variant: BF16 hi/lo operands
+constexpr int kTdOutputsPerItem = 512;  // outputs per work item
+constexpr int kTdMaxTaps = 256;         // longest template of the path
+
+// One template row of a time-domain lane.
+struct TdRow {
+    const void *frag;     // split taps: KT hi parts, then KT lo parts (16-bit)
+    const double *tmpl;   // demeaned unit-norm template (FP64), Lt samples
+    void *out;            // correlation output row (FP32)
+    long long nOut;       // outputs of this row
+    int Lt;               // template length
+    int pad;
+};
+
+// One work item: outputs [seg0, seg0 + kTdOutputsPerItem) of lane `seg`
+// for rows [rowBegin, rowBegin + nRows) (nRows <= 128) of the row table.
+struct TdItem {
+    int seg;
+    int rowBegin;
+    long long seg0;
+    int nRows;
+    int pad;
+};
+
+// Template split job (one per row).
+struct TdPrepJob {
+    const double *tmpl;
+    int Lt;
+    int pad;
+    void *frag;
+};
+
+bool td_supported(int nb);
+int td_taps(int nb);  // padded tap count KT (64, 128 or 256)
+hipError_t launch_td_prep(int kind, const TdPrepJob *dJobs, int nJobs, int KT, hipStream_t stream);
+hipError_t launch_td(int kind, int KT, const TdItem *dItems, int nItems, const Seg *dSegs, const TdRow *dRows,
+                     hipStream_t stream);
+
 // Correlation + normalisation: one launch per FFT size.  Variant 1 runs
 // block-wide LDS Stockham FFTs; variant 2 runs wave-level register FFTs
 // (one wave per template row).

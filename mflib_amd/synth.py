@@ -264,3 +264,49 @@ def uneven_batch(seed: int = SEED_BASE + 4, n_templates: int = 200, n_channels: 
         signals[c] += np.float32(rng.uniform(-100, 100))
     return UnevenBatch(fs=fs, signals=signals, templates=templates, lengths=lengths,
                        moveouts=moveouts, origins=origins)
+
+
+@dataclass
+class PrecisionStress:
+    """Short templates against channels with a large DC offset and gaps."""
+    fs: float
+    signals: list          # C float32 channels
+    templates: list        # FP64 unit-energy templates
+    gaps: list             # per channel: [(start, length)]
+    injections: list       # per channel: [(template, position)]
+
+
+def precision_stress(seed: int = SEED_BASE + 3, n_channels: int = 4, n: int = 2_000_000,
+                     n_templates: int = 256, lt=(128, 256), fs: float = 100.0, dc: float = 1e4,
+                     n_gaps: int = 20, gap=(500, 5000), n_inject: int = 30) -> PrecisionStress:
+    """n_channels float32 channels of 2-20 Hz band-passed N(0,1) noise plus
+    a DC offset of 1e4, each with n_gaps gap segments of 500-5000 samples
+    holding a constant (DC + U(-50, 50)) plus 1e-6 N(0,1); n_templates
+    unit-energy Hann-tapered 2-20 Hz bursts of U[lt] samples; each channel
+    carries n_inject injected templates at 5x the noise RMS per sample."""
+    rng = np.random.default_rng(seed)
+    lens = rng.integers(lt[0], lt[1] + 1, n_templates)
+    templates = [template_burst(np.random.default_rng([seed, 2, i]), int(L), fs, 2.0, 20.0)
+                 for i, L in enumerate(lens)]
+    signals, gaps, injections = [], [], []
+    for c in range(n_channels):
+        x = bandpassed_noise(np.random.default_rng([seed, 1, c]), n, fs, 2.0, 20.0)
+        rms = float(np.sqrt(np.mean(x[:200000] ** 2)))
+        inj = []
+        for _ in range(n_inject):
+            t = int(rng.integers(0, n_templates))
+            p = int(rng.integers(0, n - len(templates[t])))
+            w = templates[t] / np.abs(templates[t]).max()
+            x[p:p + len(w)] += 5.0 * rms * w
+            inj.append((t, p))
+        x += dc
+        g = []
+        for _ in range(n_gaps):
+            L = int(rng.integers(gap[0], gap[1] + 1))
+            s0 = int(rng.integers(0, n - L))
+            x[s0:s0 + L] = dc + rng.uniform(-50, 50) + 1e-6 * rng.standard_normal(L)
+            g.append((s0, L))
+        signals.append(x.astype(np.float32))
+        gaps.append(g)
+        injections.append(inj)
+    return PrecisionStress(fs=fs, signals=signals, templates=templates, gaps=gaps, injections=injections)

@@ -370,3 +370,40 @@ def test_graph_launch_replays_match_eager():
             h.synchronize()
         for r in range(len(tpls)):
             assert np.array_equal(graphed.rho(r), eager.rho(r)), (replay, r)
+
+
+@pytest.mark.parametrize("variant,tol", [(3, 1e-5), (4, 1e-5)])
+def test_time_domain_mfma_path_matches_oracle(variant, tol):
+    """Short templates (64-256 taps, several lengths) on the MFMA
+    time-domain path, FP16 or BF16 hi/lo operands: every row within the
+    FP32 contract of the FP64 oracle, including a constant gap with a DC
+    offset (windows recomputed exactly) and a dead channel."""
+    rng = np.random.default_rng(700 + variant)
+    n = 40_000
+    x0 = (rng.standard_normal(n) * 3 + 1e4).astype(np.float32)
+    x0[12000:14500] = np.float32(1e4 + 17.0) + (rng.standard_normal(2500) * 1e-6).astype(np.float32)
+    x1 = np.full(n, 5.0, dtype=np.float32)   # dead channel
+    lens = [64, 100, 128, 129, 200, 256, 256, 300]   # 300 stays on the FFT path
+    tpls, rch = [], []
+    for c in range(2):
+        for L in lens:
+            tpls.append(rng.standard_normal(L) * np.hanning(L))
+            rch.append(c)
+    h = capi.Handle(np.float32, "batched")
+    h.set_templates(tpls, shifts=[0] * len(tpls), weights=[1.0] * len(tpls))
+    h.set_batch_layout([n, n], rch, [0] * len(tpls), 1)
+    h.set_kernel_variant(variant)
+    h.initialize()
+    h.set_signal(0, x0)
+    h.set_signal(1, x1)
+    h.apply()
+    nout = n - max(lens) + 1
+    assert h.plan(0)[0] == 0 and h.plan(len(lens) - 1)[0] > 0  # time-domain vs FFT row
+    for r, (t, c) in enumerate(zip(tpls, rch)):
+        row = h.rho(r)
+        assert row.size == nout
+        if c == 1:
+            assert not np.any(row)
+            continue
+        ref = O.pearson(t, x0)[:nout]
+        assert np.abs(row.astype(np.float64) - ref).max() < tol, (r, len(t))
