@@ -46,6 +46,31 @@ __device__ __forceinline__ Cx<T> load_cx(gptr<const Cx<T>> p, int i)
     return Cx<T>{s[2 * i], s[2 * i + 1]};
 }
 
+// Raw buffer resource over `bytes` bytes from p (stride 0; dword 3 selects
+// 32-bit data, as for every gfx9 raw buffer access).  Stores through it at
+// byte offsets >= bytes are dropped by the hardware range check, so a
+// store needs no branch of its own and every path issues the same
+// instructions.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t buffer_rsrc(void *p, int bytes)
+{
+    return __builtin_amdgcn_make_buffer_rsrc(p, 0, bytes, 0x00020000);
+}
+__device__ __forceinline__ void buffer_store_pair(float x, float y, __amdgpu_buffer_rsrc_t r, int byteOff)
+{
+    typedef unsigned int u2 __attribute__((ext_vector_type(2)));
+    const u2 d = {__float_as_uint(x), __float_as_uint(y)};
+    __builtin_amdgcn_raw_buffer_store_b64(d, r, byteOff, 0, 0);
+}
+__device__ __forceinline__ void buffer_store_pair(double x, double y, __amdgpu_buffer_rsrc_t r, int byteOff)
+{
+    typedef unsigned int u4 __attribute__((ext_vector_type(4)));
+    const unsigned long long a = static_cast<unsigned long long>(__double_as_longlong(x));
+    const unsigned long long b = static_cast<unsigned long long>(__double_as_longlong(y));
+    const u4 d = {static_cast<unsigned>(a), static_cast<unsigned>(a >> 32), static_cast<unsigned>(b),
+                  static_cast<unsigned>(b >> 32)};
+    __builtin_amdgcn_raw_buffer_store_b128(d, r, byteOff, 0, 0);
+}
+
 template <typename T>
 __device__ __forceinline__ Cx<T> cadd(Cx<T> a, Cx<T> b) { return {a.x + b.x, a.y + b.y}; }
 template <typename T>

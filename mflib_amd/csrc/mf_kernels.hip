@@ -315,7 +315,7 @@ __global__ __launch_bounds__(NT) void corr_kernel(const Item *__restrict__ items
     const double rmsBlk = sqrt(energy / nSamp);
     const double noise = kNoiseK / Eps<T>::rhoTol * Eps<T>::value * sqrt(static_cast<double>(LOGN)) *
                          static_cast<double>(sg.hmax) * rmsBlk;
-    const int wave = tid >> 6;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);  // wave-uniform: scalar row loads and branches
     const int lane = tid & 63;
     int curLt = -1;
     for (int r = sg.rowBegin; r < sg.rowEnd; ++r) {
@@ -494,7 +494,7 @@ __global__ __launch_bounds__(NT, (Corr2Waves<T, M>::value)) void corr2_kernel(co
     const Item item = items[blockIdx.x];
     const Seg sg = segs[item.seg];
     const int tid = threadIdx.x;
-    const int wave = tid >> 6;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);  // wave-uniform: scalar row loads and branches
     const int lane = tid & 63;
     const long long base = static_cast<long long>(item.block) * sg.L;
     const int nValidSeg = static_cast<int>(min(static_cast<long long>(sg.L), sg.nOut - base));
@@ -591,36 +591,33 @@ __global__ __launch_bounds__(NT, (Corr2Waves<T, M>::value)) void corr2_kernel(co
         wave_lds_sync();
     };
 
-    // C. one template row on this wave: spectral product + inverse packing,
-    // three register passes, then the normalisation epilogue straight from
-    // the pass-3 registers.  scl is the row's scale (a block-shared buffer),
-    // or null: the wave then computes the row's scale itself, into its own
-    // exchange buffer, which is free once the inverse FFT is in registers.
-#ifndef MF_PREFETCH_H
-#define MF_PREFETCH_H 1
-#endif
-    // template spectrum pairs (H[k], H[M-k]) of the row about to be processed
-    Cx<T> hk[KP], hm[KP];
+    // C. one template row on this wave, in two halves.  startRow: spectral
+    // product + inverse packing and the three register passes (the result
+    // stays in v).  finishRow: the row's scale when it has no shared one,
+    // then the normalisation epilogue straight from the pass-3 registers.
+    // A wave's row sequence runs start(r0), then for each next row r:
+    // load H(r), finish(previous), start(r), and finally finish(last).
+    // H(r) is thus in flight during the previous row's epilogue, and the
+    // epilogue issues the same 16 (range-checked) stores on every path, so
+    // the wait for H(r) never waits for those stores.
+    Cx<T> hk[KP], hm[KP];  // template spectrum pairs (H[k], H[M-k])
     auto loadH = [&](const void *Hp) {
         const gptr<const Cx<T>> H = to_global(static_cast<const Cx<T> *>(Hp));
 #pragma unroll
         for (int t = 0; t < KP; ++t) {
-            const int k = lane + 64 * t;
-            if (k <= M / 2) {
-                hk[t] = load_cx(H, k);
-                hm[t] = load_cx(H, M - k);
-            }
+            // lanes past M/2 in the last step load a duplicate they never use
+            const int k = min(lane + 64 * t, M / 2);
+            hk[t] = load_cx(H, k);
+            hm[t] = load_cx(H, M - k);
         }
     };
-    auto processRow = [&](const Row &row, const T *sharedScale, bool anyFlagShared, const void *nextH) {
-        const int nvRow = static_cast<int>(min(static_cast<long long>(sg.L), row.nOut - base));
-        if (nvRow <= 0) {
-            if (MF_PREFETCH_H && nextH != nullptr) { loadH(nextH); }
-            return;
-        }
+    auto rowValid = [&](const Row &row) {
+        return static_cast<int>(max(0ll, min(static_cast<long long>(sg.L), row.nOut - base)));
+    };
+    auto startRow = [&](const Row &row) {
+        if (rowValid(row) == 0) { return; }
         Cx<T> *buf = xbuf[wave];
         // (k, M-k) pair per lane step (k = lane + 64 t; tw(k) = lo[lane] hi[t])
-        if (!MF_PREFETCH_H) { loadH(row.H); }
         const Cx<T> wlo = twlo[lane];
         Cx<T> *pk = buf + xpad(lane);
         Cx<T> *pm = buf + xpad(M - lane);
@@ -641,15 +638,19 @@ __global__ __launch_bounds__(NT, (Corr2Waves<T, M>::value)) void corr2_kernel(co
         wave_load<T, M, P::R1>(v, buf, lane);
         wave_lds_sync();
         wave_fft_regs<T, M, true>(v, buf, lane, wt);
-        // next row's spectrum loads overlap this row's epilogue (the loads
-        // are not live across the FFT passes, so they cost no registers there)
-        if (MF_PREFETCH_H && nextH != nullptr) { loadH(nextH); }
+    };
+    // sharedScale is the row's block-shared scale, or null: the wave then
+    // computes the row's scale itself, into its own exchange buffer, which
+    // is free once the inverse FFT is in registers.
+    auto finishRow = [&](const Row &row, const T *sharedScale, bool anyFlagShared) {
+        const int nvRow = rowValid(row);
+        Cx<T> *buf = xbuf[wave];
         const T *scale = sharedScale;
-        bool anyFlag = anyFlagShared;
+        bool anyFlag = anyFlagShared && nvRow > 0;
         if (scale == nullptr) {
             T *own = reinterpret_cast<T *>(buf);
-            wave_lds_sync();
-            const int nf = compute_scale<T>(raw, ts1, ts2, tp1, tp2, own, a, nvRow, row.Lt, noise, lane, 64);
+            int nf = 0;
+            if (nvRow > 0) { nf = compute_scale<T>(raw, ts1, ts2, tp1, tp2, own, a, nvRow, row.Lt, noise, lane, 64); }
             anyFlag = __ballot(nf > 0) != 0ull;
             wave_lds_sync();
             scale = own;
@@ -657,28 +658,35 @@ __global__ __launch_bounds__(NT, (Corr2Waves<T, M>::value)) void corr2_kernel(co
         // rho = c * scale, paired stores.  Flagged windows (scale -1) are
         // overwritten below by the exact fix-up; for the others
         // |rho| <= 1 + O(eps), so the reference's tooBig rule (|rho| > 1.1
-        // -> 0) cannot fire and is not evaluated.
-        gptr<T> outRow = to_global(static_cast<T *>(row.out)) + base;
-        gptr<T> outl = outRow + 2 * lane;
-        const T *scl = scale + 2 * lane;
+        // -> 0) cannot fire and is not evaluated.  The scale reads stay in
+        // bounds for every position (both scale buffers hold >= N entries).
+        // The store resource ends at the last whole pair of valid outputs,
+        // so invalid pairs are dropped; an odd last output is stored apart.
+        using V2 = typename Vec2<T>::type;
         constexpr int B3 = M / (P::R3 * 64);
+        constexpr int NQ = B3 * P::R3;
+        T *outRow = static_cast<T *>(row.out) + base;
+        const __amdgpu_buffer_rsrc_t rs = buffer_rsrc(outRow, static_cast<int>(sizeof(T)) * (nvRow & ~1));
+        const T *scl = scale + 2 * lane;
+        V2 f[NQ];
 #pragma unroll
-        for (int b = 0; b < B3; ++b) {
+        for (int q = 0; q < NQ; ++q) {
+            f[q] = *reinterpret_cast<const V2 *>(scl + 2 * (64 * (q / P::R3) + (q % P::R3) * (M / P::R3)));
+        }
 #pragma unroll
-            for (int rr = 0; rr < P::R3; ++rr) {
-                const int off = 2 * (64 * b + rr * (M / P::R3));
-                const int i0 = 2 * lane + off;
-                const Cx<T> c = v[b * P::R3 + rr];
-                if (i0 + 1 < nvRow) {
-                    const typename Vec2<T>::type f = *reinterpret_cast<const typename Vec2<T>::type *>(scl + off);
-                    outl[off] = c.x * f.x;
-                    outl[off + 1] = c.y * f.y;
-                } else if (i0 < nvRow) {
-                    outl[off] = c.x * scl[off];
-                }
+        for (int q = 0; q < NQ; ++q) {
+            const int i0 = 2 * lane + 2 * (64 * (q / P::R3) + (q % P::R3) * (M / P::R3));
+            buffer_store_pair(v[q].x * f[q].x, v[q].y * f[q].y, rs, static_cast<int>(sizeof(T)) * i0);
+        }
+        if (nvRow & 1) {
+            gptr<T> outG = to_global(outRow);
+#pragma unroll
+            for (int q = 0; q < NQ; ++q) {
+                const int i0 = 2 * lane + 2 * (64 * (q / P::R3) + (q % P::R3) * (M / P::R3));
+                if (i0 == nvRow - 1) { outG[i0] = v[q].x * f[q].x; }
             }
         }
-        if (anyFlag) { fixup_row_wave<T>(raw, scale, to_global(row.tmpl), row.Lt, outRow, nvRow, lane); }
+        if (anyFlag) { fixup_row_wave<T>(raw, scale, to_global(row.tmpl), row.Lt, to_global(outRow), nvRow, lane); }
         if (sharedScale == nullptr) { wave_lds_sync(); }  // the next row reuses buf
     };
 
@@ -706,9 +714,17 @@ __global__ __launch_bounds__(NT, (Corr2Waves<T, M>::value)) void corr2_kernel(co
                 myFlags = compute_scale<T>(raw, ts1, ts2, tp1, tp2, sc, a, nValidSeg, Lt, noise, tid, NT);
             }
             const bool anyFlag = block_sum<NT>(static_cast<double>(myFlags), red) > 0.0;
-            if (MF_PREFETCH_H && g0 + wave < g1) { loadH(rows[g0 + wave].H); }
-            for (int r = g0 + wave; r < g1; r += NWAVE) {
-                processRow(rows[r], sc, anyFlag, r + NWAVE < g1 ? rows[r + NWAVE].H : nullptr);
+            int r = g0 + wave;
+            if (r < g1) {
+                loadH(rows[r].H);
+                startRow(rows[r]);
+                for (int rn = r + NWAVE; rn < g1; rn += NWAVE) {
+                    loadH(rows[rn].H);
+                    finishRow(rows[r], sc, anyFlag);
+                    startRow(rows[rn]);
+                    r = rn;
+                }
+                finishRow(rows[r], sc, anyFlag);
             }
             __syncthreads();  // sc is rebuilt for the next group
         }
@@ -718,20 +734,36 @@ __global__ __launch_bounds__(NT, (Corr2Waves<T, M>::value)) void corr2_kernel(co
         if (wave == 0) { forwardFft(); }
         __syncthreads();
     }
-    int k = 0;
-    for (int g0 = sg.rowBegin; g0 < sg.rowEnd;) {
-        const int Lt = rows[g0].Lt;
-        int g1 = g0 + 1;
-        while (g1 < sg.rowEnd && rows[g1].Lt == Lt) { ++g1; }
-        if (g1 - g0 < NWAVE) {
-            for (int r = g0; r < g1; ++r, ++k) {
-                if ((k & (NWAVE - 1)) == wave) {
-                    if (MF_PREFETCH_H) { loadH(rows[r].H); }
-                    processRow(rows[r], nullptr, false, nullptr);
-                }
+    // Solo rows of this wave in order: the k-th solo row of the segment
+    // belongs to wave k mod NWAVE.  nextSolo walks the groups (scalar
+    // work) and returns the wave's next solo row, or -1.
+    int sg0 = sg.rowBegin, sg1 = sg.rowBegin, sr = sg.rowBegin, sk = 0;
+    auto nextSolo = [&]() -> int {
+        while (true) {
+            if (sr >= sg1) {
+                sg0 = sg1;
+                if (sg0 >= sg.rowEnd) { return -1; }
+                const int Lt = rows[sg0].Lt;
+                sg1 = sg0 + 1;
+                while (sg1 < sg.rowEnd && rows[sg1].Lt == Lt) { ++sg1; }
+                sr = (sg1 - sg0 >= NWAVE) ? sg1 : sg0;
+                continue;
             }
+            const int r = sr++;
+            if (((sk++) & (NWAVE - 1)) == wave) { return r; }
         }
-        g0 = g1;
+    };
+    int cur = nextSolo();
+    if (cur >= 0) {
+        loadH(rows[cur].H);
+        startRow(rows[cur]);
+        for (int nx = nextSolo(); nx >= 0; nx = nextSolo()) {
+            loadH(rows[nx].H);
+            finishRow(rows[cur], nullptr, false);
+            startRow(rows[nx]);
+            cur = nx;
+        }
+        finishRow(rows[cur], nullptr, false);
     }
 }
 

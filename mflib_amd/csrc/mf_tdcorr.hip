@@ -104,7 +104,7 @@ __global__ __launch_bounds__(NT, 2) void tdcorr_kernel(const TdItem *__restrict_
     const TdItem item = items[blockIdx.x];
     const Seg sg = segs[item.seg];
     const int tid = threadIdx.x;
-    const int wave = tid >> 6;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);  // wave-uniform: scalar row loads and branches
     const int lane = tid & 63;
     const int half = lane >> 5;
     const int col = lane & 31;
