@@ -475,7 +475,9 @@ __global__ __launch_bounds__(NT, (Corr2Waves<T, M>::value)) void corr2_kernel(co
     constexpr int SPT = N / NT;
     constexpr int LOGN = Log2<N>::value;
     constexpr int Q = M / 64;
-    constexpr int KP = (M / 2 + 1 + 63) / 64;  // (k, M-k) pair steps per lane
+    // (k, M-k) pair steps per lane for k < M/2; the self-paired bin k = M/2
+    // has a closed form (its twiddle is exactly -i) and is done apart
+    constexpr int KP = M / 2 / 64;
     constexpr int NWAVE = NT / 64;
     using P = WavePlan<M>;
     static_assert(N % NT == 0 && NWAVE == 4, "block shape");
@@ -572,21 +574,24 @@ __global__ __launch_bounds__(NT, (Corr2Waves<T, M>::value)) void corr2_kernel(co
             for (int r = 0; r < P::R3; ++r) { po[cpad(64 * b + r * (M / P::R3))] = v[b * P::R3 + r]; }
         }
         wave_lds_sync();
-        // unpack pairs (k, M-k), k = lane + 64 t
+        // unpack pairs (k, M-k), k = lane + 64 t < M/2
         const Cx<T> *pm = buf + xpad(M - lane);
         const Cx<T> wlo = twlo[lane];
 #pragma unroll
         for (int t = 0; t < KP; ++t) {
             const int k = lane + 64 * t;
-            if (k <= M / 2) {
-                const Cx<T> zk = po[68 * t];
-                const Cx<T> zm = cconj(k == 0 ? po[0] : pm[-68 * t]);
-                const Cx<T> E = cadd(zk, zm);
-                const Cx<T> O = cmulmi(csub(zk, zm));
-                const Cx<T> WO = cmul(cmul(wlo, twhi[t]), O);
-                Xs[k] = cadd(E, WO);
-                Xs[M - k] = cconj(csub(E, WO));
-            }
+            const Cx<T> zk = po[68 * t];
+            const Cx<T> zm = cconj(k == 0 ? po[0] : pm[-68 * t]);
+            const Cx<T> E = cadd(zk, zm);
+            const Cx<T> O = cmulmi(csub(zk, zm));
+            const Cx<T> WO = cmul(cmul(wlo, twhi[t]), O);
+            Xs[k] = cadd(E, WO);
+            Xs[M - k] = cconj(csub(E, WO));
+        }
+        // k = M/2: E = 2 Re z, O = 2 Im z, twiddle -i, so X' = 2 conj(z)
+        if (lane == 0) {
+            const Cx<T> z = buf[xpad(M / 2)];
+            Xs[M / 2] = Cx<T>{z.x + z.x, -(z.y + z.y)};
         }
         wave_lds_sync();
     };
@@ -596,26 +601,31 @@ __global__ __launch_bounds__(NT, (Corr2Waves<T, M>::value)) void corr2_kernel(co
     // stays in v).  finishRow: the row's scale when it has no shared one,
     // then the normalisation epilogue straight from the pass-3 registers.
     // A wave's row sequence runs start(r0), then for each next row r:
-    // load H(r), finish(previous), start(r), and finally finish(last).
-    // H(r) is thus in flight during the previous row's epilogue, and the
+    // finish(previous), start(r), and finally finish(last).  start(r)
+    // loads the spectrum of the row after r once its product is done, so
+    // the load is in flight through r's transform and epilogue; the
     // epilogue issues the same 16 (range-checked) stores on every path, so
-    // the wait for H(r) never waits for those stores.
-    Cx<T> hk[KP], hm[KP];  // template spectrum pairs (H[k], H[M-k])
+    // the wait for that spectrum never waits for those stores.
+    Cx<T> hk[KP], hm[KP];  // template spectrum pairs (H[k], H[M-k]), k < M/2
+    Cx<T> hmid;            // H[M/2]
     auto loadH = [&](const void *Hp) {
         const gptr<const Cx<T>> H = to_global(static_cast<const Cx<T> *>(Hp));
 #pragma unroll
         for (int t = 0; t < KP; ++t) {
-            // lanes past M/2 in the last step load a duplicate they never use
-            const int k = min(lane + 64 * t, M / 2);
+            const int k = lane + 64 * t;
             hk[t] = load_cx(H, k);
             hm[t] = load_cx(H, M - k);
         }
+        hmid = load_cx(H, M / 2);
     };
     auto rowValid = [&](const Row &row) {
         return static_cast<int>(max(0ll, min(static_cast<long long>(sg.L), row.nOut - base)));
     };
-    auto startRow = [&](const Row &row) {
-        if (rowValid(row) == 0) { return; }
+    auto startRow = [&](const Row &row, const void *nextH) {
+        if (rowValid(row) == 0) {
+            loadH(nextH);
+            return;
+        }
         Cx<T> *buf = xbuf[wave];
         // (k, M-k) pair per lane step (k = lane + 64 t; tw(k) = lo[lane] hi[t])
         const Cx<T> wlo = twlo[lane];
@@ -624,16 +634,22 @@ __global__ __launch_bounds__(NT, (Corr2Waves<T, M>::value)) void corr2_kernel(co
 #pragma unroll
         for (int t = 0; t < KP; ++t) {
             const int k = lane + 64 * t;
-            if (k <= M / 2) {
-                const Cx<T> yk = cmul(Xs[k], hk[t]);
-                const Cx<T> ym = cmul(Xs[M - k], hm[t]);
-                const Cx<T> S = cadd(yk, cconj(ym));
-                const Cx<T> D = csub(yk, cconj(ym));
-                const Cx<T> Pp = cmul(cconj(cmul(wlo, twhi[t])), D);
-                pk[68 * t] = cadd(S, cmuli(Pp));
-                if (k > 0) { pm[-68 * t] = cadd(cconj(S), cmuli(cconj(Pp))); }
-            }
+            const Cx<T> yk = cmul(Xs[k], hk[t]);
+            const Cx<T> ym = cmul(Xs[M - k], hm[t]);
+            const Cx<T> S = cadd(yk, cconj(ym));
+            const Cx<T> D = csub(yk, cconj(ym));
+            const Cx<T> Pp = cmul(cconj(cmul(wlo, twhi[t])), D);
+            pk[68 * t] = cadd(S, cmuli(Pp));
+            if (k > 0) { pm[-68 * t] = cadd(cconj(S), cmuli(cconj(Pp))); }
         }
+        // k = M/2 (y = X H): S = 2 Re y, D = 2i Im y, twiddle -i, so Z = 2 conj(y)
+        if (lane == 0) {
+            const Cx<T> y = cmul(Xs[M / 2], hmid);
+            buf[xpad(M / 2)] = Cx<T>{y.x + y.x, -(y.y + y.y)};
+        }
+        // the next row's spectrum is in flight through this row's transform
+        // and epilogue
+        loadH(nextH);
         wave_lds_sync();
         wave_load<T, M, P::R1>(v, buf, lane);
         wave_lds_sync();
@@ -717,11 +733,10 @@ __global__ __launch_bounds__(NT, (Corr2Waves<T, M>::value)) void corr2_kernel(co
             int r = g0 + wave;
             if (r < g1) {
                 loadH(rows[r].H);
-                startRow(rows[r]);
+                startRow(rows[r], rows[r + NWAVE < g1 ? r + NWAVE : r].H);
                 for (int rn = r + NWAVE; rn < g1; rn += NWAVE) {
-                    loadH(rows[rn].H);
                     finishRow(rows[r], sc, anyFlag);
-                    startRow(rows[rn]);
+                    startRow(rows[rn], rows[rn + NWAVE < g1 ? rn + NWAVE : rn].H);
                     r = rn;
                 }
                 finishRow(rows[r], sc, anyFlag);
@@ -756,12 +771,14 @@ __global__ __launch_bounds__(NT, (Corr2Waves<T, M>::value)) void corr2_kernel(co
     int cur = nextSolo();
     if (cur >= 0) {
         loadH(rows[cur].H);
-        startRow(rows[cur]);
-        for (int nx = nextSolo(); nx >= 0; nx = nextSolo()) {
-            loadH(rows[nx].H);
+        int nx = nextSolo();
+        startRow(rows[cur], rows[nx >= 0 ? nx : cur].H);
+        while (nx >= 0) {
+            const int nn = nextSolo();
             finishRow(rows[cur], nullptr, false);
-            startRow(rows[nx]);
+            startRow(rows[nx], rows[nn >= 0 ? nn : nx].H);
             cur = nx;
+            nx = nn;
         }
         finishRow(rows[cur], nullptr, false);
     }
