@@ -166,6 +166,17 @@ int mf_peaks(const void *x, int n, int precision, double threshold, int minDist,
 int mf_peaks_device(const void *dx, int n, int precision, double threshold,
                     int minDist, int *idx, int cap, int *nPeaks);
 
+/* No reference counterpart (precision study of normalizeSignal,
+ * src/matchedFilter.cpp:94-166): the Pearson scale
+ * sqrt(Lt) / sqrt(Lt S2 - S1^2) of each of the n - Lt + 1 windows of a
+ * device-resident FP32 signal dx, from prefix scans of the samples built
+ * with wavefront shuffle scans: FP64 scans (scanPrecision 0) or
+ * compensated FP32 (float-float) scans (scanPrecision 1).  Windows whose
+ * bracket is not positive get 0.  dScale (device) receives n - Lt + 1
+ * floats; *ms (may be NULL) the kernel time.  2 <= Lt <= 2048.  Blocks
+ * until done. */
+int mf_window_scale_device(const void *dx, long long n, int Lt, int scanPrecision, void *dScale, double *ms);
+
 /* computeOptimalFFTAndBlockLength(nb, nx)
  * (include/mflib/matchedFilterParameters.hpp:208, src/matchedFilterParameters.cpp:371-385). */
 int mf_compute_optimal_fft_and_block_length(int nb, int nx, int *nfft, int *blockLength);

@@ -71,6 +71,7 @@ _SIGNATURES = {
     "mf_peaks_device": ([_vp, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_int, _ip,
                          ctypes.c_int, _ip], ctypes.c_int),
     "mf_compute_optimal_fft_and_block_length": ([ctypes.c_int, ctypes.c_int, _ip, _ip], ctypes.c_int),
+    "mf_window_scale_device": ([_vp, ctypes.c_longlong, ctypes.c_int, ctypes.c_int, _vp, _dp], ctypes.c_int),
     "mf_get_timings": ([_vp, _dp, _dp], ctypes.c_int),
     "mf_autotune": ([ctypes.c_int, _dp, ctypes.c_int, _ip], ctypes.c_int),
     "mf_server_create": ([ctypes.POINTER(_vp), ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_int],
@@ -164,6 +165,19 @@ def peaks_device(ptr: int, n: int, dtype, threshold: float, min_dist: int, cap: 
                                 int(min_dist), idx.ctypes.data_as(_ip), cap, ctypes.byref(cnt)),
           "peaks_device")
     return idx[:cnt.value].copy()
+
+
+SCAN_PRECISION = {"f64": 0, "f32c": 1}
+
+
+def window_scale_device(ptr: int, n: int, lt: int, scan: str, out_ptr: int) -> float:
+    """Pearson window scales of a device FP32 signal into a device buffer
+    (n - lt + 1 floats) from FP64 ("f64") or compensated FP32 ("f32c")
+    prefix scans; returns the kernel time in ms."""
+    ms = ctypes.c_double()
+    check(lib().mf_window_scale_device(_vp(ptr), int(n), int(lt), SCAN_PRECISION[scan], _vp(out_ptr),
+                                       ctypes.byref(ms)), "window_scale_device")
+    return ms.value
 
 
 class Handle:

@@ -1450,6 +1450,45 @@ int mf_peaks_device(const void *dx, int n, int precision, double threshold, int 
     return guard([&] { peaks(dx, true, n, precision, threshold, minDist, idx, cap, nPeaks); });
 }
 
+int mf_window_scale_device(const void *dx, long long n, int Lt, int scanPrecision, void *dScale, double *ms)
+{
+    return guard([&] {
+        if (dx == nullptr || dScale == nullptr) { fail(MF_ERR_INVALID_ARGUMENT, "device pointer is NULL"); }
+        if (scanPrecision != mfk::kScanF64 && scanPrecision != mfk::kScanF32c) {
+            fail(MF_ERR_INVALID_ARGUMENT, "scanPrecision must be 0 (FP64) or 1 (compensated FP32)");
+        }
+        if (Lt < 2 || Lt > mfk::kScanMaxLt) {
+            fail(MF_ERR_INVALID_ARGUMENT, "template length must be in [2, " + std::to_string(mfk::kScanMaxLt) + "]");
+        }
+        if (n < Lt) { fail(MF_ERR_INVALID_ARGUMENT, "signal is shorter than the template"); }
+        requireDevice();
+        hipStream_t st = nullptr;
+        MF_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        auto cleanup = [&] {
+            if (e0) { hipEventDestroy(e0); }
+            if (e1) { hipEventDestroy(e1); }
+            hipStreamDestroy(st);
+        };
+        try {
+            MF_HIP(hipEventCreate(&e0));
+            MF_HIP(hipEventCreate(&e1));
+            MF_HIP(hipEventRecord(e0, st));
+            MF_HIP(mfk::launch_scan_scale(scanPrecision, static_cast<const float *>(dx), n, Lt,
+                                          static_cast<float *>(dScale), st));
+            MF_HIP(hipEventRecord(e1, st));
+            MF_HIP(hipEventSynchronize(e1));
+            float t = 0;
+            MF_HIP(hipEventElapsedTime(&t, e0, e1));
+            if (ms) { *ms = t; }
+        } catch (...) {
+            cleanup();
+            throw;
+        }
+        cleanup();
+    });
+}
+
 int mf_compute_optimal_fft_and_block_length(int nb, int nx, int *nfft, int *blockLength)
 {
     return guard([&] {

@@ -140,4 +140,13 @@ hipError_t launch_seg_dead(int precision, Seg *dSegs, int nSegs, long long maxN,
 hipError_t launch_minmax(int precision, const void *dx, long long n, double *dPartMin,
                          double *dPartMax, int nBlocks, hipStream_t stream);
 
+// Window scales sqrt(Lt)/sqrt(Lt S2 - S1^2) of the n - Lt + 1 windows of a
+// FP32 signal from prefix scans: FP64 scans (kScanF64) or compensated
+// FP32 (float-float) scans (kScanF32c).  2 <= Lt <= kScanMaxLt.
+constexpr int kScanF64 = 0;
+constexpr int kScanF32c = 1;
+constexpr int kScanChunk = 2048;   // outputs per block
+constexpr int kScanMaxLt = 2048;
+hipError_t launch_scan_scale(int mode, const float *dx, long long n, int Lt, float *dScale, hipStream_t stream);
+
 }  // namespace mfk

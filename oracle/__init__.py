@@ -48,6 +48,7 @@ def lib():
                                              ctypes.c_int, _llp, _dp]
         L.orc_pearson_oracle_f32x.argtypes = [ctypes.c_int, _dp, ctypes.c_int, _fp,
                                               ctypes.c_int, _llp, _dp]
+        L.orc_window_scale_f32x.argtypes = [ctypes.c_int, ctypes.c_int, _fp, ctypes.c_int, _llp, _dp]
         L.orc_mf_apply_f64.argtypes = [ctypes.c_int, ctypes.c_int, _ip,
                                        ctypes.POINTER(_dp), ctypes.c_int,
                                        ctypes.POINTER(_dp), ctypes.c_int, _dp, _ip, _ip]
@@ -129,6 +130,21 @@ def pearson(b, x, lags=None):
         out = np.empty(len(lags))
         _check(fn(len(b), _ptr(b, _dp), len(x), xp, len(lags), _ptr(lags, _llp),
                   _ptr(out, _dp)), "pearson")
+    return out
+
+
+def window_scale(x, nb: int, lags=None):
+    """FP64 two-pass 1/||w - mean(w)|| of the windows x[i:i+nb] of a
+    float32 signal (the Pearson scale of normalizeSignal); optional lags."""
+    x = np.ascontiguousarray(x, dtype=np.float32)
+    if lags is None:
+        out = np.empty(len(x) - nb + 1)
+        _check(lib().orc_window_scale_f32x(nb, len(x), _ptr(x, _fp), 0, None, _ptr(out, _dp)), "window_scale")
+    else:
+        lags = np.ascontiguousarray(lags, dtype=np.int64)
+        out = np.empty(len(lags))
+        _check(lib().orc_window_scale_f32x(nb, len(x), _ptr(x, _fp), len(lags), _ptr(lags, _llp),
+                                           _ptr(out, _dp)), "window_scale")
     return out
 
 

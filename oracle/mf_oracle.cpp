@@ -480,9 +480,49 @@ int pearsonOracle(int nb, const double *b, int nx, const X *x, int nLags,
     return 0;
 }
 
+// 1 / ||w - mean(w)|| of the window w = x[i .. i + nb) (the Pearson scale
+// sqrt(nb) / sqrt(nb S2 - S1^2) of src/matchedFilter.cpp:94-166, whose
+// bracket equals nb * sum (w - mean)^2), two-pass with compensated sums;
+// 0 for a constant window.
+template <typename X>
+double windowScale(int nb, const X *w)
+{
+    Neumaier sm;
+    for (int j = 0; j < nb; ++j) { sm.add(static_cast<double>(w[j])); }
+    const double mean = sm.value() / nb;
+    Neumaier e2;
+    for (int j = 0; j < nb; ++j) {
+        const double d = static_cast<double>(w[j]) - mean;
+        e2.add(d * d);
+    }
+    const double e = e2.value();
+    return e > 0.0 ? 1.0 / std::sqrt(e) : 0.0;
+}
+
 }  // namespace
 
 extern "C" {
+
+int orc_window_scale_f32x(int nb, int nx, const float *x, int nLags, const long long *lags, double *out)
+{
+    if (nb < 1 || nx < nb || x == nullptr || out == nullptr) { return -1; }
+    const long long nOut = static_cast<long long>(nx) - nb + 1;
+    if (lags == nullptr) {
+#ifdef _OPENMP
+#pragma omp parallel for schedule(static)
+#endif
+        for (long long i = 0; i < nOut; ++i) { out[i] = windowScale<float>(nb, x + i); }
+        return 0;
+    }
+    for (int q = 0; q < nLags; ++q) {
+        if (lags[q] < 0 || lags[q] >= nOut) { return -1; }
+    }
+#ifdef _OPENMP
+#pragma omp parallel for schedule(static)
+#endif
+    for (int q = 0; q < nLags; ++q) { out[q] = windowScale<float>(nb, x + lags[q]); }
+    return 0;
+}
 
 int orc_next_pow2(int n) { return nextPow2(n); }
 

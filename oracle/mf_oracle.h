@@ -46,6 +46,14 @@ int orc_pearson_oracle_f64(int nb, const double *b, int nx, const double *x,
 int orc_pearson_oracle_f32x(int nb, const double *b, int nx, const float *x,
                             int nLags, const long long *lags, double *xc);
 
+/* The Pearson window scale 1 / ||w - mean(w)|| (= sqrt(nb) /
+ * sqrt(nb S2 - S1^2) of normalizeSignal, src/matchedFilter.cpp:94-166) of
+ * the windows x[i .. i + nb) of a float32 signal, FP64 two-pass with
+ * compensated sums; 0 for a constant window.  At the nLags lags in lags[]
+ * (or every lag 0..nx-nb when lags == NULL). */
+int orc_window_scale_f32x(int nb, int nx, const float *x, int nLags, const long long *lags,
+                          double *out);
+
 /* Reference-faithful restatement of MatchedFilter<T>::initialize + apply
  * (src/matchedFilter.cpp:506-821, 971-1316).
  *   mode       0 = SINGLE_CHANNEL, 1 = MULTI_CHANNEL (enums.hpp:14-21)

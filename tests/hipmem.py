@@ -39,6 +39,14 @@ class DeviceArray:
         if rc != 0:
             raise RuntimeError(f"hipMemcpy failed ({rc})")
 
+    def get(self) -> np.ndarray:
+        """Copy back to a new host array."""
+        out = np.empty(self.size, dtype=self.dtype)
+        rc = hip().hipMemcpy(out.ctypes.data, self.ptr, self.nbytes, 2)  # device to host
+        if rc != 0:
+            raise RuntimeError(f"hipMemcpy failed ({rc})")
+        return out
+
     def free(self):
         if self.ptr:
             hip().hipFree(self.ptr)

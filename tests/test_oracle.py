@@ -101,6 +101,43 @@ def test_pearson_constant_window_is_zero():
     assert np.all(r[200:450] == 0.0)
 
 
+def test_window_scale_closed_forms():
+    """1/||w - mean|| pinned by closed forms: a ramp window has
+    sum (w - mean)^2 = nb (nb^2 - 1) / 12 times the slope squared, a
+    constant window has scale 0, and the scale is invariant to offsets and
+    scales as 1/|c| under x -> c x."""
+    nb = 37
+    x = (np.arange(500, dtype=np.float32) * 0.25).astype(np.float32) + 3.0
+    s = O.window_scale(x, nb)
+    want = 1.0 / np.sqrt(nb * (nb * nb - 1) / 12.0 * 0.25 ** 2)
+    assert np.abs(s / want - 1).max() < 1e-13
+    c = np.concatenate([np.zeros(100), np.full(200, 7.0), np.zeros(50)]).astype(np.float32)
+    sc = O.window_scale(c, nb)
+    assert np.all(sc[100:300 - nb + 1] == 0.0)
+    rng = np.random.default_rng(9)
+    y = rng.standard_normal(400).astype(np.float32)
+    s1 = O.window_scale(y, nb)
+    s2 = O.window_scale((y * np.float32(4.0)).astype(np.float32), nb)   # exact power-of-two scaling
+    assert np.abs(s1 / (4.0 * s2) - 1).max() < 1e-14
+    lags = np.array([0, 17, 363])
+    assert np.array_equal(O.window_scale(y, nb, lags), s1[lags])
+
+
+def test_window_scale_agrees_with_pearson_oracle():
+    """rho = <t~, w - mean(w)> * scale with t~ the demeaned unit-norm
+    template must equal the (separately pinned) Pearson oracle."""
+    rng = np.random.default_rng(10)
+    nb = 50
+    x = (rng.standard_normal(3000) * 2.0 + 40.0).astype(np.float32)
+    b = rng.standard_normal(nb)
+    t = b - b.mean()
+    t /= np.linalg.norm(t)
+    w = np.lib.stride_tricks.sliding_window_view(x.astype(np.float64), nb)
+    num = (w - w.mean(axis=1, keepdims=True)) @ t
+    rho = num * O.window_scale(x, nb)
+    assert np.abs(rho - O.pearson(b, x)).max() < 1e-12
+
+
 def test_demean_normalize_reverse_closed_form():
     b = np.array([1.0, 2.0, 4.0, 8.0])
     d = b - b.mean()
