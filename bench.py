@@ -333,24 +333,58 @@ def precision_section(args, torch):
         paths[name] = {"corr_kernel_ms": corr_ms, "corr_samples_per_s": n_corr / (corr_ms * 1e-3),
                        "plan_row0_nfft": plan[0], "rows": rows}
         log(f"precision stress {name}: corr {corr_ms:.2f} ms, {n_corr / (corr_ms * 1e-3) / 1e9:.1f} G corr samples/s")
+    # normalization scans: window scales of every audited (channel, template
+    # length) from FP64 prefix scans and from compensated FP32 scans, kept at
+    # the audit lags; throughput on a 64M-sample signal
+    scans = {}
+    for scan in ("f64", "f32c"):
+        got = {}
+        for c, t in audit_rows:
+            lt = len(wl.templates[t])
+            d_out = torch.empty(n - lt + 1, dtype=torch.float32, device="cuda:0")
+            capi.window_scale_device(dev[c].data_ptr(), n, lt, scan, d_out.data_ptr())
+            got[(c, t)] = d_out.cpu().numpy()[lag_sets[(c, t)]].astype(np.float64)
+        big = torch.randn(1 << 26, dtype=torch.float32, device="cuda:0") + 1.0e4
+        d_big = torch.empty(big.numel() - 255, dtype=torch.float32, device="cuda:0")
+        capi.window_scale_device(big.data_ptr(), big.numel(), 256, scan, d_big.data_ptr())
+        ms = float(np.median([capi.window_scale_device(big.data_ptr(), big.numel(), 256, scan, d_big.data_ptr())
+                              for _ in range(5)]))
+        nbytes = 4.0 * big.numel() + 4.0 * d_big.numel()
+        scans[scan] = {"rows": got, "kernel_ms_64M_windows_Lt256": ms,
+                       "windows_per_s": d_big.numel() / (ms * 1e-3),
+                       "hbm_GBps": nbytes / (ms * 1e-3) / 1e9, "hbm_frac_of_8TBps": nbytes / (ms * 1e-3) / 8e12}
+        del big, d_big
+        log(f"normalization scans {scan}: {ms:.3f} ms for 64M windows, {nbytes / (ms * 1e-3) / 1e9:.0f} GB/s")
     out = {
         "workload": "precision stress: 256 templates (128-256 samples, 2-20 Hz) x 4 channels x 2,000,000 FP32 "
                     "samples, DC 1e4, 20 constant gaps of 500-5000 samples per channel",
         "corr_samples_per_apply": n_corr,
         "paths": {k: {kk: vv for kk, vv in v.items() if kk != "rows"} for k, v in paths.items()},
+        "normalization_scans": {k: {kk: vv for kk, vv in v.items() if kk != "rows"} for k, v in scans.items()},
     }
-    return out, (wl, audit_rows, lag_sets, paths)
+    return out, (wl, audit_rows, lag_sets, paths, scans)
 
 
 def precision_cpu(state):
     """FP64 oracle audit of the precision-stress rows of every path, and
     peak-index agreement against PeakFinder(0.6, 200) on the oracle rows."""
     import oracle as O
-    wl, audit_rows, lag_sets, paths = state
+    wl, audit_rows, lag_sets, paths, scans = state
     res = {name: {"max_abs_drho_vs_fp64": 0.0, "peak_sets_equal": 0, "rows": len(audit_rows)} for name in paths}
+    for scan in scans:
+        res["scan_" + scan] = {"max_rel_scale_err": 0.0, "max_abs_drho_vs_fp64": 0.0, "rows": len(audit_rows),
+                               "note": "rho = exact FP64 numerator x this scale (the scan's own error only)"}
     for k, (c, t) in enumerate(audit_rows):
         lags = lag_sets[(c, t)]
         ref = O.pearson(wl.templates[t], wl.signals[c], lags)
+        sref = O.window_scale(wl.signals[c], len(wl.templates[t]), lags)
+        for scan, p in scans.items():
+            ok = sref > 0
+            rel = np.abs(p["rows"][(c, t)][ok] / sref[ok] - 1.0)
+            r = res["scan_" + scan]
+            r["max_rel_scale_err"] = max(r["max_rel_scale_err"], float(rel.max()) if rel.size else 0.0)
+            r["max_abs_drho_vs_fp64"] = max(r["max_abs_drho_vs_fp64"],
+                                            float((np.abs(ref[ok]) * rel).max()) if rel.size else 0.0)
         full = None
         if k < 2:
             full = O.pearson(wl.templates[t], wl.signals[c])[: wl.signals[c].size - 256 + 1]
@@ -362,7 +396,8 @@ def precision_cpu(state):
             if full is not None:
                 res[name]["peak_sets_equal"] += int(np.array_equal(np.sort(pk), np.sort(ref_pk)))
     for name in res:
-        res[name]["peak_rows_checked"] = 2
+        if name in paths:
+            res[name]["peak_rows_checked"] = 2
         res[name]["audited_samples"] = int(sum(lag_sets[r].size for r in audit_rows))
     return res
 

@@ -145,7 +145,6 @@ hipError_t launch_minmax(int precision, const void *dx, long long n, double *dPa
 // FP32 (float-float) scans (kScanF32c).  2 <= Lt <= kScanMaxLt.
 constexpr int kScanF64 = 0;
 constexpr int kScanF32c = 1;
-constexpr int kScanChunk = 2048;   // outputs per block
 constexpr int kScanMaxLt = 2048;
 hipError_t launch_scan_scale(int mode, const float *dx, long long n, int Lt, float *dScale, hipStream_t stream);
 

@@ -12,11 +12,14 @@
 //     (TwoSum), d^2 as an exact pair (TwoProduct by FMA) plus the cross
 //     term, and every addition of the scans and of the final bracket in
 //     float-float arithmetic (Dekker/Knuth TwoSum and FastTwoSum).
-// One block covers kScanChunk outputs; the span of samples it scans is
-// kScanChunk + Lt - 1 <= kScanSpan.
+// A block covers a chunk of outputs whose span of samples (chunk + Lt - 1)
+// fits one pass of K (8 or 16) consecutive samples per thread: a sequential prefix
+// in registers per thread, a wave shuffle scan of the thread totals and the
+// wave totals in LDS, then the prefix table in LDS for the window sums.
 #include <hip/hip_runtime.h>
 
 #include <cfloat>
+#include <cstdint>
 
 #include "mf_device.hpp"
 #include "mf_kernels.hpp"
@@ -28,8 +31,6 @@ namespace mfk {
 namespace {
 
 constexpr int NT = kThreads;
-constexpr int kScanSpan = kScanChunk + kScanMaxLt - 1;
-constexpr int kScanGroups = (kScanSpan + 63) / 64;  // 64-sample groups in a span
 
 // ---- float-float arithmetic --------------------------------------------
 // No contraction in this file: fusing a product into a later sum (say the
@@ -75,6 +76,17 @@ __device__ __forceinline__ FF ff_mul(FF a, FF b)
     return fast_two_sum(p.h, p.l);
 }
 
+// sqrt(Lt) / sqrt(den2) from the bracket rounded to FP32: hardware rsqrt
+// plus one Newton step (relative error ~2^-23, the output's precision), 0
+// when the bracket is not positive
+__device__ __forceinline__ float scale_of(float den2, int Lt)
+{
+    if (!(den2 > 0.0f)) { return 0.0f; }
+    float y = __builtin_amdgcn_rsqf(den2);
+    y = y * (1.5f - 0.5f * den2 * y * y);
+    return sqrtf(static_cast<float>(Lt)) * y;
+}
+
 // ---- scan element types ---------------------------------------------------
 template <int MODE>
 struct ScanT;
@@ -94,8 +106,7 @@ struct ScanT<kScanF64> {
     // sqrt(Lt) / sqrt(Lt S2 - S1^2), or 0
     __device__ static float scale(V s1, V s2, int Lt)
     {
-        const double den2 = Lt * s2 - s1 * s1;
-        return den2 > 0.0 ? static_cast<float>(sqrt(static_cast<double>(Lt) / den2)) : 0.0f;
+        return scale_of(static_cast<float>(Lt * s2 - s1 * s1), Lt);
     }
 };
 template <>
@@ -117,8 +128,7 @@ struct ScanT<kScanF32c> {
     {
         const FF lt = {static_cast<float>(Lt), 0.0f};
         const FF den2 = ff_add(ff_mul(lt, s2), ff_neg(ff_mul(s1, s1)));
-        const float d = den2.h + den2.l;
-        return d > 0.0f ? sqrtf(static_cast<float>(Lt) / d) : 0.0f;
+        return scale_of(den2.h + den2.l, Lt);
     }
 };
 
@@ -135,21 +145,39 @@ __device__ __forceinline__ typename ScanT<MODE>::V wave_scan(typename ScanT<MODE
     return v;
 }
 
-template <int MODE>
+// Outputs per block for template lengths up to MAXLT: the block scans one
+// span of NT * K samples (K = 8 for MAXLT 256, so 4 blocks fit a CU; 16
+// for MAXLT 2048), so chunk + MAXLT - 1 <= NT * K; the chunk is a multiple
+// of 4 so every block's first sample is 16-byte aligned when x is.
+template <int MAXLT>
+struct ScanChunk {
+    static constexpr int K = MAXLT <= 256 ? 8 : 16;  // consecutive samples per thread
+    static constexpr int value = ((NT * K - MAXLT + 1) / 4) * 4;
+};
+
+// padded LDS index: one spare slot per 16 entries, so the 16 consecutive
+// entries a thread writes sit in distinct banks across a quarter-wave
+__device__ __forceinline__ int spad(int i) { return i + (i >> 4); }
+
+template <int MODE, int MAXLT, bool VEC>
 __global__ __launch_bounds__(NT) void scan_scale_kernel(const float *__restrict__ x, long long n, int Lt,
                                                         float *__restrict__ scale)
 {
     using S = ScanT<MODE>;
     using V = typename S::V;
-    // P1/P2[j] = sum of the first j anchored samples of the span
-    __shared__ V P1[kScanSpan + 1];
-    __shared__ V P2[kScanSpan + 1];
-    __shared__ V g1[kScanGroups];
-    __shared__ V g2[kScanGroups];
+    constexpr int K = ScanChunk<MAXLT>::K;
+    constexpr int SPAN = NT * K;
+    constexpr int CH = ScanChunk<MAXLT>::value;
+    static_assert(CH + MAXLT - 1 <= SPAN, "span");
+    // P1/P2[spad(j)] = sum of the first j anchored samples of the span
+    __shared__ V P1[SPAN + SPAN / 16 + 1];
+    __shared__ V P2[SPAN + SPAN / 16 + 1];
+    __shared__ V w1[NT / 64];
+    __shared__ V w2[NT / 64];
 
     const long long nOut = n - Lt + 1;
-    const long long c0 = static_cast<long long>(blockIdx.x) * kScanChunk;
-    const int nOutBlk = static_cast<int>(min(static_cast<long long>(kScanChunk), nOut - c0));
+    const long long c0 = static_cast<long long>(blockIdx.x) * CH;
+    const int nOutBlk = static_cast<int>(min(static_cast<long long>(CH), nOut - c0));
     const int span = nOutBlk + Lt - 1;
     const int tid = threadIdx.x;
     const int lane = tid & 63;
@@ -157,55 +185,87 @@ __global__ __launch_bounds__(NT) void scan_scale_kernel(const float *__restrict_
     const gptr<const float> xg = to_global(x) + c0;
     const float a = xg[0];
 
-    // 1. per 64-sample group: wave scan of (d, d^2), group totals
-    for (int g = wave; g * 64 < span; g += NT / 64) {
-        const int j = g * 64 + lane;
+    // 1. the thread's K consecutive samples
+    const int j0 = K * tid;
+    float xv[K];
+    if (VEC && j0 + K <= span) {
+        typedef float f4 __attribute__((ext_vector_type(4)));
+        const __attribute__((address_space(1))) f4 *p = (const __attribute__((address_space(1))) f4 *)(xg + j0);
+#pragma unroll
+        for (int m = 0; m < K / 4; ++m) {
+            const f4 v = p[m];
+            xv[4 * m] = v.x;
+            xv[4 * m + 1] = v.y;
+            xv[4 * m + 2] = v.z;
+            xv[4 * m + 3] = v.w;
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < K; ++k) { xv[k] = (j0 + k < span) ? xg[j0 + k] : 0.0f; }
+    }
+    // 2. sequential inclusive prefix over the thread's samples
+    V p1[K], p2[K];
+    V r1 = S::zero(), r2 = S::zero();
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
         V d = S::zero(), d2 = S::zero();
-        if (j < span) { S::terms(xg[j], a, d, d2); }
-        d = wave_scan<MODE>(d, lane);
-        d2 = wave_scan<MODE>(d2, lane);
-        if (j < span) {
-            P1[j + 1] = d;
-            P2[j + 1] = d2;
-        }
-        if (lane == 63) {
-            g1[g] = d;
-            g2[g] = d2;
-        }
+        if (j0 + k < span) { S::terms(xv[k], a, d, d2); }
+        r1 = S::add(r1, d);
+        r2 = S::add(r2, d2);
+        p1[k] = r1;
+        p2[k] = r2;
+    }
+    // 3. exclusive offset of the thread: wave shuffle scan of the thread
+    // totals, then the totals of the waves before this one
+    const V i1 = wave_scan<MODE>(r1, lane), i2 = wave_scan<MODE>(r2, lane);
+    const V e1 = S::shfl_up(i1, 1), e2 = S::shfl_up(i2, 1);
+    if (lane == 63) {
+        w1[wave] = i1;
+        w2[wave] = i2;
+    }
+    __syncthreads();
+    V o1 = S::zero(), o2 = S::zero();
+    for (int w = 0; w < wave; ++w) {
+        o1 = S::add(o1, w1[w]);
+        o2 = S::add(o2, w2[w]);
+    }
+    if (lane > 0) {
+        o1 = S::add(o1, e1);
+        o2 = S::add(o2, e2);
+    }
+    // 4. the block's prefix table
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        P1[spad(j0 + k + 1)] = S::add(o1, p1[k]);
+        P2[spad(j0 + k + 1)] = S::add(o2, p2[k]);
     }
     if (tid == 0) {
         P1[0] = S::zero();
         P2[0] = S::zero();
     }
     __syncthreads();
-    // 2. exclusive scan of the group totals (wave 0, one group per lane)
-    static_assert(kScanGroups <= 64, "group count");
-    if (wave == 0) {
-        const int nG = (span + 63) / 64;
-        const V t1 = lane < nG ? g1[lane] : S::zero();
-        const V t2 = lane < nG ? g2[lane] : S::zero();
-        const V i1 = wave_scan<MODE>(t1, lane), i2 = wave_scan<MODE>(t2, lane);
-        const V e1 = S::shfl_up(i1, 1), e2 = S::shfl_up(i2, 1);
-        if (lane < nG) {
-            g1[lane] = lane > 0 ? e1 : S::zero();
-            g2[lane] = lane > 0 ? e2 : S::zero();
-        }
-    }
-    __syncthreads();
-    // 3. add the group offsets
-    for (int j = tid; j < span; j += NT) {
-        const int g = j >> 6;
-        P1[j + 1] = S::add(g1[g], P1[j + 1]);
-        P2[j + 1] = S::add(g2[g], P2[j + 1]);
-    }
-    __syncthreads();
-    // 4. window sums and scales
+    // 5. window sums and scales (coalesced stores)
     const gptr<float> out = to_global(scale) + c0;
     for (int i = tid; i < nOutBlk; i += NT) {
-        const V s1 = S::sub(P1[i + Lt], P1[i]);
-        const V s2 = S::sub(P2[i + Lt], P2[i]);
+        const V s1 = S::sub(P1[spad(i + Lt)], P1[spad(i)]);
+        const V s2 = S::sub(P2[spad(i + Lt)], P2[spad(i)]);
         out[i] = S::scale(s1, s2, Lt);
     }
+}
+
+template <int MODE, int MAXLT>
+hipError_t scan_launch(const float *dx, long long n, int Lt, float *dScale, hipStream_t stream)
+{
+    constexpr int CH = ScanChunk<MAXLT>::value;
+    const long long grid = (n - Lt + 1 + CH - 1) / CH;
+    if (grid > 0x7fffffffLL) { return hipErrorInvalidValue; }
+    const dim3 g(static_cast<unsigned>(grid));
+    if ((reinterpret_cast<uintptr_t>(dx) & 15u) == 0) {
+        hipLaunchKernelGGL((scan_scale_kernel<MODE, MAXLT, true>), g, dim3(NT), 0, stream, dx, n, Lt, dScale);
+    } else {
+        hipLaunchKernelGGL((scan_scale_kernel<MODE, MAXLT, false>), g, dim3(NT), 0, stream, dx, n, Lt, dScale);
+    }
+    return hipGetLastError();
 }
 
 }  // namespace
@@ -213,19 +273,16 @@ __global__ __launch_bounds__(NT) void scan_scale_kernel(const float *__restrict_
 hipError_t launch_scan_scale(int mode, const float *dx, long long n, int Lt, float *dScale, hipStream_t stream)
 {
     if (Lt < 2 || Lt > kScanMaxLt || n < Lt) { return hipErrorInvalidValue; }
-    const long long nOut = n - Lt + 1;
-    const long long grid = (nOut + kScanChunk - 1) / kScanChunk;
-    if (grid > 0x7fffffffLL) { return hipErrorInvalidValue; }
+    const bool small = Lt <= 256;
     if (mode == kScanF64) {
-        hipLaunchKernelGGL(scan_scale_kernel<kScanF64>, dim3(static_cast<unsigned>(grid)), dim3(NT), 0, stream, dx,
-                           n, Lt, dScale);
-    } else if (mode == kScanF32c) {
-        hipLaunchKernelGGL(scan_scale_kernel<kScanF32c>, dim3(static_cast<unsigned>(grid)), dim3(NT), 0, stream,
-                           dx, n, Lt, dScale);
-    } else {
-        return hipErrorInvalidValue;
+        return small ? scan_launch<kScanF64, 256>(dx, n, Lt, dScale, stream)
+                     : scan_launch<kScanF64, kScanMaxLt>(dx, n, Lt, dScale, stream);
     }
-    return hipGetLastError();
+    if (mode == kScanF32c) {
+        return small ? scan_launch<kScanF32c, 256>(dx, n, Lt, dScale, stream)
+                     : scan_launch<kScanF32c, kScanMaxLt>(dx, n, Lt, dScale, stream);
+    }
+    return hipErrorInvalidValue;
 }
 
 }  // namespace mfk
