@@ -382,6 +382,12 @@ def precision_cpu(state):
             ok = sref > 0
             rel = np.abs(p["rows"][(c, t)][ok] / sref[ok] - 1.0)
             r = res["scan_" + scan]
+            # windows that are exactly constant in FP32 (inside the gaps):
+            # rho is 0 there; a scan that leaves a positive bracket gives a
+            # spurious finite scale, which the fused kernel's flags catch
+            r["constant_windows"] = r.get("constant_windows", 0) + int((~ok).sum())
+            r["constant_windows_with_nonzero_scale"] = (r.get("constant_windows_with_nonzero_scale", 0)
+                                                        + int((p["rows"][(c, t)][~ok] != 0).sum()))
             r["max_rel_scale_err"] = max(r["max_rel_scale_err"], float(rel.max()) if rel.size else 0.0)
             r["max_abs_drho_vs_fp64"] = max(r["max_abs_drho_vs_fp64"],
                                             float((np.abs(ref[ok]) * rel).max()) if rel.size else 0.0)
