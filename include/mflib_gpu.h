@@ -177,6 +177,26 @@ int mf_peaks_device(const void *dx, int n, int precision, double threshold,
  * until done. */
 int mf_window_scale_device(const void *dx, long long n, int Lt, int scanPrecision, void *dScale, double *ms);
 
+/* Detector reduce + clamp: reduceAllCorrelations and cleanReducedTemplate
+ * (src/singleChannel/detector.cpp:190-250), the step of Detector::detect
+ * (detector.cpp:464-477) that feeds PeakFinder with the REDUCED policy.
+ * Over the rows of `group` (group 0 holds every row in single- and
+ * multi-channel mode), det[k] = max over rows of rho[k] (of |rho[k]| when
+ * absoluteValue), clamped to [-1, 1], and id[k] = the template (row) index
+ * of the first row reaching that maximum.  *len receives the group's
+ * filtered length (also when det or id is NULL, as a size query that
+ * returns MF_ERR_INVALID_ARGUMENT); det and id must hold that many
+ * entries (cap). */
+int mf_reduce_group(mf_handle *h, int group, int absoluteValue, void *det, int *id, int cap, int *len);
+/* No reference counterpart: the same into device buffers. */
+int mf_reduce_group_device(mf_handle *h, int group, int absoluteValue, void *dDet, int *dId, int cap, int *len);
+/* Detector::detect's REDUCED policy up to the peak list (detector.cpp:464-477,
+ * 542-552): the reduce above, PeakFinder<T>::apply on det (threshold,
+ * minDist) and, per peak, the template index id[peak].  idx and
+ * templateIds hold cap entries; *nPeaks the count. */
+int mf_detect_reduced(mf_handle *h, int group, int absoluteValue, double threshold, int minDist, int *idx,
+                      int *templateIds, int cap, int *nPeaks);
+
 /* computeOptimalFFTAndBlockLength(nb, nx)
  * (include/mflib/matchedFilterParameters.hpp:208, src/matchedFilterParameters.cpp:371-385). */
 int mf_compute_optimal_fft_and_block_length(int nb, int nx, int *nfft, int *blockLength);

@@ -72,6 +72,10 @@ _SIGNATURES = {
                          ctypes.c_int, _ip], ctypes.c_int),
     "mf_compute_optimal_fft_and_block_length": ([ctypes.c_int, ctypes.c_int, _ip, _ip], ctypes.c_int),
     "mf_window_scale_device": ([_vp, ctypes.c_longlong, ctypes.c_int, ctypes.c_int, _vp, _dp], ctypes.c_int),
+    "mf_reduce_group": ([_vp, ctypes.c_int, ctypes.c_int, _vp, _ip, ctypes.c_int, _ip], ctypes.c_int),
+    "mf_reduce_group_device": ([_vp, ctypes.c_int, ctypes.c_int, _vp, _ip, ctypes.c_int, _ip], ctypes.c_int),
+    "mf_detect_reduced": ([_vp, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_int, _ip, _ip, ctypes.c_int,
+                           _ip], ctypes.c_int),
     "mf_get_timings": ([_vp, _dp, _dp], ctypes.c_int),
     "mf_autotune": ([ctypes.c_int, _dp, ctypes.c_int, _ip], ctypes.c_int),
     "mf_server_create": ([ctypes.POINTER(_vp), ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_int],
@@ -334,6 +338,38 @@ class Handle:
         check(lib().mf_detect(self._h, float(threshold), int(min_dist), idx.ctypes.data_as(_ip), cap,
                               off.ctypes.data_as(_ip)), "detect")
         return [idx[off[g]:off[g + 1]].copy() for g in range(ng)]
+
+    def reduce(self, group: int = 0, absval: bool = False):
+        """Detector reduce + clamp of a group's rows: (det, template id)."""
+        n = ctypes.c_int()
+        rc = lib().mf_reduce_group(self._h, int(group), int(absval), None, None, 0, ctypes.byref(n))
+        if rc != MF_ERR_INVALID_ARGUMENT:
+            check(rc, "reduce")
+        det = np.empty(max(n.value, 1), dtype=self.dtype)
+        ids = np.empty(max(n.value, 1), dtype=np.int32)
+        check(lib().mf_reduce_group(self._h, int(group), int(absval), det.ctypes.data_as(_vp),
+                                    ids.ctypes.data_as(_ip), det.size, ctypes.byref(n)), "reduce")
+        return det[:n.value], ids[:n.value]
+
+    def reduce_device(self, group: int, absval: bool, det_ptr: int, id_ptr: int, cap: int) -> int:
+        n = ctypes.c_int()
+        check(lib().mf_reduce_group_device(self._h, int(group), int(absval), _vp(det_ptr),
+                                           ctypes.cast(_vp(id_ptr), _ip), int(cap), ctypes.byref(n)),
+              "reduce_device")
+        return n.value
+
+    def detect_reduced(self, threshold: float, min_dist: int, group: int = 0, absval: bool = False,
+                       cap: int | None = None):
+        """Detector REDUCED policy up to the peak list: (peak indices,
+        template index of each peak)."""
+        cap = cap or max(1, 1 << 20)
+        idx = np.empty(cap, dtype=np.int32)
+        tid = np.empty(cap, dtype=np.int32)
+        n = ctypes.c_int()
+        check(lib().mf_detect_reduced(self._h, int(group), int(absval), float(threshold), int(min_dist),
+                                      idx.ctypes.data_as(_ip), tid.ctypes.data_as(_ip), cap, ctypes.byref(n)),
+              "detect_reduced")
+        return idx[:n.value].copy(), tid[:n.value].copy()
 
     def stack_device(self, group: int):
         p, n = _vp(), ctypes.c_int()

@@ -98,6 +98,7 @@ struct mf_handle {
     DevBuf dSegs, dRowTab, dItems;
     DevBuf dFrag, dTdRowTab, dTdItems;  // time-domain lanes
     DevBuf dStackRows, dStackOut;
+    DevBuf dReduceRows, dRedDet, dRedId, dRedPk, dRedPkId;  // Detector reduce step
     // stacking groups (built on first stack/detect)
     struct StackInfo {
         int first = 0, count = 0, smax = 0, minShift = 0;
@@ -1390,6 +1391,108 @@ int mf_get_stack_device(mf_handle *h, int group, const void **dptr, int *len)
         }
         if (dptr) { *dptr = stackPtr(h, group); }
         if (len) { *len = static_cast<int>(h->stackInfo[group].len); }
+    });
+}
+
+namespace {
+
+// Detector reduce + clamp of one group's rows into dDet/dId (device),
+// asynchronous on the handle's stream; returns the length.
+long long reduceGroup(mf_handle *h, int group, int absval, void *dDet, int *dId, long long cap)
+{
+    requireInit(h);
+    if (!h->applied) { fail(MF_ERR_BAD_STATE, "Matched filtered signals not yet computed"); }
+    if (group < 0 || group >= static_cast<int>(h->groups.size())) {
+        fail(MF_ERR_INVALID_ARGUMENT, "group " + std::to_string(group) + " out of range");
+    }
+    const auto &gr = h->groups[group];
+    const long long len = gr.nOut;
+    if (cap < len) {
+        fail(MF_ERR_INVALID_ARGUMENT, "output holds " + std::to_string(cap) + " < " + std::to_string(len) + " lags");
+    }
+    std::vector<const void *> ptrs(gr.rows.size());
+    for (size_t k = 0; k < gr.rows.size(); ++k) {
+        ptrs[k] = static_cast<const char *>(h->dRho.p) + h->rows[gr.rows[k]].rhoOff * h->tsize;
+    }
+    growTo(h->dReduceRows, ptrs.size() * sizeof(void *));
+    MF_HIP(hipStreamSynchronize(h->stream));  // the table may be in use by an earlier reduce
+    MF_HIP(hipMemcpy(h->dReduceRows.p, ptrs.data(), ptrs.size() * sizeof(void *), hipMemcpyHostToDevice));
+    MF_HIP(mfk::launch_reduce(h->precision, static_cast<const void *const *>(h->dReduceRows.p),
+                              static_cast<int>(ptrs.size()), len, absval ? 1 : 0, dDet, dId, h->stream));
+    return len;
+}
+
+// group-local row index -> the handle's row (template) index
+void groupRowsToTemplates(const mf_handle *h, int group, int *id, long long n)
+{
+    const auto &rows = h->groups[group].rows;
+    for (long long k = 0; k < n; ++k) { id[k] = rows[id[k]]; }
+}
+
+}  // namespace
+
+int mf_reduce_group(mf_handle *h, int group, int absoluteValue, void *det, int *id, int cap, int *len)
+{
+    return guard([&] {
+        requireInit(h);
+        if (group < 0 || group >= static_cast<int>(h->groups.size())) {
+            fail(MF_ERR_INVALID_ARGUMENT, "group " + std::to_string(group) + " out of range");
+        }
+        const long long need = h->groups[group].nOut;
+        if (len) { *len = static_cast<int>(need); }  // also when the outputs are missing (size query)
+        if (det == nullptr || id == nullptr) { fail(MF_ERR_INVALID_ARGUMENT, "output is NULL"); }
+        growTo(h->dRedDet, static_cast<size_t>(std::max(need, 1LL)) * h->tsize);
+        growTo(h->dRedId, static_cast<size_t>(std::max(need, 1LL)) * sizeof(int));
+        const long long n = reduceGroup(h, group, absoluteValue, h->dRedDet.p, static_cast<int *>(h->dRedId.p), cap);
+        MF_HIP(hipMemcpyAsync(det, h->dRedDet.p, static_cast<size_t>(n) * h->tsize, hipMemcpyDeviceToHost, h->stream));
+        MF_HIP(hipMemcpyAsync(id, h->dRedId.p, static_cast<size_t>(n) * sizeof(int), hipMemcpyDeviceToHost,
+                              h->stream));
+        MF_HIP(hipStreamSynchronize(h->stream));
+        groupRowsToTemplates(h, group, id, n);
+        if (len) { *len = static_cast<int>(n); }
+    });
+}
+
+int mf_reduce_group_device(mf_handle *h, int group, int absoluteValue, void *dDet, int *dId, int cap, int *len)
+{
+    return guard([&] {
+        if (dDet == nullptr || dId == nullptr) { fail(MF_ERR_INVALID_ARGUMENT, "output is NULL"); }
+        const long long n = reduceGroup(h, group, absoluteValue, dDet, dId, cap);
+        MF_HIP(hipStreamSynchronize(h->stream));
+        if (len) { *len = static_cast<int>(n); }
+    });
+}
+
+int mf_detect_reduced(mf_handle *h, int group, int absoluteValue, double threshold, int minDist, int *idx,
+                      int *templateIds, int cap, int *nPeaks)
+{
+    return guard([&] {
+        if (nPeaks == nullptr || ((idx == nullptr || templateIds == nullptr) && cap > 0)) {
+            fail(MF_ERR_INVALID_ARGUMENT, "output is NULL");
+        }
+        requireInit(h);
+        if (group < 0 || group >= static_cast<int>(h->groups.size())) {
+            fail(MF_ERR_INVALID_ARGUMENT, "group " + std::to_string(group) + " out of range");
+        }
+        const long long need = h->groups[group].nOut;
+        growTo(h->dRedDet, static_cast<size_t>(std::max(need, 1LL)) * h->tsize);
+        growTo(h->dRedId, static_cast<size_t>(std::max(need, 1LL)) * sizeof(int));
+        const long long n = reduceGroup(h, group, absoluteValue, h->dRedDet.p, static_cast<int *>(h->dRedId.p), need);
+        MF_HIP(hipStreamSynchronize(h->stream));
+        peaks(h->dRedDet.p, true, static_cast<int>(n), h->precision, threshold, minDist, idx, cap, nPeaks);
+        const int np = *nPeaks;
+        if (np > 0) {
+            growTo(h->dRedPk, static_cast<size_t>(np) * sizeof(int));
+            growTo(h->dRedPkId, static_cast<size_t>(np) * sizeof(int));
+            MF_HIP(hipMemcpyAsync(h->dRedPk.p, idx, static_cast<size_t>(np) * sizeof(int), hipMemcpyHostToDevice,
+                                  h->stream));
+            MF_HIP(mfk::launch_gather_int(static_cast<const int *>(h->dRedId.p), static_cast<const int *>(h->dRedPk.p),
+                                          np, static_cast<int *>(h->dRedPkId.p), h->stream));
+            MF_HIP(hipMemcpyAsync(templateIds, h->dRedPkId.p, static_cast<size_t>(np) * sizeof(int),
+                                  hipMemcpyDeviceToHost, h->stream));
+            MF_HIP(hipStreamSynchronize(h->stream));
+            groupRowsToTemplates(h, group, templateIds, np);
+        }
     });
 }
 

@@ -907,6 +907,60 @@ __global__ __launch_bounds__(NT) void stack_kernel(const StackGroup *__restrict_
     }
 }
 
+// Detector reduce + clamp (src/singleChannel/detector.cpp:190-250): per lag,
+// the maximum over the rows of rho (or of |rho|), the first row reaching
+// it (strict >, rows in order) and the clamp to [-1, 1].  Same chunking as
+// the stack: each block owns kStackChunk lags, a thread U lags NT apart,
+// eight rows' loads in flight.
+template <typename T>
+__global__ __launch_bounds__(NT) void reduce_kernel(const void *const *__restrict__ rowPtrs, int nRows,
+                                                    long long len, int absval, T *__restrict__ det,
+                                                    int *__restrict__ id)
+{
+    constexpr int U = 8;
+    const long long chunk0 = static_cast<long long>(blockIdx.x) * kStackChunk;
+    if (chunk0 >= len) { return; }
+    for (int pass = 0; pass < kStackChunk / (NT * U); ++pass) {
+        const long long k0 = chunk0 + static_cast<long long>(pass) * NT * U + threadIdx.x;
+        if (k0 >= len) { break; }
+        T best[U];
+        int who[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            best[u] = T(0);
+            who[u] = 0;
+        }
+#pragma unroll 8
+        for (int r = 0; r < nRows; ++r) {
+            const gptr<const T> p = to_global(static_cast<const T *>(rowPtrs[r]));
+            T v[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const long long k = k0 + u * NT;
+                v[u] = k < len ? p[k] : T(0);
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const T a = absval ? fabs(v[u]) : v[u];
+                if (r == 0 || a > best[u]) {
+                    best[u] = a;
+                    who[u] = r;
+                }
+            }
+        }
+        const gptr<T> dg = to_global(det);
+        const gptr<int> ig = to_global(id);
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const long long k = k0 + u * NT;
+            if (k < len) {
+                dg[k] = fmax(T(-1), fmin(T(1), best[u]));
+                ig[k] = who[u];
+            }
+        }
+    }
+}
+
 // Candidates of src/peakFinder.cpp:500-521: strictly above threshold and
 // a local maximum (plateaus keep their last sample; end-point rules).
 template <typename T>
@@ -951,6 +1005,14 @@ __device__ __forceinline__ void peak_scan(gptr<const T> x, long long n, T thr, i
             }
         }
     }
+}
+
+// out[i] = src[idx[i]]
+__global__ __launch_bounds__(NT) void gather_int_kernel(const int *__restrict__ src, const int *__restrict__ idx,
+                                                        int n, int *__restrict__ out)
+{
+    const int i = blockIdx.x * NT + threadIdx.x;
+    if (i < n) { out[i] = src[idx[i]]; }
 }
 
 template <typename T>
@@ -1135,6 +1197,30 @@ hipError_t launch_stack_groups(int precision, const StackGroup *dGroups, int nGr
     } else {
         hipLaunchKernelGGL(stack_kernel<double>, dim3(grid, nGroups), dim3(NT), 0, stream, dGroups, absval);
     }
+    return hipGetLastError();
+}
+
+hipError_t launch_reduce(int precision, const void *const *dRowPtrs, int nRows, long long len, int absval,
+                         void *dDet, int *dId, hipStream_t stream)
+{
+    if (len <= 0 || nRows <= 0) { return hipSuccess; }
+    const long long chunks = (len + kStackChunk - 1) / kStackChunk;
+    if (chunks > 0x7fffffffLL) { return hipErrorInvalidValue; }
+    const int grid = static_cast<int>(chunks);
+    if (precision == 0) {
+        hipLaunchKernelGGL(reduce_kernel<float>, dim3(grid), dim3(NT), 0, stream, dRowPtrs, nRows, len, absval,
+                           static_cast<float *>(dDet), dId);
+    } else {
+        hipLaunchKernelGGL(reduce_kernel<double>, dim3(grid), dim3(NT), 0, stream, dRowPtrs, nRows, len, absval,
+                           static_cast<double *>(dDet), dId);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_gather_int(const int *dSrc, const int *dIdx, int n, int *dOut, hipStream_t stream)
+{
+    if (n <= 0) { return hipSuccess; }
+    hipLaunchKernelGGL(gather_int_kernel, dim3((n + NT - 1) / NT), dim3(NT), 0, stream, dSrc, dIdx, n, dOut);
     return hipGetLastError();
 }
 

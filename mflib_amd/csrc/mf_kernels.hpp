@@ -128,6 +128,13 @@ hipError_t launch_stack_groups(int precision, const StackGroup *dGroups, int nGr
                                int absval, hipStream_t stream);
 hipError_t launch_peak_groups(int precision, const StackGroup *dGroups, int nGroups, long long maxLen,
                               double threshold, hipStream_t stream);
+// Detector reduce + clamp over nRows device rows (pointer table on the
+// device): det[k] = clamp(max_r rho_r[k] (or |rho_r[k]|), -1, 1), id[k] = the
+// first r reaching the maximum; k < len.
+hipError_t launch_reduce(int precision, const void *const *dRowPtrs, int nRows, long long len, int absval,
+                         void *dDet, int *dId, hipStream_t stream);
+// out[i] = src[idx[i]], i < n (all device pointers).
+hipError_t launch_gather_int(const int *dSrc, const int *dIdx, int n, int *dOut, hipStream_t stream);
 // Threshold + local-maximum candidates (unordered, deterministic set).
 hipError_t launch_peaks(int precision, const void *dx, long long n, double threshold,
                         int *dCount, int *dIdx, void *dVal, int cap, hipStream_t stream);

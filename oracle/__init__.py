@@ -62,6 +62,10 @@ def lib():
         L.orc_shift_and_stack_f32.argtypes = [ctypes.c_int, _fp, ctypes.c_longlong,
                                               ctypes.c_int, _ip, _dp, ctypes.c_int,
                                               _fp, _ip]
+        L.orc_reduce_correlations_f32.argtypes = [ctypes.c_int, _fp, ctypes.c_longlong, ctypes.c_int,
+                                                  ctypes.c_int, _fp, _ip]
+        L.orc_reduce_correlations_f64.argtypes = [ctypes.c_int, _dp, ctypes.c_longlong, ctypes.c_int,
+                                                  ctypes.c_int, _dp, _ip]
         L.orc_peak_finder_f64.argtypes = [ctypes.c_int, _dp, ctypes.c_double,
                                           ctypes.c_int, _ip, _ip]
         L.orc_peak_finder_f32.argtypes = [ctypes.c_int, _fp, ctypes.c_float,
@@ -194,6 +198,24 @@ def shift_and_stack(rho, shifts, weights, absval: bool = True):
                                            ctypes.byref(ln))
     _check(rc, "shift_and_stack")
     return out[:ln.value]
+
+
+def reduce_correlations(rho, absval: bool = False, length: int | None = None):
+    """Detector reduce + clamp over the rows of rho (2-D, float32 or
+    float64): returns (det, id) of the first `length` lags."""
+    rho = np.ascontiguousarray(rho)
+    if rho.dtype not in (np.float32, np.float64):
+        rho = rho.astype(np.float64)
+    nT, ld = rho.shape
+    n = ld if length is None else int(length)
+    det = np.empty(n, dtype=rho.dtype)
+    idx = np.empty(n, dtype=np.int32)
+    if rho.dtype == np.float32:
+        rc = lib().orc_reduce_correlations_f32(nT, _ptr(rho, _fp), ld, n, int(absval), _ptr(det, _fp), _ptr(idx, _ip))
+    else:
+        rc = lib().orc_reduce_correlations_f64(nT, _ptr(rho, _dp), ld, n, int(absval), _ptr(det, _dp), _ptr(idx, _ip))
+    _check(rc, "reduce_correlations")
+    return det, idx
 
 
 def peak_finder(x, threshold: float, min_dist: int):

@@ -374,6 +374,36 @@ int shiftAndStack(int nT, const T *rho, long long ld, int nOut, const int *shift
 }
 
 /* ---------------------------------------------------------------------- */
+/* Detector reduce + clamp: reduceAllCorrelations and cleanReducedTemplate */
+/* (src/singleChannel/detector.cpp:190-250).  det[i] = max over templates  */
+/* of rho_t[i] (or |rho_t[i]|), the first template reaching it in id[i]     */
+/* (strict >), then det clamped to [-1, 1].  The workspace is calloc'd by  */
+/* the caller (detector.cpp:460-463), so id starts at 0 in both branches.  */
+/* ---------------------------------------------------------------------- */
+template <typename T>
+int reduceCorrelations(int nT, const T *rho, long long ld, int len, int absval, T *det, int *id)
+{
+    if (nT < 1 || len < 0 || rho == nullptr || det == nullptr || id == nullptr) { return -1; }
+    for (int i = 0; i < len; ++i) {
+        det[i] = absval ? std::abs(rho[i]) : rho[i];
+        id[i] = 0;
+    }
+    for (int it = 1; it < nT; ++it) {
+        const T *p = rho + it * ld;
+        for (int i = 0; i < len; ++i) {
+            const T v = absval ? std::abs(p[i]) : p[i];
+            if (v > det[i]) {
+                det[i] = v;
+                id[i] = it;
+            }
+        }
+    }
+    const T one = 1;
+    for (int i = 0; i < len; ++i) { det[i] = std::max(-one, std::min(one, det[i])); }
+    return 0;
+}
+
+/* ---------------------------------------------------------------------- */
 /* PeakFinder<T>::apply: src/peakFinder.cpp:468-586                         */
 /* ---------------------------------------------------------------------- */
 template <typename T>
@@ -591,6 +621,18 @@ int orc_shift_and_stack_f64(int nT, const double *rho, long long ld, int nOut,
                             double *out, int *outLen)
 {
     return shiftAndStack<double>(nT, rho, ld, nOut, shifts, weights, absval, out, outLen);
+}
+
+int orc_reduce_correlations_f32(int nT, const float *rho, long long ld, int len, int absval, float *det,
+                                int *id)
+{
+    return reduceCorrelations<float>(nT, rho, ld, len, absval, det, id);
+}
+
+int orc_reduce_correlations_f64(int nT, const double *rho, long long ld, int len, int absval, double *det,
+                                int *id)
+{
+    return reduceCorrelations<double>(nT, rho, ld, len, absval, det, id);
 }
 
 int orc_shift_and_stack_f32(int nT, const float *rho, long long ld, int nOut,

@@ -82,6 +82,15 @@ int orc_mf_apply_f32(int mode, int nT, const int *lens,
  * nOut samples (row stride ld).  shifts/weights as recorded at
  * src/matchedFilter.cpp:577-592.  out must hold nOut - max(shift) values;
  * the produced length is written to *outLen. */
+/* Detector reduce + clamp (src/singleChannel/detector.cpp:190-250): over
+ * the nT rows of rho (row stride ld), det[i] = max of rho_t[i] (|rho_t[i]|
+ * when absval), id[i] = the first t reaching it, then det clamped to
+ * [-1, 1]; i < len. */
+int orc_reduce_correlations_f32(int nT, const float *rho, long long ld, int len, int absval,
+                                float *det, int *id);
+int orc_reduce_correlations_f64(int nT, const double *rho, long long ld, int len, int absval,
+                                double *det, int *id);
+
 int orc_shift_and_stack_f64(int nT, const double *rho, long long ld,
                             int nOut, const int *shifts,
                             const double *weights, int absval,

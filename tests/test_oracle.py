@@ -224,6 +224,29 @@ def test_shift_and_stack_closed_form():
 
 
 # ---------------------------------------------------------------- peaks
+@pytest.mark.parametrize("absval", [False, True])
+def test_reduce_correlations_brute_force(absval):
+    """Detector reduce: det = clamp(max over templates), id = the first
+    template reaching the max (numpy argmax also returns the first)."""
+    rng = np.random.default_rng(11 + absval)
+    rho = rng.uniform(-1.2, 1.2, (7, 300))
+    rho[3, 10] = rho[1, 10] = 5.0            # tie: the first template wins
+    rho[:, 20] = -0.25                        # all equal: template 0
+    det, idx = O.reduce_correlations(rho, absval)
+    v = np.abs(rho) if absval else rho
+    assert np.array_equal(idx, np.argmax(v, axis=0))
+    assert np.array_equal(det, np.clip(v.max(axis=0), -1.0, 1.0))
+    assert idx[10] == 1 and idx[20] == 0
+    d32, i32 = O.reduce_correlations(rho.astype(np.float32), absval, length=250)
+    assert np.array_equal(i32, np.argmax(v.astype(np.float32)[:, :250], axis=0))
+
+
+def test_reduce_correlations_single_row_is_clamped_copy():
+    x = np.linspace(-2, 2, 41)
+    det, idx = O.reduce_correlations(x[None, :])
+    assert np.array_equal(det, np.clip(x, -1, 1)) and not idx.any()
+
+
 def test_peak_finder_reference_kats():
     """testing/peakFinder.cpp:16-143."""
     sig, thr, md, ref13, ref21 = R.peak_finder_signal()
