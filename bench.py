@@ -328,6 +328,31 @@ def precision_section(args, torch):
             r = h.rho(c * T + t)
             pk = capi.peaks_device(h.rho_device_ptr(c * T + t), r.size, np.float32, 0.6, 200)
             rows[(c, t)] = (r[lag_sets[(c, t)]].astype(np.float64), pk)
+        if name == "fft_fp32":
+            # Detector reduce step over each channel's 256 template rows
+            glen = h.filtered_length(0)
+            d_det = torch.empty(glen, dtype=torch.float32, device="cuda:0")
+            d_id = torch.empty(glen, dtype=torch.int32, device="cuda:0")
+            red_ms = []
+            for rep in range(3):
+                e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                e0.record()
+                for c in range(C):
+                    h.reduce_device(c, True, d_det.data_ptr(), d_id.data_ptr(), glen)
+                e1.record()
+                torch.cuda.synchronize()
+                red_ms.append(e0.elapsed_time(e1))
+            red = float(np.median(red_ms))
+            nred = C * T * glen
+            # audit: the first 20,000 lags of channel 1 (rows T..2T-1), ids are template indices
+            sl = 20_000
+            det1, id1 = h.reduce(1, True)
+            reduce_audit = (np.stack([h.rho(T + t)[:sl] for t in range(T)]), det1[:sl].copy(), id1[:sl] - T)
+            reduce_info = {"ms_all_4_channels": red, "corr_samples_reduced_per_s": nred / (red * 1e-3),
+                           "hbm_GBps": (nred * 4.0 + C * glen * 8.0) / (red * 1e-3) / 1e9,
+                           "hbm_frac_of_8TBps": (nred * 4.0 + C * glen * 8.0) / (red * 1e-3) / 8e12}
+            log(f"detector reduce (256 templates x 4 channels, |max|): {red:.2f} ms, "
+                f"{reduce_info['hbm_GBps']:.0f} GB/s")
         plan = h.plan(0)
         h.close()
         paths[name] = {"corr_kernel_ms": corr_ms, "corr_samples_per_s": n_corr / (corr_ms * 1e-3),
@@ -361,15 +386,16 @@ def precision_section(args, torch):
         "corr_samples_per_apply": n_corr,
         "paths": {k: {kk: vv for kk, vv in v.items() if kk != "rows"} for k, v in paths.items()},
         "normalization_scans": {k: {kk: vv for kk, vv in v.items() if kk != "rows"} for k, v in scans.items()},
+        "detector_reduce": reduce_info,
     }
-    return out, (wl, audit_rows, lag_sets, paths, scans)
+    return out, (wl, audit_rows, lag_sets, paths, scans, reduce_audit)
 
 
 def precision_cpu(state):
     """FP64 oracle audit of the precision-stress rows of every path, and
     peak-index agreement against PeakFinder(0.6, 200) on the oracle rows."""
     import oracle as O
-    wl, audit_rows, lag_sets, paths, scans = state
+    wl, audit_rows, lag_sets, paths, scans, reduce_audit = state
     res = {name: {"max_abs_drho_vs_fp64": 0.0, "peak_sets_equal": 0, "rows": len(audit_rows)} for name in paths}
     for scan in scans:
         res["scan_" + scan] = {"max_rel_scale_err": 0.0, "max_abs_drho_vs_fp64": 0.0, "rows": len(audit_rows),
@@ -401,7 +427,14 @@ def precision_cpu(state):
                                                     float(np.abs(vals - ref).max()))
             if full is not None:
                 res[name]["peak_sets_equal"] += int(np.array_equal(np.sort(pk), np.sort(ref_pk)))
+    rows1, det1, id1 = reduce_audit
+    rdet, rid = O.reduce_correlations(rows1, True)
+    res["detector_reduce"] = {"lags_checked": int(rdet.size), "templates": int(rows1.shape[0]),
+                              "det_bit_identical": bool(np.array_equal(rdet, det1)),
+                              "id_identical": bool(np.array_equal(rid, id1))}
     for name in res:
+        if name == "detector_reduce":
+            continue
         if name in paths:
             res[name]["peak_rows_checked"] = 2
         res[name]["audited_samples"] = int(sum(lag_sets[r].size for r in audit_rows))
