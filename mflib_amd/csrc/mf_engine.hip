@@ -109,6 +109,8 @@ struct mf_handle {
     size_t ldStack = 0;
     size_t candCap = 0;
     DevBuf dStacks, dCounts, dCandIdx, dCandVal, dGroupTab;
+    DevBuf dPackIdx, dPackVal;  // candidates of all groups, packed for one readback
+    PinnedBuf hPack;            // host side of that readback
     std::map<int, TwTables> tw;
     struct Launch {
         int logM, first, count, variant;
@@ -176,6 +178,8 @@ void dropDevice(mf_handle *h)
     h->dCounts.release();
     h->dCandIdx.release();
     h->dCandVal.release();
+    h->dPackIdx.release();
+    h->dPackVal.release();
     h->dGroupTab.release();
     h->dStackOut.release();
     h->tw.clear();
@@ -670,6 +674,8 @@ void prepareStacks(mf_handle *h)
     h->candCap = candCap;
     h->dCandIdx.alloc(candCap * nG * sizeof(int));
     h->dCandVal.alloc(candCap * nG * h->tsize);
+    h->dPackIdx.alloc((candCap + 1) * nG * sizeof(int));
+    h->dPackVal.alloc(candCap * nG * h->tsize);
     std::vector<mfk::StackGroup> tab(nG);
     for (int g = 0; g < nG; ++g) {
         const auto &gi = h->stackInfo[g];
@@ -838,20 +844,48 @@ void detectT(mf_handle *h, double threshold, int minDist, int *idx, int cap, int
     stackGroups(h, -1);
     MF_HIP(mfk::launch_peak_groups(h->precision, h->dGroupTab.as<mfk::StackGroup>(), nG, maxLen, threshold,
                                    h->stream));
-    std::vector<int> counts(nG);
-    MF_HIP(hipMemcpyAsync(counts.data(), h->dCounts.p, nG * sizeof(int), hipMemcpyDeviceToHost, h->stream));
+    MF_HIP(mfk::launch_pack_candidates(h->precision, h->dGroupTab.as<mfk::StackGroup>(), nG, h->dPackIdx.as<int>(),
+                                       h->dPackVal.p, h->stream));
+    // one readback of the counts and the first kPrefix candidates (usually
+    // all of them); a second only when there are more
+    constexpr size_t kPrefix = 8192;
+    const size_t pre = std::min(kPrefix, h->candCap * static_cast<size_t>(nG));
+    const size_t hdrBytes = (static_cast<size_t>(nG) + pre) * sizeof(int);
+    h->hPack.reserve(hdrBytes + pre * sizeof(T));
+    int *hIdx = static_cast<int *>(h->hPack.p);
+    T *hVal = reinterpret_cast<T *>(static_cast<char *>(h->hPack.p) + hdrBytes);
+    MF_HIP(hipMemcpyAsync(hIdx, h->dPackIdx.p, hdrBytes, hipMemcpyDeviceToHost, h->stream));
+    MF_HIP(hipMemcpyAsync(hVal, h->dPackVal.p, pre * sizeof(T), hipMemcpyDeviceToHost, h->stream));
     MF_HIP(hipStreamSynchronize(h->stream));
     float ms = 0;
     if (hipEventElapsedTime(&ms, h->evC, h->evD) == hipSuccess) { h->stackMs = ms; }
+    std::vector<int> counts(nG);
+    std::vector<size_t> first(nG + 1, 0);
+    for (int g = 0; g < nG; ++g) {
+        counts[g] = hIdx[g];
+        if (counts[g] > static_cast<int>(h->candCap)) { fail(MF_ERR_DEVICE, "peak candidate buffer overflow"); }
+        first[g + 1] = first[g] + static_cast<size_t>(counts[g]);
+    }
+    const size_t total = first[nG];
+    std::vector<int> ci(total);
+    std::vector<T> cv(total);
+    std::copy(hIdx + nG, hIdx + nG + std::min(total, pre), ci.begin());
+    std::copy(hVal, hVal + std::min(total, pre), cv.begin());
+    if (total > pre) {
+        MF_HIP(hipMemcpyAsync(ci.data() + pre, h->dPackIdx.as<int>() + nG + pre, (total - pre) * sizeof(int),
+                              hipMemcpyDeviceToHost, h->stream));
+        MF_HIP(hipMemcpyAsync(cv.data() + pre, static_cast<const T *>(h->dPackVal.p) + pre, (total - pre) * sizeof(T),
+                              hipMemcpyDeviceToHost, h->stream));
+        MF_HIP(hipStreamSynchronize(h->stream));
+    }
     int off = 0;
     for (int g = 0; g < nG; ++g) {
-        if (counts[g] > static_cast<int>(h->candCap)) { fail(MF_ERR_DEVICE, "peak candidate buffer overflow"); }
-        if (h->stackInfo[g].len <= 0) { counts[g] = 0; }
         offsets[g] = off;
+        const int cnt = h->stackInfo[g].len <= 0 ? 0 : counts[g];
+        std::vector<std::pair<T, int>> cand(cnt);
+        for (int i = 0; i < cnt; ++i) { cand[i] = {cv[first[g] + i], ci[first[g] + i]}; }
         int np = 0;
-        collectCandidates<T>(h->dCandIdx.as<int>() + static_cast<size_t>(g) * h->candCap,
-                             static_cast<const char *>(h->dCandVal.p) + static_cast<size_t>(g) * h->candCap * h->tsize,
-                             counts[g], minDist, idx + off, cap - off, &np, h->stream);
+        pruneCandidates<T>(cand, minDist, idx + off, cap - off, &np);
         off += np;
     }
     offsets[nG] = off;

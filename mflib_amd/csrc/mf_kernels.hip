@@ -1032,6 +1032,38 @@ __global__ __launch_bounds__(NT) void peak_groups_kernel(const StackGroup *__res
                  to_global(static_cast<T *>(g.candVal)), g.candCap);
 }
 
+// Packs every group's candidates into one array, group by group in order
+// (block g: offset = the counts of groups before g, each capped at its
+// buffer's capacity so the packed writes stay in bounds even when a count
+// overflowed), and copies the raw counts in front: packed[0..nG) = counts,
+// then the indices; values alike.  The host then reads one contiguous
+// prefix (and rejects an overflowed count).
+template <typename T>
+__global__ __launch_bounds__(NT) void pack_candidates_kernel(const StackGroup *__restrict__ groups, int nG,
+                                                             int *__restrict__ packedIdx, T *__restrict__ packedVal)
+{
+    __shared__ long long red[NT / 64];
+    const int g = blockIdx.x;
+    long long off = 0;
+    for (int q = threadIdx.x; q < g; q += NT) { off += min(*groups[q].count, groups[q].candCap); }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) { off += __shfl_xor(off, o, 64); }
+    if ((threadIdx.x & 63) == 0) { red[threadIdx.x >> 6] = off; }
+    __syncthreads();
+    off = 0;
+#pragma unroll
+    for (int w = 0; w < NT / 64; ++w) { off += red[w]; }
+    const StackGroup gr = groups[g];
+    const int cnt = min(*gr.count, gr.candCap);
+    if (threadIdx.x == 0) { packedIdx[g] = *gr.count; }
+    const gptr<const int> si = to_global(gr.candIdx);
+    const gptr<const T> sv = to_global(static_cast<const T *>(gr.candVal));
+    for (int i = threadIdx.x; i < cnt; i += NT) {
+        packedIdx[nG + off + i] = si[i];
+        packedVal[off + i] = sv[i];
+    }
+}
+
 template <typename T>
 __global__ __launch_bounds__(NT) void minmax_kernel(const T *__restrict__ x, long long n,
                                                     double *__restrict__ pmin,
@@ -1213,6 +1245,20 @@ hipError_t launch_reduce(int precision, const void *const *dRowPtrs, int nRows, 
     } else {
         hipLaunchKernelGGL(reduce_kernel<double>, dim3(grid), dim3(NT), 0, stream, dRowPtrs, nRows, len, absval,
                            static_cast<double *>(dDet), dId);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_pack_candidates(int precision, const StackGroup *dGroups, int nGroups, int *dPackedIdx,
+                                  void *dPackedVal, hipStream_t stream)
+{
+    if (nGroups <= 0) { return hipSuccess; }
+    if (precision == 0) {
+        hipLaunchKernelGGL(pack_candidates_kernel<float>, dim3(nGroups), dim3(NT), 0, stream, dGroups, nGroups,
+                           dPackedIdx, static_cast<float *>(dPackedVal));
+    } else {
+        hipLaunchKernelGGL(pack_candidates_kernel<double>, dim3(nGroups), dim3(NT), 0, stream, dGroups, nGroups,
+                           dPackedIdx, static_cast<double *>(dPackedVal));
     }
     return hipGetLastError();
 }

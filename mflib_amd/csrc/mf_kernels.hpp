@@ -133,6 +133,11 @@ hipError_t launch_peak_groups(int precision, const StackGroup *dGroups, int nGro
 // first r reaching the maximum; k < len.
 hipError_t launch_reduce(int precision, const void *const *dRowPtrs, int nRows, long long len, int absval,
                          void *dDet, int *dId, hipStream_t stream);
+// Every group's peak candidates packed group by group: dPackedIdx[0..nG)
+// = the counts, then the indices from dPackedIdx[nG]; values from
+// dPackedVal[0].  Capacity: nG + sum of the groups' candCap.
+hipError_t launch_pack_candidates(int precision, const StackGroup *dGroups, int nGroups, int *dPackedIdx,
+                                  void *dPackedVal, hipStream_t stream);
 // out[i] = src[idx[i]], i < n (all device pointers).
 hipError_t launch_gather_int(const int *dSrc, const int *dIdx, int n, int *dOut, hipStream_t stream);
 // Threshold + local-maximum candidates (unordered, deterministic set).
