@@ -100,19 +100,20 @@ __device__ __forceinline__ void tile_prefix(const double *ts1, const double *ts2
 }
 
 // Sums of d = x - a and d^2 over the window [lo, lo + Lt) of the block's
-// samples: partial tiles sample by sample, whole tiles from the tile
-// prefix sums (FP32 correlation) or tile by tile (FP64 correlation, whose
-// tighter tolerance keeps the summation order of the tiles).
-template <typename T>
+// samples (TILE-sample tiles): partial tiles sample by sample, whole tiles
+// from the tile prefix sums (FP32 correlation) or tile by tile (FP64
+// correlation, whose tighter tolerance keeps the summation order of the
+// tiles).
+template <typename T, int TILE>
 __device__ __forceinline__ void window_sums(const T *raw, const double *ts1, const double *ts2, const double *tp1,
                                             const double *tp2, double a, int lo, int Lt, double &S1, double &S2)
 {
     const int hi = lo + Lt;
-    const int q0 = (lo + 15) >> 4, q1 = hi >> 4;
+    const int q0 = (lo + TILE - 1) / TILE, q1 = hi / TILE;
     S1 = 0;
     S2 = 0;
     if (q0 < q1) {
-        for (int j = lo; j < 16 * q0; ++j) {
+        for (int j = lo; j < TILE * q0; ++j) {
             const double d = static_cast<double>(raw[j]) - a;
             S1 += d;
             S2 += d * d;
@@ -126,7 +127,7 @@ __device__ __forceinline__ void window_sums(const T *raw, const double *ts1, con
                 S2 += ts2[q];
             }
         }
-        for (int j = 16 * q1; j < hi; ++j) {
+        for (int j = TILE * q1; j < hi; ++j) {
             const double d = static_cast<double>(raw[j]) - a;
             S1 += d;
             S2 += d * d;
@@ -146,7 +147,7 @@ __device__ __forceinline__ void window_sums(const T *raw, const double *ts1, con
 // outputs.  The FP32 scale is rsqrt(den^2) in FP32 plus one Newton step
 // (relative error ~2^-23, far inside the FP32 contract); the FP64 scale is
 // sqrt/division in FP64.  Returns this thread's number of flagged outputs.
-template <typename T>
+template <typename T, int TILE>
 __device__ __forceinline__ int compute_scale(const T *raw, const double *ts1, const double *ts2,
                                              const double *tp1, const double *tp2, T *sc, double a, int nValid,
                                              int Lt, double noise, int t, int nthr)
@@ -163,9 +164,12 @@ __device__ __forceinline__ int compute_scale(const T *raw, const double *ts1, co
     int nflag = 0;
     if (i0 >= i1) { return 0; }
     double S1, S2;
-    window_sums<T>(raw, ts1, ts2, tp1, tp2, a, i0, Lt, S1, S2);
+    window_sums<T, TILE>(raw, ts1, ts2, tp1, tp2, a, i0, Lt, S1, S2);
     double S2max = S2;
     const float sqrtLtF = static_cast<float>(sqrtLt);
+    // unrolled so the sample reads of several steps are issued together
+    // ahead of the serial window-sum chain
+#pragma unroll 4
     for (int i = i0; i < i1; ++i) {
         const double den2 = Lt * S2 - S1 * S1;
         const bool flag = !(den2 > canc * S2max) || !(den2 > denThr2);
@@ -326,7 +330,7 @@ __global__ __launch_bounds__(NT) void corr_kernel(const Item *__restrict__ items
         if (Lt != curLt) {
             // 3. per-output scale for this template length
             curLt = Lt;
-            compute_scale<T>(raw, ts1, ts2, tp1, tp2, sc, a, nValidSeg, Lt, noise, tid, NT);
+            compute_scale<T, 16>(raw, ts1, ts2, tp1, tp2, sc, a, nValidSeg, Lt, noise, tid, NT);
             __syncthreads();
         }
         // 4a. spectral product + inverse packing into Ws
@@ -471,7 +475,10 @@ __global__ __launch_bounds__(NT, (Corr2Waves<T, M>::value)) void corr2_kernel(co
                                                    const Cx<T> *__restrict__ twhiG)
 {
     constexpr int N = 2 * M;
-    constexpr int NTILE = N / 16;
+    // tile of the FP64 prefix sums: 4 samples in FP32 (short partial tiles
+    // at each window's ends), 16 in FP64 (whose windows sum whole tiles)
+    constexpr int TILE = sizeof(T) == 4 ? 4 : 16;
+    constexpr int NTILE = N / TILE;
     constexpr int SPT = N / NT;
     constexpr int LOGN = Log2<N>::value;
     constexpr int Q = M / 64;
@@ -539,13 +546,13 @@ __global__ __launch_bounds__(NT, (Corr2Waves<T, M>::value)) void corr2_kernel(co
         double t1 = d, t2 = d * d;
         e += t2;
 #pragma unroll
-        for (int o = 8; o > 0; o >>= 1) {
+        for (int o = TILE / 2; o > 0; o >>= 1) {
             t1 += __shfl_xor(t1, o, 64);
             t2 += __shfl_xor(t2, o, 64);
         }
-        if ((tid & 15) == 0) {
-            ts1[j >> 4] = t1;
-            ts2[j >> 4] = t2;
+        if ((tid & (TILE - 1)) == 0) {
+            ts1[j / TILE] = t1;
+            ts2[j / TILE] = t2;
         }
         xin[2 * xpad(j >> 1) + (j & 1)] = valid ? static_cast<T>(v0[q] - aT) : T(0);
     }
@@ -666,7 +673,7 @@ __global__ __launch_bounds__(NT, (Corr2Waves<T, M>::value)) void corr2_kernel(co
         if (scale == nullptr) {
             T *own = reinterpret_cast<T *>(buf);
             int nf = 0;
-            if (nvRow > 0) { nf = compute_scale<T>(raw, ts1, ts2, tp1, tp2, own, a, nvRow, row.Lt, noise, lane, 64); }
+            if (nvRow > 0) { nf = compute_scale<T, TILE>(raw, ts1, ts2, tp1, tp2, own, a, nvRow, row.Lt, noise, lane, 64); }
             anyFlag = __ballot(nf > 0) != 0ull;
             wave_lds_sync();
             scale = own;
@@ -722,12 +729,12 @@ __global__ __launch_bounds__(NT, (Corr2Waves<T, M>::value)) void corr2_kernel(co
                 if (wave == 0) {
                     forwardFft();
                 } else {
-                    myFlags = compute_scale<T>(raw, ts1, ts2, tp1, tp2, sc, a, nValidSeg, Lt, noise, tid - 64,
+                    myFlags = compute_scale<T, TILE>(raw, ts1, ts2, tp1, tp2, sc, a, nValidSeg, Lt, noise, tid - 64,
                                                NT - 64);
                 }
                 fftDone = true;
             } else {
-                myFlags = compute_scale<T>(raw, ts1, ts2, tp1, tp2, sc, a, nValidSeg, Lt, noise, tid, NT);
+                myFlags = compute_scale<T, TILE>(raw, ts1, ts2, tp1, tp2, sc, a, nValidSeg, Lt, noise, tid, NT);
             }
             const bool anyFlag = block_sum<NT>(static_cast<double>(myFlags), red) > 0.0;
             int r = g0 + wave;
