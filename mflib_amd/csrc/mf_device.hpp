@@ -181,6 +181,19 @@ __device__ __forceinline__ void fft_lds(Cx<T> *buf, Twiddle<T> tw)
     if constexpr ((LOG & 1) != 0) { stockham_pass<T, M, 2, NT, INV>(buf, Ns, tw); }
 }
 
+// A value every lane holds the same copy of, moved to scalar registers
+// (block-wide results read back from LDS land in vector registers).
+__device__ __forceinline__ double uniform(double v)
+{
+    const int lo = __builtin_amdgcn_readfirstlane(__double2loint(v));
+    const int hi = __builtin_amdgcn_readfirstlane(__double2hiint(v));
+    return __hiloint2double(hi, lo);
+}
+__device__ __forceinline__ float uniform(float v)
+{
+    return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v)));
+}
+
 // ---- block reductions -------------------------------------------------
 __device__ __forceinline__ double wave_sum(double v)
 {

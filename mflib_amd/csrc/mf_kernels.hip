@@ -147,7 +147,7 @@ __device__ __forceinline__ void window_sums(const T *raw, const double *ts1, con
 // outputs.  The FP32 scale is rsqrt(den^2) in FP32 plus one Newton step
 // (relative error ~2^-23, far inside the FP32 contract); the FP64 scale is
 // sqrt/division in FP64.  Returns this thread's number of flagged outputs.
-template <typename T, int TILE>
+template <typename T, int TILE, int UNROLL = 1>
 __device__ __forceinline__ int compute_scale(const T *raw, const double *ts1, const double *ts2,
                                              const double *tp1, const double *tp2, T *sc, double a, int nValid,
                                              int Lt, double noise, int t, int nthr)
@@ -167,9 +167,9 @@ __device__ __forceinline__ int compute_scale(const T *raw, const double *ts1, co
     window_sums<T, TILE>(raw, ts1, ts2, tp1, tp2, a, i0, Lt, S1, S2);
     double S2max = S2;
     const float sqrtLtF = static_cast<float>(sqrtLt);
-    // unrolled so the sample reads of several steps are issued together
-    // ahead of the serial window-sum chain
-#pragma unroll 4
+    // UNROLL > 1 (a wave's own long runs) issues the sample reads of
+    // several steps together ahead of the serial window-sum chain
+#pragma unroll UNROLL
     for (int i = i0; i < i1; ++i) {
         const double den2 = Lt * S2 - S1 * S1;
         const bool flag = !(den2 > canc * S2max) || !(den2 > denThr2);
@@ -534,7 +534,7 @@ __global__ __launch_bounds__(NT, (Corr2Waves<T, M>::value)) void corr2_kernel(co
         raw[j] = val;
         s += static_cast<double>(val);
     }
-    const double a = block_sum<NT>(s, red) / nSamp;
+    const double a = uniform(block_sum<NT>(s, red) / nSamp);  // scalar registers: used by every row
     const T aT = static_cast<T>(a);
     double e = 0;
     T *xin = reinterpret_cast<T *>(xbuf[0]);
@@ -560,8 +560,8 @@ __global__ __launch_bounds__(NT, (Corr2Waves<T, M>::value)) void corr2_kernel(co
     tile_prefix<NTILE>(ts1, ts2, tp1, tp2, tid);
     __syncthreads();
     const double rmsBlk = sqrt(energy / nSamp);
-    const double noise = kNoiseK / Eps<T>::rhoTol * Eps<T>::value * sqrt(static_cast<double>(LOGN)) *
-                         static_cast<double>(sg.hmax) * rmsBlk;
+    const double noise = uniform(kNoiseK / Eps<T>::rhoTol * Eps<T>::value * sqrt(static_cast<double>(LOGN)) *
+                                 static_cast<double>(sg.hmax) * rmsBlk);
     WaveTwiddles<T, M> wt;
     wt.init(lane, tw);
     Cx<T> v[Q];
@@ -673,7 +673,7 @@ __global__ __launch_bounds__(NT, (Corr2Waves<T, M>::value)) void corr2_kernel(co
         if (scale == nullptr) {
             T *own = reinterpret_cast<T *>(buf);
             int nf = 0;
-            if (nvRow > 0) { nf = compute_scale<T, TILE>(raw, ts1, ts2, tp1, tp2, own, a, nvRow, row.Lt, noise, lane, 64); }
+            if (nvRow > 0) { nf = compute_scale<T, TILE, 4>(raw, ts1, ts2, tp1, tp2, own, a, nvRow, row.Lt, noise, lane, 64); }
             anyFlag = __ballot(nf > 0) != 0ull;
             wave_lds_sync();
             scale = own;
