@@ -232,6 +232,19 @@ def test_batched_layout_detect_matches_oracle():
         assert np.array_equal(st, st_ref)
         assert np.array_equal(peaks[g], O.peak_finder(st_ref, 0.3, 25))
         assert len(peaks[g]) >= 1
+    # threshold 0 with no minimum distance: every local maximum above 0 is a
+    # peak (tens of thousands over both groups), more than the detect
+    # readback's first batch, so the remainder is read in a second copy
+    peaks0 = h.detect(0.0, 0)
+    total = 0
+    for g in range(G):
+        members = [r for r in range(len(tpls)) if groups[r] == g]
+        st_ref = O.shift_and_stack(np.stack([rows[r] for r in members]), [shifts[r] for r in members],
+                                   [weights[r] for r in members])
+        ref0 = O.peak_finder(st_ref, 0.0, 0)
+        assert np.array_equal(peaks0[g], ref0)
+        total += len(ref0)
+    assert total > 8192
 
 
 def test_dead_and_zeroed_channels():
