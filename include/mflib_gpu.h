@@ -197,6 +197,36 @@ int mf_reduce_group_device(mf_handle *h, int group, int absoluteValue, void *dDe
 int mf_detect_reduced(mf_handle *h, int group, int absoluteValue, double threshold, int minDist, int *idx,
                       int *templateIds, int cap, int *nPeaks);
 
+/* Time interpolation kinds (values of MFLib::DetectionTimeInterpolationType). */
+#define MF_INTERP_LANCZOS 0
+#define MF_INTERP_QUADRATIC 1
+#define MF_INTERP_NONE 2
+/* Bits of mf_detection_products' flags. */
+#define MF_PRODUCT_AMPLITUDE 1 /* alphaGR / alphaSR were computed */
+#define MF_PRODUCT_TRUNCATED 2 /* peak + template length runs past the signal */
+/* The per-detection products of SingleChannel::Detector::detect
+ * (src/singleChannel/detector.cpp:537-664), for nPeaks detections, each a
+ * lag peakIdx[k] on template row templateIds[k] of the last apply():
+ *  - rho[k]: rho at the peak clamped to [-1, 1] (574-575); its sign gives
+ *    the detection's polarity (623-629);
+ *  - shift[k]: interpolated minus integer detection time, seconds
+ *    (601-614): quadraticRefinement (38-89) or lanczosRefinement (111-152)
+ *    with lanczosAlpha and lanczosPoints (DetectionTimeInterpolationParameters);
+ *    0 for MF_INTERP_NONE;
+ *  - alphaGR[k], alphaSR[k]: RelativeMagnitude's amplitude scaling factors
+ *    (src/singleChannel/relativeMagnitude.cpp:35-58,362-388) of the window
+ *    signal[peak .. peak + Lt) against the template, set with
+ *    MF_PRODUCT_AMPLITUDE in flags[k] unless the window runs past the
+ *    signal (MF_PRODUCT_TRUNCATED, detector.cpp:578-585) or is constant
+ *    (relativeMagnitude.cpp:330-338).
+ * samplingPeriods[it] is template it's 1/fs (NULL: 1, shifts in samples).
+ * absoluteValue follows the detector's maxima policy.  Arithmetic is FP64
+ * for both precisions.  Any output may be NULL. */
+int mf_detection_products(mf_handle *h, int nPeaks, const int *peakIdx, const int *templateIds,
+                          const double *samplingPeriods, int absoluteValue, int interpolation, int lanczosAlpha,
+                          int lanczosPoints, double *rho, double *shift, double *alphaGR, double *alphaSR,
+                          int *flags);
+
 /* computeOptimalFFTAndBlockLength(nb, nx)
  * (include/mflib/matchedFilterParameters.hpp:208, src/matchedFilterParameters.cpp:371-385). */
 int mf_compute_optimal_fft_and_block_length(int nb, int nx, int *nfft, int *blockLength);

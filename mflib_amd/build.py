@@ -47,7 +47,7 @@ def hipcc() -> str:
 def build_gpu_lib() -> str:
     os.makedirs(LIB, exist_ok=True)
     out = os.path.join(LIB, "libmflib_gpu.so")
-    srcs = [os.path.join(CSRC, f) for f in ("mf_kernels.hip", "mf_tdcorr.hip", "mf_scan.hip", "mf_engine.hip", "mf_server.hip")]
+    srcs = [os.path.join(CSRC, f) for f in ("mf_kernels.hip", "mf_tdcorr.hip", "mf_scan.hip", "mf_products.hip", "mf_engine.hip", "mf_server.hip")]
     objs = []
     for s in srcs:
         o = os.path.join(LIB, os.path.basename(s) + ".o")

@@ -76,6 +76,8 @@ _SIGNATURES = {
     "mf_reduce_group_device": ([_vp, ctypes.c_int, ctypes.c_int, _vp, _ip, ctypes.c_int, _ip], ctypes.c_int),
     "mf_detect_reduced": ([_vp, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_int, _ip, _ip, ctypes.c_int,
                            _ip], ctypes.c_int),
+    "mf_detection_products": ([_vp, ctypes.c_int, _ip, _ip, _dp, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                               ctypes.c_int, _dp, _dp, _dp, _dp, _ip], ctypes.c_int),
     "mf_get_timings": ([_vp, _dp, _dp], ctypes.c_int),
     "mf_autotune": ([ctypes.c_int, _dp, ctypes.c_int, _ip], ctypes.c_int),
     "mf_server_create": ([ctypes.POINTER(_vp), ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_int],
@@ -172,6 +174,10 @@ def peaks_device(ptr: int, n: int, dtype, threshold: float, min_dist: int, cap: 
 
 
 SCAN_PRECISION = {"f64": 0, "f32c": 1}
+
+# DetectionTimeInterpolationType values and mf_detection_products flag bits
+INTERP_LANCZOS, INTERP_QUADRATIC, INTERP_NONE = 0, 1, 2
+PRODUCT_AMPLITUDE, PRODUCT_TRUNCATED = 1, 2
 
 
 def window_scale_device(ptr: int, n: int, lt: int, scan: str, out_ptr: int) -> float:
@@ -370,6 +376,35 @@ class Handle:
                                       idx.ctypes.data_as(_ip), tid.ctypes.data_as(_ip), cap, ctypes.byref(n)),
               "detect_reduced")
         return idx[:n.value].copy(), tid[:n.value].copy()
+
+    def detection_products(self, peaks, template_ids, sampling_periods=None, absval: bool = False,
+                           interpolation: int = INTERP_QUADRATIC, lanczos_alpha: int = 51,
+                           lanczos_points: int = 100):
+        """Per-detection products of Detector::detect for the peaks
+        (lags) on the template rows template_ids: a dict of rho (clamped),
+        shift (seconds, or samples when sampling_periods is None), alpha_gr,
+        alpha_sr and flags (PRODUCT_AMPLITUDE, PRODUCT_TRUNCATED)."""
+        pk = np.ascontiguousarray(peaks, dtype=np.int32)
+        ti = np.ascontiguousarray(template_ids, dtype=np.int32)
+        if pk.shape != ti.shape or pk.ndim != 1:
+            raise ValueError("peaks and template_ids must be 1-D arrays of one length")
+        n = len(pk)
+        dts = None
+        if sampling_periods is not None:
+            dts = np.ascontiguousarray(sampling_periods, dtype=np.float64)
+            if dts.ndim != 1 or len(dts) != self.number_of_templates():
+                raise ValueError("sampling_periods needs one entry per template")
+        out = {k: np.empty(max(n, 1)) for k in ("rho", "shift", "alpha_gr", "alpha_sr")}
+        flags = np.empty(max(n, 1), dtype=np.int32)
+        check(lib().mf_detection_products(
+            self._h, n, pk.ctypes.data_as(_ip), ti.ctypes.data_as(_ip),
+            dts.ctypes.data_as(_dp) if dts is not None else None, int(absval), int(interpolation),
+            int(lanczos_alpha), int(lanczos_points), out["rho"].ctypes.data_as(_dp),
+            out["shift"].ctypes.data_as(_dp), out["alpha_gr"].ctypes.data_as(_dp),
+            out["alpha_sr"].ctypes.data_as(_dp), flags.ctypes.data_as(_ip)), "detection_products")
+        res = {k: v[:n].copy() for k, v in out.items()}
+        res["flags"] = flags[:n].copy()
+        return res
 
     def stack_device(self, group: int):
         p, n = _vp(), ctypes.c_int()

@@ -99,6 +99,7 @@ struct mf_handle {
     DevBuf dFrag, dTdRowTab, dTdItems;  // time-domain lanes
     DevBuf dStackRows, dStackOut;
     DevBuf dReduceRows, dRedDet, dRedId, dRedPk, dRedPkId;  // Detector reduce step
+    DevBuf dProdJobs, dProdOut;                             // per-detection products
     // stacking groups (built on first stack/detect)
     struct StackInfo {
         int first = 0, count = 0, smax = 0, minShift = 0;
@@ -1526,6 +1527,71 @@ int mf_detect_reduced(mf_handle *h, int group, int absoluteValue, double thresho
                                   hipMemcpyDeviceToHost, h->stream));
             MF_HIP(hipStreamSynchronize(h->stream));
             groupRowsToTemplates(h, group, templateIds, np);
+        }
+    });
+}
+
+int mf_detection_products(mf_handle *h, int nPeaks, const int *peakIdx, const int *templateIds,
+                          const double *samplingPeriods, int absoluteValue, int interpolation, int lanczosAlpha,
+                          int lanczosPoints, double *rho, double *shift, double *alphaGR, double *alphaSR,
+                          int *flags)
+{
+    return guard([&] {
+        requireInit(h);
+        if (!h->applied) { fail(MF_ERR_BAD_STATE, "Matched filtered signals not yet computed"); }
+        if (nPeaks < 0) { fail(MF_ERR_INVALID_ARGUMENT, "nPeaks must be non-negative"); }
+        if (nPeaks > 0 && (peakIdx == nullptr || templateIds == nullptr)) {
+            fail(MF_ERR_INVALID_ARGUMENT, "peak list is NULL");
+        }
+        if (interpolation != mfk::kInterpLanczos && interpolation != mfk::kInterpQuadratic &&
+            interpolation != mfk::kInterpNone) {
+            fail(MF_ERR_INVALID_ARGUMENT, "unknown interpolation type " + std::to_string(interpolation));
+        }
+        // DetectionTimeInterpolationParameters' own checks
+        if (interpolation == mfk::kInterpLanczos) {
+            if (lanczosAlpha < 1) { fail(MF_ERR_INVALID_ARGUMENT, "alpha must be positive"); }
+            if (lanczosPoints < 1) { fail(MF_ERR_INVALID_ARGUMENT, "Number of interpolation points must be positive"); }
+        }
+        if (nPeaks == 0) { return; }
+        std::vector<mfk::ProductJob> jobs(nPeaks);
+        for (int k = 0; k < nPeaks; ++k) {
+            const int it = templateIds[k];
+            checkRow(h, it);
+            const auto &rp = h->rows[it];
+            if (peakIdx[k] < 0 || peakIdx[k] >= rp.nOut) {
+                fail(MF_ERR_INVALID_ARGUMENT, "peak " + std::to_string(peakIdx[k]) + " outside [0," +
+                                                  std::to_string(rp.nOut - 1) + "]");
+            }
+            const double dt = samplingPeriods ? samplingPeriods[it] : 1.0;
+            if (!(dt > 0)) { fail(MF_ERR_INVALID_ARGUMENT, "sampling period must be positive"); }
+            auto &j = jobs[k];
+            j.rho = static_cast<const char *>(h->dRho.p) + rp.rhoOff * h->tsize;
+            j.sig = h->ch[rp.ch].buf.p;
+            j.tmpl = h->dTmplRaw.as<double>() + rp.tmplOff;
+            j.sigLen = h->ch[rp.ch].n;
+            j.dt = dt;
+            j.nOut = rp.nOut;
+            j.peak = peakIdx[k];
+            j.Lt = rp.Lt;
+            j.pad = 0;
+        }
+        growTo(h->dProdJobs, jobs.size() * sizeof(mfk::ProductJob));
+        growTo(h->dProdOut, jobs.size() * sizeof(mfk::ProductOut));
+        MF_HIP(hipStreamSynchronize(h->stream));  // the job table may be in use by an earlier call
+        MF_HIP(hipMemcpy(h->dProdJobs.p, jobs.data(), jobs.size() * sizeof(mfk::ProductJob), hipMemcpyHostToDevice));
+        MF_HIP(mfk::launch_products(h->precision, h->dProdJobs.as<mfk::ProductJob>(), nPeaks, absoluteValue ? 1 : 0,
+                                    interpolation, lanczosAlpha, lanczosPoints, h->dProdOut.as<mfk::ProductOut>(),
+                                    h->stream));
+        std::vector<mfk::ProductOut> res(nPeaks);
+        MF_HIP(hipMemcpyAsync(res.data(), h->dProdOut.p, res.size() * sizeof(mfk::ProductOut), hipMemcpyDeviceToHost,
+                              h->stream));
+        MF_HIP(hipStreamSynchronize(h->stream));
+        for (int k = 0; k < nPeaks; ++k) {
+            if (rho) { rho[k] = res[k].rho; }
+            if (shift) { shift[k] = res[k].shift; }
+            if (alphaGR) { alphaGR[k] = res[k].alphaGR; }
+            if (alphaSR) { alphaSR[k] = res[k].alphaSR; }
+            if (flags) { flags[k] = res[k].flags; }
         }
     });
 }

@@ -160,4 +160,26 @@ constexpr int kScanF32c = 1;
 constexpr int kScanMaxLt = 2048;
 hipError_t launch_scan_scale(int mode, const float *dx, long long n, int Lt, float *dScale, hipStream_t stream);
 
+// Per-detection products (mf_products.hip): one wavefront per detection.
+// Interpolation kinds use the values of MFLib::DetectionTimeInterpolationType.
+constexpr int kInterpLanczos = 0;
+constexpr int kInterpQuadratic = 1;
+constexpr int kInterpNone = 2;
+constexpr int kProductAmplitude = 1;  // scaling factors computed
+constexpr int kProductTruncated = 2;  // peak + Lt runs past the signal: no amplitude
+struct ProductJob {
+    const void *rho;     // the template's correlation row (T), nOut lags
+    const void *sig;     // the template's channel (T), sigLen samples
+    const double *tmpl;  // the template's samples as given (FP64), Lt
+    long long sigLen;
+    double dt;           // sampling period of the template, s
+    int nOut, peak, Lt, pad;
+};
+struct ProductOut {
+    double rho, shift, alphaGR, alphaSR;
+    int flags, pad;
+};
+hipError_t launch_products(int precision, const ProductJob *dJobs, int nJobs, int absval, int interp,
+                           int lanczosAlpha, int lanczosPoints, ProductOut *dOut, hipStream_t stream);
+
 }  // namespace mfk

@@ -71,6 +71,10 @@ def lib():
         L.orc_peak_finder_f32.argtypes = [ctypes.c_int, _fp, ctypes.c_float,
                                           ctypes.c_int, _ip, _ip]
         L.orc_rfft_f64.argtypes = [ctypes.c_int, _dp, _dp]
+        L.orc_quadratic_shift.argtypes = [ctypes.c_int, _dp, ctypes.c_int, ctypes.c_double, ctypes.c_int, _dp]
+        L.orc_lanczos_shift.argtypes = [ctypes.c_int, _dp, ctypes.c_int, ctypes.c_double, ctypes.c_int,
+                                        ctypes.c_int, ctypes.c_int, _dp, _dp]
+        L.orc_amplitude_factors.argtypes = [ctypes.c_int, _dp, _dp, _dp, _dp, _ip]
         L.orc_irfft_f64.argtypes = [ctypes.c_int, _dp, _dp]
         _lib = L
     return _lib
@@ -216,6 +220,39 @@ def reduce_correlations(rho, absval: bool = False, length: int | None = None):
         rc = lib().orc_reduce_correlations_f64(nT, _ptr(rho, _dp), ld, n, int(absval), _ptr(det, _dp), _ptr(idx, _ip))
     _check(rc, "reduce_correlations")
     return det, idx
+
+
+def quadratic_shift(xc, peak: int, dt: float = 1.0, absval: bool = False) -> float:
+    """quadraticRefinement of Detector::detect on the FP64 row xc at lag peak."""
+    xc = np.ascontiguousarray(xc, dtype=np.float64)
+    out = np.zeros(1)
+    _check(lib().orc_quadratic_shift(len(xc), _ptr(xc, _dp), int(peak), float(dt), int(absval), _ptr(out, _dp)),
+           "quadratic_shift")
+    return float(out[0])
+
+
+def lanczos_shift(xc, peak: int, dt: float = 1.0, absval: bool = False, alpha: int = 51, points: int = 100):
+    """lanczosRefinement of Detector::detect: returns (shift, profile), the
+    profile holding the interpolant at the points-1 grid shifts."""
+    xc = np.ascontiguousarray(xc, dtype=np.float64)
+    out = np.zeros(1)
+    prof = np.zeros(max(points - 1, 1))
+    _check(lib().orc_lanczos_shift(len(xc), _ptr(xc, _dp), int(peak), float(dt), int(absval), int(alpha),
+                                   int(points), _ptr(out, _dp), _ptr(prof, _dp)), "lanczos_shift")
+    return float(out[0]), prof[:max(points - 1, 0)]
+
+
+def amplitude_factors(tmpl, window):
+    """RelativeMagnitude's (Gibbons-Ringdal, Schaff-Richards) scaling factors
+    of window against tmpl; None when the window is constant."""
+    t = np.ascontiguousarray(tmpl, dtype=np.float64)
+    y = np.ascontiguousarray(window, dtype=np.float64)
+    assert len(t) == len(y)
+    gr, sr = np.zeros(1), np.zeros(1)
+    ok = ctypes.c_int()
+    _check(lib().orc_amplitude_factors(len(t), _ptr(t, _dp), _ptr(y, _dp), _ptr(gr, _dp), _ptr(sr, _dp),
+                                       ctypes.byref(ok)), "amplitude_factors")
+    return (float(gr[0]), float(sr[0])) if ok.value else None
 
 
 def peak_finder(x, threshold: float, min_dist: int):

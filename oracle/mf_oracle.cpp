@@ -404,6 +404,110 @@ int reduceCorrelations(int nT, const T *rho, long long ld, int len, int absval, 
 }
 
 /* ---------------------------------------------------------------------- */
+/* Per-detection products of Detector::detect, FP64 (the reference's        */
+/* double instantiation): src/singleChannel/detector.cpp:38-152, 537-664;   */
+/* src/singleChannel/relativeMagnitude.cpp:35-58, 76-98, 273-388.           */
+/* ---------------------------------------------------------------------- */
+
+/* quadraticRefinement (detector.cpp:38-89). */
+double quadraticShift(int n, const double *xc, int p, double dt, int absval)
+{
+    if (!(p > 0 && p < n - 1)) { return 0.0; }
+    double y1 = xc[p - 1], y2 = xc[p], y3 = xc[p + 1];
+    if (absval) {
+        y1 = std::abs(y1);
+        y2 = std::abs(y2);
+        y3 = std::abs(y3);
+    }
+    if (y2 < y1 || y2 < y3) {
+        if (y2 < y1 && y2 < y3) {
+            y1 = -y1;
+            y2 = -y2;
+            y3 = -y3;
+        } else {
+            return 0.0;
+        }
+    }
+    const double eps100 = 100 * std::numeric_limits<double>::epsilon();
+    y1 = std::min(y2 - eps100, y1);
+    y3 = std::min(y2 - eps100, y3);
+    const double num = dt * (y1 - y3);
+    const double den = 2 * (y1 - 2 * y2 + y3);
+    return num / den;
+}
+
+/* sinc / lanczos (detector.cpp:91-109). */
+double sincRef(double x)
+{
+    double r = 1.0;
+    if (std::abs(x) > std::numeric_limits<double>::epsilon() * 100) {
+        const double pix = M_PI * x;
+        r = std::sin(pix) / pix;
+    }
+    return r;
+}
+
+double lanczosRef(double x, int a)
+{
+    double r = 0.0;
+    if (static_cast<int>(std::abs(x)) <= a) { r = sincRef(x) * sincRef(x / a); }
+    return r;
+}
+
+/* The interpolant of lanczosRefinement (detector.cpp:120-139) at grid
+ * points idx = 1 .. nRefine-1 (profile[idx-1]), and the shift it picks
+ * (140-148): the first grid point of the largest value, strict '>' from
+ * -DBL_MAX; 0 when no point beats that. */
+double lanczosShift(int n, const double *xc, int p, double dt, int absval, int a, int nRefine, double *profile)
+{
+    const double dtWork = 1 / static_cast<double>(nRefine);
+    double optShift = 0;
+    double yMax = -std::numeric_limits<double>::max();
+    for (int idx = 1; idx < nRefine; ++idx) {
+        const double shift = -0.5 + dtWork * idx;
+        const double x = p + shift;
+        double y = 0;
+        for (int k = -a; k <= a; ++k) {
+            const int i = static_cast<int>(std::floor(x)) - k + 1;
+            const double xi = x - static_cast<double>(i);
+            double v = 0;
+            if (i >= 0 && i < n) { v = xc[i]; }
+            y = y + v * lanczosRef(xi, a);
+        }
+        if (absval) { y = std::abs(y); }
+        if (profile) { profile[idx - 1] = y; }
+        if (y > yMax) {
+            optShift = dt * shift;
+            yMax = y;
+        }
+    }
+    return optShift;
+}
+
+/* RelativeMagnitude: demeanNorm2 of the template (76-98, 273-299),
+ * demeanCopyComputeNorm2 of the window (50-74, 318-339; a zero norm throws
+ * there, caught by detect, so no amplitude), Gibbons & Ringdal (35-47) and
+ * Schaff & Richards (49-58).  Returns 1 when the factors were computed. */
+int amplitudeFactors(int Lt, const double *t, const double *y, double *gr, double *sr)
+{
+    const double mt = std::accumulate(t, t + Lt, 0.0) / static_cast<double>(Lt);
+    const double my = std::accumulate(y, y + Lt, 0.0) / static_cast<double>(Lt);
+    std::vector<double> x(Lt), d(Lt);
+    double xx = 0, yy = 0;
+    for (int i = 0; i < Lt; ++i) {
+        x[i] = t[i] - mt;
+        xx = xx + x[i] * x[i];
+        d[i] = y[i] - my;
+        yy = yy + d[i] * d[i];
+    }
+    if (yy == 0) { return 0; }
+    const double xy = std::inner_product(x.begin(), x.end(), d.begin(), 0.0);
+    *gr = std::abs(xy) / xx;
+    *sr = std::sqrt(yy / xx);
+    return 1;
+}
+
+/* ---------------------------------------------------------------------- */
 /* PeakFinder<T>::apply: src/peakFinder.cpp:468-586                         */
 /* ---------------------------------------------------------------------- */
 template <typename T>
@@ -633,6 +737,36 @@ int orc_reduce_correlations_f64(int nT, const double *rho, long long ld, int len
                                 int *id)
 {
     return reduceCorrelations<double>(nT, rho, ld, len, absval, det, id);
+}
+
+int orc_quadratic_shift(int n, const double *xc, int peak, double dt, int absval, double *shift)
+{
+    if (n < 1 || xc == nullptr || shift == nullptr || peak < 0 || peak >= n) { return -1; }
+    *shift = quadraticShift(n, xc, peak, dt, absval);
+    return 0;
+}
+
+int orc_lanczos_shift(int n, const double *xc, int peak, double dt, int absval, int alpha, int nRefine,
+                      double *shift, double *profile)
+{
+    if (n < 1 || xc == nullptr || shift == nullptr || peak < 0 || peak >= n || alpha < 1 || nRefine < 1) {
+        return -1;
+    }
+    *shift = lanczosShift(n, xc, peak, dt, absval, alpha, nRefine, profile);
+    return 0;
+}
+
+int orc_amplitude_factors(int Lt, const double *tmpl, const double *window, double *alphaGR, double *alphaSR,
+                          int *computed)
+{
+    if (Lt < 1 || tmpl == nullptr || window == nullptr || alphaGR == nullptr || alphaSR == nullptr ||
+        computed == nullptr) {
+        return -1;
+    }
+    *alphaGR = 0;
+    *alphaSR = 0;
+    *computed = amplitudeFactors(Lt, tmpl, window, alphaGR, alphaSR);
+    return 0;
 }
 
 int orc_shift_and_stack_f32(int nT, const float *rho, long long ld, int nOut,

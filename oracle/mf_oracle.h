@@ -91,6 +91,19 @@ int orc_reduce_correlations_f32(int nT, const float *rho, long long ld, int len,
 int orc_reduce_correlations_f64(int nT, const double *rho, long long ld, int len, int absval,
                                 double *det, int *id);
 
+/* Per-detection products of SingleChannel::Detector::detect, FP64.
+ * quadraticRefinement: src/singleChannel/detector.cpp:38-89. */
+int orc_quadratic_shift(int n, const double *xc, int peak, double dt, int absval, double *shift);
+/* lanczosRefinement: detector.cpp:91-152; profile (NULL or nRefine-1
+ * entries) receives the interpolant at every grid point. */
+int orc_lanczos_shift(int n, const double *xc, int peak, double dt, int absval, int alpha, int nRefine,
+                      double *shift, double *profile);
+/* RelativeMagnitude's scaling factors (src/singleChannel/relativeMagnitude.cpp:
+ * 35-58, 76-98, 273-388) of window[0..Lt) against tmpl[0..Lt); *computed = 0
+ * when the window is constant (the reference throws and detect skips it). */
+int orc_amplitude_factors(int Lt, const double *tmpl, const double *window, double *alphaGR, double *alphaSR,
+                          int *computed);
+
 int orc_shift_and_stack_f64(int nT, const double *rho, long long ld,
                             int nOut, const int *shifts,
                             const double *weights, int absval,

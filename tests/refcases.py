@@ -112,3 +112,42 @@ def single_channel_inputs(rng):
     wave = rng.uniform(2, 3, n)
     tpls = [wave[25:25 + 130].copy(), wave[1124:1124 + 140].copy(), wave[3601:3601 + 125].copy()]
     return wave, tpls
+
+
+def detector_inputs(seed: int = 0):
+    """testing/detector.cpp:100-227 (singleChannelDetection.detector): five
+    decaying-sine templates of 4 s at 100 Hz; templates 0, 1, 2 and 4 are
+    written, phase-shifted and scaled by 1/4 (even i) or 1/2 (odd i), three
+    times each over a 120 s trace of uniform noise in [-0.05, 0.05]
+    (template 3 is left out on purpose).  The reference draws the noise
+    unseeded; any seed serves.  Returns (signal, templates, dt, expected),
+    expected = (onset, phase-shifted onset, scale, template) of the 12
+    detections in time order."""
+    df = 100.0
+    dt = 1 / df
+    template_dur, signal_dur = 4.0, 120.0
+    n_signal = int(signal_dur / dt + 0.5) + 1
+    n_template = int(template_dur / dt + 0.5) + 1
+    amp = [1.0, 2.0, 0.5, 0.75, 3.0]
+    freq = [10.0, 5.0, 7.5, 12.5, 2.5]
+    decay = [2.8, 2.9, 2.6, 2.2, 2.3]
+    phi = [0.001, 0.002, 0.004, -0.003, -0.002]
+    inserts = {0: [4, 28, 90], 1: [8, 24, 100], 2: [33, 66, 86], 4: [16, 44, 106]}
+
+    def make(npts, a, f, dec, shift):          # createTemplateSignal, 285-301
+        t = np.arange(npts) * dt
+        return a * np.exp(-t * dec) * np.sin(2 * np.pi * f * (t - shift))
+
+    rng = np.random.default_rng(seed)
+    signal = rng.uniform(-0.05, 0.05, n_signal)
+    templates, expected = [], []
+    for i in range(5):
+        templates.append(make(n_template, amp[i], freq[i], decay[i], 0.0))
+        scale = 1.0 / (4.0 if i % 2 == 0 else 2.0)
+        w = make(n_template, scale * amp[i], freq[i], decay[i], phi[i])
+        for toff in inserts.get(i, []):
+            k = int(toff / dt + 0.5)
+            signal[k:k + n_template] = w          # written, not added (214-217)
+            expected.append((float(toff), toff + phi[i], scale, i))
+    expected.sort(key=lambda e: e[0])
+    return signal, templates, dt, expected
