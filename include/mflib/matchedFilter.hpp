@@ -6,6 +6,10 @@
 #include "mflib/waveformTemplate.hpp"
 namespace MFLib
 {
+namespace Facade
+{
+struct Access;
+}
 class MatchedFilterParameters;
 /// Pearson-correlation matched filter of templates against signals
 /// (include/mflib/matchedFilter.hpp in the reference), computed on the
@@ -35,6 +39,7 @@ public:
     bool isInitialized() const noexcept;
     std::vector<std::complex<T>> getSpectraOfTemplate(int it) const;
 private:
+    friend struct MFLib::Facade::Access;  // the engine handle, for Detector
     MatchedFilter& operator=(const MatchedFilter &mf) = delete;
     MatchedFilter(const MatchedFilter &mf) = delete;
     class MatchedFilterImpl;

@@ -6,6 +6,10 @@
 #include "mflib/waveformTemplate.hpp"
 namespace MFLib
 {
+namespace Facade
+{
+struct Access;
+}
 namespace SingleChannel
 {
 class MatchedFilterParameters;
@@ -33,6 +37,7 @@ public:
     bool haveMatchedFilteredSignals() const noexcept;
     bool isInitialized() const noexcept;
 private:
+    friend struct MFLib::Facade::Access;  // the engine handle, for Detector
     MatchedFilter(const MatchedFilter &mf) = delete;
     MatchedFilter& operator=(const MatchedFilter &mf) = delete;
     class MatchedFilterImpl;

@@ -197,6 +197,13 @@ int mf_reduce_group_device(mf_handle *h, int group, int absoluteValue, void *dDe
 int mf_detect_reduced(mf_handle *h, int group, int absoluteValue, double threshold, int minDist, int *idx,
                       int *templateIds, int cap, int *nPeaks);
 
+/* Detector::detect's SINGLE policy for template it (detector.cpp:478-491):
+ * setCorrelations (252-284: rho or |rho| of that template, clamped to
+ * [-1, 1]) and PeakFinder<T>::apply (threshold, minDist).  idx holds cap
+ * entries; *nPeaks the count. */
+int mf_detect_template(mf_handle *h, int it, int absoluteValue, double threshold, int minDist, int *idx, int cap,
+                       int *nPeaks);
+
 /* Time interpolation kinds (values of MFLib::DetectionTimeInterpolationType). */
 #define MF_INTERP_LANCZOS 0
 #define MF_INTERP_QUADRATIC 1

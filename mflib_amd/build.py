@@ -63,7 +63,7 @@ def build_gpu_lib() -> str:
 
 FACADE_SRCS = ["waveformTemplate.cpp", "networkStationPhase.cpp", "matchedFilterParameters.cpp",
                "matchedFilter.cpp", "singleChannelMatchedFilterParameters.cpp",
-               "singleChannelMatchedFilter.cpp", "peakFinder.cpp"]
+               "singleChannelMatchedFilter.cpp", "peakFinder.cpp", "detection.cpp", "detector.cpp"]
 
 
 def build_facade() -> str:

@@ -76,6 +76,8 @@ _SIGNATURES = {
     "mf_reduce_group_device": ([_vp, ctypes.c_int, ctypes.c_int, _vp, _ip, ctypes.c_int, _ip], ctypes.c_int),
     "mf_detect_reduced": ([_vp, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_int, _ip, _ip, ctypes.c_int,
                            _ip], ctypes.c_int),
+    "mf_detect_template": ([_vp, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_int, _ip, ctypes.c_int, _ip],
+                           ctypes.c_int),
     "mf_detection_products": ([_vp, ctypes.c_int, _ip, _ip, _dp, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                ctypes.c_int, _dp, _dp, _dp, _dp, _ip], ctypes.c_int),
     "mf_get_timings": ([_vp, _dp, _dp], ctypes.c_int),
@@ -376,6 +378,16 @@ class Handle:
                                       idx.ctypes.data_as(_ip), tid.ctypes.data_as(_ip), cap, ctypes.byref(n)),
               "detect_reduced")
         return idx[:n.value].copy(), tid[:n.value].copy()
+
+    def detect_template(self, it: int, threshold: float, min_dist: int, absval: bool = False,
+                        cap: int | None = None):
+        """Detector SINGLE policy for one template: its peak list."""
+        cap = cap or max(1, 1 << 20)
+        idx = np.empty(cap, dtype=np.int32)
+        n = ctypes.c_int()
+        check(lib().mf_detect_template(self._h, int(it), int(absval), float(threshold), int(min_dist),
+                                       idx.ctypes.data_as(_ip), cap, ctypes.byref(n)), "detect_template")
+        return idx[:n.value].copy()
 
     def detection_products(self, peaks, template_ids, sampling_periods=None, absval: bool = False,
                            interpolation: int = INTERP_QUADRATIC, lanczos_alpha: int = 51,

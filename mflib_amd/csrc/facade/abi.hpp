@@ -10,6 +10,15 @@
 
 namespace MFLib
 {
+template<class T> class MatchedFilter;
+namespace SingleChannel
+{
+template<class T> class MatchedFilter;
+}
+}
+
+namespace MFLib
+{
 namespace Facade
 {
 inline void check(int rc, const char *where)
@@ -23,6 +32,13 @@ inline void check(int rc, const char *where)
     default: throw std::runtime_error(msg);
     }
 }
+// The engine handle behind the reference-compatible classes (friend of
+// MFLib::MatchedFilter<T> and SingleChannel::MatchedFilter<T>).
+struct Access
+{
+    template<class T> static mf_handle *handle(const MFLib::MatchedFilter<T> &mf);
+    template<class T> static const MFLib::MatchedFilter<T> &inner(const MFLib::SingleChannel::MatchedFilter<T> &mf);
+};
 template<class T> constexpr int precisionOf();
 template<> constexpr int precisionOf<float>() { return MF_PRECISION_FLOAT; }
 template<> constexpr int precisionOf<double>() { return MF_PRECISION_DOUBLE; }

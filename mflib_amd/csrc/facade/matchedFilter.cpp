@@ -287,3 +287,11 @@ std::vector<std::complex<T>> MatchedFilter<T>::getSpectraOfTemplate(int it) cons
 
 template class MFLib::MatchedFilter<double>;
 template class MFLib::MatchedFilter<float>;
+
+template<class T>
+mf_handle *MFLib::Facade::Access::handle(const MFLib::MatchedFilter<T> &mf)
+{
+    return mf.pImpl->handle;
+}
+template mf_handle *MFLib::Facade::Access::handle<float>(const MFLib::MatchedFilter<float> &);
+template mf_handle *MFLib::Facade::Access::handle<double>(const MFLib::MatchedFilter<double> &);

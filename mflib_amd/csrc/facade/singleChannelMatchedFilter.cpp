@@ -1,6 +1,7 @@
 // SingleChannel::MatchedFilter<T>: MFLib::MatchedFilter<T> in
 // SINGLE_CHANNEL mode with the signal index fixed at 0
 // (src/singleChannel/matchedFilter.cpp:40-141 in the reference).
+#include "abi.hpp"
 #include "mflib/matchedFilter.hpp"
 #include "mflib/matchedFilterParameters.hpp"
 #include "mflib/singleChannel/matchedFilter.hpp"
@@ -56,3 +57,13 @@ bool MatchedFilter<T>::isInitialized() const noexcept { return pImpl->mf.isIniti
 
 template class MFLib::SingleChannel::MatchedFilter<double>;
 template class MFLib::SingleChannel::MatchedFilter<float>;
+
+template<class T>
+const MFLib::MatchedFilter<T> &MFLib::Facade::Access::inner(const MFLib::SingleChannel::MatchedFilter<T> &mf)
+{
+    return mf.pImpl->mf;
+}
+template const MFLib::MatchedFilter<float> &MFLib::Facade::Access::inner<float>(
+    const MFLib::SingleChannel::MatchedFilter<float> &);
+template const MFLib::MatchedFilter<double> &MFLib::Facade::Access::inner<double>(
+    const MFLib::SingleChannel::MatchedFilter<double> &);

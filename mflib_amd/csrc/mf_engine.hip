@@ -1431,6 +1431,9 @@ int mf_get_stack_device(mf_handle *h, int group, const void **dptr, int *len)
 
 namespace {
 
+long long reduceRows(mf_handle *h, const std::vector<int> &rows, long long len, int absval, void *dDet, int *dId,
+                     long long cap);
+
 // Detector reduce + clamp of one group's rows into dDet/dId (device),
 // asynchronous on the handle's stream; returns the length.
 long long reduceGroup(mf_handle *h, int group, int absval, void *dDet, int *dId, long long cap)
@@ -1441,13 +1444,19 @@ long long reduceGroup(mf_handle *h, int group, int absval, void *dDet, int *dId,
         fail(MF_ERR_INVALID_ARGUMENT, "group " + std::to_string(group) + " out of range");
     }
     const auto &gr = h->groups[group];
-    const long long len = gr.nOut;
+    return reduceRows(h, gr.rows, gr.nOut, absval, dDet, dId, cap);
+}
+
+// The same over an explicit row list of common length len.
+long long reduceRows(mf_handle *h, const std::vector<int> &rows, long long len, int absval, void *dDet, int *dId,
+                     long long cap)
+{
     if (cap < len) {
         fail(MF_ERR_INVALID_ARGUMENT, "output holds " + std::to_string(cap) + " < " + std::to_string(len) + " lags");
     }
-    std::vector<const void *> ptrs(gr.rows.size());
-    for (size_t k = 0; k < gr.rows.size(); ++k) {
-        ptrs[k] = static_cast<const char *>(h->dRho.p) + h->rows[gr.rows[k]].rhoOff * h->tsize;
+    std::vector<const void *> ptrs(rows.size());
+    for (size_t k = 0; k < rows.size(); ++k) {
+        ptrs[k] = static_cast<const char *>(h->dRho.p) + h->rows[rows[k]].rhoOff * h->tsize;
     }
     growTo(h->dReduceRows, ptrs.size() * sizeof(void *));
     MF_HIP(hipStreamSynchronize(h->stream));  // the table may be in use by an earlier reduce
@@ -1528,6 +1537,24 @@ int mf_detect_reduced(mf_handle *h, int group, int absoluteValue, double thresho
             MF_HIP(hipStreamSynchronize(h->stream));
             groupRowsToTemplates(h, group, templateIds, np);
         }
+    });
+}
+
+int mf_detect_template(mf_handle *h, int it, int absoluteValue, double threshold, int minDist, int *idx, int cap,
+                       int *nPeaks)
+{
+    return guard([&] {
+        if (nPeaks == nullptr || (idx == nullptr && cap > 0)) { fail(MF_ERR_INVALID_ARGUMENT, "output is NULL"); }
+        requireInit(h);
+        if (!h->applied) { fail(MF_ERR_BAD_STATE, "Matched filtered signals not yet computed"); }
+        checkRow(h, it);
+        const long long need = h->rows[it].nOut;
+        growTo(h->dRedDet, static_cast<size_t>(std::max(need, 1LL)) * h->tsize);
+        growTo(h->dRedId, static_cast<size_t>(std::max(need, 1LL)) * sizeof(int));
+        const long long n = reduceRows(h, std::vector<int>{it}, need, absoluteValue, h->dRedDet.p,
+                                       static_cast<int *>(h->dRedId.p), need);
+        MF_HIP(hipStreamSynchronize(h->stream));
+        peaks(h->dRedDet.p, true, static_cast<int>(n), h->precision, threshold, minDist, idx, cap, nPeaks);
     });
 }
 

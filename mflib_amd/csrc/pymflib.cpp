@@ -11,7 +11,10 @@
 //  - get_signal_length / get_filtered_signal_length;
 //  - MultiChannel.MatchedFilterParameters.set/get_stack_absolute_values;
 //  - PeakFinder;
-//  - apply() releases the GIL while the GPU works.
+//  - SingleChannel.DetectionTimeInterpolationParameters and
+//    DetectorParameters.set/get_detection_time_interpolation_parameters
+//    (the reference's bindings leave the interpolation at its default);
+//  - apply() and detect() release the GIL while the GPU works.
 #include <pybind11/numpy.h>
 #include <pybind11/pybind11.h>
 #include <pybind11/stl.h>
@@ -26,6 +29,10 @@
 #include "mflib/networkStationPhase.hpp"
 #include "mflib/singleChannel/matchedFilter.hpp"
 #include "mflib/singleChannel/matchedFilterParameters.hpp"
+#include "mflib/singleChannel/detection.hpp"
+#include "mflib/singleChannel/detectionTimeInterpolationParameters.hpp"
+#include "mflib/singleChannel/detector.hpp"
+#include "mflib/singleChannel/detectorParameters.hpp"
 #include "mflib/version.hpp"
 #include "mflib/waveformTemplate.hpp"
 #include "private/peakFinder.hpp"
@@ -261,6 +268,109 @@ void bindSingleChannel(py::module_ &sc)
     });
 }
 
+// ---- detector (python/singleChannel/detector.cpp, detectorParameters.cpp,
+//      detection.cpp) ------------------------------------------------------
+void bindDetector(py::module_ &sc)
+{
+    using I = MFLib::SingleChannel::DetectionTimeInterpolationParameters;
+    py::class_<I> ip(sc, "DetectionTimeInterpolationParameters");
+    ip.def(py::init<>());
+    ip.doc() = "Sub-sample refinement of detection times: quadratic (default), Lanczos or none.";
+    ip.def("clear", &I::clear);
+    ip.def("enable_lanczos", &I::enableLanczos);
+    ip.def("enable_quadratic", &I::enableQuadratic);
+    ip.def("disable", &I::disable);
+    ip.def("set_lanczos_alpha", &I::setLanczosAlpha);
+    ip.def("get_lanczos_alpha", &I::getLanczosAlpha);
+    ip.def("set_lanczos_number_of_interpolation_points", &I::setLanczosNumberOfInterpolationPoints);
+    ip.def("get_lanczos_number_of_interpolation_points", &I::getLanczosNumberOfInterpolationPoints);
+    ip.def("get_type", &I::getType);
+
+    using P = MFLib::SingleChannel::DetectorParameters;
+    py::class_<P> p(sc, "DetectorParameters");
+    p.def(py::init<>());
+    p.doc() = "Threshold, spacing, maxima and reduction policies, and products of the single-channel detector.";
+    p.def("clear", &P::clear);
+    p.def("get_maxima_policy", &P::getMaximaPolicy);
+    p.def("set_maxima_policy", &P::setMaximaPolicy);
+    p.def("get_matched_filtered_signal_detector_policy", &P::getMatchedFilteredSignalDetectorPolicy);
+    p.def("set_matched_filtered_signal_detector_policy", &P::setMatchedFilteredSignalDetectorPolicy);
+    p.def("get_minimum_detection_spacing", &P::getMinimumDetectionSpacing);
+    p.def("set_minimum_detection_spacing", &P::setMinimumDetectionSpacing);
+    p.def("get_detection_threshold", &P::getDetectionThreshold);
+    p.def("set_detection_threshold", &P::setDetectionThreshold);
+    p.def("enable_save_detected_waveform", &P::enableSaveDetectedWaveform);
+    p.def("disable_save_detected_waveform", &P::disableSaveDetectedWaveform);
+    p.def("want_detected_waveform", &P::wantDetectedWaveform);
+    p.def("enable_save_amplitude_scaling_factor", &P::enableSaveAmplitudeScalingFactor);
+    p.def("disable_save_amplitude_scaling_factor", &P::disableSaveAmplitudeScalingFactor);
+    p.def("want_amplitude_scaling_factor", &P::wantAmplitudeScalingFactor);
+    p.def("set_detection_time_interpolation_parameters", &P::setDetectionTimeInterpolationParameters);
+    p.def("get_detection_time_interpolation_parameters",
+          [](const P &parms) { return I(parms.getDetectionTimeInterpolationParameters()); });
+
+    using D = MFLib::SingleChannel::Detection<double>;
+    using R = MFLib::RelativeMagnitudeType;
+    py::class_<D> d(sc, "Detection");
+    d.def(py::init<>());
+    d.doc() = "One detection: template, correlation, times, polarity, amplitude scaling factors, waveform.";
+    d.def("clear", &D::clear);
+    d.def("set_template_identifier", [](D &x, const py::sequence &id) {
+        if (py::len(id) != 2) { throw std::invalid_argument("identifier must be [NetworkStationPhase, int]"); }
+        x.setTemplateIdentifier({id[0].cast<MFLib::NetworkStationPhase>(), id[1].cast<uint64_t>()});
+    });
+    d.def("get_template_identifier", [](const D &x) {
+        auto id = x.getTemplateIdentifier();
+        return py::make_tuple(id.first, id.second);
+    });
+    d.def("have_template_identifier", &D::haveTemplateIdentifier);
+    d.def("set_correlation_coefficient", &D::setCorrelationCoefficient);
+    d.def("get_correlation_coefficient", &D::getCorrelationCoefficient);
+    d.def("have_correlation_coefficient", &D::haveCorrelationCoefficient);
+    d.def("set_detected_signal", [](D &x, DArray a) { x.setDetectedSignal(arrayLength(a), a.data()); });
+    d.def("get_detected_signal", [](const D &x) {
+        return toArray(x.getDetectedSignalPointer(), x.getDetectedSignalLength());
+    });
+    d.def("have_detected_signal", &D::haveDetectedSignal);
+    d.def("set_detection_time", &D::setDetectionTime);
+    d.def("get_detection_time", &D::getDetectionTime);
+    d.def("have_detection_time", &D::haveDetectionTime);
+    d.def("set_interpolated_detection_time", &D::setInterpolatedDetectionTime);
+    d.def("get_interpolated_detection_time", &D::getInterpolatedDetectionTime);
+    d.def("have_interpolated_detection_time", &D::haveInterpolatedDetectionTime);
+    d.def("set_phase_onset_time", &D::setPhaseOnsetTime);
+    d.def("get_phase_onset_time", &D::getPhaseOnsetTime);
+    d.def("have_phase_onset_time", &D::havePhaseOnsetTime);
+    d.def("set_interpolated_phase_onset_time", &D::setInterpolatedPhaseOnsetTime);
+    d.def("get_interpolated_phase_onset_time", &D::getInterpolatedPhaseOnsetTime);
+    d.def("have_interpolated_phase_onset_time", &D::haveInterpolatedPhaseOnsetTime);
+    d.def("set_travel_time", &D::setTravelTime);
+    d.def("get_travel_time", &D::getTravelTime);
+    d.def("have_travel_time", &D::haveTravelTime);
+    d.def("set_polarity", &D::setPolarity);
+    d.def("get_polarity", &D::getPolarity);
+    d.def("set_amplitude_scaling_factor", &D::setAmplitudeScalingFactor, py::arg("value"),
+          py::arg("type") = R::GIBBONS_RINGDAL_2006);
+    d.def("get_amplitude_scaling_factor", &D::getAmplitudeScalingFactor, py::arg("type") = R::GIBBONS_RINGDAL_2006);
+    d.def("have_amplitude_scaling_factor", &D::haveAmplitudeScalingFactor,
+          py::arg("type") = R::GIBBONS_RINGDAL_2006);
+    d.def("get_magnitude_perturbation", &D::getMagnitudePerturbation, py::arg("type") = R::GIBBONS_RINGDAL_2006);
+
+    using X = MFLib::SingleChannel::Detector<double>;
+    py::class_<X> x(sc, "Detector");
+    x.def(py::init<>());
+    x.doc() = "Detections from a single-channel matched filter, computed on the GPU engine.";
+    x.def("initialize", &X::initialize);
+    x.def("detect", &X::detect, py::call_guard<py::gil_scoped_release>());
+    x.def("get_number_of_detections", &X::getNumberOfDetections);
+    x.def("get_detection", &X::getDetection);
+    x.def("__len__", &X::getNumberOfDetections);
+    x.def("__getitem__", [](const X &det, int i) {
+        if (i < 0 || i >= det.getNumberOfDetections()) { throw py::index_error("detection index out of range"); }
+        return det.getDetection(i);
+    });
+}
+
 void bindPeakFinder(py::module_ &m)
 {
     using F = MFLib::PeakFinder<double>;
@@ -315,6 +425,7 @@ PYBIND11_MODULE(pymflib, m)
         .value("none", MFLib::DetectionTimeInterpolationType::NONE);
     py::module_ sc = m.def_submodule("SingleChannel");
     bindSingleChannel(sc);
+    bindDetector(sc);
     py::module_ mc = m.def_submodule("MultiChannel");
     bindMultiChannel(mc);
     bindPeakFinder(m);

@@ -6,6 +6,8 @@
 //   testing/matchedFilter.cpp:275-445 shiftAndStack     (stack max 1 +- 1e-3 at the origin)
 //   testing/singleChannel.cpp:104-160 single channel    (rho within 1e-12)
 //   testing/peakFinder.cpp:16-120     PeakFinder KATs   (13 and 21 indices)
+//   testing/detector.cpp:100-283      Detector          (12 detections: times, templates,
+//                                                        interpolated times, scaling factors)
 // Expected correlations are the two-pass FP64 Pearson definition
 // (testing/utilities.hpp:33-69).  Prints one PASS/FAIL line per case and
 // exits non-zero on any failure.
@@ -22,6 +24,11 @@
 #include "mflib/matchedFilterParameters.hpp"
 #include "mflib/singleChannel/matchedFilter.hpp"
 #include "mflib/singleChannel/matchedFilterParameters.hpp"
+#include "mflib/networkStationPhase.hpp"
+#include "mflib/singleChannel/detection.hpp"
+#include "mflib/singleChannel/detectionTimeInterpolationParameters.hpp"
+#include "mflib/singleChannel/detector.hpp"
+#include "mflib/singleChannel/detectorParameters.hpp"
 #include "mflib/waveformTemplate.hpp"
 #include "private/peakFinder.hpp"
 
@@ -388,6 +395,124 @@ void peakFinderTest()
     expect(std::vector<int>(pf13, pf13 + pff.getNumberOfPeaks()) == ref13, "13-peak KAT (float)");
 }
 
+// testing/detector.cpp:100-283 (createTemplateSignal 285-301): five decaying
+// sines; templates 0, 1, 2, 4 written three times each, phase-shifted and
+// scaled by 1/4 (even) or 1/2 (odd), into 120 s of uniform noise.
+std::vector<double> decayingSine(int npts, double amp, double dt, double f, double decay, double shift)
+{
+    std::vector<double> r(npts);
+    for (int i = 0; i < npts; ++i)
+    {
+        const double t = i * dt;
+        r[i] = amp * std::exp(-t * decay) * std::sin(2 * M_PI * f * (t - shift));
+    }
+    return r;
+}
+
+template<class T>
+void detectorTest(MFLib::MatchedFilteredSignalDetectorPolicy policy, bool lanczos)
+{
+    const double df = 100, dt = 1 / df;
+    const int minSpacing = static_cast<int>(4.0 / 2 / dt);
+    const int nSignal = static_cast<int>(120.0 / dt + 0.5) + 1;
+    const int nTemplate = static_cast<int>(4.0 / dt + 0.5) + 1;
+    const double amp[5] = {1.0, 2.0, 0.5, 0.75, 3.0}, freq[5] = {10.0, 5.0, 7.5, 12.5, 2.5};
+    const double decay[5] = {2.8, 2.9, 2.6, 2.2, 2.3}, phi[5] = {0.001, 0.002, 0.004, -0.003, -0.002};
+    const std::vector<std::vector<double>> inserts = {{4, 28, 90}, {8, 24, 100}, {33, 66, 86}, {}, {16, 44, 106}};
+    std::mt19937_64 gen(0xDE7EC7);
+    std::uniform_real_distribution<double> noise(-0.05, 0.05);
+    std::vector<double> signal(nSignal);
+    for (auto &v : signal) { v = noise(gen); }
+    MFLib::NetworkStationPhase nsp;
+    nsp.setNetwork("UU");
+    nsp.setStation("CCUT");
+    nsp.setPhase("P");
+    struct Expected { double onset, shifted, scale; int it; };
+    std::vector<Expected> expected;
+    MFLib::SingleChannel::MatchedFilterParameters parms;
+    for (int i = 0; i < 5; ++i)
+    {
+        MFLib::WaveformTemplate wt = makeTemplate(decayingSine(nTemplate, amp[i], dt, freq[i], decay[i], 0), df);
+        wt.setIdentifier({nsp, static_cast<uint64_t>(i)});
+        parms.addTemplate(wt);
+        const double scale = 1.0 / (i % 2 == 0 ? 4.0 : 2.0);
+        const auto w = decayingSine(nTemplate, scale * amp[i], dt, freq[i], decay[i], phi[i]);
+        for (double toff : inserts[i])
+        {
+            const int k = static_cast<int>(toff / dt + 0.5);
+            std::copy(w.begin(), w.end(), signal.begin() + k);
+            expected.push_back({toff, toff + phi[i], scale, i});
+        }
+    }
+    std::sort(expected.begin(), expected.end(), [](const Expected &a, const Expected &b) { return a.onset < b.onset; });
+    parms.setSignalSize(nSignal);
+    MFLib::SingleChannel::MatchedFilter<T> mf;
+    mf.initialize(parms);
+    std::vector<T> sig(signal.begin(), signal.end());
+    mf.setSignal(nSignal, sig.data());
+    mf.apply();
+
+    MFLib::SingleChannel::DetectionTimeInterpolationParameters interp;
+    if (lanczos) { interp.enableLanczos(); }
+    expect(interp.getLanczosAlpha() == 51 && interp.getLanczosNumberOfInterpolationPoints() == 100,
+           "interpolation defaults");
+    MFLib::SingleChannel::DetectorParameters dp;
+    expect(dp.getDetectionThreshold() == 0.7 && dp.wantAmplitudeScalingFactor() && !dp.wantDetectedWaveform(),
+           "detector defaults");
+    dp.setDetectionTimeInterpolationParameters(interp);
+    dp.setMinimumDetectionSpacing(minSpacing);
+    dp.setMatchedFilteredSignalDetectorPolicy(policy);
+    dp.enableSaveDetectedWaveform();
+    dp.enableSaveAmplitudeScalingFactor();
+    MFLib::SingleChannel::Detector<T> detector;
+    expect(throwsType<std::invalid_argument>([&] { detector.detect(mf); }), "detect before initialize");
+    detector.initialize(dp);
+    detector.detect(mf);
+    const int ndet = detector.getNumberOfDetections();
+    if (policy == MFLib::MatchedFilteredSignalDetectorPolicy::REDUCED)
+    {
+        expect(ndet == static_cast<int>(expected.size()), "detections: " + std::to_string(ndet));
+    }
+    // one peak list per template also finds the other templates' events
+    // (cross-correlations above 0.7); the written ones are among them
+    int matched = 0;
+    double lastTime = -1;
+    for (int i = 0; i < ndet; ++i)
+    {
+        const auto &d = detector[i];
+        expect(d.getDetectionTime() >= lastTime, "detections sorted by time");
+        lastTime = d.getDetectionTime();
+        for (const auto &e : expected)
+        {
+            if (std::abs(d.getDetectionTime() - e.onset) > 1e-10 ||
+                d.getTemplateIdentifier().second != static_cast<uint64_t>(e.it))
+            {
+                continue;
+            }
+            ++matched;
+            expect(d.getTemplateIdentifier().first == nsp, "template identifier");
+            const double tol = lanczos ? dt / 2 : dt;
+            expect(std::abs(e.shifted - d.getInterpolatedDetectionTime()) < tol,
+                   "interpolated time " + std::to_string(d.getInterpolatedDetectionTime()) + " vs " +
+                       std::to_string(e.shifted));
+            const double a1 = d.getAmplitudeScalingFactor(MFLib::RelativeMagnitudeType::GIBBONS_RINGDAL_2006);
+            const double a2 = d.getAmplitudeScalingFactor(MFLib::RelativeMagnitudeType::SCHAFF_RICHARDS_2014);
+            expect(std::abs(a1 - e.scale) < 1e-2 && std::abs(a2 - e.scale) < 1e-2,
+                   "scaling factors " + std::to_string(a1) + ", " + std::to_string(a2));
+            expect(std::abs(d.getMagnitudePerturbation(MFLib::RelativeMagnitudeType::GIBBONS_RINGDAL_2006) -
+                            std::log10(a1)) < 1e-6, "magnitude perturbation");
+            expect(d.getCorrelationCoefficient() > 0.7 && d.getCorrelationCoefficient() <= 1, "coefficient");
+            expect(d.getPolarity() == MFLib::Polarity::UNKNOWN, "unknown polarity stays unknown");
+            expect(d.getDetectedSignalLength() == nTemplate, "detected waveform length");
+            expect(d.getDetectedSignalPointer()[0] == sig[static_cast<size_t>(d.getDetectionTime() / dt + 0.5)],
+                   "detected waveform samples");
+            expect(!d.havePhaseOnsetTime() && !d.haveTravelTime(), "no onset or travel time on these templates");
+        }
+    }
+    expect(matched == static_cast<int>(expected.size()), "matched " + std::to_string(matched) + " of 12");
+    expect(throwsType<std::invalid_argument>([&] { (void)detector[ndet]; }), "index past the end");
+}
+
 void errorConventionTest()
 {
     const auto in = basicInputs();
@@ -426,6 +551,15 @@ int main()
     runCase("shiftAndStack", shiftAndStackTest);
     runCase("singleChannel", singleChannelTest);
     runCase("peakFinder", peakFinderTest);
+    runCase("detectorReducedLanczos", [] {
+        detectorTest<double>(MFLib::MatchedFilteredSignalDetectorPolicy::REDUCED, true);
+    });
+    runCase("detectorReducedQuadraticFloat", [] {
+        detectorTest<float>(MFLib::MatchedFilteredSignalDetectorPolicy::REDUCED, false);
+    });
+    runCase("detectorSingleLanczos", [] {
+        detectorTest<double>(MFLib::MatchedFilteredSignalDetectorPolicy::SINGLE, true);
+    });
     runCase("errorConventions", errorConventionTest);
     std::printf("%d failure(s)\n", failures);
     return failures == 0 ? 0 : 1;

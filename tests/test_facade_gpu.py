@@ -129,3 +129,46 @@ def test_peak_finder_kats():
     pf.set_minimum_peak_distance(0)
     pf.apply()
     assert pf.get_peak_indices().tolist() == ref21
+
+
+def test_pymflib_detector_kat():
+    """testing/detector.cpp:100-283 through pymflib: SingleChannel.Detector
+    (REDUCED policy, Lanczos) on a SingleChannel.MatchedFilter; the
+    detections' times, templates, interpolated times and scaling factors."""
+    from refcases import detector_inputs
+    signal, tpls, dt, expected = detector_inputs(1)
+    sc = pymflib.SingleChannel
+    parms = sc.MatchedFilterParameters()
+    nsp = pymflib.NetworkStationPhase()
+    nsp.set_station("CCUT")
+    for i, t in enumerate(tpls):
+        w = pymflib.WaveformTemplate()
+        w.set_signal(t)
+        w.set_sampling_rate(1 / dt)
+        w.set_identifier([nsp, i])
+        parms.add_template(w)
+    parms.set_signal_size(len(signal))
+    mf = sc.MatchedFilter()
+    mf.initialize(parms)
+    mf.set_signal(signal)
+    mf.apply()
+    ip = sc.DetectionTimeInterpolationParameters()
+    ip.enable_lanczos()
+    dp = sc.DetectorParameters()
+    dp.set_detection_time_interpolation_parameters(ip)
+    dp.set_minimum_detection_spacing(int(4.0 / 2 / dt))
+    dp.set_matched_filtered_signal_detector_policy(pymflib.MatchedFilteredSignalDetectorPolicy.reduce)
+    dp.enable_save_detected_waveform()
+    det = sc.Detector()
+    det.initialize(dp)
+    det.detect(mf)
+    assert det.get_number_of_detections() == len(expected) == 12
+    R = pymflib.RelativeMagnitudeType
+    for d, (onset, shifted, scale, it) in zip(det, expected):
+        assert abs(d.get_detection_time() - onset) < 1e-10
+        assert d.get_template_identifier()[1] == it
+        assert abs(shifted - d.get_interpolated_detection_time()) < dt / 2
+        assert abs(d.get_amplitude_scaling_factor(R.gibbons_ringdal_2006) - scale) < 1e-2
+        assert abs(d.get_amplitude_scaling_factor(R.schaff_richards_2014) - scale) < 1e-2
+        k = int(round(onset / dt))
+        assert np.array_equal(d.get_detected_signal(), signal[k:k + len(tpls[it])])
