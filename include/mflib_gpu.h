@@ -123,6 +123,9 @@ int mf_get_signal(mf_handle *h, int channel, void *out, int cap);
  * of every template row.  Enqueues work on the handle's stream. */
 int mf_apply(mf_handle *h);
 int mf_synchronize(mf_handle *h);
+/* No reference counterpart: the HIP device the handle's buffers live on (the
+ * device current at mf_create), for callers exchanging device pointers. */
+int mf_get_device(const mf_handle *h, int *device);
 int mf_have_matched_filtered_signals(const mf_handle *h, int *yes);
 
 /* getMatchedFilteredSignal(it) (940-949): copies the row's

@@ -38,6 +38,15 @@ struct Access
 {
     template<class T> static mf_handle *handle(const MFLib::MatchedFilter<T> &mf);
     template<class T> static const MFLib::MatchedFilter<T> &inner(const MFLib::SingleChannel::MatchedFilter<T> &mf);
+    template<class T> static MFLib::MatchedFilter<T> &inner(MFLib::SingleChannel::MatchedFilter<T> &mf);
+    // Device-memory exchange (no reference counterpart), used by pymflib's
+    // DLPack methods: the signal of row it from device memory on the
+    // handle's device (the host copy behind getSignalPointer is fetched on
+    // demand), and the device address of correlation row it after apply(),
+    // its work complete; valid until the next apply(), initialize() or clear().
+    template<class T> static void setSignalDevice(MFLib::MatchedFilter<T> &mf, int it, int nSamples, const T *dx);
+    template<class T> static const T *rhoDevice(const MFLib::MatchedFilter<T> &mf, int it);
+    template<class T> static int device(const MFLib::MatchedFilter<T> &mf);
 };
 template<class T> constexpr int precisionOf();
 template<> constexpr int precisionOf<float>() { return MF_PRECISION_FLOAT; }

@@ -26,6 +26,7 @@ public:
         if (handle) { mf_destroy(handle); }
         handle = nullptr;
         signals.clear();
+        signalOnDevice.clear();
         rho.clear();
         rhoReady.clear();
         nTemplates = 0;
@@ -51,6 +52,7 @@ public:
     mf_handle *handle = nullptr;
     MatchedFilterParameters parameters;
     std::vector<std::vector<T>> signals;   // per template row, host copy
+    std::vector<uint8_t> signalOnDevice;   // row set from device memory: host copy fetched on demand
     std::vector<std::vector<T>> rho;       // per template row, host mirror
     std::vector<uint8_t> rhoReady;
     int nTemplates = 0;
@@ -138,6 +140,7 @@ void MatchedFilter<T>::initialize(const MatchedFilterParameters &parms)
     d.nSamples = n;
     d.singleChannel = single;
     d.signals.assign(nt, std::vector<T>());  // rows filled on first use
+    d.signalOnDevice.assign(nt, 0);
     d.rho.assign(nt, std::vector<T>());
     d.rhoReady.assign(nt, 0);
 }
@@ -167,14 +170,22 @@ void MatchedFilter<T>::setSignal(int it, int nSamples, const T signal[])
         Facade::check(mf_set_signal(d.handle, it, nSamples, signal), "setSignal");
     }
     d.signals[it].assign(signal, signal + nSamples);
+    d.signalOnDevice[it] = 0;
 }
 
 template<class T>
 const T* MatchedFilter<T>::getSignalPointer(int it) const
 {
     pImpl->checkIndex(it);
-    auto &row = pImpl->signals[it];
-    if (row.empty()) { row.assign(pImpl->nSamples, T(0)); }
+    auto &d = *pImpl;
+    auto &row = d.signals[it];
+    if (d.signalOnDevice[it])
+    {
+        row.assign(d.nSamples, T(0));
+        Facade::check(mf_get_signal(d.handle, d.singleChannel ? 0 : it, row.data(), d.nSamples), "getSignalPointer");
+        d.signalOnDevice[it] = 0;
+    }
+    if (row.empty()) { row.assign(d.nSamples, T(0)); }
     return row.data();
 }
 
@@ -185,6 +196,7 @@ void MatchedFilter<T>::zeroSignal(int it)
     d.checkIndex(it);
     if (!d.singleChannel || it == 0) { Facade::check(mf_zero_signal(d.handle, it), "zeroSignal"); }
     d.signals[it].assign(d.nSamples, T(0));
+    d.signalOnDevice[it] = 0;
 }
 
 template<class T>
@@ -295,3 +307,51 @@ mf_handle *MFLib::Facade::Access::handle(const MFLib::MatchedFilter<T> &mf)
 }
 template mf_handle *MFLib::Facade::Access::handle<float>(const MFLib::MatchedFilter<float> &);
 template mf_handle *MFLib::Facade::Access::handle<double>(const MFLib::MatchedFilter<double> &);
+
+template<class T>
+void MFLib::Facade::Access::setSignalDevice(MFLib::MatchedFilter<T> &mf, int it, int nSamples, const T *dx)
+{
+    auto &d = *mf.pImpl;
+    d.checkIndex(it);
+    if (nSamples != d.nSamples)
+    {
+        throw std::invalid_argument("Number of samples = " + std::to_string(nSamples) + " must equal " +
+                                    std::to_string(d.nSamples));
+    }
+    if (dx == nullptr) { throw std::invalid_argument("signal is NULL"); }
+    if (!d.singleChannel || it == 0)
+    {
+        Facade::check(mf_set_signal_device(d.handle, d.singleChannel ? 0 : it, nSamples, dx), "setSignal");
+    }
+    d.signals[it].clear();
+    d.signalOnDevice[it] = 1;
+}
+
+template<class T>
+const T *MFLib::Facade::Access::rhoDevice(const MFLib::MatchedFilter<T> &mf, int it)
+{
+    auto &d = *mf.pImpl;
+    d.checkIndex(it);
+    if (!d.applied) { throw std::runtime_error("Matched filtered signals not yet computed"); }
+    const void *p = nullptr;
+    Facade::check(mf_get_rho_device(d.handle, it, &p), "getMatchedFilterSignalPointer");
+    Facade::check(mf_synchronize(d.handle), "getMatchedFilterSignalPointer");
+    return static_cast<const T *>(p);
+}
+
+template<class T>
+int MFLib::Facade::Access::device(const MFLib::MatchedFilter<T> &mf)
+{
+    mf.pImpl->requireInitialized();
+    int dev = 0;
+    Facade::check(mf_get_device(mf.pImpl->handle, &dev), "device");
+    return dev;
+}
+
+template void MFLib::Facade::Access::setSignalDevice<float>(MFLib::MatchedFilter<float> &, int, int, const float *);
+template void MFLib::Facade::Access::setSignalDevice<double>(MFLib::MatchedFilter<double> &, int, int,
+                                                             const double *);
+template const float *MFLib::Facade::Access::rhoDevice<float>(const MFLib::MatchedFilter<float> &, int);
+template const double *MFLib::Facade::Access::rhoDevice<double>(const MFLib::MatchedFilter<double> &, int);
+template int MFLib::Facade::Access::device<float>(const MFLib::MatchedFilter<float> &);
+template int MFLib::Facade::Access::device<double>(const MFLib::MatchedFilter<double> &);

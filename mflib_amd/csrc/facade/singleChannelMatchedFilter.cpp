@@ -67,3 +67,12 @@ template const MFLib::MatchedFilter<float> &MFLib::Facade::Access::inner<float>(
     const MFLib::SingleChannel::MatchedFilter<float> &);
 template const MFLib::MatchedFilter<double> &MFLib::Facade::Access::inner<double>(
     const MFLib::SingleChannel::MatchedFilter<double> &);
+
+template<class T>
+MFLib::MatchedFilter<T> &MFLib::Facade::Access::inner(MFLib::SingleChannel::MatchedFilter<T> &mf)
+{
+    return mf.pImpl->mf;
+}
+template MFLib::MatchedFilter<float> &MFLib::Facade::Access::inner<float>(MFLib::SingleChannel::MatchedFilter<float> &);
+template MFLib::MatchedFilter<double> &MFLib::Facade::Access::inner<double>(
+    MFLib::SingleChannel::MatchedFilter<double> &);

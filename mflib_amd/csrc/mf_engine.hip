@@ -36,6 +36,7 @@ using namespace mfh;
 struct mf_handle {
     int precision = MF_PRECISION_FLOAT;
     int mode = MF_MODE_MULTI_CHANNEL;
+    int device = 0;  // the HIP device current at mf_create; all buffers live there
     size_t tsize = 4;
     int signalSize = 0;
     int absval = 1;
@@ -1055,7 +1056,8 @@ int mf_create(mf_handle **h, int precision, int mode)
         p->precision = precision;
         p->mode = mode;
         p->tsize = precision == MF_PRECISION_FLOAT ? 4 : 8;
-        hipError_t e = hipStreamCreate(&p->stream);
+        hipError_t e = hipGetDevice(&p->device);
+        if (e == hipSuccess) { e = hipStreamCreate(&p->stream); }
         if (e == hipSuccess) { e = hipEventCreate(&p->evA); }
         if (e == hipSuccess) { e = hipEventCreate(&p->evB); }
         if (e == hipSuccess) { e = hipEventCreate(&p->evC); }
@@ -1620,6 +1622,15 @@ int mf_detection_products(mf_handle *h, int nPeaks, const int *peakIdx, const in
             if (alphaSR) { alphaSR[k] = res[k].alphaSR; }
             if (flags) { flags[k] = res[k].flags; }
         }
+    });
+}
+
+int mf_get_device(const mf_handle *h, int *device)
+{
+    return guard([&] {
+        requireHandle(h);
+        if (device == nullptr) { fail(MF_ERR_INVALID_ARGUMENT, "output is NULL"); }
+        *device = h->device;
     });
 }
 

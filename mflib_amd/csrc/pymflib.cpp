@@ -14,11 +14,16 @@
 //  - SingleChannel.DetectionTimeInterpolationParameters and
 //    DetectorParameters.set/get_detection_time_interpolation_parameters
 //    (the reference's bindings leave the interpolation at its default);
-//  - apply() and detect() release the GIL while the GPU works.
+//  - apply() and detect() release the GIL while the GPU works;
+//  - device arrays through DLPack: set_signal also takes a float64 array
+//    on the engine's GPU (a torch tensor, say), copied HBM to HBM with no
+//    host round trip, and get_matched_filtered_signal_device returns a
+//    zero-copy DLPack view of a correlation row (torch.from_dlpack).
 #include <pybind11/numpy.h>
 #include <pybind11/pybind11.h>
 #include <pybind11/stl.h>
 
+#include <cstdint>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -36,6 +41,7 @@
 #include "mflib/version.hpp"
 #include "mflib/waveformTemplate.hpp"
 #include "private/peakFinder.hpp"
+#include "facade/abi.hpp"
 
 namespace py = pybind11;
 using DArray = py::array_t<double, py::array::c_style | py::array::forcecast>;
@@ -62,6 +68,164 @@ int arrayLength(const DArray &x)
     if (x.ndim() != 1) { throw std::invalid_argument("signal must be one-dimensional"); }
     if (x.size() > 0x7fffffff) { throw std::invalid_argument("signal is too long"); }
     return static_cast<int>(x.size());
+}
+
+// ---- DLPack -------------------------------------------------------------
+// The array-exchange structures of dlpack.h (ABI 0.x, as torch, numpy and
+// cupy use them): a "dltensor" capsule holds a DLManagedTensor.
+namespace dl
+{
+struct Device
+{
+    int32_t type;
+    int32_t id;
+};
+struct DataType
+{
+    uint8_t code;
+    uint8_t bits;
+    uint16_t lanes;
+};
+struct Tensor
+{
+    void *data;
+    Device device;
+    int32_t ndim;
+    DataType dtype;
+    int64_t *shape;
+    int64_t *strides;
+    uint64_t byteOffset;
+};
+struct Managed
+{
+    Tensor tensor;
+    void *ctx;
+    void (*deleter)(Managed *);
+};
+constexpr int32_t kCPU = 1;
+constexpr int32_t kROCm = 10;
+constexpr uint8_t kFloat = 2;
+}  // namespace dl
+
+// A correlation row in HBM, exported through __dlpack__.  `owner` keeps the
+// matched filter (and so its device buffers) alive while any view exists.
+struct DeviceRow
+{
+    py::object owner;
+    const double *data = nullptr;
+    int64_t n = 0;
+    int device = 0;
+};
+
+struct RowCtx
+{
+    py::object owner;
+    int64_t shape[1];
+    int64_t strides[1];
+};
+
+void rowDeleter(dl::Managed *m)
+{
+    // consumers may call this without the GIL
+    PyGILState_STATE g = PyGILState_Ensure();
+    delete static_cast<RowCtx *>(m->ctx);
+    delete m;
+    PyGILState_Release(g);
+}
+
+void capsuleDestructor(PyObject *cap)
+{
+    // a capsule still named "dltensor" was never consumed: free it here
+    if (PyCapsule_IsValid(cap, "dltensor"))
+    {
+        auto *m = static_cast<dl::Managed *>(PyCapsule_GetPointer(cap, "dltensor"));
+        if (m && m->deleter) { m->deleter(m); }
+    }
+}
+
+py::object exportRow(const DeviceRow &r)
+{
+    auto *ctx = new RowCtx{r.owner, {r.n}, {1}};
+    auto *m = new dl::Managed{};
+    m->tensor.data = const_cast<double *>(r.data);
+    m->tensor.device = {dl::kROCm, r.device};
+    m->tensor.ndim = 1;
+    m->tensor.dtype = {dl::kFloat, 64, 1};
+    m->tensor.shape = ctx->shape;
+    m->tensor.strides = ctx->strides;
+    m->tensor.byteOffset = 0;
+    m->ctx = ctx;
+    m->deleter = rowDeleter;
+    PyObject *cap = PyCapsule_New(m, "dltensor", capsuleDestructor);
+    if (!cap)
+    {
+        rowDeleter(m);
+        throw py::error_already_set();
+    }
+    return py::reinterpret_steal<py::object>(cap);
+}
+
+void bindDeviceRow(py::module_ &m)
+{
+    py::class_<DeviceRow> c(m, "DeviceRow");
+    c.doc() = "A correlation row in GPU memory (float64), exported zero-copy through DLPack.  The next "
+              "apply() overwrites it in place; initialize() or clear() on its filter invalidates it.";
+    c.def("__dlpack__", [](const DeviceRow &r, py::object, py::object, py::object, py::object) {
+        return exportRow(r);
+    }, py::arg("stream") = py::none(), py::arg("max_version") = py::none(), py::arg("dl_device") = py::none(),
+       py::arg("copy") = py::none());
+    c.def("__dlpack_device__", [](const DeviceRow &r) { return py::make_tuple(dl::kROCm, r.device); });
+    c.def("__len__", [](const DeviceRow &r) { return r.n; });
+    c.def_property_readonly("device", [](const DeviceRow &r) { return r.device; });
+}
+
+// A float64 signal on the GPU given through DLPack: (device pointer, n).
+// Returns false when x is not a GPU array (the caller treats it as a host
+// array).  The producer's pending work is ordered before the engine's by
+// passing stream 0 (the default stream, with which the engine's blocking
+// stream synchronizes).
+bool deviceSignal(const py::object &x, int device, py::object &capsule, const double *&ptr, int &n)
+{
+    if (!py::hasattr(x, "__dlpack_device__") || !py::hasattr(x, "__dlpack__")) { return false; }
+    auto dev = x.attr("__dlpack_device__")().cast<py::tuple>();
+    if (dev[0].cast<int>() != dl::kROCm) { return false; }
+    if (dev[1].cast<int>() != device)
+    {
+        throw std::invalid_argument("signal is on GPU " + std::to_string(dev[1].cast<int>()) +
+                                    ", the filter on GPU " + std::to_string(device));
+    }
+    capsule = x.attr("__dlpack__")(py::arg("stream") = 0);
+    auto *m = static_cast<dl::Managed *>(PyCapsule_GetPointer(capsule.ptr(), "dltensor"));
+    if (!m) { throw py::error_already_set(); }
+    const auto &t = m->tensor;
+    if (t.ndim != 1) { throw std::invalid_argument("signal must be one-dimensional"); }
+    if (t.dtype.code != dl::kFloat || t.dtype.bits != 64 || t.dtype.lanes != 1)
+    {
+        throw std::invalid_argument("a GPU signal must be float64");
+    }
+    if (t.strides && t.shape[0] > 1 && t.strides[0] != 1) { throw std::invalid_argument("signal must be contiguous"); }
+    if (t.shape[0] > 0x7fffffff) { throw std::invalid_argument("signal is too long"); }
+    n = static_cast<int>(t.shape[0]);
+    ptr = reinterpret_cast<const double *>(static_cast<const char *>(t.data) + t.byteOffset);
+    return true;
+}
+
+// setSignal from a host array or a GPU array (DLPack) for the FP64 filter
+// behind either Python class.  The capsule (and the producer's buffer)
+// lives until the copy into HBM has completed.
+void setSignalAny(MFLib::MatchedFilter<double> &mf, int it, const py::object &x)
+{
+    if (!mf.isInitialized()) { throw std::runtime_error("Class is not initialized"); }
+    py::object capsule;
+    const double *ptr = nullptr;
+    int n = 0;
+    if (deviceSignal(x, MFLib::Facade::Access::device(mf), capsule, ptr, n))
+    {
+        MFLib::Facade::Access::setSignalDevice(mf, it, n, ptr);
+        return;
+    }
+    DArray a = x.cast<DArray>();
+    mf.setSignal(it, arrayLength(a), a.data());
 }
 
 // ---- NetworkStationPhase (python/networkStationPhase.cpp) ---------------
@@ -222,10 +386,7 @@ void bindMultiChannel(py::module_ &mc)
         if (!parms.isValid()) { throw std::invalid_argument("Parameters are invalid"); }
         m.initialize(parms);
     });
-    c.def("set_signal", [](M &m, int it, DArray x) {
-        if (!m.isInitialized()) { throw std::runtime_error("Class is not initialized"); }
-        m.setSignal(it, arrayLength(x), x.data());
-    });
+    c.def("set_signal", [](M &m, int it, const py::object &x) { setSignalAny(m, it, x); });
     c.def("zero_signal", &M::zeroSignal);
     c.def("get_number_of_templates", &M::getNumberOfTemplates);
     c.def("get_signal_length", &M::getSignalLength);
@@ -236,6 +397,11 @@ void bindMultiChannel(py::module_ &mc)
         return toArray(m.getMatchedFilterSignalPointer(it), m.getFilteredSignalLength());
     });
     c.def("shift_and_stack", [](M &m) { return toArray(m.shiftAndStack()); });
+    c.def("get_matched_filtered_signal_device", [](py::object self, int it) {
+        const M &m = self.cast<const M &>();
+        return DeviceRow{self, MFLib::Facade::Access::rhoDevice(m, it), m.getFilteredSignalLength(),
+                         MFLib::Facade::Access::device(m)};
+    });
 }
 
 void bindSingleChannel(py::module_ &sc)
@@ -253,9 +419,9 @@ void bindSingleChannel(py::module_ &sc)
         if (!parms.isValid()) { throw std::invalid_argument("Parameters are invalid"); }
         m.initialize(parms);
     });
-    c.def("set_signal", [](M &m, DArray x) {
+    c.def("set_signal", [](M &m, const py::object &x) {
         if (!m.isInitialized()) { throw std::runtime_error("Class is not initialized"); }
-        m.setSignal(arrayLength(x), x.data());
+        setSignalAny(MFLib::Facade::Access::inner(m), 0, x);
     });
     c.def("zero_signal", &M::zeroSignal);
     c.def("get_number_of_templates", &M::getNumberOfTemplates);
@@ -265,6 +431,11 @@ void bindSingleChannel(py::module_ &sc)
     c.def("have_matched_filtered_signals", &M::haveMatchedFilteredSignals);
     c.def("get_matched_filtered_signal", [](const M &m, int it) {
         return toArray(m.getMatchedFilterSignalPointer(it), m.getFilteredSignalLength());
+    });
+    c.def("get_matched_filtered_signal_device", [](py::object self, int it) {
+        const auto &inner = MFLib::Facade::Access::inner(self.cast<const M &>());
+        return DeviceRow{self, MFLib::Facade::Access::rhoDevice(inner, it), inner.getFilteredSignalLength(),
+                         MFLib::Facade::Access::device(inner)};
     });
 }
 
@@ -402,6 +573,7 @@ void bindPeakFinder(py::module_ &m)
 PYBIND11_MODULE(pymflib, m)
 {
     m.attr("__version__") = MFLIB_VERSION;
+    bindDeviceRow(m);
     bindWaveformTemplate(m);
     bindNetworkStationPhase(m);
     py::enum_<MFLib::RelativeMagnitudeType>(m, "RelativeMagnitudeType")

@@ -10,12 +10,28 @@ import numpy as np
 _hip = None
 
 
+def _loaded_runtime():
+    """Path of the HIP runtime this process has already loaded (torch's own,
+    when torch was imported first), or None."""
+    try:
+        with open("/proc/self/maps") as f:
+            for line in f:
+                path = line.split()[-1]
+                if "libamdhip64.so" in os.path.basename(path):
+                    return path
+    except OSError:
+        pass
+    return None
+
+
 def hip():
     global _hip
     if _hip is None:
-        path = "/opt/rocm/lib/libamdhip64.so"
-        if not os.path.exists(path):
-            path = ctypes.util.find_library("amdhip64") or "libamdhip64.so"
+        path = _loaded_runtime()
+        if path is None:
+            path = "/opt/rocm/lib/libamdhip64.so"
+            if not os.path.exists(path):
+                path = ctypes.util.find_library("amdhip64") or "libamdhip64.so"
         h = ctypes.CDLL(path)
         h.hipMalloc.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_size_t]
         h.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]

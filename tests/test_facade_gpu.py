@@ -172,3 +172,92 @@ def test_pymflib_detector_kat():
         assert abs(d.get_amplitude_scaling_factor(R.schaff_richards_2014) - scale) < 1e-2
         k = int(round(onset / dt))
         assert np.array_equal(d.get_detected_signal(), signal[k:k + len(tpls[it])])
+
+
+def test_pymflib_dlpack_device_exchange():
+    """set_signal from a GPU array (torch, through DLPack) gives the rows a
+    host array gives, bit for bit; get_matched_filtered_signal_device is a
+    zero-copy view that torch.from_dlpack reads, overwritten in place by the
+    next apply()."""
+    torch = pytest.importorskip("torch")
+    rng = np.random.default_rng(17)
+    n = 50_000
+    tpls = [rng.standard_normal(300), rng.standard_normal(250)]
+    sigs = [rng.standard_normal(n) + 3.0 for _ in tpls]
+
+    def build():
+        parms = pymflib.MultiChannel.MatchedFilterParameters()
+        for t in tpls:
+            w = pymflib.WaveformTemplate()
+            w.set_signal(t)
+            w.set_sampling_rate(100.0)
+            parms.add_template(w)
+        parms.set_signal_size(n)
+        mf = pymflib.MultiChannel.MatchedFilter()
+        mf.initialize(parms)
+        return mf
+
+    host, dev = build(), build()
+    for i, x in enumerate(sigs):
+        host.set_signal(i, x)
+        dev.set_signal(i, torch.from_numpy(x).to("cuda"))
+    host.apply()
+    dev.apply()
+    for i in range(len(tpls)):
+        ref = host.get_matched_filtered_signal(i)
+        assert np.array_equal(dev.get_matched_filtered_signal(i), ref)
+        view = dev.get_matched_filtered_signal_device(i)
+        t = torch.from_dlpack(view)
+        assert t.is_cuda and t.dtype == torch.float64 and t.numel() == len(ref) == len(view)
+        assert np.array_equal(t.cpu().numpy(), ref)
+        # two views share the engine's buffer
+        assert torch.from_dlpack(dev.get_matched_filtered_signal_device(i)).data_ptr() == t.data_ptr()
+    # the view aliases the row: the next apply() rewrites it in place
+    t0 = torch.from_dlpack(dev.get_matched_filtered_signal_device(0))
+    dev.set_signal(0, torch.from_numpy(sigs[0][::-1].copy()).to("cuda"))
+    dev.apply()
+    assert np.array_equal(t0.cpu().numpy(), dev.get_matched_filtered_signal(0))
+    with pytest.raises(ValueError):
+        dev.set_signal(0, torch.zeros(n, dtype=torch.float32, device="cuda"))
+    with pytest.raises(ValueError):
+        dev.set_signal(0, torch.zeros((2, n // 2), dtype=torch.float64, device="cuda"))
+    with pytest.raises(ValueError):
+        dev.set_signal(0, torch.zeros(n - 1, dtype=torch.float64, device="cuda"))
+    # a host torch tensor takes the host path
+    dev.set_signal(1, torch.from_numpy(sigs[1]))
+    dev.apply()
+    assert np.array_equal(dev.get_matched_filtered_signal(1), host.get_matched_filtered_signal(1))
+
+
+def test_pymflib_single_channel_device_signal_feeds_detector_waveforms():
+    """A single-channel signal given as a GPU array: the detector's saved
+    waveforms (read through getSignalPointer's host copy, fetched from HBM
+    on demand) equal the signal."""
+    torch = pytest.importorskip("torch")
+    from refcases import detector_inputs
+    signal, tpls, dt, expected = detector_inputs(2)
+    sc = pymflib.SingleChannel
+    parms = sc.MatchedFilterParameters()
+    for t in tpls:
+        w = pymflib.WaveformTemplate()
+        w.set_signal(t)
+        w.set_sampling_rate(1 / dt)
+        parms.add_template(w)
+    parms.set_signal_size(len(signal))
+    mf = sc.MatchedFilter()
+    mf.initialize(parms)
+    mf.set_signal(torch.from_numpy(signal).to("cuda"))
+    mf.apply()
+    dp = sc.DetectorParameters()
+    dp.set_minimum_detection_spacing(200)
+    dp.set_matched_filtered_signal_detector_policy(pymflib.MatchedFilteredSignalDetectorPolicy.reduce)
+    dp.enable_save_detected_waveform()
+    det = sc.Detector()
+    det.initialize(dp)
+    det.detect(mf)
+    assert len(det) == len(expected)
+    for d, (onset, _, _, it) in zip(det, expected):
+        k = int(round(onset / dt))
+        assert np.array_equal(d.get_detected_signal(), signal[k:k + len(tpls[it])])
+    view = mf.get_matched_filtered_signal_device(0)
+    assert np.array_equal(torch.from_dlpack(view).cpu().numpy(), mf.get_matched_filtered_signal(0))

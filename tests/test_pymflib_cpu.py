@@ -234,3 +234,15 @@ def test_detector_requires_initialize():
     assert det.get_number_of_detections() == 0 and len(det) == 0
     with pytest.raises(IndexError):
         det[0]
+
+
+def test_set_signal_accepts_host_arrays_of_any_kind():
+    """Host arrays (numpy of any dtype, lists) take the host path; only a
+    GPU array goes through DLPack.  Without an initialized filter both fail
+    the reference's way."""
+    mf = pymflib.MultiChannel.MatchedFilter()
+    with pytest.raises(RuntimeError):
+        mf.set_signal(0, np.zeros(10))
+    with pytest.raises(RuntimeError):
+        mf.set_signal(0, [0.0, 1.0])
+    assert hasattr(pymflib, "DeviceRow")
