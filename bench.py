@@ -353,6 +353,34 @@ def precision_section(args, torch):
                            "hbm_frac_of_8TBps": (nred * 4.0 + C * glen * 8.0) / (red * 1e-3) / 8e12}
             log(f"detector reduce (256 templates x 4 channels, |max|): {red:.2f} ms, "
                 f"{reduce_info['hbm_GBps']:.0f} GB/s")
+            # per-detection products (Detector::detect's refinement and scaling
+            # factors) for 16,384 detections on random rows and lags, |rho|
+            prng = np.random.default_rng(0x51)
+            npk, alpha = 16384, 51
+            p_tid = prng.integers(0, C * T, npk).astype(np.int32)
+            p_lag = prng.integers(alpha + 1, glen - alpha - 2, npk).astype(np.int32)
+            dts = np.full(C * T, 0.01)
+            products_info = {"detections": npk}
+            for kind, interp in (("quadratic", capi.INTERP_QUADRATIC), ("lanczos_51_100", capi.INTERP_LANCZOS)):
+                got = h.detection_products(p_lag, p_tid, dts, True, interp, alpha, 100)
+                walls = []
+                for _ in range(3):
+                    t0 = time.perf_counter()
+                    got = h.detection_products(p_lag, p_tid, dts, True, interp, alpha, 100)
+                    walls.append((time.perf_counter() - t0) * 1e3)
+                wall = float(np.median(walls))
+                products_info[kind] = {"ms_per_call": wall, "detections_per_s": npk / (wall * 1e-3)}
+                log(f"detection products {kind}: {wall:.2f} ms per call of {npk} detections")
+            # audit sample: 24 detections, their lags' neighbourhoods and windows
+            sample = []
+            for k in range(24):
+                r, pk = int(p_tid[k]), int(p_lag[k])
+                c, t = divmod(r, T)
+                lt = len(wl.templates[t])
+                sample.append((h.rho(r)[pk - alpha:pk + alpha + 2].astype(np.float64), c, t, pk,
+                               wl.signals[c][pk:pk + lt].astype(np.float64),
+                               {kk: float(v[k]) for kk, v in got.items()}))
+            products_audit = sample
         plan = h.plan(0)
         h.close()
         paths[name] = {"corr_kernel_ms": corr_ms, "corr_samples_per_s": n_corr / (corr_ms * 1e-3),
@@ -387,15 +415,16 @@ def precision_section(args, torch):
         "paths": {k: {kk: vv for kk, vv in v.items() if kk != "rows"} for k, v in paths.items()},
         "normalization_scans": {k: {kk: vv for kk, vv in v.items() if kk != "rows"} for k, v in scans.items()},
         "detector_reduce": reduce_info,
+        "detection_products": products_info,
     }
-    return out, (wl, audit_rows, lag_sets, paths, scans, reduce_audit)
+    return out, (wl, audit_rows, lag_sets, paths, scans, reduce_audit, products_audit)
 
 
 def precision_cpu(state):
     """FP64 oracle audit of the precision-stress rows of every path, and
     peak-index agreement against PeakFinder(0.6, 200) on the oracle rows."""
     import oracle as O
-    wl, audit_rows, lag_sets, paths, scans, reduce_audit = state
+    wl, audit_rows, lag_sets, paths, scans, reduce_audit, products_audit = state
     res = {name: {"max_abs_drho_vs_fp64": 0.0, "peak_sets_equal": 0, "rows": len(audit_rows)} for name in paths}
     for scan in scans:
         res["scan_" + scan] = {"max_rel_scale_err": 0.0, "max_abs_drho_vs_fp64": 0.0, "rows": len(audit_rows),
@@ -432,8 +461,22 @@ def precision_cpu(state):
     res["detector_reduce"] = {"lags_checked": int(rdet.size), "templates": int(rows1.shape[0]),
                               "det_bit_identical": bool(np.array_equal(rdet, det1)),
                               "id_identical": bool(np.array_equal(rid, id1))}
+    # products (Lanczos 51/100, |rho|): the GPU's grid point reaches the
+    # oracle interpolant's maximum; scaling factors vs the oracle
+    worst_gap, worst_rel, ok_flags = 0.0, 0.0, True
+    for seg, c, t, pk, win, got in products_audit:
+        _, prof = O.lanczos_shift(seg, 51, 0.01, True, 51, 100)   # the segment holds every tap
+        j = int(round((got["shift"] / 0.01 + 0.5) * 100)) - 1
+        worst_gap = max(worst_gap, float(prof.max() - prof[j]))
+        ref = O.amplitude_factors(wl.templates[t], win)
+        ok_flags &= (ref is not None) == bool(int(got["flags"]) & 1)
+        if ref is not None:
+            worst_rel = max(worst_rel, abs(got["alpha_gr"] / ref[0] - 1), abs(got["alpha_sr"] / ref[1] - 1))
+    res["detection_products"] = {"detections_checked": len(products_audit),
+                                 "max_lanczos_gap_to_oracle_max": worst_gap,
+                                 "max_rel_err_scaling_factors": worst_rel, "amplitude_flags_equal": bool(ok_flags)}
     for name in res:
-        if name == "detector_reduce":
+        if name in ("detector_reduce", "detection_products"):
             continue
         if name in paths:
             res[name]["peak_rows_checked"] = 2
