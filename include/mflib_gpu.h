@@ -93,6 +93,22 @@ int mf_set_kernel_variant(mf_handle *h, int variant);
  * replays that graph on every later apply, until the next mf_initialize. */
 int mf_set_graph_launch(mf_handle *h, int on);
 
+/* No reference counterpart (streaming state for continuous data): with
+ * on != 0, set before mf_initialize, every channel keeps the last
+ * carry = (longest template - 1) + (largest moveout shift) samples in HBM
+ * ahead of its newest chunk, and mf_set_signal{,_device} push a chunk of
+ * the signal size (the channel length in MF_MODE_BATCHED).  Rows then hold
+ * chunk + carry - nb + 1 lags and stacks chunk + carry - nb + 1 - smax
+ * (nb, smax: the group's longest template and largest shift); the LAST
+ * `chunk` entries of every row and stack are the lags new with this push
+ * (windows whose last sample lies in the chunk), so the tails of
+ * consecutive pushes tile the lag axis with no gap or overlap.  Before the
+ * first push the carry is zeros; mf_zero_signal restarts a channel.  The
+ * dead-trace rule applies to carry + chunk.  mf_get_signal returns the
+ * newest chunk. */
+int mf_set_streaming(mf_handle *h, int on);
+int mf_get_stream_carry(const mf_handle *h, int *carry);
+
 /* MatchedFilter<T>::initialize (src/matchedFilter.cpp:506-821): template
  * demean/normalize (FP64), plans, template spectra, device buffers. */
 int mf_initialize(mf_handle *h);

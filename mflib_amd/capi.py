@@ -77,6 +77,8 @@ _SIGNATURES = {
     "mf_detect_reduced": ([_vp, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_int, _ip, _ip, ctypes.c_int,
                            _ip], ctypes.c_int),
     "mf_get_device": ([_vp, _ip], ctypes.c_int),
+    "mf_set_streaming": ([_vp, ctypes.c_int], ctypes.c_int),
+    "mf_get_stream_carry": ([_vp, _ip], ctypes.c_int),
     "mf_detect_template": ([_vp, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_int, _ip, ctypes.c_int, _ip],
                            ctypes.c_int),
     "mf_detection_products": ([_vp, ctypes.c_int, _ip, _ip, _dp, ctypes.c_int, ctypes.c_int, ctypes.c_int,
@@ -251,6 +253,14 @@ class Handle:
 
     def set_kernel_variant(self, variant: int):
         check(lib().mf_set_kernel_variant(self._h, int(variant)), "set_kernel_variant")
+
+    def set_streaming(self, on: bool = True):
+        """Streaming state (before initialize): channels keep their last
+        carry samples ahead of each pushed chunk."""
+        check(lib().mf_set_streaming(self._h, int(bool(on))), "set_streaming")
+
+    def stream_carry(self) -> int:
+        return self._int(lib().mf_get_stream_carry)
 
     def initialize(self):
         check(lib().mf_initialize(self._h), "initialize")

@@ -43,6 +43,11 @@ struct mf_handle {
     int userNfft = 0;
     int userVariant = 0;  // 0: best available correlation kernel
     bool useGraph = false;            // replay apply's launches as one HIP graph
+    // streaming: every channel keeps its last `carry` samples in front of the
+    // newest chunk, so consecutive chunks give continuous rows and stacks
+    bool streaming = false;
+    long long carry = 0;
+    DevBuf dCarry;
     hipGraph_t graph = nullptr;
     hipGraphExec_t graphExec = nullptr;
     // templates
@@ -184,6 +189,8 @@ void dropDevice(mf_handle *h)
     h->dPackVal.release();
     h->dGroupTab.release();
     h->dStackOut.release();
+    h->dCarry.release();
+    h->carry = 0;
     h->tw.clear();
     h->launches.clear();
     h->segHost.clear();
@@ -230,6 +237,17 @@ void initialize(mf_handle *h)
     int nGroups = 0;
     deriveLayout(h, rowCh, rowGroup, chLen, nGroups);
     const int nC = static_cast<int>(chLen.size());
+    h->carry = 0;
+    if (h->streaming) {
+        // nb - 1 samples for the windows ending in the chunk, plus the
+        // largest moveout for the stacks of the chunk's lags
+        int nbMax = 0, sMax = 0;
+        for (int r = 0; r < nT; ++r) { nbMax = std::max(nbMax, static_cast<int>(h->tmpl[r].size())); }
+        if (h->haveShifts) { sMax = *std::max_element(h->shifts.begin(), h->shifts.end()); }
+        h->carry = static_cast<long long>(nbMax - 1) + std::max(sMax, 0);
+        for (auto &n : chLen) { n += h->carry; }
+        if (h->carry > 0) { h->dCarry.alloc(static_cast<size_t>(h->carry) * h->tsize); }
+    }
 
     // groups: nb = max template length, nOut = min channel length - nb + 1
     h->groups.assign(nGroups, {});
@@ -1254,12 +1272,81 @@ int mf_get_plan(const mf_handle *h, int row, int *nfft, int *blockLength)
     });
 }
 
+namespace {
+
+// Streaming push: the last `carry` samples of the channel move to its
+// front (through a scratch buffer: the ranges overlap when the chunk is
+// shorter than the carry), the chunk follows, and the dead-trace rule is
+// evaluated on the device over the whole buffer (carry + chunk).
+void pushChunk(mf_handle *h, int channel, int n, const void *x, hipMemcpyKind kind)
+{
+    auto &c = h->ch[channel];
+    const size_t ts = h->tsize;
+    char *buf = static_cast<char *>(c.buf.p);
+    if (h->carry > 0) {
+        MF_HIP(hipMemcpyAsync(h->dCarry.p, buf + static_cast<size_t>(n) * ts, static_cast<size_t>(h->carry) * ts,
+                              hipMemcpyDeviceToDevice, h->stream));
+        MF_HIP(hipMemcpyAsync(buf, h->dCarry.p, static_cast<size_t>(h->carry) * ts, hipMemcpyDeviceToDevice,
+                              h->stream));
+    }
+    MF_HIP(hipMemcpyAsync(buf + static_cast<size_t>(h->carry) * ts, x, static_cast<size_t>(n) * ts, kind, h->stream));
+    const int nb = static_cast<int>(std::min<long long>((c.n + 255) / 256, 1024));
+    DevBuf pmin, pmax;
+    pmin.alloc(nb * sizeof(double));
+    pmax.alloc(nb * sizeof(double));
+    MF_HIP(mfk::launch_minmax(h->precision, c.buf.p, c.n, pmin.as<double>(), pmax.as<double>(), nb, h->stream));
+    std::vector<double> lo(nb), hi(nb);
+    MF_HIP(hipMemcpyAsync(lo.data(), pmin.p, nb * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    MF_HIP(hipMemcpyAsync(hi.data(), pmax.p, nb * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    MF_HIP(hipStreamSynchronize(h->stream));
+    const double mn = *std::min_element(lo.begin(), lo.end());
+    const double mx = *std::max_element(hi.begin(), hi.end());
+    c.dead = std::abs(mn) == std::abs(mx);
+    c.set = true;
+    h->tablesDirty = true;
+}
+
+void checkChunkLength(const mf_handle *h, int c, int n)
+{
+    const long long chunk = h->ch[c].n - h->carry;
+    if (n != chunk) {
+        fail(MF_ERR_INVALID_ARGUMENT, "Number of samples = " + std::to_string(n) + " must equal the chunk length " +
+                                          std::to_string(chunk));
+    }
+}
+
+}  // namespace
+
+int mf_set_streaming(mf_handle *h, int on)
+{
+    return guard([&] {
+        requireHandle(h);
+        if (h->initialized) { fail(MF_ERR_BAD_STATE, "streaming must be chosen before initialize"); }
+        h->streaming = on != 0;
+    });
+}
+
+int mf_get_stream_carry(const mf_handle *h, int *carry)
+{
+    return guard([&] {
+        requireInit(h);
+        if (carry == nullptr) { fail(MF_ERR_INVALID_ARGUMENT, "output is NULL"); }
+        *carry = static_cast<int>(h->carry);
+    });
+}
+
 int mf_set_signal(mf_handle *h, int channel, int n, const void *x)
 {
     return guard([&] {
         requireInit(h);
         h->applied = false;
         checkChannel(h, channel);
+        if (h->streaming) {
+            checkChunkLength(h, channel, n);
+            if (x == nullptr) { fail(MF_ERR_INVALID_ARGUMENT, "signal is NULL"); }
+            pushChunk(h, channel, n, x, hipMemcpyHostToDevice);
+            return;
+        }
         checkSignalLength(h, channel, n);
         if (x == nullptr) { fail(MF_ERR_INVALID_ARGUMENT, "signal is NULL"); }
         auto &c = h->ch[channel];
@@ -1279,6 +1366,12 @@ int mf_set_signal_device(mf_handle *h, int channel, int n, const void *dx)
         requireInit(h);
         h->applied = false;
         checkChannel(h, channel);
+        if (h->streaming) {
+            checkChunkLength(h, channel, n);
+            if (dx == nullptr) { fail(MF_ERR_INVALID_ARGUMENT, "signal is NULL"); }
+            pushChunk(h, channel, n, dx, hipMemcpyDeviceToDevice);
+            return;
+        }
         checkSignalLength(h, channel, n);
         if (dx == nullptr) { fail(MF_ERR_INVALID_ARGUMENT, "signal is NULL"); }
         auto &c = h->ch[channel];
@@ -1321,8 +1414,10 @@ int mf_get_signal(mf_handle *h, int channel, void *out, int cap)
         requireInit(h);
         checkChannel(h, channel);
         const auto &c = h->ch[channel];
-        if (out == nullptr || cap < c.n) { fail(MF_ERR_INVALID_ARGUMENT, "output buffer too small"); }
-        MF_HIP(hipMemcpyAsync(out, c.buf.p, static_cast<size_t>(c.n) * h->tsize, hipMemcpyDeviceToHost, h->stream));
+        const long long n = c.n - h->carry;  // streaming: the newest chunk
+        if (out == nullptr || cap < n) { fail(MF_ERR_INVALID_ARGUMENT, "output buffer too small"); }
+        MF_HIP(hipMemcpyAsync(out, static_cast<const char *>(c.buf.p) + static_cast<size_t>(h->carry) * h->tsize,
+                              static_cast<size_t>(n) * h->tsize, hipMemcpyDeviceToHost, h->stream));
         MF_HIP(hipStreamSynchronize(h->stream));
     });
 }

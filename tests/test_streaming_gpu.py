@@ -1,0 +1,133 @@
+"""Streaming state (mf_set_streaming): a continuous signal pushed in
+fixed-size chunks.  The tails of the rows and stacks of consecutive pushes
+tile the lag axis, and equal the FP64 oracle over the whole concatenated
+signal (Pearson definition, testing/utilities.hpp:33-69; shiftAndStack,
+src/matchedFilter.cpp:1318-1364) at the accuracy contract: 1e-5 for FP32
+instances, 1e-11 for FP64.  Lags whose windows start before the stream read
+the zero carry and are not compared."""
+import numpy as np
+import pytest
+
+import oracle as O
+from hipmem import DeviceArray
+from mflib_amd import capi
+
+pytestmark = pytest.mark.gpu
+
+
+def _signal(rng, n, dc):
+    x = rng.standard_normal(n) * 0.5 + dc
+    x[n // 3:n // 3 + 700] = dc + 0.25          # a constant stretch inside one chunk
+    return x
+
+
+def test_multichannel_stream_rows_and_stacks_match_oracle():
+    rng = np.random.default_rng(41)
+    lens, shifts, weights = [300, 250, 300], [0, 7, 19], [1.0, 0.5, 0.75]
+    tpls = [rng.standard_normal(L) * np.hanning(L) for L in lens]
+    chunk, pushes = 10_000, 6
+    total = chunk * pushes
+    sigs = [_signal(rng, total, 50.0 * c) for c in range(3)]
+    for c, t in enumerate(tpls):                  # events straddling chunk boundaries
+        for pos in (chunk - 120, 3 * chunk - 5, 4 * chunk + 999):
+            sigs[c][pos + shifts[c]:pos + shifts[c] + len(t)] += 3 * t
+    h = capi.Handle(np.float32, "multi")
+    h.set_templates(tpls, shifts=shifts, weights=weights)
+    h.set_signal_size(chunk)
+    h.set_streaming(True)
+    h.initialize()
+    nb, smax = max(lens), max(shifts)
+    assert h.stream_carry() == nb - 1 + smax
+    assert h.filtered_length(0) == chunk + smax
+    rows_tail = [[] for _ in tpls]
+    stack_tail = []
+    for p in range(pushes):
+        for c in range(3):
+            h.set_signal(c, sigs[c][p * chunk:(p + 1) * chunk].astype(np.float32))
+        h.apply()
+        for r in range(3):
+            rows_tail[r].append(h.rho(r)[-chunk:])
+        st = h.shift_and_stack()
+        assert len(st) == chunk
+        stack_tail.append(st)
+        assert np.array_equal(h.get_signal(0, chunk), sigs[0][p * chunk:(p + 1) * chunk].astype(np.float32))
+    # the oracle over the whole signal (rows of the common length total - nb + 1)
+    nout = total - nb + 1
+    full = np.stack([O.pearson(tpls[r], sigs[r].astype(np.float32).astype(np.float64))[:nout] for r in range(3)])
+    ref_stack = O.shift_and_stack(full, shifts, weights, absval=True)
+    for r in range(3):
+        got = np.concatenate(rows_tail[r])        # lag j of push p: window start p*chunk - nb + 1 + j
+        starts = np.arange(total) - nb + 1
+        ok = (starts >= 0) & (starts < nout)
+        assert np.abs(got[ok] - full[r][starts[ok]]).max() < 1e-5
+    got = np.concatenate(stack_tail)              # stack k of push p: start p*chunk - nb + 1 - smax + k
+    starts = np.arange(total) - nb + 1 - smax
+    ok = (starts >= 0) & (starts < len(ref_stack))
+    assert ok.sum() > total - 2 * nb
+    assert np.abs(got[ok] - ref_stack[starts[ok]]).max() < 1e-5
+    h.close()
+
+
+def test_single_channel_stream_chunk_shorter_than_carry_fp64():
+    """FP64, uneven templates, chunks shorter than the carry (the carry
+    moves through the scratch buffer), pushed from device memory."""
+    rng = np.random.default_rng(43)
+    lens = [400, 333, 120]
+    tpls = [rng.standard_normal(L) for L in lens]
+    chunk, pushes = 300, 12
+    total = chunk * pushes
+    x = _signal(rng, total, 7.0)
+    h = capi.Handle(np.float64, "single")
+    h.set_templates(tpls)
+    h.set_signal_size(chunk)
+    h.set_streaming(True)
+    h.initialize()
+    nb = max(lens)
+    assert h.stream_carry() == nb - 1 > chunk
+    tails = [[] for _ in tpls]
+    for p in range(pushes):
+        d = DeviceArray(x[p * chunk:(p + 1) * chunk].copy())
+        h.set_signal_device(0, d.ptr, chunk)
+        d.free()
+        h.apply()
+        for r in range(len(tpls)):
+            tails[r].append(h.rho(r)[-chunk:])
+    nout = total - nb + 1
+    starts = np.arange(total) - nb + 1
+    ok = (starts >= 0) & (starts < nout)
+    for r, t in enumerate(tpls):
+        ref = O.pearson(t, x)[:nout]
+        got = np.concatenate(tails[r])
+        assert np.abs(got[ok] - ref[starts[ok]]).max() < 1e-11
+    h.close()
+
+
+def test_stream_restart_and_errors():
+    rng = np.random.default_rng(47)
+    tpls = [rng.standard_normal(200)]
+    chunk = 5000
+    a, b = rng.standard_normal(chunk), rng.standard_normal(chunk)
+
+    def fresh():
+        h = capi.Handle(np.float32, "multi")
+        h.set_templates(tpls)
+        h.set_signal_size(chunk)
+        h.set_streaming(True)
+        h.initialize()
+        return h
+
+    h1, h2 = fresh(), fresh()
+    h1.set_signal(0, a.astype(np.float32))
+    h1.apply()
+    h1.zero_signal(0)                             # restart: the carry is zeros again
+    h1.set_signal(0, b.astype(np.float32))
+    h1.apply()
+    h2.set_signal(0, b.astype(np.float32))
+    h2.apply()
+    assert np.array_equal(h1.rho(0), h2.rho(0))
+    with pytest.raises(ValueError):
+        h1.set_signal(0, np.zeros(chunk + 1, np.float32))
+    with pytest.raises(RuntimeError):
+        h1.set_streaming(False)                   # only before initialize
+    h1.close()
+    h2.close()
