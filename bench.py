@@ -282,6 +282,53 @@ def uneven_section(args, torch):
     }
 
 
+def streaming_section(args, torch, wl, day_row0):
+    """Streaming: the multi-channel day pushed in 60 s chunks (6,000 samples
+    per channel per push) into a streaming handle of the same layout; per
+    push: 24 device chunks in, all 384 rows, 16 stacks and PeakFinder.
+    Audit: the row tails of template event 0 on channel 0 against the
+    full-day row of the same engine (only the FFT blocking differs)."""
+    from mflib_amd import capi
+    E, C = wl.n_events, wl.n_channels
+    chunk, pushes = 6000, 40
+    h = capi.Handle(np.float32, "batched")
+    h.set_templates([wl.templates[e][c] for e in range(E) for c in range(C)],
+                    shifts=[int(wl.moveouts[e, c]) for e in range(E) for c in range(C)], weights=[1.0] * (E * C))
+    h.set_batch_layout([chunk] * C, [c for e in range(E) for c in range(C)], [e for e in range(E) for c in range(C)], E)
+    h.set_streaming(True)
+    h.initialize()
+    carry = h.stream_carry()
+    nb = len(wl.templates[0][0])
+    dev = [torch.from_numpy(wl.signals[c][:chunk * pushes]).to("cuda:0") for c in range(C)]
+    torch.cuda.synchronize()
+    lat, tails = [], []
+    for p in range(pushes):
+        t0 = time.perf_counter()
+        for c in range(C):
+            h.set_signal_device(c, dev[c][p * chunk:].data_ptr(), chunk)
+        h.apply()
+        h.detect(0.5, 512)
+        lat.append((time.perf_counter() - t0) * 1e3)
+        tails.append(h.rho(0)[-chunk:])
+    got = np.concatenate(tails)
+    starts = np.arange(chunk * pushes) - nb + 1
+    ok = starts >= 0
+    drho = float(np.abs(got[ok] - day_row0[starts[ok]]).max())
+    lat = np.array(lat[2:])                        # the first pushes include allocation
+    n_corr = E * C * chunk
+    h.close()
+    res = {"workload": "streaming: 16 templates x 24 channels, 60 s chunks (6000 samples) of the multi-channel "
+                       "day, 384 rows + 16 stacks + PeakFinder per push",
+           "chunk_samples": chunk, "pushes": pushes, "carry_samples": carry,
+           "ms_per_push_p50": float(np.percentile(lat, 50)), "ms_per_push_p99": float(np.percentile(lat, 99)),
+           "realtime_factor": 60e3 / float(np.percentile(lat, 50)),
+           "new_corr_samples_per_s": n_corr / (float(np.percentile(lat, 50)) * 1e-3),
+           "max_abs_drho_vs_day_row": drho}
+    log(f"streaming: {res['ms_per_push_p50']:.2f} ms per 60 s push (p99 {res['ms_per_push_p99']:.2f}), "
+        f"{res['realtime_factor']:.0f}x real time; |drho| vs day row {drho:.2e}")
+    return res
+
+
 def precision_section(args, torch):
     """Precision stress: 256 templates of 128-256 samples against 4 channels
     of 2,000,000 FP32 samples with a 1e4 DC offset and constant gaps, on
@@ -546,6 +593,7 @@ def main() -> int:
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-serve", action="store_true", help="skip the served-stream measurement")
     ap.add_argument("--no-uneven", action="store_true", help="skip the uneven batched workload")
+    ap.add_argument("--no-stream", action="store_true", help="skip the streaming workload")
     ap.add_argument("--no-precision", action="store_true", help="skip the precision-stress workload")
     ap.add_argument("--precision-only", action="store_true", help="only the precision-stress workload")
     ap.add_argument("--uneven-only", action="store_true", help="only the uneven batched workload")
@@ -698,6 +746,8 @@ def main() -> int:
     }
     audit_rows = [h.rho(e * C + c) for e, c in [(e, c) for e in (0, E - 1) for c in (0, C // 2, C - 1)]]
     h.close()
+    if not args.no_stream:
+        out["streaming"] = streaming_section(args, torch, wl, audit_rows[0])
     sc_out, sc_state = single_channel_section(args, torch)
     out["single_channel"] = sc_out
     serve_state = None
