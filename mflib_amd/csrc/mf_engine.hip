@@ -48,6 +48,7 @@ struct mf_handle {
     bool streaming = false;
     long long carry = 0;
     DevBuf dCarry;
+    DevBuf dDeadPart;  // streaming: per-chunk min/max of the dead-trace pass on the device
     hipGraph_t graph = nullptr;
     hipGraphExec_t graphExec = nullptr;
     // templates
@@ -190,6 +191,7 @@ void dropDevice(mf_handle *h)
     h->dGroupTab.release();
     h->dStackOut.release();
     h->dCarry.release();
+    h->dDeadPart.release();
     h->carry = 0;
     h->tw.clear();
     h->launches.clear();
@@ -245,7 +247,13 @@ void initialize(mf_handle *h)
         for (int r = 0; r < nT; ++r) { nbMax = std::max(nbMax, static_cast<int>(h->tmpl[r].size())); }
         if (h->haveShifts) { sMax = *std::max_element(h->shifts.begin(), h->shifts.end()); }
         h->carry = static_cast<long long>(nbMax - 1) + std::max(sMax, 0);
-        for (auto &n : chLen) { n += h->carry; }
+        for (auto &n : chLen) {
+            n += h->carry;
+            // the device dead-trace pass covers 65535 chunks of kDeadChunk samples
+            if ((n + mfk::kDeadChunk - 1) / mfk::kDeadChunk > 65535) {
+                fail(MF_ERR_UNSUPPORTED, "streaming chunk of " + std::to_string(n) + " samples is too long");
+            }
+        }
         if (h->carry > 0) { h->dCarry.alloc(static_cast<size_t>(h->carry) * h->tsize); }
     }
 
@@ -626,6 +634,17 @@ void apply(mf_handle *h)
         }
         MF_HIP(hipStreamEndCapture(h->stream, &h->graph));
         MF_HIP(hipGraphInstantiate(&h->graphExec, h->graph, nullptr, nullptr, 0));
+    }
+    if (h->streaming && !h->segHost.empty()) {
+        // streaming pushes leave the dead-trace rule to the device: every
+        // segment's flag from its channel's carry + chunk, before the launches
+        long long maxN = 0;
+        for (const auto &c : h->ch) { maxN = std::max(maxN, c.n); }
+        const size_t chunks = static_cast<size_t>((maxN + mfk::kDeadChunk - 1) / mfk::kDeadChunk);
+        const size_t partBytes = 2 * sizeof(double) * h->segHost.size() * chunks;
+        if (h->dDeadPart.bytes < partBytes) { h->dDeadPart.alloc(partBytes); }
+        MF_HIP(mfk::launch_seg_dead(h->precision, h->dSegs.as<mfk::Seg>(), static_cast<int>(h->segHost.size()), maxN,
+                                    h->dDeadPart.as<double>(), h->stream));
     }
     MF_HIP(hipEventRecord(h->evA, h->stream));
     if (h->useGraph && h->graphExec != nullptr) {
@@ -1276,8 +1295,8 @@ namespace {
 
 // Streaming push: the last `carry` samples of the channel move to its
 // front (through a scratch buffer: the ranges overlap when the chunk is
-// shorter than the carry), the chunk follows, and the dead-trace rule is
-// evaluated on the device over the whole buffer (carry + chunk).
+// shorter than the carry) and the chunk follows.  The dead-trace rule over
+// the whole buffer (carry + chunk) is evaluated on the device by apply().
 void pushChunk(mf_handle *h, int channel, int n, const void *x, hipMemcpyKind kind)
 {
     auto &c = h->ch[channel];
@@ -1290,20 +1309,9 @@ void pushChunk(mf_handle *h, int channel, int n, const void *x, hipMemcpyKind ki
                               h->stream));
     }
     MF_HIP(hipMemcpyAsync(buf + static_cast<size_t>(h->carry) * ts, x, static_cast<size_t>(n) * ts, kind, h->stream));
-    const int nb = static_cast<int>(std::min<long long>((c.n + 255) / 256, 1024));
-    DevBuf pmin, pmax;
-    pmin.alloc(nb * sizeof(double));
-    pmax.alloc(nb * sizeof(double));
-    MF_HIP(mfk::launch_minmax(h->precision, c.buf.p, c.n, pmin.as<double>(), pmax.as<double>(), nb, h->stream));
-    std::vector<double> lo(nb), hi(nb);
-    MF_HIP(hipMemcpyAsync(lo.data(), pmin.p, nb * sizeof(double), hipMemcpyDeviceToHost, h->stream));
-    MF_HIP(hipMemcpyAsync(hi.data(), pmax.p, nb * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    // the caller may reuse its buffer once this returns
     MF_HIP(hipStreamSynchronize(h->stream));
-    const double mn = *std::min_element(lo.begin(), lo.end());
-    const double mx = *std::max_element(hi.begin(), hi.end());
-    c.dead = std::abs(mn) == std::abs(mx);
     c.set = true;
-    h->tablesDirty = true;
 }
 
 void checkChunkLength(const mf_handle *h, int c, int n)

@@ -125,6 +125,14 @@ def test_stream_restart_and_errors():
     h2.set_signal(0, b.astype(np.float32))
     h2.apply()
     assert np.array_equal(h1.rho(0), h2.rho(0))
+    # dead-trace rule over carry + chunk, evaluated on the device at apply
+    h1.zero_signal(0)
+    h1.set_signal(0, np.full(chunk, 0.0, np.float32))
+    h1.apply()
+    assert not np.any(h1.rho(0))                  # |min| == |max|: every row of the channel is 0
+    h1.set_signal(0, b.astype(np.float32))        # live again: the carry of zeros no longer makes it dead
+    h1.apply()
+    assert np.array_equal(h1.rho(0), h2.rho(0))
     with pytest.raises(ValueError):
         h1.set_signal(0, np.zeros(chunk + 1, np.float32))
     with pytest.raises(RuntimeError):
