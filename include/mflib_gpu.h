@@ -115,7 +115,8 @@ int mf_initialize(mf_handle *h);
 int mf_is_initialized(const mf_handle *h, int *yes);
 /* getNumberOfTemplates / getSignalLength / getFilteredSignalLength
  * (src/matchedFilter.cpp:1366-1402).  For MF_MODE_BATCHED the lengths are
- * those of channel 0 / group 0. */
+ * those of channel 0 / group 0.  In streaming mode the signal length is the
+ * chunk length. */
 int mf_get_number_of_templates(const mf_handle *h, int *n);
 int mf_get_signal_length(const mf_handle *h, int *n);
 int mf_get_filtered_length(const mf_handle *h, int row, int *n);

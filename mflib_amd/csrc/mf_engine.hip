@@ -1267,7 +1267,7 @@ int mf_get_signal_length(const mf_handle *h, int *n)
 {
     return guard([&] {
         requireInit(h);
-        if (n) { *n = static_cast<int>(h->ch[0].n); }
+        if (n) { *n = static_cast<int>(h->ch[0].n - h->carry); }  // streaming: the chunk length
     });
 }
 

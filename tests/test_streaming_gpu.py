@@ -39,6 +39,7 @@ def test_multichannel_stream_rows_and_stacks_match_oracle():
     nb, smax = max(lens), max(shifts)
     assert h.stream_carry() == nb - 1 + smax
     assert h.filtered_length(0) == chunk + smax
+    assert h.signal_length() == chunk
     rows_tail = [[] for _ in tpls]
     stack_tail = []
     for p in range(pushes):
