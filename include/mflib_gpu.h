@@ -248,7 +248,9 @@ int mf_detect_template(mf_handle *h, int it, int absoluteValue, double threshold
  *    (relativeMagnitude.cpp:330-338).
  * samplingPeriods[it] is template it's 1/fs (NULL: 1, shifts in samples).
  * absoluteValue follows the detector's maxima policy.  Arithmetic is FP64
- * for both precisions.  Any output may be NULL. */
+ * for both precisions.  Any output may be NULL.  Lanczos work is bounded:
+ * (2 lanczosAlpha + 1) * (lanczosPoints - 1) <= 2^22 terms per detection
+ * (MF_ERR_UNSUPPORTED beyond; the defaults 51 / 100 give 10,197). */
 int mf_detection_products(mf_handle *h, int nPeaks, const int *peakIdx, const int *templateIds,
                           const double *samplingPeriods, int absoluteValue, int interpolation, int lanczosAlpha,
                           int lanczosPoints, double *rho, double *shift, double *alphaGR, double *alphaSR,

@@ -1683,6 +1683,12 @@ int mf_detection_products(mf_handle *h, int nPeaks, const int *peakIdx, const in
         if (interpolation == mfk::kInterpLanczos) {
             if (lanczosAlpha < 1) { fail(MF_ERR_INVALID_ARGUMENT, "alpha must be positive"); }
             if (lanczosPoints < 1) { fail(MF_ERR_INVALID_ARGUMENT, "Number of interpolation points must be positive"); }
+            if ((2LL * lanczosAlpha + 1) * (lanczosPoints - 1) > mfk::kLanczosMaxTerms) {
+                fail(MF_ERR_UNSUPPORTED, "(2 alpha + 1) x (points - 1) = " +
+                                             std::to_string((2LL * lanczosAlpha + 1) * (lanczosPoints - 1)) +
+                                             " interpolation terms per detection exceed " +
+                                             std::to_string(mfk::kLanczosMaxTerms));
+            }
         }
         if (nPeaks == 0) { return; }
         std::vector<mfk::ProductJob> jobs(nPeaks);

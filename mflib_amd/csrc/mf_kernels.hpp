@@ -165,6 +165,10 @@ hipError_t launch_scan_scale(int mode, const float *dx, long long n, int Lt, flo
 constexpr int kInterpLanczos = 0;
 constexpr int kInterpQuadratic = 1;
 constexpr int kInterpNone = 2;
+// Lanczos work per detection, (2 alpha + 1) taps at (points - 1) grid
+// shifts, is bounded so one wavefront's loop stays within milliseconds
+// (the default 51 / 100 is 10,197 terms).
+constexpr long long kLanczosMaxTerms = 1LL << 22;
 constexpr int kProductAmplitude = 1;  // scaling factors computed
 constexpr int kProductTruncated = 2;  // peak + Lt runs past the signal: no amplitude
 struct ProductJob {

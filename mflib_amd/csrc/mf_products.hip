@@ -106,7 +106,7 @@ __device__ double lanczos_shift(gptr<const T> xc, int n, int p, double dt, int a
         const int fl = static_cast<int>(floor(x));
         double y = 0.0;
         for (int k = -a; k <= a; ++k) {
-            const int i = fl - k + 1;
+            const long long i = static_cast<long long>(fl) - k + 1;  // no int overflow near 2^31 lags
             const double xi = x - static_cast<double>(i);
             double v = 0.0;
             if (i >= 0 && i < n) { v = static_cast<double>(xc[i]); }

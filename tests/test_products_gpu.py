@@ -169,6 +169,8 @@ def test_products_argument_errors():
         h.detection_products([5], [0], interpolation=capi.INTERP_LANCZOS, lanczos_points=0)
     with pytest.raises(ValueError):
         h.detection_products([5], [0], sampling_periods=np.zeros(len(tpls)))
+    with pytest.raises(RuntimeError):                          # beyond the bounded Lanczos work
+        h.detection_products([5], [0], interpolation=capi.INTERP_LANCZOS, lanczos_alpha=5000, lanczos_points=1000)
     assert h.detection_products([], [])["rho"].size == 0
     h.close()
     h2 = capi.Handle(np.float32, "single")
