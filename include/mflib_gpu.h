@@ -255,6 +255,16 @@ int mf_detection_products(mf_handle *h, int nPeaks, const int *peakIdx, const in
                           const double *samplingPeriods, int absoluteValue, int interpolation, int lanczosAlpha,
                           int lanczosPoints, double *rho, double *shift, double *alphaGR, double *alphaSR,
                           int *flags);
+/* RelativeMagnitude<T> (src/singleChannel/relativeMagnitude.cpp:35-58,
+ * 273-388) for a batch of detected windows against one template, on the
+ * GPU: window k is windows[k*ld .. k*ld + Lt) (host samples of
+ * `precision`), tmpl the template's Lt FP64 samples.  alphaGR[k] =
+ * |<x, y>| / <x, x> (Gibbons & Ringdal 2006) and alphaSR[k] =
+ * sqrt(<y, y> / <x, x>) (Schaff & Richards 2014) of the demeaned template x
+ * and window y, in FP64; computed[k] = 0 for a constant window, which the
+ * reference's setDetectedSignal rejects.  Outputs may be NULL. */
+int mf_amplitude_factors(int precision, int Lt, const double *tmpl, int nWindows, const void *windows,
+                         long long ld, double *alphaGR, double *alphaSR, int *computed);
 
 /* computeOptimalFFTAndBlockLength(nb, nx)
  * (include/mflib/matchedFilterParameters.hpp:208, src/matchedFilterParameters.cpp:371-385). */

@@ -63,7 +63,8 @@ def build_gpu_lib() -> str:
 
 FACADE_SRCS = ["waveformTemplate.cpp", "networkStationPhase.cpp", "matchedFilterParameters.cpp",
                "matchedFilter.cpp", "singleChannelMatchedFilterParameters.cpp",
-               "singleChannelMatchedFilter.cpp", "peakFinder.cpp", "detection.cpp", "detector.cpp"]
+               "singleChannelMatchedFilter.cpp", "peakFinder.cpp", "detection.cpp", "detector.cpp",
+               "relativeMagnitude.cpp"]
 
 
 def build_facade() -> str:

@@ -81,6 +81,8 @@ _SIGNATURES = {
     "mf_get_stream_carry": ([_vp, _ip], ctypes.c_int),
     "mf_detect_template": ([_vp, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_int, _ip, ctypes.c_int, _ip],
                            ctypes.c_int),
+    "mf_amplitude_factors": ([ctypes.c_int, ctypes.c_int, _dp, ctypes.c_int, _vp, ctypes.c_longlong, _dp, _dp,
+                              _ip], ctypes.c_int),
     "mf_detection_products": ([_vp, ctypes.c_int, _ip, _ip, _dp, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                ctypes.c_int, _dp, _dp, _dp, _dp, _ip], ctypes.c_int),
     "mf_get_timings": ([_vp, _dp, _dp], ctypes.c_int),
@@ -193,6 +195,23 @@ def window_scale_device(ptr: int, n: int, lt: int, scan: str, out_ptr: int) -> f
     check(lib().mf_window_scale_device(_vp(ptr), int(n), int(lt), SCAN_PRECISION[scan], _vp(out_ptr),
                                        ctypes.byref(ms)), "window_scale_device")
     return ms.value
+
+
+def amplitude_factors(template, windows, dtype=np.float64):
+    """RelativeMagnitude's scaling factors of each row of `windows` (2-D,
+    one detected window per row) against `template`, on the GPU: (alpha_gr,
+    alpha_sr, computed)."""
+    t = np.ascontiguousarray(template, dtype=np.float64)
+    w = np.ascontiguousarray(np.atleast_2d(windows), dtype=dtype)
+    n, lt = w.shape
+    if lt != len(t):
+        raise ValueError("windows must have the template's length")
+    gr, sr = np.empty(max(n, 1)), np.empty(max(n, 1))
+    ok = np.empty(max(n, 1), dtype=np.int32)
+    check(lib().mf_amplitude_factors(PRECISION[w.dtype], lt, t.ctypes.data_as(_dp), n, w.ctypes.data_as(_vp), lt,
+                                     gr.ctypes.data_as(_dp), sr.ctypes.data_as(_dp), ok.ctypes.data_as(_ip)),
+          "amplitude_factors")
+    return gr[:n].copy(), sr[:n].copy(), ok[:n].astype(bool)
 
 
 class Handle:

@@ -1789,6 +1789,50 @@ int mf_template_spectrum(mf_handle *h, int row, void *outInterleaved, int cap, i
     });
 }
 
+int mf_amplitude_factors(int precision, int Lt, const double *tmpl, int nWindows, const void *windows,
+                         long long ld, double *alphaGR, double *alphaSR, int *computed)
+{
+    return guard([&] {
+        if (precision != MF_PRECISION_FLOAT && precision != MF_PRECISION_DOUBLE) {
+            fail(MF_ERR_INVALID_ARGUMENT, "unknown precision " + std::to_string(precision));
+        }
+        if (Lt < 1) { fail(MF_ERR_INVALID_ARGUMENT, "Template signal length = " + std::to_string(Lt) + " must be positive"); }
+        if (nWindows < 0 || ld < Lt) { fail(MF_ERR_INVALID_ARGUMENT, "nWindows must be >= 0 and ld >= Lt"); }
+        if (tmpl == nullptr || (nWindows > 0 && windows == nullptr)) { fail(MF_ERR_INVALID_ARGUMENT, "input is NULL"); }
+        if (nWindows == 0) { return; }
+        requireDevice();
+        const size_t ts = precision == MF_PRECISION_FLOAT ? 4 : 8;
+        const size_t winElems = static_cast<size_t>(nWindows - 1) * static_cast<size_t>(ld) + static_cast<size_t>(Lt);
+        DevBuf dT, dW, dOut;
+        dT.alloc(static_cast<size_t>(Lt) * sizeof(double));
+        dW.alloc(winElems * ts);
+        dOut.alloc(static_cast<size_t>(nWindows) * sizeof(mfk::ProductOut));
+        hipStream_t st = nullptr;
+        MF_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+        std::vector<mfk::ProductOut> res(nWindows);
+        const hipError_t e = [&] {
+            hipError_t r = hipMemcpyAsync(dT.p, tmpl, static_cast<size_t>(Lt) * sizeof(double), hipMemcpyHostToDevice, st);
+            if (r == hipSuccess) { r = hipMemcpyAsync(dW.p, windows, winElems * ts, hipMemcpyHostToDevice, st); }
+            if (r == hipSuccess) {
+                r = mfk::launch_amplitude(precision, dT.as<double>(), Lt, dW.p, ld, nWindows,
+                                          dOut.as<mfk::ProductOut>(), st);
+            }
+            if (r == hipSuccess) {
+                r = hipMemcpyAsync(res.data(), dOut.p, res.size() * sizeof(mfk::ProductOut), hipMemcpyDeviceToHost, st);
+            }
+            if (r == hipSuccess) { r = hipStreamSynchronize(st); }
+            return r;
+        }();
+        (void)hipStreamDestroy(st);
+        MF_HIP(e);
+        for (int k = 0; k < nWindows; ++k) {
+            if (alphaGR) { alphaGR[k] = res[k].alphaGR; }
+            if (alphaSR) { alphaSR[k] = res[k].alphaSR; }
+            if (computed) { computed[k] = res[k].flags & mfk::kProductAmplitude ? 1 : 0; }
+        }
+    });
+}
+
 int mf_peaks(const void *x, int n, int precision, double threshold, int minDist, int *idx, int cap, int *nPeaks)
 {
     return guard([&] { peaks(x, false, n, precision, threshold, minDist, idx, cap, nPeaks); });

@@ -185,5 +185,10 @@ struct ProductOut {
 };
 hipError_t launch_products(int precision, const ProductJob *dJobs, int nJobs, int absval, int interp,
                            int lanczosAlpha, int lanczosPoints, ProductOut *dOut, hipStream_t stream);
+// RelativeMagnitude's factors of nWindows windows (window k at
+// dWindows + k * ld elements, Lt samples) against one template: alphaGR,
+// alphaSR and kProductAmplitude in dOut[k] (0 flags: constant window).
+hipError_t launch_amplitude(int precision, const double *dTmpl, int Lt, const void *dWindows, long long ld,
+                            int nWindows, ProductOut *dOut, hipStream_t stream);
 
 }  // namespace mfk

@@ -144,6 +144,37 @@ __device__ double lanczos_shift(gptr<const T> xc, int n, int p, double dt, int a
     return dt * shift;
 }
 
+// RelativeMagnitude's factors of y[0..Lt) against t[0..Lt) over one
+// wavefront, two passes: the means, then the demeaned sums of squares and
+// products.  Returns false for a constant window (no factors).
+template <typename T>
+__device__ bool amplitude_factors(gptr<const double> t, gptr<const T> y, int Lt, double &gr, double &sr)
+{
+    const int lane = threadIdx.x;
+    double st = 0.0, sy = 0.0;
+    for (int i = lane; i < Lt; i += kWave) {
+        st += t[i];
+        sy += static_cast<double>(y[i]);
+    }
+    const double mt = wave_sum(st) / Lt;
+    const double my = wave_sum(sy) / Lt;
+    double xx = 0.0, yy = 0.0, xy = 0.0;
+    for (int i = lane; i < Lt; i += kWave) {
+        const double dx = t[i] - mt;
+        const double dy = static_cast<double>(y[i]) - my;
+        xx += dx * dx;
+        yy += dy * dy;
+        xy += dx * dy;
+    }
+    xx = wave_sum(xx);
+    yy = wave_sum(yy);
+    xy = wave_sum(xy);
+    if (yy == 0.0) { return false; }  // a constant window throws in the reference
+    gr = fabs(xy) / xx;
+    sr = sqrt(yy / xx);
+    return true;
+}
+
 template <typename T>
 __global__ __launch_bounds__(kWave) void products_kernel(const ProductJob *__restrict__ jobs, int absval,
                                                         int interp, int lanczosAlpha, int lanczosPoints,
@@ -168,32 +199,9 @@ __global__ __launch_bounds__(kWave) void products_kernel(const ProductJob *__res
     const int Lt = job.Lt;
     if (static_cast<long long>(p) + Lt > job.sigLen) {
         flags |= kProductTruncated;
-    } else {
-        const gptr<const double> t = to_global(job.tmpl);
-        const gptr<const T> y = to_global(static_cast<const T *>(job.sig)) + p;
-        double st = 0.0, sy = 0.0;
-        for (int i = lane; i < Lt; i += kWave) {
-            st += t[i];
-            sy += static_cast<double>(y[i]);
-        }
-        const double mt = wave_sum(st) / Lt;
-        const double my = wave_sum(sy) / Lt;
-        double xx = 0.0, yy = 0.0, xy = 0.0;
-        for (int i = lane; i < Lt; i += kWave) {
-            const double dx = t[i] - mt;
-            const double dy = static_cast<double>(y[i]) - my;
-            xx += dx * dx;
-            yy += dy * dy;
-            xy += dx * dy;
-        }
-        xx = wave_sum(xx);
-        yy = wave_sum(yy);
-        xy = wave_sum(xy);
-        if (yy != 0.0) {  // a constant window throws in the reference and gets no amplitude
-            flags |= kProductAmplitude;
-            gr = fabs(xy) / xx;
-            sr = sqrt(yy / xx);
-        }
+    } else if (amplitude_factors<T>(to_global(job.tmpl), to_global(static_cast<const T *>(job.sig)) + p, Lt, gr,
+                                    sr)) {
+        flags |= kProductAmplitude;
     }
 
     if (lane == 0) {
@@ -209,7 +217,38 @@ __global__ __launch_bounds__(kWave) void products_kernel(const ProductJob *__res
     }
 }
 
+// One wavefront per window: window k is windows[k * ld .. k * ld + Lt).
+template <typename T>
+__global__ __launch_bounds__(kWave) void amplitude_kernel(const double *__restrict__ tmpl, int Lt,
+                                                         const T *__restrict__ windows, long long ld,
+                                                         ProductOut *__restrict__ out)
+{
+    double gr = 0.0, sr = 0.0;
+    const bool ok = amplitude_factors<T>(to_global(tmpl), to_global(windows) + blockIdx.x * ld, Lt, gr, sr);
+    if (threadIdx.x == 0) {
+        ProductOut o{};
+        o.alphaGR = gr;
+        o.alphaSR = sr;
+        o.flags = ok ? kProductAmplitude : 0;
+        out[blockIdx.x] = o;
+    }
+}
+
 }  // namespace
+
+hipError_t launch_amplitude(int precision, const double *dTmpl, int Lt, const void *dWindows, long long ld,
+                            int nWindows, ProductOut *dOut, hipStream_t stream)
+{
+    if (nWindows <= 0) { return hipSuccess; }
+    if (precision == 0) {
+        hipLaunchKernelGGL(amplitude_kernel<float>, dim3(nWindows), dim3(kWave), 0, stream, dTmpl, Lt,
+                           static_cast<const float *>(dWindows), ld, dOut);
+    } else {
+        hipLaunchKernelGGL(amplitude_kernel<double>, dim3(nWindows), dim3(kWave), 0, stream, dTmpl, Lt,
+                           static_cast<const double *>(dWindows), ld, dOut);
+    }
+    return hipGetLastError();
+}
 
 hipError_t launch_products(int precision, const ProductJob *dJobs, int nJobs, int absval, int interp,
                            int lanczosAlpha, int lanczosPoints, ProductOut *dOut, hipStream_t stream)

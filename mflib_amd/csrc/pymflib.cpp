@@ -38,6 +38,7 @@
 #include "mflib/singleChannel/detectionTimeInterpolationParameters.hpp"
 #include "mflib/singleChannel/detector.hpp"
 #include "mflib/singleChannel/detectorParameters.hpp"
+#include "mflib/singleChannel/relativeMagnitude.hpp"
 #include "mflib/version.hpp"
 #include "mflib/waveformTemplate.hpp"
 #include "private/peakFinder.hpp"
@@ -526,6 +527,20 @@ void bindDetector(py::module_ &sc)
     d.def("have_amplitude_scaling_factor", &D::haveAmplitudeScalingFactor,
           py::arg("type") = R::GIBBONS_RINGDAL_2006);
     d.def("get_magnitude_perturbation", &D::getMagnitudePerturbation, py::arg("type") = R::GIBBONS_RINGDAL_2006);
+
+    using RM = MFLib::SingleChannel::RelativeMagnitude<double>;
+    py::class_<RM> rm(sc, "RelativeMagnitude");
+    rm.def(py::init<>());
+    rm.doc() = "Amplitude of a detected window relative to its template (scored on the GPU, FP64).";
+    rm.def("initialize", &RM::initialize);
+    rm.def("is_initialized", &RM::isInitialized);
+    rm.def("get_detected_signal_length", &RM::getDetectedSignalLength);
+    rm.def("set_detected_signal", [](RM &m, DArray y) { m.setDetectedSignal(arrayLength(y), y.data()); });
+    rm.def("have_detected_signal", &RM::haveDetectedSignal);
+    rm.def("compute_amplitude_scaling_factor", &RM::computeAmplitudeScalingFactor,
+           py::arg("type") = R::GIBBONS_RINGDAL_2006);
+    rm.def("compute_magnitude_perturbation", &RM::computeMagnitudePerturbation,
+           py::arg("type") = R::GIBBONS_RINGDAL_2006);
 
     using X = MFLib::SingleChannel::Detector<double>;
     py::class_<X> x(sc, "Detector");

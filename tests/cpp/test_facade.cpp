@@ -8,6 +8,7 @@
 //   testing/peakFinder.cpp:16-120     PeakFinder KATs   (13 and 21 indices)
 //   testing/detector.cpp:100-283      Detector          (12 detections: times, templates,
 //                                                        interpolated times, scaling factors)
+//   testing/relativeMagnitude.cpp:20-124 RelativeMagnitude (log10 factors within 1e-12)
 // Expected correlations are the two-pass FP64 Pearson definition
 // (testing/utilities.hpp:33-69).  Prints one PASS/FAIL line per case and
 // exits non-zero on any failure.
@@ -29,6 +30,7 @@
 #include "mflib/singleChannel/detectionTimeInterpolationParameters.hpp"
 #include "mflib/singleChannel/detector.hpp"
 #include "mflib/singleChannel/detectorParameters.hpp"
+#include "mflib/singleChannel/relativeMagnitude.hpp"
 #include "mflib/waveformTemplate.hpp"
 #include "private/peakFinder.hpp"
 
@@ -513,6 +515,74 @@ void detectorTest(MFLib::MatchedFilteredSignalDetectorPolicy policy, bool lanczo
     expect(throwsType<std::invalid_argument>([&] { (void)detector[ndet]; }), "index past the end");
 }
 
+// testing/relativeMagnitude.cpp:20-124: factors of random windows against a
+// random template, checked through log10 against the test's own
+// least-squares and standard-deviation-ratio formulas (get_dmag_1/2).
+void relativeMagnitudeTest()
+{
+    const int n = 400;
+    std::mt19937_64 gen(0x4E1A7);
+    auto uniform = [&](double lo, double hi) {
+        std::uniform_real_distribution<double> u(lo, hi);
+        std::vector<double> v(n);
+        for (auto &x : v) { x = u(gen); }
+        return v;
+    };
+    auto demean = [](std::vector<double> v) {
+        double m = 0;
+        for (double x : v) { m += x; }
+        m /= static_cast<double>(v.size());
+        for (auto &x : v) { x -= m; }
+        return v;
+    };
+    const auto t = uniform(-2, 1);
+    MFLib::WaveformTemplate wt;
+    wt.setSignal(n, t.data());
+    MFLib::SingleChannel::RelativeMagnitude<double> mag;
+    expect(throwsType<std::runtime_error>([&] { mag.getDetectedSignalLength(); }), "length before initialize");
+    mag.initialize(wt);
+    expect(mag.isInitialized() && mag.getDetectedSignalLength() == n && !mag.haveDetectedSignal(), "state");
+    expect(throwsType<std::runtime_error>([&] {
+               mag.computeAmplitudeScalingFactor(MFLib::RelativeMagnitudeType::GIBBONS_RINGDAL_2006);
+           }), "factor before a window");
+    const auto x = demean(t);
+    for (auto range : {std::pair<double, double>(-3, 8), std::pair<double, double>(-7, 1)})
+    {
+        const auto sig = uniform(range.first, range.second);
+        mag.setDetectedSignal(n, sig.data());
+        expect(mag.haveDetectedSignal(), "window set");
+        const auto y = demean(sig);
+        double xy = 0, xx = 0, yy = 0;
+        for (int i = 0; i < n; ++i)
+        {
+            xy += x[i] * y[i];
+            xx += x[i] * x[i];
+            yy += y[i] * y[i];
+        }
+        const double d1 = std::log10(std::abs(xy) / xx);
+        const double d2 = std::log10(std::sqrt(yy / n) / std::sqrt(xx / n));
+        MFLib::SingleChannel::RelativeMagnitude<double> copy(mag);
+        const double m1 = copy.computeMagnitudePerturbation(MFLib::RelativeMagnitudeType::GIBBONS_RINGDAL_2006);
+        const double m2 = copy.computeMagnitudePerturbation(MFLib::RelativeMagnitudeType::SCHAFF_RICHARDS_2014);
+        expect(std::abs(m1 - d1) < 1e-12 && std::abs(m2 - d2) < 1e-12,
+               "magnitudes " + std::to_string(m1 - d1) + ", " + std::to_string(m2 - d2));
+    }
+    std::vector<double> flat(n, 2.5);
+    expect(throwsType<std::invalid_argument>([&] { mag.setDetectedSignal(n, flat.data()); }), "constant window");
+    expect(!mag.haveDetectedSignal(), "a rejected window clears the state");
+    expect(throwsType<std::invalid_argument>([&] { mag.setDetectedSignal(n - 1, t.data()); }), "window length");
+    expect(throwsType<std::invalid_argument>([&] {
+               MFLib::SingleChannel::convertAmplitudeScalingFactorToMagnitudePerturbation(0.0);
+           }), "non-positive factor");
+    MFLib::SingleChannel::RelativeMagnitude<float> magf;
+    magf.initialize(wt);
+    std::vector<float> sf(t.begin(), t.end());
+    for (auto &v : sf) { v = 3.0f * v + 1.0f; }
+    magf.setDetectedSignal(n, sf.data());
+    expect(std::abs(magf.computeAmplitudeScalingFactor(MFLib::RelativeMagnitudeType::SCHAFF_RICHARDS_2014) - 3.0f) <
+               1e-5f, "float instance, scaled copy");
+}
+
 void errorConventionTest()
 {
     const auto in = basicInputs();
@@ -560,6 +630,7 @@ int main()
     runCase("detectorSingleLanczos", [] {
         detectorTest<double>(MFLib::MatchedFilteredSignalDetectorPolicy::SINGLE, true);
     });
+    runCase("relativeMagnitude", relativeMagnitudeTest);
     runCase("errorConventions", errorConventionTest);
     std::printf("%d failure(s)\n", failures);
     return failures == 0 ? 0 : 1;

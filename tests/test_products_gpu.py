@@ -180,3 +180,54 @@ def test_products_argument_errors():
     with pytest.raises(RuntimeError):                          # before apply
         h2.detection_products([5], [0])
     h2.close()
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_amplitude_factors_batch_matches_oracle(dtype):
+    """mf_amplitude_factors (RelativeMagnitude on the GPU) against the
+    oracle's restatement, window by window, including a constant window."""
+    rng = np.random.default_rng(5 + (dtype == np.float64))
+    lt, nw = 333, 40
+    t = rng.standard_normal(lt)
+    wins = rng.standard_normal((nw, lt)) * rng.uniform(0.1, 5, (nw, 1)) + rng.uniform(-50, 50, (nw, 1))
+    wins[3] = 2.0 * t + 1.0
+    wins[7] = 4.25                                   # constant: no factors
+    wins = wins.astype(dtype)
+    gr, sr, ok = capi.amplitude_factors(t, wins, dtype)
+    for k in range(nw):
+        ref = O.amplitude_factors(t, wins[k].astype(np.float64))
+        if ref is None:
+            assert not ok[k], k
+            continue
+        assert ok[k]
+        assert gr[k] == pytest.approx(ref[0], rel=1e-11, abs=1e-11 * ref[1]), k
+        assert sr[k] == pytest.approx(ref[1], rel=1e-11), k
+    assert not ok[7] and ok.sum() == nw - 1
+    if dtype == np.float64:
+        assert gr[3] == pytest.approx(2.0, rel=1e-12) and sr[3] == pytest.approx(2.0, rel=1e-12)
+
+
+def test_pymflib_relative_magnitude():
+    """python/singleChannel/relativeMagnitude.cpp's names over the GPU path:
+    testing/relativeMagnitude.cpp's formulas (get_dmag_1/2) at 1e-12."""
+    pymflib = pytest.importorskip("mflib_amd.pymflib")
+    rng = np.random.default_rng(9)
+    t = rng.uniform(-2, 1, 400)
+    w = pymflib.WaveformTemplate()
+    w.set_signal(t)
+    w.set_sampling_rate(100.0)
+    mag = pymflib.SingleChannel.RelativeMagnitude()
+    mag.initialize(w)
+    assert mag.is_initialized() and mag.get_detected_signal_length() == 400
+    y = rng.uniform(-3, 8, 400)
+    mag.set_detected_signal(y)
+    assert mag.have_detected_signal()
+    xd, yd = t - t.mean(), y - y.mean()
+    R = pymflib.RelativeMagnitudeType
+    d1 = np.log10(abs(xd @ yd) / (xd @ xd))
+    d2 = np.log10(np.sqrt((yd @ yd) / 400) / np.sqrt((xd @ xd) / 400))
+    assert abs(mag.compute_magnitude_perturbation(R.gibbons_ringdal_2006) - d1) < 1e-12
+    assert abs(mag.compute_magnitude_perturbation(R.schaff_richards_2014) - d2) < 1e-12
+    with pytest.raises(ValueError):
+        mag.set_detected_signal(np.full(400, 1.5))    # constant
+    assert not mag.have_detected_signal()

@@ -246,3 +246,16 @@ def test_set_signal_accepts_host_arrays_of_any_kind():
     with pytest.raises(RuntimeError):
         mf.set_signal(0, [0.0, 1.0])
     assert hasattr(pymflib, "DeviceRow")
+
+
+def test_relative_magnitude_state_checks():
+    """src/singleChannel/relativeMagnitude.cpp:273-359: use before
+    initialize and an empty template fail the reference's way (no GPU call)."""
+    mag = pymflib.SingleChannel.RelativeMagnitude()
+    assert not mag.is_initialized() and not mag.have_detected_signal()
+    with pytest.raises(RuntimeError):
+        mag.get_detected_signal_length()
+    with pytest.raises(RuntimeError):
+        mag.compute_amplitude_scaling_factor(pymflib.RelativeMagnitudeType.gibbons_ringdal_2006)
+    with pytest.raises(RuntimeError):
+        mag.initialize(pymflib.WaveformTemplate())   # no signal on the template
