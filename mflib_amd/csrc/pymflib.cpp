@@ -527,6 +527,59 @@ void bindDetector(py::module_ &sc)
     d.def("have_amplitude_scaling_factor", &D::haveAmplitudeScalingFactor,
           py::arg("type") = R::GIBBONS_RINGDAL_2006);
     d.def("get_magnitude_perturbation", &D::getMagnitudePerturbation, py::arg("type") = R::GIBBONS_RINGDAL_2006);
+    // pickling (python/singleChannel/detection.cpp:497-660 pickles the same
+    // fields; the detected waveform is kept here too)
+    d.def(py::pickle(
+        [](const D &x) {
+            auto opt = [](bool have, double v) { return have ? py::object(py::float_(v)) : py::object(py::none()); };
+            py::object id = py::none();
+            if (x.haveTemplateIdentifier())
+            {
+                auto t = x.getTemplateIdentifier();
+                id = py::make_tuple(t.first, t.second);
+            }
+            const bool gr = x.haveAmplitudeScalingFactor(R::GIBBONS_RINGDAL_2006);
+            const bool sr = x.haveAmplitudeScalingFactor(R::SCHAFF_RICHARDS_2014);
+            py::object wave = x.haveDetectedSignal()
+                                  ? py::object(toArray(x.getDetectedSignalPointer(), x.getDetectedSignalLength()))
+                                  : py::object(py::none());
+            return py::make_tuple(
+                opt(x.haveCorrelationCoefficient(), x.haveCorrelationCoefficient() ? x.getCorrelationCoefficient() : 0),
+                id, opt(x.haveDetectionTime(), x.haveDetectionTime() ? x.getDetectionTime() : 0),
+                opt(x.haveInterpolatedDetectionTime(),
+                    x.haveInterpolatedDetectionTime() ? x.getInterpolatedDetectionTime() : 0),
+                opt(x.havePhaseOnsetTime(), x.havePhaseOnsetTime() ? x.getPhaseOnsetTime() : 0),
+                opt(x.haveInterpolatedPhaseOnsetTime(),
+                    x.haveInterpolatedPhaseOnsetTime() ? x.getInterpolatedPhaseOnsetTime() : 0),
+                opt(x.haveTravelTime(), x.haveTravelTime() ? x.getTravelTime() : 0),
+                opt(gr, gr ? x.getAmplitudeScalingFactor(R::GIBBONS_RINGDAL_2006) : 0),
+                opt(sr, sr ? x.getAmplitudeScalingFactor(R::SCHAFF_RICHARDS_2014) : 0),
+                static_cast<int>(x.getPolarity()), wave);
+        },
+        [](const py::tuple &t) {
+            if (t.size() != 11) { throw std::runtime_error("invalid Detection state"); }
+            D x;
+            if (!t[0].is_none()) { x.setCorrelationCoefficient(t[0].cast<double>()); }
+            if (!t[1].is_none())
+            {
+                py::tuple id = t[1].cast<py::tuple>();
+                x.setTemplateIdentifier({id[0].cast<MFLib::NetworkStationPhase>(), id[1].cast<uint64_t>()});
+            }
+            if (!t[2].is_none()) { x.setDetectionTime(t[2].cast<double>()); }
+            if (!t[3].is_none()) { x.setInterpolatedDetectionTime(t[3].cast<double>()); }
+            if (!t[4].is_none()) { x.setPhaseOnsetTime(t[4].cast<double>()); }
+            if (!t[5].is_none()) { x.setInterpolatedPhaseOnsetTime(t[5].cast<double>()); }
+            if (!t[6].is_none()) { x.setTravelTime(t[6].cast<double>()); }
+            if (!t[7].is_none()) { x.setAmplitudeScalingFactor(t[7].cast<double>(), R::GIBBONS_RINGDAL_2006); }
+            if (!t[8].is_none()) { x.setAmplitudeScalingFactor(t[8].cast<double>(), R::SCHAFF_RICHARDS_2014); }
+            x.setPolarity(static_cast<MFLib::Polarity>(t[9].cast<int>()));
+            if (!t[10].is_none())
+            {
+                DArray a = t[10].cast<DArray>();
+                x.setDetectedSignal(arrayLength(a), a.data());
+            }
+            return x;
+        }));
 
     using RM = MFLib::SingleChannel::RelativeMagnitude<double>;
     py::class_<RM> rm(sc, "RelativeMagnitude");

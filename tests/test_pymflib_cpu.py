@@ -259,3 +259,30 @@ def test_relative_magnitude_state_checks():
         mag.compute_amplitude_scaling_factor(pymflib.RelativeMagnitudeType.gibbons_ringdal_2006)
     with pytest.raises(RuntimeError):
         mag.initialize(pymflib.WaveformTemplate())   # no signal on the template
+
+
+def test_detection_pickles():
+    sc = pymflib.SingleChannel
+    R = pymflib.RelativeMagnitudeType
+    d = sc.Detection()
+    nsp = pymflib.NetworkStationPhase()
+    nsp.set_station("CCUT")
+    d.set_template_identifier([nsp, 3])
+    d.set_correlation_coefficient(0.91)
+    d.set_detection_time(4.0)
+    d.set_interpolated_detection_time(4.001)
+    d.set_travel_time(2.5)
+    d.set_amplitude_scaling_factor(0.25, R.schaff_richards_2014)
+    d.set_polarity(pymflib.Polarity.compression)
+    d.set_detected_signal(np.linspace(0, 1, 7))
+    e = pickle.loads(pickle.dumps(d))
+    assert e.get_correlation_coefficient() == 0.91 and e.get_template_identifier()[1] == 3
+    assert e.get_template_identifier()[0] == nsp
+    assert e.get_detection_time() == 4.0 and e.get_interpolated_detection_time() == 4.001
+    assert e.get_travel_time() == 2.5 and not e.have_phase_onset_time()
+    assert e.get_amplitude_scaling_factor(R.schaff_richards_2014) == 0.25
+    assert not e.have_amplitude_scaling_factor(R.gibbons_ringdal_2006)
+    assert e.get_polarity() == pymflib.Polarity.compression
+    assert np.array_equal(e.get_detected_signal(), np.linspace(0, 1, 7))
+    empty = pickle.loads(pickle.dumps(sc.Detection()))
+    assert not empty.have_detection_time() and not empty.have_detected_signal()
