@@ -140,3 +140,42 @@ def test_stream_restart_and_errors():
         h1.set_streaming(False)                   # only before initialize
     h1.close()
     h2.close()
+
+
+@pytest.mark.parametrize("variant", [0, 3])
+def test_batched_stream_with_graph_launch_and_mfma_rows(variant):
+    """Streaming in the batched layout (two channels, two stacking groups)
+    with graph launch on; under variant 3 the short templates (<= 256 taps)
+    take the time-domain MFMA path.  Row tails against the FP64 oracle at
+    1e-5."""
+    rng = np.random.default_rng(53 + variant)
+    lens = [200, 150, 256, 200]
+    tpls = [rng.standard_normal(L) for L in lens]
+    row_ch, row_grp, shifts = [0, 0, 1, 1], [0, 0, 1, 1], [0, 11, 0, 5]
+    chunk, pushes = 8000, 5
+    total = chunk * pushes
+    sigs = [_signal(rng, total, 3.0 + 20 * c) for c in range(2)]
+    h = capi.Handle(np.float32, "batched")
+    h.set_templates(tpls, shifts=shifts, weights=[1.0] * 4)
+    h.set_batch_layout([chunk, chunk], row_ch, row_grp, 2)
+    h.set_kernel_variant(variant)
+    h.set_graph_launch(True)
+    h.set_streaming(True)
+    h.initialize()
+    carry = h.stream_carry()
+    assert carry == max(lens) - 1 + max(shifts)
+    tails = [[] for _ in tpls]
+    for p in range(pushes):
+        for c in range(2):
+            h.set_signal(c, sigs[c][p * chunk:(p + 1) * chunk].astype(np.float32))
+        h.apply()
+        for r in range(4):
+            tails[r].append(h.rho(r)[-chunk:])
+    for r in range(4):
+        nb = max(lens[k] for k in range(4) if row_grp[k] == row_grp[r])
+        ref = O.pearson(tpls[r], sigs[row_ch[r]].astype(np.float32).astype(np.float64))
+        got = np.concatenate(tails[r])
+        starts = np.arange(total) - nb + 1
+        ok = (starts >= 0) & (starts < total - nb + 1)
+        assert np.abs(got[ok] - ref[starts[ok]]).max() < 1e-5, r
+    h.close()
