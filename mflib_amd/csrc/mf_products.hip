@@ -35,8 +35,9 @@ namespace {
 
 constexpr int kWave = 64;
 
-// sinc and the Lanczos kernel (detector.cpp:91-109): sinc(0) = 1 below
-// |x| <= 100 eps; the window keeps every x with int(|x|) <= a.
+// sinc of the Lanczos kernel sinc(x) sinc(x / a) (detector.cpp:91-109):
+// sinc(0) = 1 below |x| <= 100 eps; the window keeps every x with
+// int(|x|) <= a.
 __device__ __forceinline__ double sinc(double x)
 {
     double r = 1.0;
@@ -44,13 +45,6 @@ __device__ __forceinline__ double sinc(double x)
         const double pix = M_PI * x;
         r = sin(pix) / pix;
     }
-    return r;
-}
-
-__device__ __forceinline__ double lanczos(double x, int a)
-{
-    double r = 0.0;
-    if (static_cast<int>(fabs(x)) <= a) { r = sinc(x) * sinc(x / a); }
     return r;
 }
 
@@ -104,13 +98,24 @@ __device__ double lanczos_shift(gptr<const T> xc, int n, int p, double dt, int a
         const double shift = -0.5 + dtWork * idx;
         const double x = p + shift;
         const int fl = static_cast<int>(floor(x));
+        // The taps sit at xi = frac + k - 1 (frac = x - floor(x), exact), so
+        // sin(pi xi) = (-1)^(k-1) sin(pi frac): one sine per grid point for
+        // the first sinc factor instead of one per tap.  The window's own
+        // factor sinc(xi / a) keeps its sine per tap.
+        const double sFrac = sin(M_PI * (x - static_cast<double>(fl)));
         double y = 0.0;
         for (int k = -a; k <= a; ++k) {
             const long long i = static_cast<long long>(fl) - k + 1;  // no int overflow near 2^31 lags
             const double xi = x - static_cast<double>(i);
             double v = 0.0;
             if (i >= 0 && i < n) { v = static_cast<double>(xc[i]); }
-            y = y + v * lanczos(xi, a);
+            double w = 0.0;
+            if (static_cast<int>(fabs(xi)) <= a) {
+                const double s1 = ((k - 1) & 1) ? -sFrac : sFrac;
+                const double sinc1 = fabs(xi) > DBL_EPSILON * 100 ? s1 / (M_PI * xi) : 1.0;
+                w = sinc1 * sinc(xi / a);
+            }
+            y = y + v * w;
         }
         if (absval) { y = fabs(y); }
         // a lane's own points are visited in increasing idx: strict '>' keeps the first
