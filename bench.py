@@ -285,7 +285,8 @@ def uneven_section(args, torch):
 def streaming_section(args, torch, wl, day_row0):
     """Streaming: the multi-channel day pushed in 60 s chunks (6,000 samples
     per channel per push) into a streaming handle of the same layout; per
-    push: 24 device chunks in, all 384 rows, 16 stacks and PeakFinder.
+    push: 24 device chunks in (one batched push), all 384 rows, 16 stacks
+    and PeakFinder.
     Audit: the row tails of template event 0 on channel 0 against the
     full-day row of the same engine (only the FFT blocking differs)."""
     from mflib_amd import capi
@@ -304,8 +305,7 @@ def streaming_section(args, torch, wl, day_row0):
     lat, tails = [], []
     for p in range(pushes):
         t0 = time.perf_counter()
-        for c in range(C):
-            h.set_signal_device(c, dev[c][p * chunk:].data_ptr(), chunk)
+        h.push_chunks_device([dev[c][p * chunk:].data_ptr() for c in range(C)], chunk)
         h.apply()
         h.detect(0.5, 512)
         lat.append((time.perf_counter() - t0) * 1e3)

@@ -108,6 +108,10 @@ int mf_set_graph_launch(mf_handle *h, int on);
  * newest chunk. */
 int mf_set_streaming(mf_handle *h, int on);
 int mf_get_stream_carry(const mf_handle *h, int *carry);
+/* No reference counterpart: one streaming push of every channel at once
+ * from device memory (dx[c] holds channel c's chunk of n samples), waiting
+ * once for all the copies instead of once per channel. */
+int mf_push_chunks_device(mf_handle *h, int nChannels, int n, const void *const *dx);
 
 /* MatchedFilter<T>::initialize (src/matchedFilter.cpp:506-821): template
  * demean/normalize (FP64), plans, template spectra, device buffers. */

@@ -79,6 +79,7 @@ _SIGNATURES = {
     "mf_get_device": ([_vp, _ip], ctypes.c_int),
     "mf_set_streaming": ([_vp, ctypes.c_int], ctypes.c_int),
     "mf_get_stream_carry": ([_vp, _ip], ctypes.c_int),
+    "mf_push_chunks_device": ([_vp, ctypes.c_int, ctypes.c_int, ctypes.POINTER(_vp)], ctypes.c_int),
     "mf_detect_template": ([_vp, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_int, _ip, ctypes.c_int, _ip],
                            ctypes.c_int),
     "mf_amplitude_factors": ([ctypes.c_int, ctypes.c_int, _dp, ctypes.c_int, _vp, ctypes.c_longlong, _dp, _dp,
@@ -280,6 +281,11 @@ class Handle:
 
     def stream_carry(self) -> int:
         return self._int(lib().mf_get_stream_carry)
+
+    def push_chunks_device(self, ptrs, n: int):
+        """Streaming push of every channel from device pointers ptrs[c]."""
+        arr = (_vp * len(ptrs))(*[_vp(int(p)) for p in ptrs])
+        check(lib().mf_push_chunks_device(self._h, len(ptrs), int(n), arr), "push_chunks_device")
 
     def initialize(self):
         check(lib().mf_initialize(self._h), "initialize")

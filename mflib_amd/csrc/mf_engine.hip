@@ -1297,7 +1297,7 @@ namespace {
 // front (through a scratch buffer: the ranges overlap when the chunk is
 // shorter than the carry) and the chunk follows.  The dead-trace rule over
 // the whole buffer (carry + chunk) is evaluated on the device by apply().
-void pushChunk(mf_handle *h, int channel, int n, const void *x, hipMemcpyKind kind)
+void pushChunk(mf_handle *h, int channel, int n, const void *x, hipMemcpyKind kind, bool wait = true)
 {
     auto &c = h->ch[channel];
     const size_t ts = h->tsize;
@@ -1309,9 +1309,9 @@ void pushChunk(mf_handle *h, int channel, int n, const void *x, hipMemcpyKind ki
                               h->stream));
     }
     MF_HIP(hipMemcpyAsync(buf + static_cast<size_t>(h->carry) * ts, x, static_cast<size_t>(n) * ts, kind, h->stream));
-    // the caller may reuse its buffer once this returns
-    MF_HIP(hipStreamSynchronize(h->stream));
     c.set = true;
+    // the caller may reuse its buffer once the push returns
+    if (wait) { MF_HIP(hipStreamSynchronize(h->stream)); }
 }
 
 void checkChunkLength(const mf_handle *h, int c, int n)
@@ -1324,6 +1324,25 @@ void checkChunkLength(const mf_handle *h, int c, int n)
 }
 
 }  // namespace
+
+int mf_push_chunks_device(mf_handle *h, int nChannels, int n, const void *const *dx)
+{
+    return guard([&] {
+        requireInit(h);
+        if (!h->streaming) { fail(MF_ERR_BAD_STATE, "mf_push_chunks_device needs streaming mode"); }
+        if (nChannels != static_cast<int>(h->ch.size())) {
+            fail(MF_ERR_INVALID_ARGUMENT, "a push covers all " + std::to_string(h->ch.size()) + " channels");
+        }
+        if (dx == nullptr) { fail(MF_ERR_INVALID_ARGUMENT, "chunk table is NULL"); }
+        for (int c = 0; c < nChannels; ++c) {
+            checkChunkLength(h, c, n);
+            if (dx[c] == nullptr) { fail(MF_ERR_INVALID_ARGUMENT, "chunk of channel " + std::to_string(c) + " is NULL"); }
+        }
+        h->applied = false;
+        for (int c = 0; c < nChannels; ++c) { pushChunk(h, c, n, dx[c], hipMemcpyDeviceToDevice, false); }
+        MF_HIP(hipStreamSynchronize(h->stream));
+    });
+}
 
 int mf_set_streaming(mf_handle *h, int on)
 {

@@ -179,3 +179,46 @@ def test_batched_stream_with_graph_launch_and_mfma_rows(variant):
         ok = (starts >= 0) & (starts < total - nb + 1)
         assert np.abs(got[ok] - ref[starts[ok]]).max() < 1e-5, r
     h.close()
+
+
+def test_batched_push_equals_per_channel_pushes():
+    """mf_push_chunks_device (every channel in one call, one wait) gives the
+    rows and stacks of one mf_set_signal_device per channel, bit for bit."""
+    rng = np.random.default_rng(59)
+    tpls = [rng.standard_normal(L) for L in (240, 180, 240)]
+    chunk, pushes = 4000, 4
+    sigs = [_signal(rng, chunk * pushes, 5.0 * c) for c in range(3)]
+
+    def fresh():
+        h = capi.Handle(np.float32, "multi")
+        h.set_templates(tpls, shifts=[0, 4, 9], weights=[1.0, 1.0, 1.0])
+        h.set_signal_size(chunk)
+        h.set_streaming(True)
+        h.initialize()
+        return h
+
+    a, b = fresh(), fresh()
+    for p in range(pushes):
+        chunks = [DeviceArray(sigs[c][p * chunk:(p + 1) * chunk].astype(np.float32)) for c in range(3)]
+        for c in range(3):
+            a.set_signal_device(c, chunks[c].ptr, chunk)
+        b.push_chunks_device([d.ptr for d in chunks], chunk)
+        for d in chunks:
+            d.free()
+        a.apply()
+        b.apply()
+        for r in range(3):
+            assert np.array_equal(a.rho(r), b.rho(r)), (p, r)
+        assert np.array_equal(a.shift_and_stack(), b.shift_and_stack()), p
+    with pytest.raises(ValueError):
+        b.push_chunks_device([0, 0], chunk)            # not every channel
+    a.close()
+    b.close()
+    h = capi.Handle(np.float32, "multi")
+    h.set_templates(tpls)
+    h.set_signal_size(chunk)
+    h.initialize()
+    with pytest.raises(RuntimeError):
+        h.push_chunks_device([0, 0, 0], chunk)         # not a streaming handle
+    h.close()
+
