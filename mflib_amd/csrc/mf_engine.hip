@@ -107,6 +107,7 @@ struct mf_handle {
     DevBuf dStackRows, dStackOut;
     DevBuf dReduceRows, dRedDet, dRedId, dRedPk, dRedPkId;  // Detector reduce step
     DevBuf dProdJobs, dProdOut;                             // per-detection products
+    PinnedBuf hProd;                                        // their host staging (jobs, then results)
     // stacking groups (built on first stack/detect)
     struct StackInfo {
         int first = 0, count = 0, smax = 0, minShift = 0;
@@ -1710,7 +1711,11 @@ int mf_detection_products(mf_handle *h, int nPeaks, const int *peakIdx, const in
             }
         }
         if (nPeaks == 0) { return; }
-        std::vector<mfk::ProductJob> jobs(nPeaks);
+        const size_t jobBytes = static_cast<size_t>(nPeaks) * sizeof(mfk::ProductJob);
+        const size_t outBytes = static_cast<size_t>(nPeaks) * sizeof(mfk::ProductOut);
+        MF_HIP(hipStreamSynchronize(h->stream));  // the staging and job table may be in use by an earlier call
+        h->hProd.reserve(std::max(jobBytes, outBytes));
+        auto *jobs = static_cast<mfk::ProductJob *>(h->hProd.p);
         for (int k = 0; k < nPeaks; ++k) {
             const int it = templateIds[k];
             checkRow(h, it);
@@ -1732,16 +1737,15 @@ int mf_detection_products(mf_handle *h, int nPeaks, const int *peakIdx, const in
             j.Lt = rp.Lt;
             j.pad = 0;
         }
-        growTo(h->dProdJobs, jobs.size() * sizeof(mfk::ProductJob));
-        growTo(h->dProdOut, jobs.size() * sizeof(mfk::ProductOut));
-        MF_HIP(hipStreamSynchronize(h->stream));  // the job table may be in use by an earlier call
-        MF_HIP(hipMemcpy(h->dProdJobs.p, jobs.data(), jobs.size() * sizeof(mfk::ProductJob), hipMemcpyHostToDevice));
+        growTo(h->dProdJobs, jobBytes);
+        growTo(h->dProdOut, outBytes);
+        MF_HIP(hipMemcpyAsync(h->dProdJobs.p, jobs, jobBytes, hipMemcpyHostToDevice, h->stream));
         MF_HIP(mfk::launch_products(h->precision, h->dProdJobs.as<mfk::ProductJob>(), nPeaks, absoluteValue ? 1 : 0,
                                     interpolation, lanczosAlpha, lanczosPoints, h->dProdOut.as<mfk::ProductOut>(),
                                     h->stream));
-        std::vector<mfk::ProductOut> res(nPeaks);
-        MF_HIP(hipMemcpyAsync(res.data(), h->dProdOut.p, res.size() * sizeof(mfk::ProductOut), hipMemcpyDeviceToHost,
-                              h->stream));
+        // the results reuse the staging buffer: the copy above is ordered before them on the stream
+        auto *res = static_cast<mfk::ProductOut *>(h->hProd.p);
+        MF_HIP(hipMemcpyAsync(res, h->dProdOut.p, outBytes, hipMemcpyDeviceToHost, h->stream));
         MF_HIP(hipStreamSynchronize(h->stream));
         for (int k = 0; k < nPeaks; ++k) {
             if (rho) { rho[k] = res[k].rho; }
