@@ -153,7 +153,10 @@ int mf_have_matched_filtered_signals(const mf_handle *h, int *yes);
  * getFilteredSignalLength() correlation samples to host memory. */
 int mf_get_rho(mf_handle *h, int row, void *out, int cap);
 /* No reference counterpart: device pointer of row `row` (valid until the
- * next mf_initialize / mf_clear / mf_destroy). */
+ * next mf_initialize / mf_clear / mf_destroy).  It does not wait: the row
+ * holds the last mf_apply's values once mf_synchronize has returned (or any
+ * call that returns results to the host); work on it from another stream
+ * (mf_peaks_device, say) must come after that. */
 int mf_get_rho_device(mf_handle *h, int row, const void **dptr);
 
 /* shiftAndStack() (src/matchedFilter.cpp:1318-1364) of stacking group

@@ -824,6 +824,7 @@ void pruneCandidates(std::vector<std::pair<T, int>> &cand, int minDist, int *idx
 struct PeakWork {
     hipStream_t stream = nullptr;
     DevBuf count, idx, val, staged;
+    PinnedBuf host;  // count and the first candidates, read back together
 };
 PeakWork &peakWork()
 {
@@ -865,11 +866,28 @@ void peaksDevice(const void *dx, long long n, double threshold, int minDist, int
     MF_HIP(hipMemsetAsync(w.count.p, 0, sizeof(int), stream));
     MF_HIP(mfk::launch_peaks(sizeof(T) == 4 ? 0 : 1, dx, n, threshold, w.count.as<int>(), w.idx.as<int>(), w.val.p,
                              candCap, stream));
-    int count = 0;
-    MF_HIP(hipMemcpyAsync(&count, w.count.p, sizeof(int), hipMemcpyDeviceToHost, stream));
+    // one readback of the count and the first kPrefix candidates (usually
+    // all of them); the rest, if any, in a second
+    constexpr int kPrefix = 8192;
+    const int pre = std::min(kPrefix, candCap);
+    const size_t valOff = (1 + static_cast<size_t>(pre)) * sizeof(int);
+    w.host.reserve(valOff + static_cast<size_t>(pre) * sizeof(T));
+    int *hCount = static_cast<int *>(w.host.p);
+    int *hIdx = hCount + 1;
+    T *hVal = reinterpret_cast<T *>(static_cast<char *>(w.host.p) + valOff);
+    MF_HIP(hipMemcpyAsync(hCount, w.count.p, sizeof(int), hipMemcpyDeviceToHost, stream));
+    MF_HIP(hipMemcpyAsync(hIdx, w.idx.p, static_cast<size_t>(pre) * sizeof(int), hipMemcpyDeviceToHost, stream));
+    MF_HIP(hipMemcpyAsync(hVal, w.val.p, static_cast<size_t>(pre) * sizeof(T), hipMemcpyDeviceToHost, stream));
     MF_HIP(hipStreamSynchronize(stream));
+    const int count = *hCount;
     if (count > candCap) { fail(MF_ERR_DEVICE, "peak candidate buffer overflow"); }
-    collectCandidates<T>(w.idx.as<int>(), w.val.p, count, minDist, idx, cap, nPeaks, stream);
+    if (count > pre) {
+        collectCandidates<T>(w.idx.as<int>(), w.val.p, count, minDist, idx, cap, nPeaks, stream);
+        return;
+    }
+    std::vector<std::pair<T, int>> cand(count);
+    for (int i = 0; i < count; ++i) { cand[i] = {hVal[i], hIdx[i]}; }
+    pruneCandidates<T>(cand, minDist, idx, cap, nPeaks);
 }
 
 // Stack every group and find the peaks of every stack (one launch each),
