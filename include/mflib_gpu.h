@@ -189,7 +189,10 @@ int mf_template_spectrum(mf_handle *h, int row, void *outInterleaved, int cap, i
  * index order.  idx must hold cap entries; *nPeaks receives the count. */
 int mf_peaks(const void *x, int n, int precision, double threshold, int minDist,
              int *idx, int cap, int *nPeaks);
-/* No reference counterpart: the same on a device-resident signal. */
+/* No reference counterpart: the same on a device-resident signal.  It runs
+ * on the peak finder's own blocking stream, which is ordered after work on
+ * the default stream but not after a handle's stream: call mf_synchronize
+ * on the handle first when dx is one of its rows. */
 int mf_peaks_device(const void *dx, int n, int precision, double threshold,
                     int minDist, int *idx, int cap, int *nPeaks);
 
@@ -320,7 +323,10 @@ int mf_server_add_templates(mf_server *s, int n, const int *lengths, const doubl
                             int *firstId);
 /* Queues a request.  channels[c] points to lengths[c] samples of the
  * server precision in device memory (onDevice != 0) or host memory; the
- * memory must stay valid until the request completes.  Errors in the
+ * memory must stay valid until the request completes.  Device channels must
+ * already hold their samples: the server reads them from its own streams,
+ * which are not ordered after the caller's, so finish (synchronize) the
+ * work that writes them before submitting.  Errors in the
  * request (bad ids, channels shorter than a template, a request larger
  * than the whole budget) are reported here. */
 int mf_server_submit(mf_server *s, int nChannels, const int *lengths, const void *const *channels,

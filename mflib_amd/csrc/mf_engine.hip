@@ -1897,8 +1897,10 @@ int mf_window_scale_device(const void *dx, long long n, int Lt, int scanPrecisio
         }
         if (n < Lt) { fail(MF_ERR_INVALID_ARGUMENT, "signal is shorter than the template"); }
         requireDevice();
+        // a blocking stream: work the caller queued on the default stream
+        // (producing dx, say) completes before the scan reads it
         hipStream_t st = nullptr;
-        MF_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+        MF_HIP(hipStreamCreate(&st));
         hipEvent_t e0 = nullptr, e1 = nullptr;
         auto cleanup = [&] {
             if (e0) { hipEventDestroy(e0); }
