@@ -116,6 +116,23 @@ def test_peak_finder_kats():
     assert len(capi.peaks(np.full(4000, 0.4), thr, md)) == 0
 
 
+def test_peak_finder_many_candidates_vs_oracle():
+    """Tens of thousands of candidates: more than the first readback holds
+    (8192), so the rest come back in a second copy; host and device inputs."""
+    from hipmem import DeviceArray
+    rng = np.random.default_rng(77)
+    x = np.round(rng.uniform(0, 1, 120_000), 4)         # white noise: a local maximum every ~3 samples
+    for md in (0, 3, 40):
+        ref = O.peak_finder(x, 0.0, md)
+        assert np.array_equal(capi.peaks(x, 0.0, md), ref)
+        d = DeviceArray(x)
+        try:
+            assert np.array_equal(capi.peaks_device(d.ptr, x.size, np.float64, 0.0, md), ref)
+        finally:
+            d.free()
+    assert len(O.peak_finder(x, 0.0, 0)) > 8192
+
+
 @pytest.mark.parametrize("seed", [1, 2, 3])
 def test_peak_finder_random_vs_oracle(seed):
     rng = np.random.default_rng(seed)
