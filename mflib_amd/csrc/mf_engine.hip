@@ -837,6 +837,19 @@ void growTo(DevBuf &b, size_t bytes)
     if (b.bytes < bytes) { b.alloc(bytes); }
 }
 
+// Device workspace of mf_amplitude_factors, kept for the life of the
+// process (guarded by g_amplitudeMutex).
+struct AmplitudeWork {
+    hipStream_t stream = nullptr;
+    DevBuf tmpl, windows, out;
+};
+std::mutex g_amplitudeMutex;
+AmplitudeWork &amplitudeWork()
+{
+    static AmplitudeWork *w = new AmplitudeWork();  // never destroyed: outlives static teardown order
+    return *w;
+}
+
 // Copy `count` candidates to the host and apply the reference pruning.
 template <typename T>
 void collectCandidates(const int *dIdx, const void *dVal, int count, int minDist, int *idx, int cap,
@@ -1844,28 +1857,23 @@ int mf_amplitude_factors(int precision, int Lt, const double *tmpl, int nWindows
         requireDevice();
         const size_t ts = precision == MF_PRECISION_FLOAT ? 4 : 8;
         const size_t winElems = static_cast<size_t>(nWindows - 1) * static_cast<size_t>(ld) + static_cast<size_t>(Lt);
-        DevBuf dT, dW, dOut;
-        dT.alloc(static_cast<size_t>(Lt) * sizeof(double));
-        dW.alloc(winElems * ts);
-        dOut.alloc(static_cast<size_t>(nWindows) * sizeof(mfk::ProductOut));
-        hipStream_t st = nullptr;
-        MF_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+        // a process-wide workspace, grown on demand (one window per call is
+        // the common case: RelativeMagnitude scores each window as it is set)
+        std::lock_guard<std::mutex> lock(g_amplitudeMutex);
+        AmplitudeWork &w = amplitudeWork();
+        if (w.stream == nullptr) { MF_HIP(hipStreamCreate(&w.stream)); }
+        growTo(w.tmpl, static_cast<size_t>(Lt) * sizeof(double));
+        growTo(w.windows, winElems * ts);
+        growTo(w.out, static_cast<size_t>(nWindows) * sizeof(mfk::ProductOut));
         std::vector<mfk::ProductOut> res(nWindows);
-        const hipError_t e = [&] {
-            hipError_t r = hipMemcpyAsync(dT.p, tmpl, static_cast<size_t>(Lt) * sizeof(double), hipMemcpyHostToDevice, st);
-            if (r == hipSuccess) { r = hipMemcpyAsync(dW.p, windows, winElems * ts, hipMemcpyHostToDevice, st); }
-            if (r == hipSuccess) {
-                r = mfk::launch_amplitude(precision, dT.as<double>(), Lt, dW.p, ld, nWindows,
-                                          dOut.as<mfk::ProductOut>(), st);
-            }
-            if (r == hipSuccess) {
-                r = hipMemcpyAsync(res.data(), dOut.p, res.size() * sizeof(mfk::ProductOut), hipMemcpyDeviceToHost, st);
-            }
-            if (r == hipSuccess) { r = hipStreamSynchronize(st); }
-            return r;
-        }();
-        (void)hipStreamDestroy(st);
-        MF_HIP(e);
+        MF_HIP(hipMemcpyAsync(w.tmpl.p, tmpl, static_cast<size_t>(Lt) * sizeof(double), hipMemcpyHostToDevice,
+                              w.stream));
+        MF_HIP(hipMemcpyAsync(w.windows.p, windows, winElems * ts, hipMemcpyHostToDevice, w.stream));
+        MF_HIP(mfk::launch_amplitude(precision, w.tmpl.as<double>(), Lt, w.windows.p, ld, nWindows,
+                                     w.out.as<mfk::ProductOut>(), w.stream));
+        MF_HIP(hipMemcpyAsync(res.data(), w.out.p, res.size() * sizeof(mfk::ProductOut), hipMemcpyDeviceToHost,
+                              w.stream));
+        MF_HIP(hipStreamSynchronize(w.stream));
         for (int k = 0; k < nWindows; ++k) {
             if (alphaGR) { alphaGR[k] = res[k].alphaGR; }
             if (alphaSR) { alphaSR[k] = res[k].alphaSR; }
