@@ -420,6 +420,20 @@ def precision_section(args, torch):
                 wall = float(np.median(walls))
                 products_info[kind] = {"ms_per_call": wall, "detections_per_s": npk / (wall * 1e-3)}
                 log(f"detection products {kind}: {wall:.2f} ms per call of {npk} detections")
+            # the Detector's default (SINGLE) policy over all 1024 rows: one
+            # batch against the per-template loop (timed on 32 rows, scaled)
+            h.detect_templates(0.5, 200, True)
+            t0 = time.perf_counter()
+            lists = h.detect_templates(0.5, 200, True)
+            single_batch = (time.perf_counter() - t0) * 1e3
+            t0 = time.perf_counter()
+            same = all(np.array_equal(h.detect_template(r, 0.5, 200, True), lists[r]) for r in range(32))
+            single_loop = (time.perf_counter() - t0) * 1e3 / 32 * (C * T)
+            products_info["single_policy_peak_lists"] = {
+                "rows": C * T, "ms_batched": single_batch, "ms_per_template_loop_scaled": single_loop,
+                "peaks": int(sum(len(x) for x in lists)), "first_32_rows_identical": bool(same)}
+            log(f"detector SINGLE policy, {C * T} rows: {single_batch:.1f} ms batched, "
+                f"{single_loop:.1f} ms as a per-template loop")
             # audit sample: 24 detections, their lags' neighbourhoods and windows
             sample = []
             for k in range(24):

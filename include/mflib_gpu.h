@@ -233,6 +233,12 @@ int mf_detect_reduced(mf_handle *h, int group, int absoluteValue, double thresho
  * entries; *nPeaks the count. */
 int mf_detect_template(mf_handle *h, int it, int absoluteValue, double threshold, int minDist, int *idx, int cap,
                        int *nPeaks);
+/* The same for every template at once (detector.cpp:478-491, the loop over
+ * it): the peaks of template it are idx[offsets[it] .. offsets[it+1]) in
+ * ascending order; offsets holds nTemplates + 1 entries, idx cap.  Rows are
+ * scanned in chunks with one readback each. */
+int mf_detect_templates(mf_handle *h, int absoluteValue, double threshold, int minDist, int *idx, int cap,
+                        int *offsets);
 
 /* Time interpolation kinds (values of MFLib::DetectionTimeInterpolationType). */
 #define MF_INTERP_LANCZOS 0

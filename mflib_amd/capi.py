@@ -80,6 +80,8 @@ _SIGNATURES = {
     "mf_set_streaming": ([_vp, ctypes.c_int], ctypes.c_int),
     "mf_get_stream_carry": ([_vp, _ip], ctypes.c_int),
     "mf_push_chunks_device": ([_vp, ctypes.c_int, ctypes.c_int, ctypes.POINTER(_vp)], ctypes.c_int),
+    "mf_detect_templates": ([_vp, ctypes.c_int, ctypes.c_double, ctypes.c_int, _ip, ctypes.c_int, _ip],
+                            ctypes.c_int),
     "mf_detect_template": ([_vp, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_int, _ip, ctypes.c_int, _ip],
                            ctypes.c_int),
     "mf_amplitude_factors": ([ctypes.c_int, ctypes.c_int, _dp, ctypes.c_int, _vp, ctypes.c_longlong, _dp, _dp,
@@ -424,6 +426,16 @@ class Handle:
         check(lib().mf_detect_template(self._h, int(it), int(absval), float(threshold), int(min_dist),
                                        idx.ctypes.data_as(_ip), cap, ctypes.byref(n)), "detect_template")
         return idx[:n.value].copy()
+
+    def detect_templates(self, threshold: float, min_dist: int, absval: bool = False, cap: int | None = None):
+        """Detector SINGLE policy for every template: a list of peak arrays."""
+        nt = self.number_of_templates()
+        cap = cap or max(1, sum(self.filtered_length(r) // 2 + 1 for r in range(nt)))
+        idx = np.empty(cap, dtype=np.int32)
+        off = np.empty(nt + 1, dtype=np.int32)
+        check(lib().mf_detect_templates(self._h, int(absval), float(threshold), int(min_dist),
+                                        idx.ctypes.data_as(_ip), cap, off.ctypes.data_as(_ip)), "detect_templates")
+        return [idx[off[r]:off[r + 1]].copy() for r in range(nt)]
 
     def detection_products(self, peaks, template_ids, sampling_periods=None, absval: bool = False,
                            interpolation: int = INTERP_QUADRATIC, lanczos_alpha: int = 51,

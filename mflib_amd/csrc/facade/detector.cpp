@@ -8,8 +8,8 @@
 // that holds the filter's correlations:
 //  - REDUCED policy: mf_detect_reduced (reduceAllCorrelations + clamp +
 //    PeakFinder, and the template of every peak);
-//  - SINGLE policy: mf_detect_template per template (setCorrelations +
-//    PeakFinder);
+//  - SINGLE policy: mf_detect_templates, every template's peak list in one
+//    batch (setCorrelations + PeakFinder per template);
 //  - mf_detection_products for all peaks at once: clamped rho, the refined
 //    time shift and the amplitude scaling factors.
 // The host assembles the Detection objects in the reference's order (peak
@@ -273,19 +273,18 @@ void Detector<T>::detect(const MatchedFilter<T> &mf)
     }
     else
     {
-        std::vector<int> all, allT, one(peaks.size());
+        // every template's peak list in one batch; at most nOut/2 + 1 peaks per row
+        std::vector<int> all(static_cast<size_t>(nt) * (detectionLength / 2 + 1) + 1), offsets(nt + 1);
+        MFLib::Facade::check(mf_detect_templates(h, absval, threshold, minSpacing, all.data(),
+                                                 static_cast<int>(all.size()), offsets.data()),
+                             "detect");
+        np = offsets[nt];
+        peaks.assign(all.begin(), all.begin() + np);
+        tids.resize(np);
         for (int it = 0; it < nt; ++it)
         {
-            int n = 0;
-            MFLib::Facade::check(mf_detect_template(h, it, absval, threshold, minSpacing, one.data(),
-                                                    static_cast<int>(one.size()), &n),
-                                 "detect");
-            all.insert(all.end(), one.begin(), one.begin() + n);
-            allT.insert(allT.end(), n, it);
+            std::fill(tids.begin() + offsets[it], tids.begin() + offsets[it + 1], it);
         }
-        peaks.swap(all);
-        tids.swap(allT);
-        np = static_cast<int>(peaks.size());
     }
     if (np < 1) { return; }
 

@@ -108,6 +108,9 @@ struct mf_handle {
     DevBuf dReduceRows, dRedDet, dRedId, dRedPk, dRedPkId;  // Detector reduce step
     DevBuf dProdJobs, dProdOut;                             // per-detection products
     PinnedBuf hProd;                                        // their host staging (jobs, then results)
+    // the Detector's per-template peak lists (rows taken in chunks)
+    DevBuf dDetGroupTab, dDetCounts, dDetCandIdx, dDetCandVal, dDetPackIdx, dDetPackVal;
+    PinnedBuf hDetPack;
     // stacking groups (built on first stack/detect)
     struct StackInfo {
         int first = 0, count = 0, smax = 0, minShift = 0;
@@ -1692,6 +1695,115 @@ int mf_detect_reduced(mf_handle *h, int group, int absoluteValue, double thresho
                                   hipMemcpyDeviceToHost, h->stream));
             MF_HIP(hipStreamSynchronize(h->stream));
             groupRowsToTemplates(h, group, templateIds, np);
+        }
+    });
+}
+
+extern "C++" {
+namespace {
+
+// Detector::detect's SINGLE policy for every template row: clamp(rho) or
+// clamp(|rho|), threshold + local maximum (one launch per chunk of rows),
+// the candidates packed and read back together, then the reference
+// pruning per row on the host.  Rows go in chunks whose candidate buffers
+// (worst case nOut/2 + 2 per row) stay within about 512 MB.
+template <typename T>
+void detectRowsT(mf_handle *h, int absval, double threshold, int minDist, int *idx, int cap, int *offsets)
+{
+    const int nT = static_cast<int>(h->rows.size());
+    long long maxOut = 1;
+    for (const auto &rp : h->rows) { maxOut = std::max<long long>(maxOut, rp.nOut); }
+    const size_t capR = static_cast<size_t>(maxOut / 2 + 2);
+    const size_t perRow = capR * (sizeof(int) + sizeof(T));
+    const int chunk = static_cast<int>(std::max<size_t>(1, std::min<size_t>(64, (size_t(512) << 20) / (2 * perRow))));
+    growTo(h->dDetGroupTab, chunk * sizeof(mfk::StackGroup));
+    growTo(h->dDetCounts, chunk * sizeof(int));
+    growTo(h->dDetCandIdx, chunk * capR * sizeof(int));
+    growTo(h->dDetCandVal, chunk * capR * sizeof(T));
+    growTo(h->dDetPackIdx, (chunk + chunk * capR) * sizeof(int));
+    growTo(h->dDetPackVal, chunk * capR * sizeof(T));
+    constexpr size_t kPrefix = 8192;
+    std::vector<mfk::StackGroup> tab(chunk);
+    int off = 0;
+    for (int r0 = 0; r0 < nT; r0 += chunk) {
+        const int nG = std::min(chunk, nT - r0);
+        long long len = 0;
+        for (int g = 0; g < nG; ++g) {
+            const auto &rp = h->rows[r0 + g];
+            auto &t = tab[g];
+            t.rows = nullptr;
+            t.nRows = 0;
+            t.candCap = static_cast<int>(capR);
+            t.len = rp.nOut;
+            t.out = static_cast<char *>(h->dRho.p) + rp.rhoOff * h->tsize;
+            t.count = h->dDetCounts.as<int>() + g;
+            t.candIdx = h->dDetCandIdx.as<int>() + g * capR;
+            t.candVal = static_cast<char *>(h->dDetCandVal.p) + g * capR * sizeof(T);
+            len = std::max<long long>(len, rp.nOut);
+        }
+        // tab is not touched again before this chunk's wait below
+        MF_HIP(hipMemcpyAsync(h->dDetGroupTab.p, tab.data(), nG * sizeof(mfk::StackGroup), hipMemcpyHostToDevice,
+                              h->stream));
+        MF_HIP(hipMemsetAsync(h->dDetCounts.p, 0, nG * sizeof(int), h->stream));
+        const auto *dTab = h->dDetGroupTab.as<mfk::StackGroup>();
+        MF_HIP(mfk::launch_peak_rows(h->precision, dTab, nG, len, threshold, absval, h->stream));
+        MF_HIP(mfk::launch_pack_candidates(h->precision, dTab, nG, h->dDetPackIdx.as<int>(), h->dDetPackVal.p,
+                                           h->stream));
+        const size_t pre = std::min(kPrefix, capR * static_cast<size_t>(nG));
+        const size_t hdrBytes = (static_cast<size_t>(nG) + pre) * sizeof(int);
+        h->hDetPack.reserve(hdrBytes + pre * sizeof(T));
+        int *hIdx = static_cast<int *>(h->hDetPack.p);
+        T *hVal = reinterpret_cast<T *>(static_cast<char *>(h->hDetPack.p) + hdrBytes);
+        MF_HIP(hipMemcpyAsync(hIdx, h->dDetPackIdx.p, hdrBytes, hipMemcpyDeviceToHost, h->stream));
+        MF_HIP(hipMemcpyAsync(hVal, h->dDetPackVal.p, pre * sizeof(T), hipMemcpyDeviceToHost, h->stream));
+        MF_HIP(hipStreamSynchronize(h->stream));
+        std::vector<size_t> first(nG + 1, 0);
+        for (int g = 0; g < nG; ++g) {
+            if (hIdx[g] > static_cast<int>(capR)) { fail(MF_ERR_DEVICE, "peak candidate buffer overflow"); }
+            first[g + 1] = first[g] + static_cast<size_t>(hIdx[g]);
+        }
+        const size_t total = first[nG];
+        std::vector<int> ci(total);
+        std::vector<T> cv(total);
+        std::copy(hIdx + nG, hIdx + nG + std::min(total, pre), ci.begin());
+        std::copy(hVal, hVal + std::min(total, pre), cv.begin());
+        if (total > pre) {
+            MF_HIP(hipMemcpyAsync(ci.data() + pre, h->dDetPackIdx.as<int>() + nG + pre, (total - pre) * sizeof(int),
+                                  hipMemcpyDeviceToHost, h->stream));
+            MF_HIP(hipMemcpyAsync(cv.data() + pre, static_cast<const T *>(h->dDetPackVal.p) + pre,
+                                  (total - pre) * sizeof(T), hipMemcpyDeviceToHost, h->stream));
+            MF_HIP(hipStreamSynchronize(h->stream));
+        }
+        for (int g = 0; g < nG; ++g) {
+            offsets[r0 + g] = off;
+            const size_t cnt = first[g + 1] - first[g];
+            std::vector<std::pair<T, int>> cand(cnt);
+            for (size_t i = 0; i < cnt; ++i) { cand[i] = {cv[first[g] + i], ci[first[g] + i]}; }
+            int np = 0;
+            pruneCandidates<T>(cand, minDist, idx + off, cap - off, &np);
+            off += np;
+        }
+    }
+    offsets[nT] = off;
+}
+
+}  // namespace
+}  // extern "C++"
+
+int mf_detect_templates(mf_handle *h, int absoluteValue, double threshold, int minDist, int *idx, int cap,
+                        int *offsets)
+{
+    return guard([&] {
+        if (offsets == nullptr || (idx == nullptr && cap > 0)) { fail(MF_ERR_INVALID_ARGUMENT, "output is NULL"); }
+        requireInit(h);
+        if (!h->applied) { fail(MF_ERR_BAD_STATE, "Matched filtered signals not yet computed"); }
+        if (threshold < 0 || threshold > 1) {
+            fail(MF_ERR_INVALID_ARGUMENT, "Threshold = " + std::to_string(threshold) + " must be in the range [0,1]");
+        }
+        if (h->precision == MF_PRECISION_FLOAT) {
+            detectRowsT<float>(h, absoluteValue ? 1 : 0, threshold, minDist, idx, cap, offsets);
+        } else {
+            detectRowsT<double>(h, absoluteValue ? 1 : 0, threshold, minDist, idx, cap, offsets);
         }
     });
 }

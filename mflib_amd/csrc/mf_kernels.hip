@@ -968,9 +968,25 @@ __global__ __launch_bounds__(NT) void reduce_kernel(const void *const *__restric
     }
 }
 
+// How peak_scan reads its signal: as stored, or as the Detector's
+// setCorrelations (src/singleChannel/detector.cpp:252-284) sees a row:
+// clamped to [-1, 1] (NaN -> 1, as std::max(-1, std::min(1, x))), of rho
+// or of |rho|.
+constexpr int kPeakRaw = 0;
+constexpr int kPeakClamp = 1;
+constexpr int kPeakClampAbs = 2;
+template <int XF, typename T>
+__device__ __forceinline__ T peak_value(gptr<const T> x, long long i)
+{
+    T v = x[i];
+    if constexpr (XF == kPeakClampAbs) { v = fabs(v); }
+    if constexpr (XF != kPeakRaw) { v = fmax(T(-1), fmin(T(1), v)); }
+    return v;
+}
+
 // Candidates of src/peakFinder.cpp:500-521: strictly above threshold and
 // a local maximum (plateaus keep their last sample; end-point rules).
-template <typename T>
+template <typename T, int XF = kPeakRaw>
 __device__ __forceinline__ void peak_scan(gptr<const T> x, long long n, T thr, int *count, gptr<int> idx,
                                           gptr<T> val, int cap)
 {
@@ -983,16 +999,16 @@ __device__ __forceinline__ void peak_scan(gptr<const T> x, long long n, T thr, i
         bool cand = false;
         T xi = T(0);
         if (i < n) {
-            xi = x[i];
+            xi = peak_value<XF>(x, i);
             if (xi > thr) {
                 if (n == 1) {
                     cand = true;
                 } else if (i == 0) {
-                    cand = x[1] < xi;
+                    cand = peak_value<XF>(x, 1) < xi;
                 } else if (i == n - 1) {
-                    cand = xi > x[n - 2];
+                    cand = xi > peak_value<XF>(x, n - 2);
                 } else {
-                    cand = (xi >= x[i - 1]) && (x[i + 1] < xi);
+                    cand = (xi >= peak_value<XF>(x, i - 1)) && (peak_value<XF>(x, i + 1) < xi);
                 }
             }
         }
@@ -1030,13 +1046,14 @@ __global__ __launch_bounds__(NT) void peak_kernel(const T *__restrict__ x, long 
     peak_scan<T>(to_global(x), n, thr, count, to_global(idx), to_global(val), cap);
 }
 
-// The candidates of every group's stack (grid.y = group).
-template <typename T>
+// The candidates of every group's stack (grid.y = group); with XF, of
+// clamped rows (the Detector's per-template policy).
+template <typename T, int XF = kPeakRaw>
 __global__ __launch_bounds__(NT) void peak_groups_kernel(const StackGroup *__restrict__ groups, T thr)
 {
     const StackGroup g = groups[blockIdx.y];
-    peak_scan<T>(to_global(static_cast<const T *>(g.out)), g.len, thr, g.count, to_global(g.candIdx),
-                 to_global(static_cast<T *>(g.candVal)), g.candCap);
+    peak_scan<T, XF>(to_global(static_cast<const T *>(g.out)), g.len, thr, g.count, to_global(g.candIdx),
+                     to_global(static_cast<T *>(g.candVal)), g.candCap);
 }
 
 // Packs every group's candidates into one array, group by group in order
@@ -1289,6 +1306,33 @@ hipError_t launch_peak_groups(int precision, const StackGroup *dGroups, int nGro
     } else {
         hipLaunchKernelGGL(peak_groups_kernel<double>, dim3(grid, nGroups), dim3(NT), 0, stream, dGroups,
                            threshold);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_peak_rows(int precision, const StackGroup *dGroups, int nGroups, long long maxLen,
+                            double threshold, int absval, hipStream_t stream)
+{
+    if (maxLen <= 0 || nGroups <= 0) { return hipSuccess; }
+    if (nGroups > 65535) { return hipErrorInvalidValue; }
+    const long long want = (maxLen + NT - 1) / NT;
+    const int grid = static_cast<int>(want < 8192 ? want : 8192);
+    if (precision == 0) {
+        if (absval) {
+            hipLaunchKernelGGL((peak_groups_kernel<float, kPeakClampAbs>), dim3(grid, nGroups), dim3(NT), 0, stream,
+                               dGroups, static_cast<float>(threshold));
+        } else {
+            hipLaunchKernelGGL((peak_groups_kernel<float, kPeakClamp>), dim3(grid, nGroups), dim3(NT), 0, stream,
+                               dGroups, static_cast<float>(threshold));
+        }
+    } else {
+        if (absval) {
+            hipLaunchKernelGGL((peak_groups_kernel<double, kPeakClampAbs>), dim3(grid, nGroups), dim3(NT), 0,
+                               stream, dGroups, threshold);
+        } else {
+            hipLaunchKernelGGL((peak_groups_kernel<double, kPeakClamp>), dim3(grid, nGroups), dim3(NT), 0, stream,
+                               dGroups, threshold);
+        }
     }
     return hipGetLastError();
 }

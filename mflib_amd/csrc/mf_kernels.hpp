@@ -128,6 +128,10 @@ hipError_t launch_stack_groups(int precision, const StackGroup *dGroups, int nGr
                                int absval, hipStream_t stream);
 hipError_t launch_peak_groups(int precision, const StackGroup *dGroups, int nGroups, long long maxLen,
                               double threshold, hipStream_t stream);
+// The same over template rows as the Detector's per-template policy sees
+// them: clamp(rho) or clamp(|rho|) to [-1, 1] (group.out = the row).
+hipError_t launch_peak_rows(int precision, const StackGroup *dGroups, int nGroups, long long maxLen,
+                            double threshold, int absval, hipStream_t stream);
 // Detector reduce + clamp over nRows device rows (pointer table on the
 // device): det[k] = clamp(max_r rho_r[k] (or |rho_r[k]|), -1, 1), id[k] = the
 // first r reaching the maximum; k < len.

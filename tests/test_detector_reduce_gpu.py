@@ -73,3 +73,23 @@ def test_reduce_batched_groups_report_template_indices():
     with pytest.raises(ValueError):
         h.reduce(2)
     h.close()
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+@pytest.mark.parametrize("absval", [False, True])
+def test_detect_templates_batch_matches_per_template_and_oracle(dtype, absval):
+    """mf_detect_templates (every template's SINGLE-policy peak list in one
+    batch) equals mf_detect_template row by row, and the oracle PeakFinder on
+    the clamped rows (setCorrelations, src/singleChannel/detector.cpp:252-284)."""
+    h, nt = _single_channel(dtype, 61 + absval)
+    lists = h.detect_templates(0.25, 30, absval=absval)
+    assert len(lists) == nt
+    for it in range(nt):
+        one = h.detect_template(it, 0.25, 30, absval=absval)
+        assert np.array_equal(lists[it], one), it
+        row = h.rho(it)
+        det, _ = O.reduce_correlations(row[None, :], absval)   # one row: clamp(rho) or clamp(|rho|)
+        assert np.array_equal(lists[it], O.peak_finder(det, 0.25, 30)), it
+    assert sum(len(x) for x in lists) > 0
+    with pytest.raises(ValueError):
+        h.detect_templates(1.5, 0)
