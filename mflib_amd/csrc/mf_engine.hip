@@ -1699,23 +1699,41 @@ int mf_detect_reduced(mf_handle *h, int group, int absoluteValue, double thresho
     });
 }
 
+namespace {
+
+// Detector::detect's SINGLE policy for template row it: the reduce kernel
+// over the one row (clamp), then the standalone peak finder.
+void detectOneRow(mf_handle *h, int it, int absval, double threshold, int minDist, int *idx, int cap, int *nPeaks)
+{
+    const long long need = h->rows[it].nOut;
+    growTo(h->dRedDet, static_cast<size_t>(std::max(need, 1LL)) * h->tsize);
+    growTo(h->dRedId, static_cast<size_t>(std::max(need, 1LL)) * sizeof(int));
+    const long long n = reduceRows(h, std::vector<int>{it}, need, absval, h->dRedDet.p,
+                                   static_cast<int *>(h->dRedId.p), need);
+    MF_HIP(hipStreamSynchronize(h->stream));
+    peaks(h->dRedDet.p, true, static_cast<int>(n), h->precision, threshold, minDist, idx, cap, nPeaks);
+}
+
+}  // namespace
+
 extern "C++" {
 namespace {
 
 // Detector::detect's SINGLE policy for every template row: clamp(rho) or
 // clamp(|rho|), threshold + local maximum (one launch per chunk of rows),
 // the candidates packed and read back together, then the reference
-// pruning per row on the host.  Rows go in chunks whose candidate buffers
-// (worst case nOut/2 + 2 per row) stay within about 512 MB.
+// pruning per row on the host.  Each row gets room for kRowCands
+// candidates (rows are usually far below); a row that overflows is redone
+// alone with the full nOut/2 + 2.  Rows go in chunks of up to 256.
 template <typename T>
 void detectRowsT(mf_handle *h, int absval, double threshold, int minDist, int *idx, int cap, int *offsets)
 {
     const int nT = static_cast<int>(h->rows.size());
     long long maxOut = 1;
     for (const auto &rp : h->rows) { maxOut = std::max<long long>(maxOut, rp.nOut); }
-    const size_t capR = static_cast<size_t>(maxOut / 2 + 2);
-    const size_t perRow = capR * (sizeof(int) + sizeof(T));
-    const int chunk = static_cast<int>(std::max<size_t>(1, std::min<size_t>(64, (size_t(512) << 20) / (2 * perRow))));
+    constexpr size_t kRowCands = 65536;
+    const size_t capR = std::min(kRowCands, static_cast<size_t>(maxOut / 2 + 2));
+    const int chunk = std::min(nT, 256);
     growTo(h->dDetGroupTab, chunk * sizeof(mfk::StackGroup));
     growTo(h->dDetCounts, chunk * sizeof(int));
     growTo(h->dDetCandIdx, chunk * capR * sizeof(int));
@@ -1758,9 +1776,11 @@ void detectRowsT(mf_handle *h, int absval, double threshold, int minDist, int *i
         MF_HIP(hipMemcpyAsync(hVal, h->dDetPackVal.p, pre * sizeof(T), hipMemcpyDeviceToHost, h->stream));
         MF_HIP(hipStreamSynchronize(h->stream));
         std::vector<size_t> first(nG + 1, 0);
+        std::vector<char> overflow(nG, 0);
         for (int g = 0; g < nG; ++g) {
-            if (hIdx[g] > static_cast<int>(capR)) { fail(MF_ERR_DEVICE, "peak candidate buffer overflow"); }
-            first[g + 1] = first[g] + static_cast<size_t>(hIdx[g]);
+            overflow[g] = hIdx[g] > static_cast<int>(capR);
+            // an overflowed row packed its first capR candidates
+            first[g + 1] = first[g] + std::min(static_cast<size_t>(hIdx[g]), capR);
         }
         const size_t total = first[nG];
         std::vector<int> ci(total);
@@ -1776,6 +1796,12 @@ void detectRowsT(mf_handle *h, int absval, double threshold, int minDist, int *i
         }
         for (int g = 0; g < nG; ++g) {
             offsets[r0 + g] = off;
+            if (overflow[g]) {
+                int np = 0;
+                detectOneRow(h, r0 + g, absval, threshold, minDist, idx + off, cap - off, &np);
+                off += np;
+                continue;
+            }
             const size_t cnt = first[g + 1] - first[g];
             std::vector<std::pair<T, int>> cand(cnt);
             for (size_t i = 0; i < cnt; ++i) { cand[i] = {cv[first[g] + i], ci[first[g] + i]}; }
@@ -1816,13 +1842,7 @@ int mf_detect_template(mf_handle *h, int it, int absoluteValue, double threshold
         requireInit(h);
         if (!h->applied) { fail(MF_ERR_BAD_STATE, "Matched filtered signals not yet computed"); }
         checkRow(h, it);
-        const long long need = h->rows[it].nOut;
-        growTo(h->dRedDet, static_cast<size_t>(std::max(need, 1LL)) * h->tsize);
-        growTo(h->dRedId, static_cast<size_t>(std::max(need, 1LL)) * sizeof(int));
-        const long long n = reduceRows(h, std::vector<int>{it}, need, absoluteValue, h->dRedDet.p,
-                                       static_cast<int *>(h->dRedId.p), need);
-        MF_HIP(hipStreamSynchronize(h->stream));
-        peaks(h->dRedDet.p, true, static_cast<int>(n), h->precision, threshold, minDist, idx, cap, nPeaks);
+        detectOneRow(h, it, absoluteValue, threshold, minDist, idx, cap, nPeaks);
     });
 }
 

@@ -93,3 +93,29 @@ def test_detect_templates_batch_matches_per_template_and_oracle(dtype, absval):
     assert sum(len(x) for x in lists) > 0
     with pytest.raises(ValueError):
         h.detect_templates(1.5, 0)
+
+
+def test_detect_templates_rows_beyond_the_candidate_room():
+    """Threshold 0 on 400,000-lag rows of noise: ~130,000 candidates per row,
+    more than the batch's per-row room (65,536), so every row is redone alone
+    with full capacity; the lists still equal the per-template path and the
+    oracle."""
+    rng = np.random.default_rng(67)
+    n = 400_000
+    x = rng.standard_normal(n).astype(np.float32)
+    tpls = [rng.standard_normal(L) for L in (64, 80)]
+    h = capi.Handle(np.float32, "single")
+    h.set_templates(tpls)
+    h.set_signal_size(n)
+    h.initialize()
+    h.set_signal(0, x)
+    h.apply()
+    lists = h.detect_templates(0.0, 0)
+    for it in range(len(tpls)):
+        row = h.rho(it)
+        det, _ = O.reduce_correlations(row[None, :], False)
+        ref = O.peak_finder(det, 0.0, 0)
+        assert len(ref) > 65536
+        assert np.array_equal(lists[it], ref), it
+        assert np.array_equal(lists[it], h.detect_template(it, 0.0, 0)), it
+    h.close()
