@@ -112,6 +112,14 @@ int mf_get_stream_carry(const mf_handle *h, int *carry);
  * from device memory (dx[c] holds channel c's chunk of n samples), waiting
  * once for all the copies instead of once per channel. */
 int mf_push_chunks_device(mf_handle *h, int nChannels, int n, const void *const *dx);
+/* No reference counterpart (checkpoint / resume of a stream): the carry
+ * samples of a channel that its next push continues from (*len = carry;
+ * a size query when out is NULL), and the restore of such a state into a
+ * streaming handle of the same templates and chunk length.  A handle
+ * restored from a snapshot gives the next pushes' rows and stacks
+ * bit-identical to the handle the snapshot came from. */
+int mf_get_stream_state(mf_handle *h, int channel, void *out, int cap, int *len);
+int mf_set_stream_state(mf_handle *h, int channel, const void *state, int n);
 
 /* MatchedFilter<T>::initialize (src/matchedFilter.cpp:506-821): template
  * demean/normalize (FP64), plans, template spectra, device buffers. */

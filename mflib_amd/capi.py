@@ -79,6 +79,8 @@ _SIGNATURES = {
     "mf_get_device": ([_vp, _ip], ctypes.c_int),
     "mf_set_streaming": ([_vp, ctypes.c_int], ctypes.c_int),
     "mf_get_stream_carry": ([_vp, _ip], ctypes.c_int),
+    "mf_get_stream_state": ([_vp, ctypes.c_int, _vp, ctypes.c_int, _ip], ctypes.c_int),
+    "mf_set_stream_state": ([_vp, ctypes.c_int, _vp, ctypes.c_int], ctypes.c_int),
     "mf_push_chunks_device": ([_vp, ctypes.c_int, ctypes.c_int, ctypes.POINTER(_vp)], ctypes.c_int),
     "mf_detect_templates": ([_vp, ctypes.c_int, ctypes.c_double, ctypes.c_int, _ip, ctypes.c_int, _ip],
                             ctypes.c_int),
@@ -283,6 +285,18 @@ class Handle:
 
     def stream_carry(self) -> int:
         return self._int(lib().mf_get_stream_carry)
+
+    def stream_state(self, channel: int) -> np.ndarray:
+        """The carry samples channel `channel` continues from (a checkpoint)."""
+        out = np.empty(max(self.stream_carry(), 1), dtype=self.dtype)
+        ln = ctypes.c_int()
+        check(lib().mf_get_stream_state(self._h, int(channel), out.ctypes.data_as(_vp), len(out), ctypes.byref(ln)),
+              "stream_state")
+        return out[:ln.value].copy()
+
+    def set_stream_state(self, channel: int, state):
+        st = np.ascontiguousarray(state, dtype=self.dtype)
+        check(lib().mf_set_stream_state(self._h, int(channel), st.ctypes.data_as(_vp), len(st)), "set_stream_state")
 
     def push_chunks_device(self, ptrs, n: int):
         """Streaming push of every channel from device pointers ptrs[c]."""

@@ -1388,6 +1388,43 @@ int mf_set_streaming(mf_handle *h, int on)
     });
 }
 
+int mf_get_stream_state(mf_handle *h, int channel, void *out, int cap, int *len)
+{
+    return guard([&] {
+        requireInit(h);
+        if (!h->streaming) { fail(MF_ERR_BAD_STATE, "stream state needs streaming mode"); }
+        checkChannel(h, channel);
+        if (len) { *len = static_cast<int>(h->carry); }
+        if (h->carry == 0) { return; }
+        if (out == nullptr || cap < h->carry) { fail(MF_ERR_INVALID_ARGUMENT, "output holds fewer than carry samples"); }
+        // the samples the next push moves to the front: the buffer's tail
+        const auto &c = h->ch[channel];
+        MF_HIP(hipMemcpyAsync(out, static_cast<const char *>(c.buf.p) + static_cast<size_t>(c.n - h->carry) * h->tsize,
+                              static_cast<size_t>(h->carry) * h->tsize, hipMemcpyDeviceToHost, h->stream));
+        MF_HIP(hipStreamSynchronize(h->stream));
+    });
+}
+
+int mf_set_stream_state(mf_handle *h, int channel, const void *state, int n)
+{
+    return guard([&] {
+        requireInit(h);
+        if (!h->streaming) { fail(MF_ERR_BAD_STATE, "stream state needs streaming mode"); }
+        checkChannel(h, channel);
+        if (n != h->carry) {
+            fail(MF_ERR_INVALID_ARGUMENT, "state holds " + std::to_string(n) + " samples, the carry is " +
+                                              std::to_string(h->carry));
+        }
+        if (n == 0) { return; }
+        if (state == nullptr) { fail(MF_ERR_INVALID_ARGUMENT, "state is NULL"); }
+        auto &c = h->ch[channel];
+        MF_HIP(hipMemcpyAsync(static_cast<char *>(c.buf.p) + static_cast<size_t>(c.n - h->carry) * h->tsize, state,
+                              static_cast<size_t>(n) * h->tsize, hipMemcpyHostToDevice, h->stream));
+        MF_HIP(hipStreamSynchronize(h->stream));
+        h->applied = false;
+    });
+}
+
 int mf_get_stream_carry(const mf_handle *h, int *carry)
 {
     return guard([&] {
