@@ -273,11 +273,20 @@ void Detector<T>::detect(const MatchedFilter<T> &mf)
     }
     else
     {
-        // every template's peak list in one batch; at most nOut/2 + 1 peaks per row
-        std::vector<int> all(static_cast<size_t>(nt) * (detectionLength / 2 + 1) + 1), offsets(nt + 1);
-        MFLib::Facade::check(mf_detect_templates(h, absval, threshold, minSpacing, all.data(),
-                                                 static_cast<int>(all.size()), offsets.data()),
-                             "detect");
+        // every template's peak list in one batch.  Room for 4 M peaks first;
+        // only a pathological threshold needs the worst case (nOut/2 + 1 per
+        // row), and then the batch runs again with that room.
+        const long long worst = static_cast<long long>(nt) * (detectionLength / 2 + 1) + 1;
+        std::vector<int> all(static_cast<size_t>(std::min<long long>(worst, 1LL << 22))), offsets(nt + 1);
+        int rc = mf_detect_templates(h, absval, threshold, minSpacing, all.data(), static_cast<int>(all.size()),
+                                     offsets.data());
+        if (rc == MF_ERR_INVALID_ARGUMENT && static_cast<long long>(all.size()) < worst)
+        {
+            all.assign(static_cast<size_t>(std::min<long long>(worst, 0x7fffffffLL)), 0);
+            rc = mf_detect_templates(h, absval, threshold, minSpacing, all.data(), static_cast<int>(all.size()),
+                                     offsets.data());
+        }
+        MFLib::Facade::check(rc, "detect");
         np = offsets[nt];
         peaks.assign(all.begin(), all.begin() + np);
         tids.resize(np);
