@@ -638,8 +638,8 @@ def main() -> int:
                     help="only copy BYTES device-to-device 3 times (calibrates rocprof HBM counters)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); the "
-              "multi-GPU sharding of the path is described in DESIGN.md", file=sys.stderr)
+        print(f"bench.py: runs on one GPU only (asked for --gpus {args.gpus}); sharding the path across GPUs "
+              "(by template rows or time blocks) is not implemented", file=sys.stderr)
         return 2
     if args.copy_probe:
         import torch

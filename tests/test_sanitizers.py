@@ -1,5 +1,4 @@
-"""Host-side sanitizer run of the oracle (SURVEY's race-detection /
-sanitizer item): tests/cpp/oracle_sanitize.cpp built with g++
+"""Host-side sanitizer run of the oracle: tests/cpp/oracle_sanitize.cpp built with g++
 -fsanitize=address,undefined and run; any sanitizer finding aborts it.
 CPU only: the oracle is the test suite's checker, so its memory safety
 underwrites every parity claim."""
