@@ -231,3 +231,23 @@ def test_pymflib_relative_magnitude():
     with pytest.raises(ValueError):
         mag.set_detected_signal(np.full(400, 1.5))    # constant
     assert not mag.have_detected_signal()
+
+
+@pytest.mark.parametrize("points,alpha", [(1, 51), (2, 51), (3, 1), (7, 2)])
+def test_lanczos_degenerate_grids_match_oracle(points, alpha):
+    """Grids of 0, 1, 2 and 6 shifts (points - 1), and windows of one or two
+    lobes: the GPU's pick equals the oracle's (a single grid point, or none
+    at all, leaves nothing to tie)."""
+    h, sigs, tpls, chan = _single_channel(np.float64, 13)
+    nout = h.filtered_length(0)
+    lags = np.array([0, 1, 2, nout // 3, nout - 2, nout - 1], dtype=np.int32)
+    tids = np.array([0, 1, 2, 3, 4, 0], dtype=np.int32)
+    got = h.detection_products(lags, tids, None, absval=False, interpolation=capi.INTERP_LANCZOS,
+                               lanczos_alpha=alpha, lanczos_points=points)
+    for k, (p, it) in enumerate(zip(lags, tids)):
+        ref, prof = O.lanczos_shift(h.rho(int(it)).astype(np.float64), int(p), 1.0, False, alpha, points)
+        if points <= 2:
+            assert got["shift"][k] == ref == 0.0
+        elif np.sort(prof)[-2] < prof.max() - 1e-9 * max(1.0, abs(prof.max())):
+            assert got["shift"][k] == ref, (k, p)
+    h.close()
