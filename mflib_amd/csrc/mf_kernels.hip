@@ -972,6 +972,7 @@ __global__ __launch_bounds__(NT) void reduce_kernel(const void *const *__restric
 // setCorrelations (src/singleChannel/detector.cpp:252-284) sees a row:
 // clamped to [-1, 1] (NaN -> 1, as std::max(-1, std::min(1, x))), of rho
 // or of |rho|.
+constexpr int kPeakPerThread = 8;  // grid-stride iterations a launch aims for
 constexpr int kPeakRaw = 0;
 constexpr int kPeakClamp = 1;
 constexpr int kPeakClampAbs = 2;
@@ -1298,7 +1299,8 @@ hipError_t launch_peak_groups(int precision, const StackGroup *dGroups, int nGro
                               double threshold, hipStream_t stream)
 {
     if (maxLen <= 0 || nGroups <= 0) { return hipSuccess; }
-    const long long want = (maxLen + NT - 1) / NT;
+    // about kPeakPerThread samples per thread: far fewer workgroups to dispatch
+    const long long want = (maxLen + NT * kPeakPerThread - 1) / (NT * kPeakPerThread);
     const int grid = static_cast<int>(want < 8192 ? want : 8192);
     if (precision == 0) {
         hipLaunchKernelGGL(peak_groups_kernel<float>, dim3(grid, nGroups), dim3(NT), 0, stream, dGroups,
@@ -1315,7 +1317,8 @@ hipError_t launch_peak_rows(int precision, const StackGroup *dGroups, int nGroup
 {
     if (maxLen <= 0 || nGroups <= 0) { return hipSuccess; }
     if (nGroups > 65535) { return hipErrorInvalidValue; }
-    const long long want = (maxLen + NT - 1) / NT;
+    // about kPeakPerThread samples per thread: far fewer workgroups to dispatch
+    const long long want = (maxLen + NT * kPeakPerThread - 1) / (NT * kPeakPerThread);
     const int grid = static_cast<int>(want < 8192 ? want : 8192);
     if (precision == 0) {
         if (absval) {
@@ -1358,7 +1361,7 @@ hipError_t launch_peaks(int precision, const void *dx, long long n, double thres
                         int *dIdx, void *dVal, int cap, hipStream_t stream)
 {
     if (n <= 0) { return hipSuccess; }
-    const long long want = (n + NT - 1) / NT;
+    const long long want = (n + NT * kPeakPerThread - 1) / (NT * kPeakPerThread);
     const int grid = static_cast<int>(want < 65536 ? want : 65536);
     if (precision == 0) {
         hipLaunchKernelGGL(peak_kernel<float>, dim3(grid), dim3(NT), 0, stream,
