@@ -120,6 +120,25 @@ int mf_push_chunks_device(mf_handle *h, int nChannels, int n, const void *const 
  * bit-identical to the handle the snapshot came from. */
 int mf_get_stream_state(mf_handle *h, int channel, void *out, int cap, int *len);
 int mf_set_stream_state(mf_handle *h, int channel, const void *state, int n);
+/* No reference counterpart (PeakFinder over a stream, the per-push form of
+ * PeakFinder<T>::apply, src/peakFinder.cpp:468-586, on group `group`'s
+ * stacks): call once after every apply of a streaming handle, with the same
+ * threshold and minDist each time.  It returns in peaks[0..*nPeaks) the
+ * stream lags that became final peaks with this push, in increasing order
+ * across calls; with last != 0 the push ends the signal (its last lag is
+ * judged by the end rule) and every remaining peak is returned, and the
+ * next call starts a new stream.  Over a whole stream the returned lags are
+ * exactly PeakFinder's list of the whole signal's stack.  Lag K of a
+ * stream is the stack lag whose windows start K samples after the first
+ * sample of the stream's first push; earlier lags (windows reaching into
+ * the carry that push found) are not part of the stream.  A peak is final
+ * once no later sample can change it, so it may be returned a few pushes
+ * after its own.  When cap is too small the call fails with *nPeaks set and
+ * the state unchanged (call again with room).  Needs one chunk length on
+ * every channel.  The pending candidates are host state of the handle, not
+ * part of mf_get_stream_state. */
+int mf_stream_peaks(mf_handle *h, int group, double threshold, int minDist, int last, long long *peaks, int cap,
+                    int *nPeaks);
 
 /* MatchedFilter<T>::initialize (src/matchedFilter.cpp:506-821): template
  * demean/normalize (FP64), plans, template spectra, device buffers. */

@@ -82,6 +82,8 @@ _SIGNATURES = {
     "mf_get_stream_state": ([_vp, ctypes.c_int, _vp, ctypes.c_int, _ip], ctypes.c_int),
     "mf_set_stream_state": ([_vp, ctypes.c_int, _vp, ctypes.c_int], ctypes.c_int),
     "mf_push_chunks_device": ([_vp, ctypes.c_int, ctypes.c_int, ctypes.POINTER(_vp)], ctypes.c_int),
+    "mf_stream_peaks": ([_vp, ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_int,
+                         ctypes.POINTER(ctypes.c_longlong), ctypes.c_int, _ip], ctypes.c_int),
     "mf_detect_templates": ([_vp, ctypes.c_int, ctypes.c_double, ctypes.c_int, _ip, ctypes.c_int, _ip],
                             ctypes.c_int),
     "mf_detect_template": ([_vp, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_int, _ip, ctypes.c_int, _ip],
@@ -302,6 +304,22 @@ class Handle:
         """Streaming push of every channel from device pointers ptrs[c]."""
         arr = (_vp * len(ptrs))(*[_vp(int(p)) for p in ptrs])
         check(lib().mf_push_chunks_device(self._h, len(ptrs), int(n), arr), "push_chunks_device")
+
+    def stream_peaks(self, group: int, threshold: float, min_dist: int, last: bool = False):
+        """PeakFinder over the stream of group's stacks: the lags that became
+        final peaks with the latest push (int64, increasing across calls);
+        last=True ends the stream.  Call once after every apply."""
+        cap = 1024
+        while True:
+            out = np.empty(cap, dtype=np.int64)
+            n = ctypes.c_int()
+            rc = lib().mf_stream_peaks(self._h, int(group), float(threshold), int(min_dist), 1 if last else 0,
+                                       out.ctypes.data_as(ctypes.POINTER(ctypes.c_longlong)), cap, ctypes.byref(n))
+            if rc == MF_ERR_INVALID_ARGUMENT and n.value > cap:
+                cap = n.value
+                continue
+            check(rc, "stream_peaks")
+            return out[:n.value].copy()
 
     def initialize(self):
         check(lib().mf_initialize(self._h), "initialize")

@@ -17,6 +17,7 @@
 #include "mflib_gpu.h"
 #include "mf_host.hpp"
 #include "mf_kernels.hpp"
+#include "stream_peaks.hpp"
 
 #include <algorithm>
 #include <cmath>
@@ -47,6 +48,17 @@ struct mf_handle {
     // newest chunk, so consecutive chunks give continuous rows and stacks
     bool streaming = false;
     long long carry = 0;
+    // streaming PeakFinder (mf_stream_peaks): one state per stacking group
+    long long applies = 0;  // applies since initialize
+    struct StreamPeakState {
+        bool active = false;
+        long long origin = 0, lastApply = 0;  // apply count of the stream's first and latest push
+        double threshold = 0;
+        int minDist = 0;
+        std::unique_ptr<mfstream::StreamPeaks<float>> f;
+        std::unique_ptr<mfstream::StreamPeaks<double>> d;
+    };
+    std::vector<StreamPeakState> streamPeaks;
     DevBuf dCarry;
     DevBuf dDeadPart;  // streaming: per-chunk min/max of the dead-trace pass on the device
     hipGraph_t graph = nullptr;
@@ -197,6 +209,8 @@ void dropDevice(mf_handle *h)
     h->dCarry.release();
     h->dDeadPart.release();
     h->carry = 0;
+    h->applies = 0;
+    h->streamPeaks.clear();
     h->tw.clear();
     h->launches.clear();
     h->segHost.clear();
@@ -658,6 +672,7 @@ void apply(mf_handle *h)
     }
     MF_HIP(hipEventRecord(h->evB, h->stream));
     h->applied = true;
+    ++h->applies;
 }
 
 // Stack weights exactly as shiftAndStack forms them
@@ -853,26 +868,11 @@ AmplitudeWork &amplitudeWork()
     return *w;
 }
 
-// Copy `count` candidates to the host and apply the reference pruning.
+// Threshold + local-maximum candidates of a device signal (value, index;
+// unordered), read back to the host.
 template <typename T>
-void collectCandidates(const int *dIdx, const void *dVal, int count, int minDist, int *idx, int cap,
-                       int *nPeaks, hipStream_t stream)
-{
-    std::vector<int> ci(count);
-    std::vector<T> cv(count);
-    if (count > 0) {
-        MF_HIP(hipMemcpyAsync(ci.data(), dIdx, count * sizeof(int), hipMemcpyDeviceToHost, stream));
-        MF_HIP(hipMemcpyAsync(cv.data(), dVal, count * sizeof(T), hipMemcpyDeviceToHost, stream));
-        MF_HIP(hipStreamSynchronize(stream));
-    }
-    std::vector<std::pair<T, int>> cand(count);
-    for (int i = 0; i < count; ++i) { cand[i] = {cv[i], ci[i]}; }
-    pruneCandidates<T>(cand, minDist, idx, cap, nPeaks);
-}
-
-template <typename T>
-void peaksDevice(const void *dx, long long n, double threshold, int minDist, int *idx, int cap, int *nPeaks,
-                 hipStream_t stream)
+void rawCandidates(const void *dx, long long n, double threshold, std::vector<std::pair<T, int>> &cand,
+                   hipStream_t stream)
 {
     PeakWork &w = peakWork();
     const int candCap = static_cast<int>(std::min<long long>(n / 2 + 2, 0x7fffffffLL));
@@ -897,12 +897,25 @@ void peaksDevice(const void *dx, long long n, double threshold, int minDist, int
     MF_HIP(hipStreamSynchronize(stream));
     const int count = *hCount;
     if (count > candCap) { fail(MF_ERR_DEVICE, "peak candidate buffer overflow"); }
-    if (count > pre) {
-        collectCandidates<T>(w.idx.as<int>(), w.val.p, count, minDist, idx, cap, nPeaks, stream);
+    cand.resize(count);
+    if (count <= pre) {
+        for (int i = 0; i < count; ++i) { cand[i] = {hVal[i], hIdx[i]}; }
         return;
     }
-    std::vector<std::pair<T, int>> cand(count);
-    for (int i = 0; i < count; ++i) { cand[i] = {hVal[i], hIdx[i]}; }
+    std::vector<int> ci(count);
+    std::vector<T> cv(count);
+    MF_HIP(hipMemcpyAsync(ci.data(), w.idx.p, count * sizeof(int), hipMemcpyDeviceToHost, stream));
+    MF_HIP(hipMemcpyAsync(cv.data(), w.val.p, count * sizeof(T), hipMemcpyDeviceToHost, stream));
+    MF_HIP(hipStreamSynchronize(stream));
+    for (int i = 0; i < count; ++i) { cand[i] = {cv[i], ci[i]}; }
+}
+
+template <typename T>
+void peaksDevice(const void *dx, long long n, double threshold, int minDist, int *idx, int cap, int *nPeaks,
+                 hipStream_t stream)
+{
+    std::vector<std::pair<T, int>> cand;
+    rawCandidates<T>(dx, n, threshold, cand, stream);
     pruneCandidates<T>(cand, minDist, idx, cap, nPeaks);
 }
 
@@ -1422,6 +1435,109 @@ int mf_set_stream_state(mf_handle *h, int channel, const void *state, int n)
                               static_cast<size_t>(n) * h->tsize, hipMemcpyHostToDevice, h->stream));
         MF_HIP(hipStreamSynchronize(h->stream));
         h->applied = false;
+    });
+}
+
+extern "C++" {
+namespace {
+
+// One push of the streaming PeakFinder over group g's stack.  The stream's
+// lag K is the stack lag whose windows start K samples after the first
+// sample of the stream's first push: push p's new stack entries (the last
+// `chunk` of the stack) are K = p * chunk + (len - chunk) - carry + k, and
+// lags before K = 0 (windows reaching into the carry the first push found)
+// are not part of the stream.  Works on a copy of the state; the caller
+// keeps it only when the peaks fit the output.
+template <typename T>
+void streamPeaksPush(mf_handle *h, int group, mfstream::StreamPeaks<T> &sp, long long p, bool last,
+                     std::vector<long long> &out)
+{
+    const long long chunk = h->ch[0].n - h->carry;
+    int lenInt = 0;
+    stack(h, group, nullptr, std::numeric_limits<int>::max(), &lenInt);  // waits for the stack
+    const long long len = lenInt;
+    const long long newStart = len - chunk;
+    if (newStart < 0) { fail(MF_ERR_DEVICE, "stack shorter than the chunk"); }
+    const long long base = p * chunk + newStart - h->carry;
+    const long long v0 = std::min(chunk, std::max(0LL, -base));
+    const long long n = chunk - v0;
+    if (n > 0) {
+        const char *dx = static_cast<const char *>(stackPtr(h, group)) + static_cast<size_t>(newStart + v0) * h->tsize;
+        std::vector<std::pair<T, int>> cand;
+        T edge[4];
+        {
+            std::lock_guard<std::mutex> lock(g_peakMutex);
+            PeakWork &w = peakWork();
+            if (w.stream == nullptr) { MF_HIP(hipStreamCreate(&w.stream)); }
+            rawCandidates<T>(dx, n, static_cast<T>(sp.threshold()), cand, w.stream);
+            const long long at[4] = {0, std::min(1LL, n - 1), std::max(n - 2, 0LL), n - 1};
+            for (int i = 0; i < 4; ++i) {
+                MF_HIP(hipMemcpyAsync(&edge[i], dx + at[i] * sizeof(T), sizeof(T), hipMemcpyDeviceToHost, w.stream));
+            }
+            MF_HIP(hipStreamSynchronize(w.stream));
+        }
+        sp.push(base + v0, static_cast<int>(n), std::move(cand), edge[0], edge[1], edge[2], edge[3], out);
+    }
+    if (last) { sp.flush(out); }
+}
+
+}  // namespace
+}  // extern "C++"
+
+int mf_stream_peaks(mf_handle *h, int group, double threshold, int minDist, int last, long long *peaks, int cap,
+                    int *nPeaks)
+{
+    return guard([&] {
+        checkStackable(h, group);
+        if (!h->streaming) { fail(MF_ERR_BAD_STATE, "mf_stream_peaks needs streaming mode"); }
+        if (group < 0) { fail(MF_ERR_INVALID_ARGUMENT, "group must be non-negative"); }
+        if (nPeaks == nullptr || (peaks == nullptr && cap > 0)) { fail(MF_ERR_INVALID_ARGUMENT, "peaks is NULL"); }
+        if (!(threshold >= 0 && threshold <= 1)) {
+            fail(MF_ERR_INVALID_ARGUMENT, "Threshold = " + std::to_string(threshold) + " must be in the range [0,1]");
+        }
+        for (const auto &c : h->ch) {
+            if (c.n != h->ch[0].n) { fail(MF_ERR_UNSUPPORTED, "stream peaks need one chunk length on every channel"); }
+        }
+        if (h->streamPeaks.size() != h->groups.size()) { h->streamPeaks.resize(h->groups.size()); }
+        auto &st = h->streamPeaks[group];
+        if (st.active) {
+            if (threshold != st.threshold || minDist != st.minDist) {
+                fail(MF_ERR_INVALID_ARGUMENT, "threshold and minimum distance are fixed until the stream's last push");
+            }
+            if (h->applies != st.lastApply + 1) {
+                fail(MF_ERR_BAD_STATE, "mf_stream_peaks must be called once after every apply of the stream");
+            }
+        }
+        const long long origin = st.active ? st.origin : h->applies;
+        const long long p = h->applies - origin;
+        std::vector<long long> out;
+        if (h->precision == MF_PRECISION_FLOAT) {
+            mfstream::StreamPeaks<float> sp = st.active ? *st.f : mfstream::StreamPeaks<float>(static_cast<float>(threshold), minDist);
+            streamPeaksPush<float>(h, group, sp, p, last != 0, out);
+            *nPeaks = static_cast<int>(out.size());
+            if (static_cast<long long>(out.size()) > cap) {
+                fail(MF_ERR_INVALID_ARGUMENT, "peaks holds " + std::to_string(cap) + " < " + std::to_string(out.size()));
+            }
+            st.f = std::make_unique<mfstream::StreamPeaks<float>>(std::move(sp));
+        } else {
+            mfstream::StreamPeaks<double> sp = st.active ? *st.d : mfstream::StreamPeaks<double>(threshold, minDist);
+            streamPeaksPush<double>(h, group, sp, p, last != 0, out);
+            *nPeaks = static_cast<int>(out.size());
+            if (static_cast<long long>(out.size()) > cap) {
+                fail(MF_ERR_INVALID_ARGUMENT, "peaks holds " + std::to_string(cap) + " < " + std::to_string(out.size()));
+            }
+            st.d = std::make_unique<mfstream::StreamPeaks<double>>(std::move(sp));
+        }
+        std::copy(out.begin(), out.end(), peaks);
+        if (last) {
+            st = mf_handle::StreamPeakState{};  // the next call starts a new stream
+        } else {
+            st.active = true;
+            st.origin = origin;
+            st.lastApply = h->applies;
+            st.threshold = threshold;
+            st.minDist = minDist;
+        }
     });
 }
 

@@ -265,3 +265,101 @@ def test_stream_checkpoint_and_resume():
     a.close()
     b.close()
 
+
+
+def _stream_stack_and_peaks(h, groups, sigs, chunk, pushes, thr, md):
+    """Pushes the channels chunk by chunk; per group, the stream's stack
+    (lags K >= 0) and the peaks mf_stream_peaks returned over the pushes."""
+    carry = h.stream_carry()
+    tails = {g: [] for g in groups}
+    got = {g: [] for g in groups}
+    lead = {}
+    for p in range(pushes):
+        for c, s in enumerate(sigs):
+            h.set_signal(c, s[p * chunk:(p + 1) * chunk])
+        h.apply()
+        for g in groups:
+            st = h.shift_and_stack(g)
+            lead[g] = carry - (len(st) - chunk)          # the group's nb - 1 + max shift
+            tails[g].append(st[-chunk:])
+            got[g].append(h.stream_peaks(g, thr, md, last=(p == pushes - 1)))
+    out = {}
+    for g in groups:
+        stack = np.concatenate(tails[g])[lead[g]:]        # lag K = p * chunk - lead + k, kept from K = 0
+        out[g] = (stack, np.concatenate(got[g]))
+    return out
+
+
+@pytest.mark.parametrize("dtype,thr,md", [(np.float32, 0.25, 150), (np.float32, 0.2, 2500),
+                                          (np.float32, 0.5, 0), (np.float64, 0.3, 400)])
+def test_stream_peaks_equal_whole_signal_peak_finder(dtype, thr, md):
+    """mf_stream_peaks over every push equals the oracle's PeakFinder
+    (src/peakFinder.cpp:468-586) on the stream's whole stack, index for
+    index, in increasing order; a minDist longer than a push makes fates
+    wait several pushes."""
+    rng = np.random.default_rng(83)
+    lens, shifts = [240, 180, 240], [0, 9, 23]
+    tpls = [rng.standard_normal(L) * np.hanning(L) for L in lens]
+    chunk, pushes = 1500, 8
+    sigs = [_signal(rng, chunk * pushes, 3.0 * c) for c in range(3)]
+    for c, t in enumerate(tpls):
+        for pos in (1400, 5990, 8950, 11450):               # events across push boundaries
+            sigs[c][pos + shifts[c]:pos + shifts[c] + len(t)] += 2.5 * t
+    sigs = [s.astype(dtype) for s in sigs]
+    h = capi.Handle(dtype, "multi")
+    h.set_templates(tpls, shifts=shifts, weights=[1.0, 0.8, 0.6])
+    h.set_signal_size(chunk)
+    h.set_streaming(True)
+    h.initialize()
+    stack, got = _stream_stack_and_peaks(h, [0], sigs, chunk, pushes, thr, md)[0]
+    assert len(stack) == chunk * pushes - max(lens) + 1 - max(shifts)
+    ref = O.peak_finder(stack, thr, md)
+    assert len(ref) >= 3                                 # the injected events at least
+    assert list(got) == list(ref)
+    assert np.all(np.diff(got) > 0)
+    h.close()
+
+
+def test_stream_peaks_batched_groups_and_errors():
+    """Two stacking groups of different lengths and moveouts (so each
+    group's stream starts at its own lag of the stack), then the call
+    conventions."""
+    rng = np.random.default_rng(89)
+    lens = [200, 150, 256, 200]
+    tpls = [rng.standard_normal(L) for L in lens]
+    row_ch, row_grp, shifts = [0, 0, 1, 1], [0, 0, 1, 1], [0, 11, 0, 5]
+    chunk, pushes = 2500, 6
+    sigs = [_signal(rng, chunk * pushes, 2.0 + 10 * c).astype(np.float32) for c in range(2)]
+    h = capi.Handle(np.float32, "batched")
+    h.set_templates(tpls, shifts=shifts, weights=[1.0] * 4)
+    h.set_batch_layout([chunk, chunk], row_ch, row_grp, 2)
+    h.set_streaming(True)
+    h.initialize()
+    res = _stream_stack_and_peaks(h, [0, 1], sigs, chunk, pushes, 0.12, 120)
+    for g in (0, 1):
+        stack, got = res[g]
+        ref = O.peak_finder(stack, 0.12, 120)
+        assert len(ref) > 3, g
+        assert list(got) == list(ref), g
+    # a new stream after the last push; one call per apply; fixed parameters
+    h.set_signal(0, sigs[0][:chunk])
+    h.set_signal(1, sigs[1][:chunk])
+    h.apply()
+    h.stream_peaks(0, 0.12, 120)
+    with pytest.raises(RuntimeError):
+        h.stream_peaks(0, 0.12, 120)                   # twice after one apply
+    h.apply()
+    with pytest.raises(ValueError):
+        h.stream_peaks(0, 0.3, 120)                    # another threshold mid-stream
+    h.stream_peaks(0, 0.12, 120, last=True)
+    h.close()
+    m = capi.Handle(np.float32, "multi")
+    m.set_templates(tpls[:2], shifts=[0, 3], weights=[1.0, 1.0])
+    m.set_signal_size(chunk)
+    m.initialize()
+    m.set_signal(0, sigs[0][:chunk])
+    m.set_signal(1, sigs[1][:chunk])
+    m.apply()
+    with pytest.raises(RuntimeError):
+        m.stream_peaks(0, 0.5, 10)                     # not a streaming handle
+    m.close()
