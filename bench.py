@@ -294,12 +294,18 @@ def streaming_section(args, torch, wl, day_row0):
     from mflib_amd import capi
     E, C = wl.n_events, wl.n_channels
     chunk, pushes = 6000, 40
-    h = capi.Handle(np.float32, "batched")
-    h.set_templates([wl.templates[e][c] for e in range(E) for c in range(C)],
-                    shifts=[int(wl.moveouts[e, c]) for e in range(E) for c in range(C)], weights=[1.0] * (E * C))
-    h.set_batch_layout([chunk] * C, [c for e in range(E) for c in range(C)], [e for e in range(E) for c in range(C)], E)
-    h.set_streaming(True)
-    h.initialize()
+
+    def handle():
+        h = capi.Handle(np.float32, "batched")
+        h.set_templates([wl.templates[e][c] for e in range(E) for c in range(C)],
+                        shifts=[int(wl.moveouts[e, c]) for e in range(E) for c in range(C)], weights=[1.0] * (E * C))
+        h.set_batch_layout([chunk] * C, [c for e in range(E) for c in range(C)],
+                           [e for e in range(E) for c in range(C)], E)
+        h.set_streaming(True)
+        h.initialize()
+        return h
+
+    h = handle()
     carry = h.stream_carry()
     nb = len(wl.templates[0][0])
     dev = [torch.from_numpy(wl.signals[c][:chunk * pushes]).to("cuda:0") for c in range(C)]
@@ -319,15 +325,41 @@ def streaming_section(args, torch, wl, day_row0):
     lat = np.array(lat[2:])                        # the first pushes include allocation
     n_corr = E * C * chunk
     h.close()
+    # the same pushes with peaks continuous across pushes (mf_stream_peaks_all)
+    h = handle()
+    lat_sp, stacks, got = [], [[] for _ in range(E)], [[] for _ in range(E)]
+    for p in range(pushes):
+        t0 = time.perf_counter()
+        h.push_chunks_device([dev[c][p * chunk:].data_ptr() for c in range(C)], chunk)
+        h.apply()
+        for g, pk in enumerate(h.stream_peaks_all(0.5, 512, last=(p == pushes - 1))):
+            got[g].append(pk)
+        lat_sp.append((time.perf_counter() - t0) * 1e3)
+        for g in range(E):
+            stacks[g].append(h.shift_and_stack(g))
+    # every group's stream peaks against PeakFinder over the stream's whole stack
+    same = True
+    for g in range(E):
+        lead = carry - (len(stacks[g][0]) - chunk)
+        whole = np.concatenate([st[-chunk:] for st in stacks[g]])[lead:]
+        same = same and list(np.concatenate(got[g])) == list(capi.peaks(whole, 0.5, 512))
+    h.close()
+    lat_sp = np.array(lat_sp[2:])
     res = {"workload": "streaming: 16 templates x 24 channels, 60 s chunks (6000 samples) of the multi-channel "
                        "day, 384 rows + 16 stacks + PeakFinder per push",
            "chunk_samples": chunk, "pushes": pushes, "carry_samples": carry,
            "ms_per_push_p50": float(np.percentile(lat, 50)), "ms_per_push_p99": float(np.percentile(lat, 99)),
            "realtime_factor": 60e3 / float(np.percentile(lat, 50)),
            "new_corr_samples_per_s": n_corr / (float(np.percentile(lat, 50)) * 1e-3),
-           "max_abs_drho_vs_day_row": drho}
+           "max_abs_drho_vs_day_row": drho,
+           "continuous_peaks_ms_per_push_p50": float(np.percentile(lat_sp, 50)),
+           "continuous_peaks_ms_per_push_p99": float(np.percentile(lat_sp, 99)),
+           "continuous_peaks_equal_whole_stack_peak_finder": bool(same)}
     log(f"streaming: {res['ms_per_push_p50']:.2f} ms per 60 s push (p99 {res['ms_per_push_p99']:.2f}), "
         f"{res['realtime_factor']:.0f}x real time; |drho| vs day row {drho:.2e}")
+    log(f"streaming with peaks continuous across pushes (16 groups, one call): "
+        f"{res['continuous_peaks_ms_per_push_p50']:.2f} ms per push (p99 "
+        f"{res['continuous_peaks_ms_per_push_p99']:.2f}); equal to PeakFinder over the whole stacks: {same}")
     return res
 
 

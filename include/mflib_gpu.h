@@ -139,6 +139,15 @@ int mf_set_stream_state(mf_handle *h, int channel, const void *state, int n);
  * part of mf_get_stream_state. */
 int mf_stream_peaks(mf_handle *h, int group, double threshold, int minDist, int last, long long *peaks, int cap,
                     int *nPeaks);
+/* No reference counterpart: mf_stream_peaks for every group of the handle
+ * in one call (one stack launch, one candidate scan, one wait), as
+ * mf_detect is to PeakFinder: group g's peaks are
+ * peaks[offsets[g] .. offsets[g + 1]), offsets holding nGroups + 1 entries.
+ * When cap is too small the call fails with offsets filled and no group's
+ * state changed.  A group's stream may be driven by either call, one per
+ * apply. */
+int mf_stream_peaks_all(mf_handle *h, double threshold, int minDist, int last, long long *peaks, int cap,
+                        int *offsets);
 
 /* MatchedFilter<T>::initialize (src/matchedFilter.cpp:506-821): template
  * demean/normalize (FP64), plans, template spectra, device buffers. */

@@ -84,6 +84,8 @@ _SIGNATURES = {
     "mf_push_chunks_device": ([_vp, ctypes.c_int, ctypes.c_int, ctypes.POINTER(_vp)], ctypes.c_int),
     "mf_stream_peaks": ([_vp, ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_int,
                          ctypes.POINTER(ctypes.c_longlong), ctypes.c_int, _ip], ctypes.c_int),
+    "mf_stream_peaks_all": ([_vp, ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_longlong),
+                             ctypes.c_int, _ip], ctypes.c_int),
     "mf_detect_templates": ([_vp, ctypes.c_int, ctypes.c_double, ctypes.c_int, _ip, ctypes.c_int, _ip],
                             ctypes.c_int),
     "mf_detect_template": ([_vp, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_int, _ip, ctypes.c_int, _ip],
@@ -320,6 +322,23 @@ class Handle:
                 continue
             check(rc, "stream_peaks")
             return out[:n.value].copy()
+
+    def stream_peaks_all(self, threshold: float, min_dist: int, last: bool = False):
+        """stream_peaks for every group in one call: a list of int64 arrays,
+        one per group."""
+        ng = self.number_of_groups()
+        off = np.zeros(ng + 1, dtype=np.int32)
+        cap = 4096
+        while True:
+            out = np.empty(cap, dtype=np.int64)
+            rc = lib().mf_stream_peaks_all(self._h, float(threshold), int(min_dist), 1 if last else 0,
+                                           out.ctypes.data_as(ctypes.POINTER(ctypes.c_longlong)), cap,
+                                           off.ctypes.data_as(_ip))
+            if rc == MF_ERR_INVALID_ARGUMENT and off[ng] > cap:
+                cap = int(off[ng])
+                continue
+            check(rc, "stream_peaks_all")
+            return [out[off[g]:off[g + 1]].copy() for g in range(ng)]
 
     def initialize(self):
         check(lib().mf_initialize(self._h), "initialize")

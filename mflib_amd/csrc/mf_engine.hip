@@ -59,6 +59,7 @@ struct mf_handle {
         std::unique_ptr<mfstream::StreamPeaks<double>> d;
     };
     std::vector<StreamPeakState> streamPeaks;
+    DevBuf dStreamTab, dStreamIdx, dStreamEdge;  // mf_stream_peaks_all: segment table, edge gather
     DevBuf dCarry;
     DevBuf dDeadPart;  // streaming: per-chunk min/max of the dead-trace pass on the device
     hipGraph_t graph = nullptr;
@@ -134,6 +135,7 @@ struct mf_handle {
     size_t candCap = 0;
     DevBuf dStacks, dCounts, dCandIdx, dCandVal, dGroupTab;
     DevBuf dPackIdx, dPackVal;  // candidates of all groups, packed for one readback
+    std::vector<mfk::StackGroup> hGroupTab;  // host copy of dGroupTab
     PinnedBuf hPack;            // host side of that readback
     std::map<int, TwTables> tw;
     struct Launch {
@@ -748,6 +750,7 @@ void prepareStacks(mf_handle *h)
     }
     h->dGroupTab.alloc(nG * sizeof(mfk::StackGroup));
     MF_HIP(hipMemcpy(h->dGroupTab.p, tab.data(), tab.size() * sizeof(mfk::StackGroup), hipMemcpyHostToDevice));
+    h->hGroupTab = tab;
     h->stacksReady = true;
 }
 
@@ -872,7 +875,7 @@ AmplitudeWork &amplitudeWork()
 // unordered), read back to the host.
 template <typename T>
 void rawCandidates(const void *dx, long long n, double threshold, std::vector<std::pair<T, int>> &cand,
-                   hipStream_t stream)
+                   hipStream_t stream, T *edges = nullptr)
 {
     PeakWork &w = peakWork();
     const int candCap = static_cast<int>(std::min<long long>(n / 2 + 2, 0x7fffffffLL));
@@ -894,6 +897,13 @@ void rawCandidates(const void *dx, long long n, double threshold, std::vector<st
     MF_HIP(hipMemcpyAsync(hCount, w.count.p, sizeof(int), hipMemcpyDeviceToHost, stream));
     MF_HIP(hipMemcpyAsync(hIdx, w.idx.p, static_cast<size_t>(pre) * sizeof(int), hipMemcpyDeviceToHost, stream));
     MF_HIP(hipMemcpyAsync(hVal, w.val.p, static_cast<size_t>(pre) * sizeof(T), hipMemcpyDeviceToHost, stream));
+    if (edges != nullptr) {  // the signal's first two and last two samples, in the same wait
+        const long long at[4] = {0, std::min(1LL, n - 1), std::max(n - 2, 0LL), n - 1};
+        for (int i = 0; i < 4; ++i) {
+            MF_HIP(hipMemcpyAsync(&edges[i], static_cast<const T *>(dx) + at[i], sizeof(T), hipMemcpyDeviceToHost,
+                                  stream));
+        }
+    }
     MF_HIP(hipStreamSynchronize(stream));
     const int count = *hCount;
     if (count > candCap) { fail(MF_ERR_DEVICE, "peak candidate buffer overflow"); }
@@ -1453,9 +1463,8 @@ void streamPeaksPush(mf_handle *h, int group, mfstream::StreamPeaks<T> &sp, long
                      std::vector<long long> &out)
 {
     const long long chunk = h->ch[0].n - h->carry;
-    int lenInt = 0;
-    stack(h, group, nullptr, std::numeric_limits<int>::max(), &lenInt);  // waits for the stack
-    const long long len = lenInt;
+    stackGroups(h, group);  // on the handle's stream; evD marks its end
+    const long long len = h->stackInfo[group].len;
     const long long newStart = len - chunk;
     if (newStart < 0) { fail(MF_ERR_DEVICE, "stack shorter than the chunk"); }
     const long long base = p * chunk + newStart - h->carry;
@@ -1469,16 +1478,194 @@ void streamPeaksPush(mf_handle *h, int group, mfstream::StreamPeaks<T> &sp, long
             std::lock_guard<std::mutex> lock(g_peakMutex);
             PeakWork &w = peakWork();
             if (w.stream == nullptr) { MF_HIP(hipStreamCreate(&w.stream)); }
-            rawCandidates<T>(dx, n, static_cast<T>(sp.threshold()), cand, w.stream);
-            const long long at[4] = {0, std::min(1LL, n - 1), std::max(n - 2, 0LL), n - 1};
-            for (int i = 0; i < 4; ++i) {
-                MF_HIP(hipMemcpyAsync(&edge[i], dx + at[i] * sizeof(T), sizeof(T), hipMemcpyDeviceToHost, w.stream));
-            }
-            MF_HIP(hipStreamSynchronize(w.stream));
+            MF_HIP(hipStreamWaitEvent(w.stream, h->evD, 0));  // the scan reads the stack
+            rawCandidates<T>(dx, n, static_cast<T>(sp.threshold()), cand, w.stream, edge);
         }
         sp.push(base + v0, static_cast<int>(n), std::move(cand), edge[0], edge[1], edge[2], edge[3], out);
     }
     if (last) { sp.flush(out); }
+}
+
+// One push of the streaming PeakFinder over every group at once: one stack
+// launch, one candidate scan of all the new segments, one gather of their
+// edge samples and one wait.  Same lag convention as streamPeaksPush.
+template <typename T>
+void streamPeaksPushAll(mf_handle *h, std::vector<mfstream::StreamPeaks<T>> &sps, const std::vector<long long> &p,
+                        bool last, std::vector<std::vector<long long>> &out)
+{
+    const int nG = static_cast<int>(h->groups.size());
+    const long long chunk = h->ch[0].n - h->carry;
+    const int words = static_cast<int>(sizeof(T) / 4);  // 32-bit words per sample
+    if (static_cast<long long>(h->ldStack) * nG * words > 0x7fffffffLL) {
+        fail(MF_ERR_UNSUPPORTED, "stacks too long for the batched stream peak finder; call per group");
+    }
+    stackGroups(h, -1);
+    std::vector<mfk::StackGroup> tab(h->hGroupTab);
+    std::vector<long long> g0(nG), segN(nG);
+    std::vector<int> edgeIdx(static_cast<size_t>(4) * nG * words);
+    long long maxN = 0;
+    for (int g = 0; g < nG; ++g) {
+        const long long len = h->stackInfo[g].len;
+        const long long newStart = len - chunk;
+        if (newStart < 0) { fail(MF_ERR_DEVICE, "stack shorter than the chunk"); }
+        const long long base = p[g] * chunk + newStart - h->carry;
+        const long long v0 = std::min(chunk, std::max(0LL, -base));
+        const long long n = chunk - v0;
+        g0[g] = base + v0;
+        segN[g] = n;
+        maxN = std::max(maxN, n);
+        // the segment's first sample (any sample of the group's row when it is empty)
+        const long long first = static_cast<long long>(g) * h->ldStack + (n > 0 ? newStart + v0 : 0);
+        tab[g].out = static_cast<char *>(tab[g].out) + static_cast<size_t>(newStart + v0) * sizeof(T);
+        tab[g].len = n;
+        const long long at[4] = {0, std::min(1LL, std::max(n - 1, 0LL)), std::max(n - 2, 0LL), std::max(n - 1, 0LL)};
+        for (int i = 0; i < 4; ++i) {
+            for (int w = 0; w < words; ++w) {
+                edgeIdx[(static_cast<size_t>(g) * 4 + i) * words + w] = static_cast<int>((first + at[i]) * words + w);
+            }
+        }
+    }
+    growTo(h->dStreamTab, tab.size() * sizeof(mfk::StackGroup));
+    growTo(h->dStreamIdx, edgeIdx.size() * sizeof(int));
+    growTo(h->dStreamEdge, edgeIdx.size() * sizeof(int));
+    MF_HIP(hipMemcpyAsync(h->dStreamTab.p, tab.data(), tab.size() * sizeof(mfk::StackGroup), hipMemcpyHostToDevice,
+                          h->stream));
+    MF_HIP(hipMemcpyAsync(h->dStreamIdx.p, edgeIdx.data(), edgeIdx.size() * sizeof(int), hipMemcpyHostToDevice,
+                          h->stream));
+    MF_HIP(mfk::launch_gather_int(h->dStacks.as<int>(), h->dStreamIdx.as<int>(), static_cast<int>(edgeIdx.size()),
+                                  h->dStreamEdge.as<int>(), h->stream));
+    MF_HIP(hipMemsetAsync(h->dCounts.p, 0, nG * sizeof(int), h->stream));
+    const mfk::StackGroup *dTab = h->dStreamTab.as<mfk::StackGroup>();
+    MF_HIP(mfk::launch_peak_groups(h->precision, dTab, nG, maxN, static_cast<double>(sps[0].threshold()), h->stream));
+    MF_HIP(mfk::launch_pack_candidates(h->precision, dTab, nG, h->dPackIdx.as<int>(), h->dPackVal.p, h->stream));
+    // one readback: counts, the first kPrefix candidates and the edges
+    constexpr size_t kPrefix = 8192;
+    const size_t pre = std::min(kPrefix, h->candCap * static_cast<size_t>(nG));
+    const size_t hdrBytes = (static_cast<size_t>(nG) + pre) * sizeof(int);
+    h->hPack.reserve(hdrBytes + pre * sizeof(T));
+    int *hIdx = static_cast<int *>(h->hPack.p);
+    T *hVal = reinterpret_cast<T *>(static_cast<char *>(h->hPack.p) + hdrBytes);
+    std::vector<T> edges(static_cast<size_t>(4) * nG);
+    MF_HIP(hipMemcpyAsync(hIdx, h->dPackIdx.p, hdrBytes, hipMemcpyDeviceToHost, h->stream));
+    MF_HIP(hipMemcpyAsync(hVal, h->dPackVal.p, pre * sizeof(T), hipMemcpyDeviceToHost, h->stream));
+    MF_HIP(hipMemcpyAsync(edges.data(), h->dStreamEdge.p, edges.size() * sizeof(T), hipMemcpyDeviceToHost, h->stream));
+    MF_HIP(hipStreamSynchronize(h->stream));
+    std::vector<size_t> first(nG + 1, 0);
+    for (int g = 0; g < nG; ++g) {
+        if (hIdx[g] > static_cast<int>(h->candCap)) { fail(MF_ERR_DEVICE, "peak candidate buffer overflow"); }
+        first[g + 1] = first[g] + static_cast<size_t>(hIdx[g]);
+    }
+    const size_t total = first[nG];
+    std::vector<int> ci(total);
+    std::vector<T> cv(total);
+    std::copy(hIdx + nG, hIdx + nG + std::min(total, pre), ci.begin());
+    std::copy(hVal, hVal + std::min(total, pre), cv.begin());
+    if (total > pre) {
+        MF_HIP(hipMemcpyAsync(ci.data() + pre, h->dPackIdx.as<int>() + nG + pre, (total - pre) * sizeof(int),
+                              hipMemcpyDeviceToHost, h->stream));
+        MF_HIP(hipMemcpyAsync(cv.data() + pre, static_cast<const T *>(h->dPackVal.p) + pre, (total - pre) * sizeof(T),
+                              hipMemcpyDeviceToHost, h->stream));
+        MF_HIP(hipStreamSynchronize(h->stream));
+    }
+    out.assign(nG, {});
+    for (int g = 0; g < nG; ++g) {
+        if (segN[g] > 0) {
+            std::vector<std::pair<T, int>> cand(first[g + 1] - first[g]);
+            for (size_t i = 0; i < cand.size(); ++i) { cand[i] = {cv[first[g] + i], ci[first[g] + i]}; }
+            const T *e = edges.data() + static_cast<size_t>(4) * g;
+            sps[g].push(g0[g], static_cast<int>(segN[g]), std::move(cand), e[0], e[1], e[2], e[3], out[g]);
+        }
+        if (last) { sps[g].flush(out[g]); }
+    }
+}
+
+
+// The checks and the state bookkeeping shared by mf_stream_peaks{,_all}:
+// returns the push number of the group's stream for this apply.
+long long streamPeakCheck(mf_handle *h, int group, double threshold, int minDist)
+{
+    auto &st = h->streamPeaks[group];
+    if (st.active) {
+        if (threshold != st.threshold || minDist != st.minDist) {
+            fail(MF_ERR_INVALID_ARGUMENT, "threshold and minimum distance are fixed until the stream's last push");
+        }
+        if (h->applies != st.lastApply + 1) {
+            fail(MF_ERR_BAD_STATE, "mf_stream_peaks must be called once after every apply of the stream");
+        }
+    }
+    return h->applies - (st.active ? st.origin : h->applies);
+}
+
+void streamPeakCommit(mf_handle *h, int group, double threshold, int minDist, bool last)
+{
+    auto &st = h->streamPeaks[group];
+    if (last) {
+        st = mf_handle::StreamPeakState{};  // the next call starts a new stream
+        return;
+    }
+    if (!st.active) { st.origin = h->applies; }
+    st.active = true;
+    st.lastApply = h->applies;
+    st.threshold = threshold;
+    st.minDist = minDist;
+}
+
+template <typename T>
+std::unique_ptr<mfstream::StreamPeaks<T>> &statePtr(mf_handle::StreamPeakState &st);
+template <>
+std::unique_ptr<mfstream::StreamPeaks<float>> &statePtr<float>(mf_handle::StreamPeakState &st) { return st.f; }
+template <>
+std::unique_ptr<mfstream::StreamPeaks<double>> &statePtr<double>(mf_handle::StreamPeakState &st) { return st.d; }
+
+template <typename T>
+void streamPeaksAllT(mf_handle *h, double threshold, int minDist, bool last, long long *peaks, int cap, int *offsets)
+{
+    const int nG = static_cast<int>(h->groups.size());
+    std::vector<long long> p(nG);
+    std::vector<mfstream::StreamPeaks<T>> sps;
+    sps.reserve(nG);
+    for (int g = 0; g < nG; ++g) {
+        p[g] = streamPeakCheck(h, g, threshold, minDist);
+        auto &sp = statePtr<T>(h->streamPeaks[g]);
+        sps.push_back(h->streamPeaks[g].active ? *sp
+                                               : mfstream::StreamPeaks<T>(static_cast<T>(threshold), minDist));
+    }
+    std::vector<std::vector<long long>> out;
+    streamPeaksPushAll<T>(h, sps, p, last, out);
+    long long total = 0;
+    for (int g = 0; g < nG; ++g) {
+        offsets[g] = static_cast<int>(total);
+        total += static_cast<long long>(out[g].size());
+    }
+    offsets[nG] = static_cast<int>(total);
+    if (total > cap) {
+        fail(MF_ERR_INVALID_ARGUMENT, "peaks holds " + std::to_string(cap) + " < " + std::to_string(total));
+    }
+    for (int g = 0; g < nG; ++g) {
+        std::copy(out[g].begin(), out[g].end(), peaks + offsets[g]);
+        statePtr<T>(h->streamPeaks[g]) = std::make_unique<mfstream::StreamPeaks<T>>(std::move(sps[g]));
+        streamPeakCommit(h, g, threshold, minDist, last);
+    }
+}
+
+template <typename T>
+void streamPeaksOneT(mf_handle *h, int group, double threshold, int minDist, bool last, long long *peaks, int cap,
+                     int *nPeaks)
+{
+    const long long p = streamPeakCheck(h, group, threshold, minDist);
+    auto &sp0 = statePtr<T>(h->streamPeaks[group]);
+    mfstream::StreamPeaks<T> sp = h->streamPeaks[group].active
+                                      ? *sp0
+                                      : mfstream::StreamPeaks<T>(static_cast<T>(threshold), minDist);
+    std::vector<long long> out;
+    streamPeaksPush<T>(h, group, sp, p, last, out);
+    *nPeaks = static_cast<int>(out.size());
+    if (static_cast<long long>(out.size()) > cap) {
+        fail(MF_ERR_INVALID_ARGUMENT, "peaks holds " + std::to_string(cap) + " < " + std::to_string(out.size()));
+    }
+    std::copy(out.begin(), out.end(), peaks);
+    sp0 = std::make_unique<mfstream::StreamPeaks<T>>(std::move(sp));
+    streamPeakCommit(h, group, threshold, minDist, last);
 }
 
 }  // namespace
@@ -1499,44 +1686,32 @@ int mf_stream_peaks(mf_handle *h, int group, double threshold, int minDist, int 
             if (c.n != h->ch[0].n) { fail(MF_ERR_UNSUPPORTED, "stream peaks need one chunk length on every channel"); }
         }
         if (h->streamPeaks.size() != h->groups.size()) { h->streamPeaks.resize(h->groups.size()); }
-        auto &st = h->streamPeaks[group];
-        if (st.active) {
-            if (threshold != st.threshold || minDist != st.minDist) {
-                fail(MF_ERR_INVALID_ARGUMENT, "threshold and minimum distance are fixed until the stream's last push");
-            }
-            if (h->applies != st.lastApply + 1) {
-                fail(MF_ERR_BAD_STATE, "mf_stream_peaks must be called once after every apply of the stream");
-            }
-        }
-        const long long origin = st.active ? st.origin : h->applies;
-        const long long p = h->applies - origin;
-        std::vector<long long> out;
         if (h->precision == MF_PRECISION_FLOAT) {
-            mfstream::StreamPeaks<float> sp = st.active ? *st.f : mfstream::StreamPeaks<float>(static_cast<float>(threshold), minDist);
-            streamPeaksPush<float>(h, group, sp, p, last != 0, out);
-            *nPeaks = static_cast<int>(out.size());
-            if (static_cast<long long>(out.size()) > cap) {
-                fail(MF_ERR_INVALID_ARGUMENT, "peaks holds " + std::to_string(cap) + " < " + std::to_string(out.size()));
-            }
-            st.f = std::make_unique<mfstream::StreamPeaks<float>>(std::move(sp));
+            streamPeaksOneT<float>(h, group, threshold, minDist, last != 0, peaks, cap, nPeaks);
         } else {
-            mfstream::StreamPeaks<double> sp = st.active ? *st.d : mfstream::StreamPeaks<double>(threshold, minDist);
-            streamPeaksPush<double>(h, group, sp, p, last != 0, out);
-            *nPeaks = static_cast<int>(out.size());
-            if (static_cast<long long>(out.size()) > cap) {
-                fail(MF_ERR_INVALID_ARGUMENT, "peaks holds " + std::to_string(cap) + " < " + std::to_string(out.size()));
-            }
-            st.d = std::make_unique<mfstream::StreamPeaks<double>>(std::move(sp));
+            streamPeaksOneT<double>(h, group, threshold, minDist, last != 0, peaks, cap, nPeaks);
         }
-        std::copy(out.begin(), out.end(), peaks);
-        if (last) {
-            st = mf_handle::StreamPeakState{};  // the next call starts a new stream
+    });
+}
+
+int mf_stream_peaks_all(mf_handle *h, double threshold, int minDist, int last, long long *peaks, int cap,
+                        int *offsets)
+{
+    return guard([&] {
+        checkStackable(h, -1);
+        if (!h->streaming) { fail(MF_ERR_BAD_STATE, "mf_stream_peaks_all needs streaming mode"); }
+        if (offsets == nullptr || (peaks == nullptr && cap > 0)) { fail(MF_ERR_INVALID_ARGUMENT, "peaks is NULL"); }
+        if (!(threshold >= 0 && threshold <= 1)) {
+            fail(MF_ERR_INVALID_ARGUMENT, "Threshold = " + std::to_string(threshold) + " must be in the range [0,1]");
+        }
+        for (const auto &c : h->ch) {
+            if (c.n != h->ch[0].n) { fail(MF_ERR_UNSUPPORTED, "stream peaks need one chunk length on every channel"); }
+        }
+        if (h->streamPeaks.size() != h->groups.size()) { h->streamPeaks.resize(h->groups.size()); }
+        if (h->precision == MF_PRECISION_FLOAT) {
+            streamPeaksAllT<float>(h, threshold, minDist, last != 0, peaks, cap, offsets);
         } else {
-            st.active = true;
-            st.origin = origin;
-            st.lastApply = h->applies;
-            st.threshold = threshold;
-            st.minDist = minDist;
+            streamPeaksAllT<double>(h, threshold, minDist, last != 0, peaks, cap, offsets);
         }
     });
 }

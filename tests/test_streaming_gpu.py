@@ -363,3 +363,75 @@ def test_stream_peaks_batched_groups_and_errors():
     with pytest.raises(RuntimeError):
         m.stream_peaks(0, 0.5, 10)                     # not a streaming handle
     m.close()
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_stream_peaks_all_groups_equals_per_group_calls(dtype):
+    """mf_stream_peaks_all (every group, one scan and one wait) returns, push
+    by push, the lists of one mf_stream_peaks call per group."""
+    rng = np.random.default_rng(97)
+    lens = [200, 150, 256, 200, 120, 180]
+    tpls = [rng.standard_normal(L) for L in lens]
+    row_ch, row_grp, shifts = [0, 1, 0, 1, 0, 1], [0, 0, 1, 1, 2, 2], [0, 11, 0, 5, 3, 40]
+    chunk, pushes = 1800, 7
+
+    def fresh():
+        h = capi.Handle(dtype, "batched")
+        h.set_templates(tpls, shifts=shifts, weights=[1.0, 0.5, 1.0, 1.0, 0.7, 0.7])
+        h.set_batch_layout([chunk, chunk], row_ch, row_grp, 3)
+        h.set_streaming(True)
+        h.initialize()
+        return h
+
+    sigs = [_signal(rng, chunk * pushes, 1.0 + 6 * c).astype(dtype) for c in range(2)]
+    a, b = fresh(), fresh()
+    seen = 0
+    for p in range(pushes):
+        last = p == pushes - 1
+        for h in (a, b):
+            for c in range(2):
+                h.set_signal(c, sigs[c][p * chunk:(p + 1) * chunk])
+            h.apply()
+        per = [a.stream_peaks(g, 0.1, 90, last=last) for g in range(3)]
+        allg = b.stream_peaks_all(0.1, 90, last=last)
+        for g in range(3):
+            assert np.array_equal(per[g], allg[g]), (p, g)
+            seen += len(per[g])
+    assert seen > 10
+    a.close()
+    b.close()
+
+
+def test_stream_peaks_all_with_chunks_shorter_than_the_carry():
+    """Chunks of 100 samples under a carry of 269: the first pushes hold no
+    lag of the stream yet (their segments are empty) and every later peak's
+    pruning window spans several pushes.  The lists equal the oracle's
+    PeakFinder over the stream's whole stack."""
+    rng = np.random.default_rng(101)
+    lens, shifts = [240, 180], [0, 30]
+    tpls = [rng.standard_normal(L) * np.hanning(L) for L in lens]
+    chunk, pushes = 100, 40
+    sigs = [_signal(rng, chunk * pushes, 2.0 * c) for c in range(2)]
+    for c, t in enumerate(tpls):
+        for pos in (700, 2150, 3300):
+            sigs[c][pos + shifts[c]:pos + shifts[c] + len(t)] += 3 * t
+    sigs = [s.astype(np.float32) for s in sigs]
+    h = capi.Handle(np.float32, "multi")
+    h.set_templates(tpls, shifts=shifts, weights=[1.0, 1.0])
+    h.set_signal_size(chunk)
+    h.set_streaming(True)
+    h.initialize()
+    carry = h.stream_carry()
+    assert carry == 239 + 30 > chunk
+    tails, got = [], []
+    for p in range(pushes):
+        for c in range(2):
+            h.set_signal(c, sigs[c][p * chunk:(p + 1) * chunk])
+        h.apply()
+        got.append(h.stream_peaks_all(0.15, 250, last=(p == pushes - 1))[0])
+        tails.append(h.shift_and_stack(0)[-chunk:])
+    stack = np.concatenate(tails)[carry:]
+    ref = O.peak_finder(stack, 0.15, 250)
+    assert len(ref) >= 3
+    assert list(np.concatenate(got)) == list(ref)
+    h.close()
