@@ -435,3 +435,33 @@ def test_stream_peaks_all_with_chunks_shorter_than_the_carry():
     assert len(ref) >= 3
     assert list(np.concatenate(got)) == list(ref)
     h.close()
+
+
+def test_stream_peaks_retry_after_a_full_output_keeps_the_state():
+    """Threshold 0 and no minimum distance: about a third of all lags are
+    peaks, more than the first output buffer holds, so every call fails once
+    with the count and is repeated with room (Handle.stream_peaks does
+    this).  The failed calls leave the state unchanged: the lists still
+    equal PeakFinder over the whole stream stack."""
+    rng = np.random.default_rng(103)
+    tpls = [rng.standard_normal(L) for L in (150, 120)]
+    chunk, pushes = 5000, 4
+    sigs = [_signal(rng, chunk * pushes, 1.0).astype(np.float32) for _ in range(2)]
+    h = capi.Handle(np.float32, "multi")
+    h.set_templates(tpls, shifts=[0, 2], weights=[1.0, 1.0])
+    h.set_signal_size(chunk)
+    h.set_streaming(True)
+    h.initialize()
+    carry = h.stream_carry()
+    tails, got = [], []
+    for p in range(pushes):
+        for c in range(2):
+            h.set_signal(c, sigs[c][p * chunk:(p + 1) * chunk])
+        h.apply()
+        pk = h.stream_peaks(0, 0.0, 0, last=(p == pushes - 1))
+        assert len(pk) > 1024
+        got.append(pk)
+        tails.append(h.shift_and_stack(0)[-chunk:])
+    stack = np.concatenate(tails)[carry:]
+    assert list(np.concatenate(got)) == list(O.peak_finder(stack, 0.0, 0))
+    h.close()
