@@ -60,6 +60,7 @@ struct mf_handle {
     };
     std::vector<StreamPeakState> streamPeaks;
     DevBuf dStreamTab, dStreamIdx, dStreamEdge;  // mf_stream_peaks_all: segment table, edge gather
+    DevBuf dPushScratch;  // mf_push_chunks_device: every channel's carry while its buffer is rewritten
     DevBuf dCarry;
     DevBuf dDeadPart;  // streaming: per-chunk min/max of the dead-trace pass on the device
     hipGraph_t graph = nullptr;
@@ -209,6 +210,7 @@ void dropDevice(mf_handle *h)
     h->dGroupTab.release();
     h->dStackOut.release();
     h->dCarry.release();
+    h->dPushScratch.release();
     h->dDeadPart.release();
     h->carry = 0;
     h->applies = 0;
@@ -1397,7 +1399,21 @@ int mf_push_chunks_device(mf_handle *h, int nChannels, int n, const void *const 
             if (dx[c] == nullptr) { fail(MF_ERR_INVALID_ARGUMENT, "chunk of channel " + std::to_string(c) + " is NULL"); }
         }
         h->applied = false;
-        for (int c = 0; c < nChannels; ++c) { pushChunk(h, c, n, dx[c], hipMemcpyDeviceToDevice, false); }
+        if (nChannels <= mfk::kPushMaxChannels) {
+            // two launches for every channel instead of three copies per channel
+            mfk::PushTable tab{};
+            for (int c = 0; c < nChannels; ++c) {
+                tab.buf[c] = h->ch[c].buf.p;
+                tab.src[c] = dx[c];
+                h->ch[c].set = true;
+            }
+            const long long words = static_cast<long long>(h->tsize / 4);
+            growTo(h->dPushScratch, static_cast<size_t>(nChannels) * static_cast<size_t>(h->carry) * h->tsize);
+            MF_HIP(mfk::launch_push(tab, nChannels, n * words, h->carry * words, h->dPushScratch.as<unsigned>(),
+                                    h->stream));
+        } else {
+            for (int c = 0; c < nChannels; ++c) { pushChunk(h, c, n, dx[c], hipMemcpyDeviceToDevice, false); }
+        }
         MF_HIP(hipStreamSynchronize(h->stream));
     });
 }

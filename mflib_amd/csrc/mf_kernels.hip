@@ -1031,6 +1031,37 @@ __device__ __forceinline__ void peak_scan(gptr<const T> x, long long n, T thr, i
     }
 }
 
+// Streaming push of every channel, in 32-bit words, two launches: first
+// each channel's last `carry` words (the buffer's tail) go to its slot of
+// the scratch area, then each buffer is rewritten as carry + new chunk.
+// Two launches because the ranges overlap when the chunk is shorter than
+// the carry.  grid.y = channel.
+__global__ __launch_bounds__(NT) void push_save_kernel(PushTable tab, long long chunkWords, long long carryWords,
+                                                       unsigned *__restrict__ scratch)
+{
+    const int c = blockIdx.y;
+    const unsigned *__restrict__ buf = static_cast<const unsigned *>(tab.buf[c]);
+    unsigned *__restrict__ dst = scratch + static_cast<long long>(c) * carryWords;
+    for (long long i = static_cast<long long>(blockIdx.x) * NT + threadIdx.x; i < carryWords;
+         i += static_cast<long long>(gridDim.x) * NT) {
+        dst[i] = buf[chunkWords + i];
+    }
+}
+
+__global__ __launch_bounds__(NT) void push_write_kernel(PushTable tab, long long chunkWords, long long carryWords,
+                                                        const unsigned *__restrict__ scratch)
+{
+    const int c = blockIdx.y;
+    unsigned *__restrict__ buf = static_cast<unsigned *>(tab.buf[c]);
+    const unsigned *__restrict__ src = static_cast<const unsigned *>(tab.src[c]);
+    const unsigned *__restrict__ saved = scratch + static_cast<long long>(c) * carryWords;
+    const long long total = carryWords + chunkWords;
+    for (long long j = static_cast<long long>(blockIdx.x) * NT + threadIdx.x; j < total;
+         j += static_cast<long long>(gridDim.x) * NT) {
+        buf[j] = j < carryWords ? saved[j] : src[j - carryWords];
+    }
+}
+
 // out[i] = src[idx[i]]
 __global__ __launch_bounds__(NT) void gather_int_kernel(const int *__restrict__ src, const int *__restrict__ idx,
                                                         int n, int *__restrict__ out)
@@ -1285,6 +1316,26 @@ hipError_t launch_pack_candidates(int precision, const StackGroup *dGroups, int 
         hipLaunchKernelGGL(pack_candidates_kernel<double>, dim3(nGroups), dim3(NT), 0, stream, dGroups, nGroups,
                            dPackedIdx, static_cast<double *>(dPackedVal));
     }
+    return hipGetLastError();
+}
+
+hipError_t launch_push(const PushTable &tab, int nChannels, long long chunkWords, long long carryWords,
+                       unsigned *dScratch, hipStream_t stream)
+{
+    if (nChannels <= 0 || nChannels > kPushMaxChannels || chunkWords <= 0 || carryWords < 0) {
+        return hipErrorInvalidValue;
+    }
+    // about four words per thread per launch, at most 1024 blocks per channel
+    auto blocks = [](long long words) {
+        const long long b = (words + NT * 4 - 1) / (NT * 4);
+        return static_cast<unsigned>(b < 1 ? 1 : (b > 1024 ? 1024 : b));
+    };
+    if (carryWords > 0) {
+        hipLaunchKernelGGL(push_save_kernel, dim3(blocks(carryWords), nChannels), dim3(NT), 0, stream, tab,
+                           chunkWords, carryWords, dScratch);
+    }
+    hipLaunchKernelGGL(push_write_kernel, dim3(blocks(carryWords + chunkWords), nChannels), dim3(NT), 0, stream, tab,
+                       chunkWords, carryWords, dScratch);
     return hipGetLastError();
 }
 

@@ -142,6 +142,17 @@ hipError_t launch_reduce(int precision, const void *const *dRowPtrs, int nRows, 
 // dPackedVal[0].  Capacity: nG + sum of the groups' candCap.
 hipError_t launch_pack_candidates(int precision, const StackGroup *dGroups, int nGroups, int *dPackedIdx,
                                   void *dPackedVal, hipStream_t stream);
+// Streaming push of up to kPushMaxChannels channels at once (the table
+// travels as a kernel argument): buffer c, carry + chunk words long, keeps
+// its last carryWords words in front of the chunkWords words of src c.
+// dScratch holds nChannels * carryWords words.
+constexpr int kPushMaxChannels = 64;
+struct PushTable {
+    void *buf[kPushMaxChannels];
+    const void *src[kPushMaxChannels];
+};
+hipError_t launch_push(const PushTable &tab, int nChannels, long long chunkWords, long long carryWords,
+                       unsigned *dScratch, hipStream_t stream);
 // out[i] = src[idx[i]], i < n (all device pointers).
 hipError_t launch_gather_int(const int *dSrc, const int *dIdx, int n, int *dOut, hipStream_t stream);
 // Threshold + local-maximum candidates (unordered, deterministic set).

@@ -181,16 +181,19 @@ def test_batched_stream_with_graph_launch_and_mfma_rows(variant):
     h.close()
 
 
-def test_batched_push_equals_per_channel_pushes():
-    """mf_push_chunks_device (every channel in one call, one wait) gives the
-    rows and stacks of one mf_set_signal_device per channel, bit for bit."""
+@pytest.mark.parametrize("dtype,chunk", [(np.float32, 4000), (np.float64, 150)])
+def test_batched_push_equals_per_channel_pushes(dtype, chunk):
+    """mf_push_chunks_device (every channel in one call: the carries saved,
+    then every buffer rewritten, one wait) gives the rows and stacks of one
+    mf_set_signal_device per channel, bit for bit; with a chunk of 150
+    samples under a carry of 248 the saved carry overlaps the new one."""
     rng = np.random.default_rng(59)
     tpls = [rng.standard_normal(L) for L in (240, 180, 240)]
-    chunk, pushes = 4000, 4
+    pushes = 4 if chunk > 1000 else 9
     sigs = [_signal(rng, chunk * pushes, 5.0 * c) for c in range(3)]
 
     def fresh():
-        h = capi.Handle(np.float32, "multi")
+        h = capi.Handle(dtype, "multi")
         h.set_templates(tpls, shifts=[0, 4, 9], weights=[1.0, 1.0, 1.0])
         h.set_signal_size(chunk)
         h.set_streaming(True)
@@ -199,7 +202,7 @@ def test_batched_push_equals_per_channel_pushes():
 
     a, b = fresh(), fresh()
     for p in range(pushes):
-        chunks = [DeviceArray(sigs[c][p * chunk:(p + 1) * chunk].astype(np.float32)) for c in range(3)]
+        chunks = [DeviceArray(sigs[c][p * chunk:(p + 1) * chunk].astype(dtype)) for c in range(3)]
         for c in range(3):
             a.set_signal_device(c, chunks[c].ptr, chunk)
         b.push_chunks_device([d.ptr for d in chunks], chunk)
@@ -214,12 +217,12 @@ def test_batched_push_equals_per_channel_pushes():
         b.push_chunks_device([0, 0], chunk)            # not every channel
     a.close()
     b.close()
-    h = capi.Handle(np.float32, "multi")
+    h = capi.Handle(dtype, "multi")
     h.set_templates(tpls)
-    h.set_signal_size(chunk)
+    h.set_signal_size(4000)
     h.initialize()
     with pytest.raises(RuntimeError):
-        h.push_chunks_device([0, 0, 0], chunk)         # not a streaming handle
+        h.push_chunks_device([0, 0, 0], 4000)          # not a streaming handle
     h.close()
 
 
