@@ -523,10 +523,15 @@ class Handle:
         return p.value, n.value
 
     def template_spectrum(self, row: int):
-        nfft, _ = self.plan(row)
-        out = np.empty(2 * (nfft // 2 + 1), dtype=self.dtype)
-        ln = ctypes.c_int()
-        check(lib().mf_template_spectrum(self._h, int(row), out.ctypes.data_as(_vp), nfft // 2 + 1,
+        """getSpectraOfTemplate(row): nfft/2 + 1 complex bins at the FFT
+        length the engine exports for the row (the size comes from the
+        call itself, so time-domain rows work too)."""
+        ln = ctypes.c_int(-1)
+        rc = lib().mf_template_spectrum(self._h, int(row), None, 0, ctypes.byref(ln))
+        if ln.value < 1:
+            check(rc, "template_spectrum")
+        out = np.empty(2 * ln.value, dtype=self.dtype)
+        check(lib().mf_template_spectrum(self._h, int(row), out.ctypes.data_as(_vp), ln.value,
                                          ctypes.byref(ln)), "template_spectrum")
         return out[0::2] + 1j * out[1::2]
 

@@ -288,9 +288,12 @@ std::vector<std::complex<T>> MatchedFilter<T>::getSpectraOfTemplate(int it) cons
 {
     auto &d = *pImpl;
     d.checkIndex(it);
-    int nfft = 0, L = 0, len = 0;
-    Facade::check(mf_get_plan(d.handle, it, &nfft, &L), "getSpectraOfTemplate");
-    std::vector<std::complex<T>> out(nfft / 2 + 1);
+    // size query first: the exported length is the engine's, not always
+    // the row's plan (time-domain rows have no FFT plan)
+    int len = 0;
+    const int rc = mf_template_spectrum(d.handle, it, nullptr, 0, &len);
+    if (len < 1) { Facade::check(rc, "getSpectraOfTemplate"); }
+    std::vector<std::complex<T>> out(len);
     Facade::check(mf_template_spectrum(d.handle, it, out.data(), static_cast<int>(out.size()), &len),
                   "getSpectraOfTemplate");
     out.resize(len);

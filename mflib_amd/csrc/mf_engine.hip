@@ -27,6 +27,7 @@
 #include <mutex>
 #include <new>
 #include <numeric>
+#include <set>
 #include <stdexcept>
 #include <string>
 #include <utility>
@@ -823,14 +824,19 @@ void pruneCandidates(std::vector<std::pair<T, int>> &cand, int minDist, int *idx
     std::sort(cand.begin(), cand.end(), [](const auto &a, const auto &b) { return a.second < b.second; });
     std::vector<int> keep;
     if (minDist > 0 && !cand.empty()) {
+        // greedy in order of decreasing value (ties: ascending index): a
+        // candidate is kept when no kept index lies within minDist of it
+        // (src/peakFinder.cpp:536-574).  The kept indices live in an ordered
+        // set, so each test is one lower_bound instead of a scan over every
+        // kept index.
         std::stable_sort(cand.begin(), cand.end(), [](const auto &a, const auto &b) { return a.first > b.first; });
-        keep.push_back(cand[0].second);
+        std::set<long long> kept;
         for (const auto &p : cand) {
-            int dmin = std::abs(keep[0] - p.second);
-            for (size_t i = 1; i < keep.size(); ++i) { dmin = std::min(dmin, std::abs(keep[i] - p.second)); }
-            if (dmin > minDist) { keep.push_back(p.second); }
+            const long long i = p.second;
+            const auto it = kept.lower_bound(i - minDist);
+            if (it == kept.end() || *it > i + minDist) { kept.insert(i); }
         }
-        std::sort(keep.begin(), keep.end());
+        keep.assign(kept.begin(), kept.end());
     } else {
         for (const auto &p : cand) { keep.push_back(p.second); }
     }
