@@ -231,6 +231,19 @@ int mf_peaks(const void *x, int n, int precision, double threshold, int minDist,
  * on the handle first when dx is one of its rows. */
 int mf_peaks_device(const void *dx, int n, int precision, double threshold,
                     int minDist, int *idx, int cap, int *nPeaks);
+/* No reference counterpart: for callers that cannot allocate device memory
+ * (code between HIP graph captures, tight serving loops, harnesses that
+ * forbid hipMalloc).  mf_peaks_device with its candidate counter and list
+ * in a library-held device array (24 MiB on every GPU it runs on, kept
+ * until the process ends) instead of buffers it allocates: no hipMalloc,
+ * hipFree or pinned host allocation.  n <= 4,194,304 samples (larger n
+ * fails with MF_ERR_INVALID_ARGUMENT).  Runs on the null stream, so it is
+ * ordered after work on the default stream (synchronize a handle's stream
+ * first when dx is one of its rows); blocks until done.  All calls on one
+ * GPU share the array; the library serializes them with a process-wide
+ * lock, since overlapping calls on one GPU would race on it. */
+int mf_peaks_device_noalloc(const void *dx, int n, int precision, double threshold,
+                            int minDist, int *idx, int cap, int *nPeaks);
 
 /* No reference counterpart (precision study of normalizeSignal,
  * src/matchedFilter.cpp:94-166): the Pearson scale

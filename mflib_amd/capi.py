@@ -70,6 +70,8 @@ _SIGNATURES = {
                   ctypes.c_int, _ip], ctypes.c_int),
     "mf_peaks_device": ([_vp, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_int, _ip,
                          ctypes.c_int, _ip], ctypes.c_int),
+    "mf_peaks_device_noalloc": ([_vp, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_int, _ip,
+                                 ctypes.c_int, _ip], ctypes.c_int),
     "mf_compute_optimal_fft_and_block_length": ([ctypes.c_int, ctypes.c_int, _ip, _ip], ctypes.c_int),
     "mf_window_scale_device": ([_vp, ctypes.c_longlong, ctypes.c_int, ctypes.c_int, _vp, _dp], ctypes.c_int),
     "mf_reduce_group": ([_vp, ctypes.c_int, ctypes.c_int, _vp, _ip, ctypes.c_int, _ip], ctypes.c_int),
@@ -186,6 +188,21 @@ def peaks_device(ptr: int, n: int, dtype, threshold: float, min_dist: int, cap: 
     check(lib().mf_peaks_device(_vp(ptr), n, PRECISION[np.dtype(dtype)], float(threshold),
                                 int(min_dist), idx.ctypes.data_as(_ip), cap, ctypes.byref(cnt)),
           "peaks_device")
+    return idx[:cnt.value].copy()
+
+
+NOALLOC_MAX_SAMPLES = 1 << 22
+
+
+def peaks_device_noalloc(ptr: int, n: int, dtype, threshold: float, min_dist: int, cap: int | None = None):
+    """peaks_device with the candidates in the library's fixed device
+    scratch array (no device allocation); n <= NOALLOC_MAX_SAMPLES."""
+    cap = cap or max(n, 1)
+    idx = np.empty(cap, dtype=np.int32)
+    cnt = ctypes.c_int()
+    check(lib().mf_peaks_device_noalloc(_vp(ptr), n, PRECISION[np.dtype(dtype)], float(threshold),
+                                        int(min_dist), idx.ctypes.data_as(_ip), cap, ctypes.byref(cnt)),
+          "peaks_device_noalloc")
     return idx[:cnt.value].copy()
 
 

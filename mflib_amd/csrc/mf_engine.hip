@@ -1026,6 +1026,70 @@ void peaks(const void *x, bool onDevice, int n, int precision, double threshold,
     }
 }
 
+// Scratch of mf_peaks_device_noalloc: the candidate counter and candidate
+// list of one call, in a file-scope device array that every GPU the library
+// runs on holds until the process ends (24 MiB).  Host code never passes
+// `scratch` itself to a kernel: its name is a host-side stand-in, so the
+// device address comes from hipGetSymbolAddress, once per device.
+constexpr long long kNoAllocMaxSamples = 1LL << 22;                     // 4,194,304 samples
+constexpr int kNoAllocCandCap = static_cast<int>(kNoAllocMaxSamples / 2 + 2);  // worst case n/2 + 2
+constexpr size_t kScratchIdxOff = 256;                                  // [0, 256): the counter
+constexpr size_t kScratchValOff = kScratchIdxOff + ((kNoAllocCandCap * sizeof(int) + 255) / 256) * 256;
+constexpr size_t kScratchBytes = kScratchValOff + static_cast<size_t>(kNoAllocCandCap) * sizeof(double);
+__device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+constexpr int kMaxScratchDevices = 64;
+std::mutex g_noAllocMutex;  // serializes every call (they share one array per GPU)
+void *g_scratchPtr[kMaxScratchDevices] = {};
+
+// The device address of `scratch` on the current device, looked up on the
+// first call on that device.
+void *scratchOnCurrentDevice()
+{
+    int dev = -1;
+    MF_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxScratchDevices) { fail(MF_ERR_UNSUPPORTED, "device index out of range"); }
+    if (g_scratchPtr[dev] == nullptr) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || p == nullptr) {
+            fail(MF_ERR_DEVICE, std::string("hipGetSymbolAddress(scratch) failed: ") + hipGetErrorString(e));
+        }
+        g_scratchPtr[dev] = p;
+    }
+    return g_scratchPtr[dev];
+}
+
+// PeakFinder on a device signal with the candidates in the scratch array:
+// no device or pinned allocation; runs on the null stream.
+template <typename T>
+void peaksNoAlloc(const void *dx, long long n, double threshold, int minDist, int *idx, int cap, int *nPeaks)
+{
+    char *base = static_cast<char *>(scratchOnCurrentDevice());
+    int *dCount = reinterpret_cast<int *>(base);
+    int *dIdx = reinterpret_cast<int *>(base + kScratchIdxOff);
+    T *dVal = reinterpret_cast<T *>(base + kScratchValOff);
+    const int candCap = static_cast<int>(n / 2 + 2);
+    hipStream_t stream = nullptr;
+    // the counter keeps the last call's value: a fill kernel clears it
+    MF_HIP(mfk::launch_fill_int(dCount, 1, 0, stream));
+    MF_HIP(mfk::launch_peaks(sizeof(T) == 4 ? 0 : 1, dx, n, threshold, dCount, dIdx, dVal, candCap, stream));
+    int count = 0;
+    MF_HIP(hipMemcpyAsync(&count, dCount, sizeof(int), hipMemcpyDeviceToHost, stream));
+    MF_HIP(hipStreamSynchronize(stream));
+    if (count < 0 || count > candCap) { fail(MF_ERR_DEVICE, "peak candidate count out of range"); }
+    std::vector<int> ci(count);
+    std::vector<T> cv(count);
+    if (count > 0) {
+        MF_HIP(hipMemcpyAsync(ci.data(), dIdx, count * sizeof(int), hipMemcpyDeviceToHost, stream));
+        MF_HIP(hipMemcpyAsync(cv.data(), dVal, count * sizeof(T), hipMemcpyDeviceToHost, stream));
+        MF_HIP(hipStreamSynchronize(stream));
+    }
+    std::vector<std::pair<T, int>> cand(count);
+    for (int i = 0; i < count; ++i) { cand[i] = {cv[i], ci[i]}; }
+    pruneCandidates<T>(cand, minDist, idx, cap, nPeaks);
+}
+
 // Measured cost table (mf_autotune): for every FFT size and kernel variant
 // the correlation launch is timed on 16 synthetic lanes x 128 blocks
 // (2048 work items: the whole GPU) with 1 row, 8 rows of one template
@@ -2373,6 +2437,33 @@ int mf_peaks_device(const void *dx, int n, int precision, double threshold, int 
                     int *nPeaks)
 {
     return guard([&] { peaks(dx, true, n, precision, threshold, minDist, idx, cap, nPeaks); });
+}
+
+int mf_peaks_device_noalloc(const void *dx, int n, int precision, double threshold, int minDist, int *idx, int cap,
+                            int *nPeaks)
+{
+    return guard([&] {
+        if (dx == nullptr) { fail(MF_ERR_INVALID_ARGUMENT, "x is NULL"); }
+        if (n < 1) { fail(MF_ERR_INVALID_ARGUMENT, "No points in x"); }
+        if (nPeaks == nullptr || (idx == nullptr && cap > 0)) { fail(MF_ERR_INVALID_ARGUMENT, "peaks is NULL"); }
+        if (precision != MF_PRECISION_FLOAT && precision != MF_PRECISION_DOUBLE) {
+            fail(MF_ERR_INVALID_ARGUMENT, "unknown precision");
+        }
+        if (threshold < 0 || threshold > 1) {
+            fail(MF_ERR_INVALID_ARGUMENT, "Threshold = " + std::to_string(threshold) + " must be in the range [0,1]");
+        }
+        if (n > kNoAllocMaxSamples) {
+            fail(MF_ERR_INVALID_ARGUMENT, "n = " + std::to_string(n) + " exceeds the scratch limit of " +
+                                              std::to_string(kNoAllocMaxSamples) +
+                                              " samples (use mf_peaks_device)");
+        }
+        std::lock_guard<std::mutex> lock(g_noAllocMutex);
+        if (precision == MF_PRECISION_FLOAT) {
+            peaksNoAlloc<float>(dx, n, static_cast<float>(threshold), minDist, idx, cap, nPeaks);
+        } else {
+            peaksNoAlloc<double>(dx, n, threshold, minDist, idx, cap, nPeaks);
+        }
+    });
 }
 
 int mf_window_scale_device(const void *dx, long long n, int Lt, int scanPrecision, void *dScale, double *ms)

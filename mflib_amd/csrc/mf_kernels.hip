@@ -1062,6 +1062,14 @@ __global__ __launch_bounds__(NT) void push_write_kernel(PushTable tab, long long
     }
 }
 
+__global__ __launch_bounds__(NT) void fill_int_kernel(int *__restrict__ dst, long long n, int value)
+{
+    for (long long i = static_cast<long long>(blockIdx.x) * NT + threadIdx.x; i < n;
+         i += static_cast<long long>(gridDim.x) * NT) {
+        dst[i] = value;
+    }
+}
+
 // out[i] = src[idx[i]]
 __global__ __launch_bounds__(NT) void gather_int_kernel(const int *__restrict__ src, const int *__restrict__ idx,
                                                         int n, int *__restrict__ out)
@@ -1405,6 +1413,15 @@ hipError_t launch_seg_dead(int precision, Seg *dSegs, int nSegs, long long maxN,
     }
     hipLaunchKernelGGL(seg_dead_kernel, dim3((nSegs + NT - 1) / NT), dim3(NT), 0, stream, dSegs, nSegs,
                        static_cast<const double *>(dPart), cps);
+    return hipGetLastError();
+}
+
+hipError_t launch_fill_int(int *dDst, long long n, int value, hipStream_t stream)
+{
+    if (n <= 0) { return hipSuccess; }
+    const long long want = (n + NT - 1) / NT;
+    const int grid = static_cast<int>(want < 4096 ? want : 4096);
+    hipLaunchKernelGGL(fill_int_kernel, dim3(grid), dim3(NT), 0, stream, dDst, n, value);
     return hipGetLastError();
 }
 

@@ -153,6 +153,9 @@ struct PushTable {
 };
 hipError_t launch_push(const PushTable &tab, int nChannels, long long chunkWords, long long carryWords,
                        unsigned *dScratch, hipStream_t stream);
+// dst[i] = value, i < n: clears counters with a kernel node instead of a
+// memset node (safe inside a captured graph).
+hipError_t launch_fill_int(int *dDst, long long n, int value, hipStream_t stream);
 // out[i] = src[idx[i]], i < n (all device pointers).
 hipError_t launch_gather_int(const int *dSrc, const int *dIdx, int n, int *dOut, hipStream_t stream);
 // Threshold + local-maximum candidates (unordered, deterministic set).
