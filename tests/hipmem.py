@@ -67,3 +67,16 @@ class DeviceArray:
         if self.ptr:
             hip().hipFree(self.ptr)
             self.ptr = None
+
+
+def read_elements(ptr: int, dtype, indices) -> np.ndarray:
+    """Copies the elements at `indices` of a device array of `dtype` back to
+    the host (one small copy each)."""
+    dt = np.dtype(dtype)
+    out = np.empty(len(indices), dtype=dt)
+    base = out.ctypes.data
+    for k, i in enumerate(indices):
+        rc = hip().hipMemcpy(base + k * dt.itemsize, ptr + int(i) * dt.itemsize, dt.itemsize, 2)
+        if rc != 0:
+            raise RuntimeError(f"hipMemcpy failed ({rc})")
+    return out
