@@ -159,24 +159,30 @@ def single_channel_section(args, torch):
 
 def single_channel_cpu(state, threads):
     """The oracle's restated SingleChannel MatchedFilter<double> apply and
-    PeakFinder on the host (about 2 s of repeats), and |drho| of the GPU row
-    against the FP64 two-pass Pearson oracle."""
+    PeakFinder on the host (median of 5 runs of 20 repeats, after one
+    warm-up run), and |drho| of the GPU row against the FP64 two-pass
+    Pearson oracle."""
     import oracle as O
     x, t, rho, pk = state
-    reps, t0 = 0, time.perf_counter()
-    while True:
-        ref_rho, nfft, L = O.mf_apply([t], [x], mode="single", dtype=np.float64, threads=threads)
-        ref_pk = O.peak_finder(ref_rho[0], 0.7, 200)
-        reps += 1
-        if time.perf_counter() - t0 > 2.0:
-            break
-    dt = (time.perf_counter() - t0) / reps
+    reps = 20
+    box = {}
+
+    def run():
+        for _ in range(reps):
+            box["rho"], _, _ = O.mf_apply([t], [x], mode="single", dtype=np.float64, threads=threads)
+            box["pk"] = O.peak_finder(box["rho"][0], 0.7, 200)
+
+    dt, runs = _median_timed(run)
+    dt /= reps
+    ref_pk = box["pk"]
     lags = np.random.default_rng(0x51).integers(0, rho.size, 20000)
     exact = O.pearson(t, x, lags)
     return {
         "value": rho.size / dt, "unit": "corr samples/s", "cores": threads, "kind": "port",
-        "sample": f"the whole hour ({rho.size} corr samples), {reps} repeats of apply + PeakFinder",
+        "sample": f"the whole hour ({rho.size} corr samples): median of 5 runs of {reps} repeats of apply + "
+                  "PeakFinder, after 1 warm-up run",
         "ms_per_request": dt * 1e3,
+        "host": host_info(threads),
     }, {
         "max_abs_drho_vs_fp64": float(np.abs(rho[lags] - exact).max()),
         "audited_samples": int(lags.size),
@@ -234,6 +240,13 @@ def uneven_section(args, torch):
     alg_bytes = 4.0 * n_corr + 4.0 * sum(ns)
     sample_rows = list(range(0, len(rows), max(1, len(rows) // 12)))
     batched_rows = {r: h.rho(r) for r in sample_rows}
+    # the stack of every group at each of its injection origins (for the
+    # FP64 audit of the stacks and of what the detector can find)
+    stack_at_origins = {}
+    for t in range(len(wl.templates)):
+        if len(wl.origins[t]):
+            ptr, _ = h.stack_device(t)
+            stack_at_origins[t] = _device_gather_f32(ptr, wl.origins[t])
     h.close()
     log(f"uneven batch: {len(rows)} rows, {n_corr / 1e9:.2f} G corr samples, {dt * 1e3:.1f} ms per step "
         f"(corr {corr_ms:.1f} ms), {n_corr / dt / 1e9:.1f} G corr samples/s, plans {dict(plans)}")
@@ -264,6 +277,7 @@ def uneven_section(args, torch):
     diff = max(float(np.abs(batched_rows[r].astype(np.float64) - plain_rows[r]).max()) for r in sample_rows)
     del row_index
     log(f"uneven batch: plain per-template loop {plain_dt:.2f} s, {total / plain_dt / 1e9:.2f} G corr samples/s")
+    state = (wl, sample_rows, batched_rows, stack_at_origins, peaks)
     return {
         "workload": "uneven batch: 200 templates (200-3000 samples) x 1-50 of 50 channels (1-24 h at 100 Hz), "
                     "moveouts 0-500, 0-30 injections per template, FP32, stack + PeakFinder(0.4, 500)",
@@ -281,7 +295,71 @@ def uneven_section(args, torch):
         "max_abs_drho_batched_vs_plain": diff,
         "injections_recovered": hits,
         "injections": n_inj,
-    }
+    }, state
+
+
+def _device_gather_f32(ptr: int, idx):
+    """Values of a device float32 array at the given indices (one small copy
+    per index through the HIP runtime torch loaded)."""
+    import ctypes
+    path = "libamdhip64.so"
+    with open("/proc/self/maps") as f:       # the runtime torch already loaded
+        for line in f:
+            if "libamdhip64.so" in line.split()[-1]:
+                path = line.split()[-1]
+                break
+    hip = ctypes.CDLL(path)
+    hip.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
+    out = np.empty(len(idx), dtype=np.float32)
+    for k, i in enumerate(np.asarray(idx).tolist()):
+        if hip.hipMemcpy(out.ctypes.data + 4 * k, ptr + 4 * int(i), 4, 2) != 0:
+            raise RuntimeError("hipMemcpy failed")
+    return out
+
+
+def uneven_audit(state):
+    """FP64 oracle audit of the uneven batch: rho of the sampled rows at
+    seeded lags, and every group's stack at its injection origins against
+    the oracle's stack value there (mean over the template's channels of
+    |rho| at origin + moveout).  The oracle's stack values also say which
+    injections the detector can find at all: the generator injects at a
+    peak amplitude of 2-8 x the noise RMS, so many bursts stay below the
+    threshold even in the exact stack."""
+    import oracle as O
+    wl, sample_rows, batched_rows, stack_at_origins, peaks = state
+    rows = wl.rows()
+    rng = np.random.default_rng(0xA0D4)
+    worst, n_aud = 0.0, 0
+    for r in sample_rows:
+        t, c = rows[r]
+        g = batched_rows[r]
+        lags = rng.integers(0, g.size, 2000)
+        ref = O.pearson(wl.templates[t][c], wl.signals[c], lags)
+        worst = max(worst, float(np.abs(g[lags].astype(np.float64) - ref).max()))
+        n_aud += lags.size
+    st_err, above, above_found, margin, margin_found = 0.0, 0, 0, 0, 0
+    for t, got in stack_at_origins.items():
+        chans = sorted(wl.templates[t])
+        ref = np.zeros(len(wl.origins[t]))
+        for c in chans:
+            lags = np.asarray(wl.origins[t], dtype=np.int64) + wl.moveouts[t][c]
+            ref += np.abs(O.pearson(wl.templates[t][c], wl.signals[c], lags))
+        ref /= len(chans)
+        st_err = max(st_err, float(np.abs(got.astype(np.float64) - ref).max()))
+        det = np.asarray(peaks[t])
+        for o, v in zip(wl.origins[t], ref):
+            hit = int(det.size > 0 and np.min(np.abs(det - o)) <= 2)
+            if v > 0.4:
+                above += 1
+                above_found += hit
+            if v > 0.45:
+                margin += 1
+                margin_found += hit
+    return {"max_abs_drho_vs_fp64": worst, "audited_rows": len(sample_rows), "audited_samples": n_aud,
+            "max_abs_dstack_at_origins_vs_fp64": st_err,
+            "injections_with_fp64_stack_above_threshold_0.4": above,
+            "of_those_recovered": above_found,
+            "injections_with_fp64_stack_above_0.45": margin, "of_those_recovered_0.45": margin_found}
 
 
 def streaming_section(args, torch, wl, day_row0):
@@ -594,29 +672,64 @@ def serve_audit(wl, audit):
     return {"max_abs_drho_vs_fp64": worst, "audited_rows": len(audit), "audited_samples": n}
 
 
-def cpu_baseline(wl, threads: int, accuracy_rows, gpu_rho_rows):
+def host_info(threads: int) -> dict:
+    """Host facts the CPU figures depend on."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"threads_used": threads, "nproc": os.cpu_count(), "affinity_cpus": len(os.sched_getaffinity(0)),
+            "omp_num_threads_env": os.environ.get("OMP_NUM_THREADS"), "cpu_model": model}
+
+
+def _median_timed(fn, runs: int = 5):
+    """One untimed warm-up call, then the median wall time of `runs` calls."""
+    fn()
+    ts = []
+    for _ in range(runs):
+        t0 = time.perf_counter()
+        fn()
+        ts.append(time.perf_counter() - t0)
+    return float(np.median(ts)), ts
+
+
+def cpu_baseline(wl, threads: int, accuracy_rows, gpu_rho_rows, sample_samples: int = 720_000):
     """Reference-faithful CPU restatement (oracle) on a bounded sample: one
-    template event (24 channels x full day) in the reference's
-    MatchedFilter<float> multi-channel form.  Also audits the GPU rows
+    template event on all 24 channels over the first `sample_samples`
+    samples (2 h at 100 Hz; the restated sizing picks the same nfft 8192 as
+    for the whole day) in the reference's MatchedFilter<float> multi-channel
+    form; median of 5 runs after one warm-up.  Also audits the GPU rows
     against the FP64 Pearson oracle at seeded lags."""
     import oracle as O
     O.lib()
     ev = 0
     tmpl = wl.templates[ev]
-    t0 = time.perf_counter()
-    rho, nfft, L = O.mf_apply(tmpl, wl.signals, mode="multi", dtype=np.float32, threads=threads)
-    dt = time.perf_counter() - t0
+    sigs = [np.ascontiguousarray(s[:sample_samples]) for s in wl.signals]
+    box = {}
+
+    def ref_threading():
+        box["rho"], box["nfft"], box["L"] = O.mf_apply(tmpl, sigs, mode="multi", dtype=np.float32, threads=threads)
+
+    dt, runs = _median_timed(ref_threading)
+    rho, nfft = box["rho"], box["nfft"]
     n_corr = rho.shape[0] * rho.shape[1]
     # all-cores form: the same event as 24 independent one-channel instances,
     # one per worker, each with one OpenMP thread (ctypes releases the GIL)
     from concurrent.futures import ThreadPoolExecutor
-    t0 = time.perf_counter()
-    with ThreadPoolExecutor(max_workers=threads) as ex:
-        parts = list(ex.map(lambda c: O.mf_apply([tmpl[c]], [wl.signals[c]], mode="multi", dtype=np.float32,
-                                                 threads=1)[0], range(len(tmpl))))
-    dt_all = time.perf_counter() - t0
-    all_same = all(np.array_equal(parts[c][0], rho[c]) for c in range(len(tmpl)))
-    del parts
+
+    def all_cores():
+        with ThreadPoolExecutor(max_workers=threads) as ex:
+            box["parts"] = list(ex.map(lambda c: O.mf_apply([tmpl[c]], [sigs[c]], mode="multi", dtype=np.float32,
+                                                            threads=1)[0], range(len(tmpl))))
+
+    dt_all, runs_all = _median_timed(all_cores)
+    all_same = all(np.array_equal(box["parts"][c][0], rho[c]) for c in range(len(tmpl)))
+    del box["parts"]
     # accuracy audit of the GPU result vs the FP64 oracle
     rng = np.random.default_rng(0xA0D17)
     worst = 0.0
@@ -631,15 +744,19 @@ def cpu_baseline(wl, threads: int, accuracy_rows, gpu_rho_rows):
         "unit": "corr samples/s",
         "cores": threads,
         "kind": "port",
-        "sample": (f"1 of {wl.n_events} template events ({len(tmpl)} channels x {wl.signals[0].size} "
-                   f"samples, {n_corr} corr samples) through the restated MatchedFilter<float> "
-                   f"apply (own radix-2 FFT, nfft={nfft}, serial block loop as mkl_sequential, "
-                   f"OpenMP normalization on {threads} threads); MKL is not available"),
-        "seconds": dt,
-        "all_cores": {"value": n_corr / dt_all, "cores": threads, "seconds": dt_all,
+        "sample": (f"1 of {wl.n_events} template events, {len(tmpl)} channels x the first {sample_samples} "
+                   f"samples ({n_corr} corr samples), through the restated MatchedFilter<float> apply "
+                   f"(own radix-2 FFT, nfft={nfft} as for the whole day, serial block loop as "
+                   f"mkl_sequential, OpenMP normalization on {threads} threads); median of 5 runs after "
+                   "1 warm-up. MKL is not available, so an MKL build of the reference would be faster"),
+        "seconds_median": dt,
+        "seconds_runs": runs,
+        "host": host_info(threads),
+        "all_cores": {"value": n_corr / dt_all, "cores": threads, "seconds_median": dt_all,
+                      "seconds_runs": runs_all,
                       "sample": f"the same event as {len(tmpl)} independent one-channel instances on {threads} "
                                 "worker threads, one OpenMP thread each (a stronger baseline than the "
-                                "reference's own threading)",
+                                "reference's own threading); median of 5 runs after 1 warm-up",
                       "rows_identical_to_reference_threading": bool(all_same)},
     }, {"max_abs_drho_vs_fp64": worst, "audited_samples": n_aud}
 
@@ -699,7 +816,10 @@ def main() -> int:
         print(json.dumps({"precision_stress": out}), flush=True)
         return 0
     if args.uneven_only:
-        print(json.dumps({"uneven_batch": uneven_section(args, torch)}), flush=True)
+        uneven, ustate = uneven_section(args, torch)
+        if not args.no_cpu_baseline:
+            uneven["accuracy"] = uneven_audit(ustate)
+        print(json.dumps({"uneven_batch": uneven}), flush=True)
         return 0
     if args.serve_only:
         serving, (swl, audit) = serve_section(args, torch)
@@ -820,14 +940,17 @@ def main() -> int:
         lat = min(serving["poisson"], key=lambda r: abs(r["lambda_per_s"] - 1000.0))
         out["latency"] = {"p50_ms": lat["p50_ms"], "p99_ms": lat["p99_ms"],
                           "lambda_per_s": lat["lambda_per_s"], "workload": "served stream (see serving)"}
+    uneven_state = None
     if not args.no_uneven:
-        out["uneven_batch"] = uneven_section(args, torch)
+        out["uneven_batch"], uneven_state = uneven_section(args, torch)
     prec_state = None
     if not args.no_precision:
         out["precision_stress"], prec_state = precision_section(args, torch)
     if not args.no_cpu_baseline:
         log("CPU baseline (bounded sample) and accuracy audit")
-        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        # the host threads of this GPU's share of the box: OMP_NUM_THREADS
+        # when the environment sets it, else every CPU this process may use
+        threads = args.cpu_threads or int(os.environ.get("OMP_NUM_THREADS") or len(os.sched_getaffinity(0)))
         rows = [(e, c) for e in (0, E - 1) for c in (0, C // 2, C - 1)]
         cb, acc = cpu_baseline(wl, threads, rows, audit_rows)
         out["cpu_baseline"] = cb
@@ -836,6 +959,8 @@ def main() -> int:
             out["serving"]["accuracy"] = serve_audit(*serve_state)
         if prec_state is not None:
             out["precision_stress"]["accuracy"] = precision_cpu(prec_state)
+        if uneven_state is not None:
+            out["uneven_batch"]["accuracy"] = uneven_audit(uneven_state)
         sc_cpu, sc_acc = single_channel_cpu(sc_state, threads)
         out["single_channel"]["cpu_baseline"] = sc_cpu
         out["single_channel"]["accuracy"] = sc_acc
