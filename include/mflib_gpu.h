@@ -109,15 +109,19 @@ int mf_set_graph_launch(mf_handle *h, int on);
 int mf_set_streaming(mf_handle *h, int on);
 int mf_get_stream_carry(const mf_handle *h, int *carry);
 /* No reference counterpart: one streaming push of every channel at once
- * from device memory (dx[c] holds channel c's chunk of n samples), waiting
- * once for all the copies instead of once per channel. */
+ * (dx[c] holds channel c's chunk of n samples), waiting once for all the
+ * copies instead of once per channel.  When every dx[c] is device memory of
+ * the handle's device, two kernels move all channels; otherwise (host
+ * memory, another device) each chunk goes through the copy engine. */
 int mf_push_chunks_device(mf_handle *h, int nChannels, int n, const void *const *dx);
 /* No reference counterpart (checkpoint / resume of a stream): the carry
  * samples of a channel that its next push continues from (*len = carry;
  * a size query when out is NULL), and the restore of such a state into a
  * streaming handle of the same templates and chunk length.  A handle
  * restored from a snapshot gives the next pushes' rows and stacks
- * bit-identical to the handle the snapshot came from. */
+ * bit-identical to the handle the snapshot came from.  A restore ends the
+ * stream-peak state of every group (mf_set_stream_peaks_state restores
+ * it afterwards). */
 int mf_get_stream_state(mf_handle *h, int channel, void *out, int cap, int *len);
 int mf_set_stream_state(mf_handle *h, int channel, const void *state, int n);
 /* No reference counterpart (PeakFinder over a stream, the per-push form of
@@ -135,10 +139,27 @@ int mf_set_stream_state(mf_handle *h, int channel, const void *state, int n);
  * once no later sample can change it, so it may be returned a few pushes
  * after its own.  When cap is too small the call fails with *nPeaks set and
  * the state unchanged (call again with room).  Needs one chunk length on
- * every channel.  The pending candidates are host state of the handle, not
- * part of mf_get_stream_state. */
+ * every channel.  The pending candidates are host state of the handle:
+ * mf_get_stream_peaks_state saves them; mf_set_stream_state (a channel's
+ * carry) ends every group's stream, so restore the carries first. */
 int mf_stream_peaks(mf_handle *h, int group, double threshold, int minDist, int last, long long *peaks, int cap,
                     int *nPeaks);
+/* No reference counterpart (checkpoint of mf_stream_peaks, the streamed
+ * PeakFinder<T>::apply, src/peakFinder.cpp:468-586): group `group`'s
+ * stream-peak state as opaque bytes (*len bytes; a size query when out is
+ * NULL or cap too small, which fails with MF_ERR_INVALID_ARGUMENT and *len
+ * set), and its restore into a streaming handle of the same layout and
+ * precision whose channel carries were restored with mf_set_stream_state.
+ * The restored group's next mf_stream_peaks call must follow the handle's
+ * next apply; it then continues the saved stream exactly.  Bytes that are
+ * not such a state fail with MF_ERR_INVALID_ARGUMENT and change nothing. */
+int mf_get_stream_peaks_state(mf_handle *h, int group, void *out, long long cap, long long *len);
+int mf_set_stream_peaks_state(mf_handle *h, int group, const void *state, long long n);
+/* No reference counterpart: drops group `group`'s stream-peak state
+ * (group -1: every group) whatever its push count, for instance after a
+ * missed mf_stream_peaks call, which otherwise fails with MF_ERR_BAD_STATE;
+ * the group's next call starts a new stream. */
+int mf_stream_peaks_reset(mf_handle *h, int group);
 /* No reference counterpart: mf_stream_peaks for every group of the handle
  * in one call (one stack launch, one candidate scan, one wait), as
  * mf_detect is to PeakFinder: group g's peaks are

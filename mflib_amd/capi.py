@@ -88,6 +88,10 @@ _SIGNATURES = {
                          ctypes.POINTER(ctypes.c_longlong), ctypes.c_int, _ip], ctypes.c_int),
     "mf_stream_peaks_all": ([_vp, ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_longlong),
                              ctypes.c_int, _ip], ctypes.c_int),
+    "mf_get_stream_peaks_state": ([_vp, ctypes.c_int, _vp, ctypes.c_longlong, ctypes.POINTER(ctypes.c_longlong)],
+                                  ctypes.c_int),
+    "mf_set_stream_peaks_state": ([_vp, ctypes.c_int, _vp, ctypes.c_longlong], ctypes.c_int),
+    "mf_stream_peaks_reset": ([_vp, ctypes.c_int], ctypes.c_int),
     "mf_detect_templates": ([_vp, ctypes.c_int, ctypes.c_double, ctypes.c_int, _ip, ctypes.c_int, _ip],
                             ctypes.c_int),
     "mf_detect_template": ([_vp, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_int, _ip, ctypes.c_int, _ip],
@@ -356,6 +360,25 @@ class Handle:
                 continue
             check(rc, "stream_peaks_all")
             return [out[off[g]:off[g + 1]].copy() for g in range(ng)]
+
+    def stream_peaks_state(self, group: int) -> bytes:
+        """Checkpoint of a group's stream-peak finder (opaque bytes)."""
+        n = ctypes.c_longlong()
+        lib().mf_get_stream_peaks_state(self._h, int(group), None, 0, ctypes.byref(n))
+        if n.value <= 0:
+            check(lib().mf_get_stream_peaks_state(self._h, int(group), None, 0, ctypes.byref(n)),
+                  "stream_peaks_state")
+        buf = ctypes.create_string_buffer(n.value)
+        check(lib().mf_get_stream_peaks_state(self._h, int(group), buf, n.value, ctypes.byref(n)),
+              "stream_peaks_state")
+        return buf.raw[:n.value]
+
+    def set_stream_peaks_state(self, group: int, state: bytes):
+        check(lib().mf_set_stream_peaks_state(self._h, int(group), ctypes.c_char_p(bytes(state)), len(state)),
+              "set_stream_peaks_state")
+
+    def stream_peaks_reset(self, group: int = -1):
+        check(lib().mf_stream_peaks_reset(self._h, int(group)), "stream_peaks_reset")
 
     def initialize(self):
         check(lib().mf_initialize(self._h), "initialize")

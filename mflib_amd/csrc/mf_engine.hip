@@ -1444,6 +1444,19 @@ void pushChunk(mf_handle *h, int channel, int n, const void *x, hipMemcpyKind ki
     if (wait) { MF_HIP(hipStreamSynchronize(h->stream)); }
 }
 
+// True when kernels on `device` can read p directly: device (or managed)
+// memory of that device.  Host pointers and other devices' memory go
+// through the copy engine instead.
+bool readableOnDevice(const void *p, int device)
+{
+    hipPointerAttribute_t a{};
+    if (hipPointerGetAttributes(&a, p) != hipSuccess) {
+        (void)hipGetLastError();  // an unknown (pageable host) pointer: clear the error it leaves
+        return false;
+    }
+    return (a.type == hipMemoryTypeDevice || a.type == hipMemoryTypeManaged) && a.device == device;
+}
+
 void checkChunkLength(const mf_handle *h, int c, int n)
 {
     const long long chunk = h->ch[c].n - h->carry;
@@ -1469,7 +1482,9 @@ int mf_push_chunks_device(mf_handle *h, int nChannels, int n, const void *const 
             if (dx[c] == nullptr) { fail(MF_ERR_INVALID_ARGUMENT, "chunk of channel " + std::to_string(c) + " is NULL"); }
         }
         h->applied = false;
-        if (nChannels <= mfk::kPushMaxChannels) {
+        bool direct = nChannels <= mfk::kPushMaxChannels;
+        for (int c = 0; c < nChannels && direct; ++c) { direct = readableOnDevice(dx[c], h->device); }
+        if (direct) {
             // two launches for every channel instead of three copies per channel
             mfk::PushTable tab{};
             for (int c = 0; c < nChannels; ++c) {
@@ -1481,8 +1496,8 @@ int mf_push_chunks_device(mf_handle *h, int nChannels, int n, const void *const 
             growTo(h->dPushScratch, static_cast<size_t>(nChannels) * static_cast<size_t>(h->carry) * h->tsize);
             MF_HIP(mfk::launch_push(tab, nChannels, n * words, h->carry * words, h->dPushScratch.as<unsigned>(),
                                     h->stream));
-        } else {
-            for (int c = 0; c < nChannels; ++c) { pushChunk(h, c, n, dx[c], hipMemcpyDeviceToDevice, false); }
+        } else {  // copies handle any pointer the runtime can reach
+            for (int c = 0; c < nChannels; ++c) { pushChunk(h, c, n, dx[c], hipMemcpyDefault, false); }
         }
         MF_HIP(hipStreamSynchronize(h->stream));
     });
@@ -1531,6 +1546,9 @@ int mf_set_stream_state(mf_handle *h, int channel, const void *state, int n)
                               static_cast<size_t>(n) * h->tsize, hipMemcpyHostToDevice, h->stream));
         MF_HIP(hipStreamSynchronize(h->stream));
         h->applied = false;
+        // the stacks now continue another stream position: every group's
+        // stream-peak finder starts over (or is restored after the carries)
+        for (auto &st : h->streamPeaks) { st = mf_handle::StreamPeakState{}; }
     });
 }
 
@@ -1756,6 +1774,123 @@ void streamPeaksOneT(mf_handle *h, int group, double threshold, int minDist, boo
 
 }  // namespace
 }  // extern "C++"
+
+extern "C++" {
+namespace {
+
+// Checkpoint of one group's stream-peak finder: a small header (whether a
+// stream is open, how many pushes it has had, the threshold and minimum
+// distance it was opened with) followed by the finder's own state.
+constexpr uint32_t kStreamPeakMagic = 0x4753464Du;  // "MFSG"
+struct StreamPeakHeader {
+    uint32_t magic, version, active, tsize;
+    int64_t pushes;
+    double threshold;
+    int32_t minDist, pad;
+};
+
+template <typename T>
+void getStreamPeakStateT(mf_handle *h, int group, std::vector<unsigned char> &out)
+{
+    auto &st = h->streamPeaks[group];
+    StreamPeakHeader hd{kStreamPeakMagic, 1, st.active ? 1u : 0u, static_cast<uint32_t>(sizeof(T)), 0, 0.0, 0, 0};
+    std::vector<unsigned char> body;
+    if (st.active) {
+        hd.pushes = st.lastApply - st.origin + 1;
+        hd.threshold = st.threshold;
+        hd.minDist = st.minDist;
+        statePtr<T>(st)->save(body);
+    }
+    out.resize(sizeof(hd) + body.size());
+    std::memcpy(out.data(), &hd, sizeof(hd));
+    if (!body.empty()) { std::memcpy(out.data() + sizeof(hd), body.data(), body.size()); }
+}
+
+template <typename T>
+void setStreamPeakStateT(mf_handle *h, int group, const unsigned char *p, size_t n)
+{
+    StreamPeakHeader hd{};
+    if (n < sizeof(hd)) { fail(MF_ERR_INVALID_ARGUMENT, "stream-peak state too short"); }
+    std::memcpy(&hd, p, sizeof(hd));
+    if (hd.magic != kStreamPeakMagic || hd.version != 1 || hd.active > 1) {
+        fail(MF_ERR_INVALID_ARGUMENT, "not a stream-peak state");
+    }
+    if (hd.tsize != sizeof(T)) { fail(MF_ERR_INVALID_ARGUMENT, "stream-peak state of another precision"); }
+    mf_handle::StreamPeakState st{};
+    if (hd.active) {
+        if (hd.pushes < 1 || hd.minDist < 0 || !(hd.threshold >= 0 && hd.threshold <= 1)) {
+            fail(MF_ERR_INVALID_ARGUMENT, "stream-peak state out of range");
+        }
+        auto sp = std::make_unique<mfstream::StreamPeaks<T>>(T(0), 0);
+        if (!sp->load(p + sizeof(hd), n - sizeof(hd))) { fail(MF_ERR_INVALID_ARGUMENT, "corrupt stream-peak state"); }
+        st.active = true;
+        // the next call must follow the next apply, as push number `pushes`
+        st.lastApply = h->applies;
+        st.origin = h->applies - hd.pushes + 1;
+        st.threshold = hd.threshold;
+        st.minDist = hd.minDist;
+        statePtr<T>(st) = std::move(sp);
+    } else if (n != sizeof(hd)) {
+        fail(MF_ERR_INVALID_ARGUMENT, "stream-peak state has trailing bytes");
+    }
+    h->streamPeaks[group] = std::move(st);
+}
+
+void checkStreamPeakGroup(mf_handle *h, int group, bool allowAll)
+{
+    requireInit(h);
+    if (!h->streaming) { fail(MF_ERR_BAD_STATE, "stream peaks need streaming mode"); }
+    const int nG = static_cast<int>(h->groups.size());
+    if (group < (allowAll ? -1 : 0) || group >= nG) {
+        fail(MF_ERR_INVALID_ARGUMENT, "group " + std::to_string(group) + " must be in [0, " + std::to_string(nG) + ")");
+    }
+    if (h->streamPeaks.size() != h->groups.size()) { h->streamPeaks.resize(h->groups.size()); }
+}
+
+}  // namespace
+}  // extern "C++"
+
+int mf_get_stream_peaks_state(mf_handle *h, int group, void *out, long long cap, long long *len)
+{
+    return guard([&] {
+        checkStreamPeakGroup(h, group, false);
+        std::vector<unsigned char> blob;
+        if (h->precision == MF_PRECISION_FLOAT) {
+            getStreamPeakStateT<float>(h, group, blob);
+        } else {
+            getStreamPeakStateT<double>(h, group, blob);
+        }
+        if (len) { *len = static_cast<long long>(blob.size()); }
+        if (out == nullptr || cap < static_cast<long long>(blob.size())) {
+            fail(MF_ERR_INVALID_ARGUMENT, "state needs " + std::to_string(blob.size()) + " bytes");
+        }
+        std::memcpy(out, blob.data(), blob.size());
+    });
+}
+
+int mf_set_stream_peaks_state(mf_handle *h, int group, const void *state, long long n)
+{
+    return guard([&] {
+        checkStreamPeakGroup(h, group, false);
+        if (state == nullptr || n <= 0) { fail(MF_ERR_INVALID_ARGUMENT, "state is empty"); }
+        const auto *p = static_cast<const unsigned char *>(state);
+        if (h->precision == MF_PRECISION_FLOAT) {
+            setStreamPeakStateT<float>(h, group, p, static_cast<size_t>(n));
+        } else {
+            setStreamPeakStateT<double>(h, group, p, static_cast<size_t>(n));
+        }
+    });
+}
+
+int mf_stream_peaks_reset(mf_handle *h, int group)
+{
+    return guard([&] {
+        checkStreamPeakGroup(h, group, true);
+        for (int g = 0; g < static_cast<int>(h->streamPeaks.size()); ++g) {
+            if (group < 0 || g == group) { h->streamPeaks[g] = mf_handle::StreamPeakState{}; }
+        }
+    });
+}
 
 int mf_stream_peaks(mf_handle *h, int group, double threshold, int minDist, int last, long long *peaks, int cap,
                     int *nPeaks)
@@ -2484,9 +2619,9 @@ int mf_window_scale_device(const void *dx, long long n, int Lt, int scanPrecisio
         MF_HIP(hipStreamCreate(&st));
         hipEvent_t e0 = nullptr, e1 = nullptr;
         auto cleanup = [&] {
-            if (e0) { hipEventDestroy(e0); }
-            if (e1) { hipEventDestroy(e1); }
-            hipStreamDestroy(st);
+            if (e0) { (void)hipEventDestroy(e0); }
+            if (e1) { (void)hipEventDestroy(e1); }
+            (void)hipStreamDestroy(st);
         };
         try {
             MF_HIP(hipEventCreate(&e0));

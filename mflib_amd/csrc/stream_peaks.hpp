@@ -25,6 +25,7 @@
 
 #include <algorithm>
 #include <cstdint>
+#include <cstring>
 #include <limits>
 #include <set>
 #include <utility>
@@ -90,7 +91,100 @@ public:
         settle(std::numeric_limits<long long>::max(), out);
     }
 
+    // Checkpoint: the whole pending state as bytes, and its restore.
+    // Values are stored as double (exact for float).  Layout: magic, version,
+    // sizeof(T), minDist, threshold, the two held-back samples and their
+    // flags, the last index seen, then every live candidate (index, value,
+    // fate, emitted).
+    void save(std::vector<unsigned char> &out) const
+    {
+        out.clear();
+        put<uint32_t>(out, kMagic);
+        put<uint32_t>(out, kVersion);
+        put<uint32_t>(out, static_cast<uint32_t>(sizeof(T)));
+        put<int32_t>(out, md_);
+        put<double>(out, static_cast<double>(thr_));
+        put<double>(out, static_cast<double>(prev1_));
+        put<double>(out, static_cast<double>(prev2_));
+        put<int64_t>(out, lastIdx_);
+        put<uint32_t>(out, (havePrev_ ? 1u : 0u) | (havePrev2_ ? 2u : 0u));
+        put<uint32_t>(out, 0);
+        put<int64_t>(out, static_cast<int64_t>(live_.size()));
+        for (const auto &c : live_) {
+            put<int64_t>(out, c.idx);
+            put<double>(out, static_cast<double>(c.val));
+            put<int32_t>(out, c.state);
+            put<int32_t>(out, c.emitted ? 1 : 0);
+        }
+    }
+
+    // Restores a state written by save() for the same T; false (and the
+    // state unchanged) when the bytes are not such a state.
+    bool load(const unsigned char *p, size_t n)
+    {
+        size_t at = 0;
+        uint32_t magic = 0, version = 0, tsize = 0, flags = 0, pad = 0;
+        int32_t md = 0;
+        double thr = 0, p1 = 0, p2 = 0;
+        int64_t last = 0, count = 0;
+        if (!get(p, n, at, magic) || !get(p, n, at, version) || !get(p, n, at, tsize) || !get(p, n, at, md) ||
+            !get(p, n, at, thr) || !get(p, n, at, p1) || !get(p, n, at, p2) || !get(p, n, at, last) ||
+            !get(p, n, at, flags) || !get(p, n, at, pad) || !get(p, n, at, count)) {
+            return false;
+        }
+        if (magic != kMagic || version != kVersion || tsize != sizeof(T) || md < 0 || flags > 3 || count < 0 ||
+            !(thr >= 0 && thr <= 1)) {
+            return false;
+        }
+        constexpr size_t kEntry = 8 + 8 + 4 + 4;
+        if (n - at != static_cast<size_t>(count) * kEntry) { return false; }
+        std::vector<Cand> live;
+        live.reserve(static_cast<size_t>(count));
+        for (int64_t i = 0; i < count; ++i) {
+            int64_t idx = 0;
+            double v = 0;
+            int32_t st = 0, em = 0;
+            get(p, n, at, idx);
+            get(p, n, at, v);
+            get(p, n, at, st);
+            get(p, n, at, em);
+            if (st < 0 || st > 2 || em < 0 || em > 1 || idx > last || (!live.empty() && idx <= live.back().idx)) {
+                return false;
+            }
+            live.push_back(Cand{idx, static_cast<T>(v), static_cast<int8_t>(st), em != 0});
+        }
+        thr_ = static_cast<T>(thr);
+        md_ = md;
+        prev1_ = static_cast<T>(p1);
+        prev2_ = static_cast<T>(p2);
+        lastIdx_ = last;
+        havePrev_ = (flags & 1u) != 0;
+        havePrev2_ = (flags & 2u) != 0;
+        live_ = std::move(live);
+        return true;
+    }
+
 private:
+    static constexpr uint32_t kMagic = 0x5053464Du;  // "MFSP"
+    static constexpr uint32_t kVersion = 1;
+
+    template <typename V>
+    static void put(std::vector<unsigned char> &out, V v)
+    {
+        const size_t at = out.size();
+        out.resize(at + sizeof(V));
+        std::memcpy(out.data() + at, &v, sizeof(V));
+    }
+
+    template <typename V>
+    static bool get(const unsigned char *p, size_t n, size_t &at, V &v)
+    {
+        if (n < sizeof(V) || at > n - sizeof(V)) { return false; }
+        std::memcpy(&v, p + at, sizeof(V));
+        at += sizeof(V);
+        return true;
+    }
+
     struct Cand {
         long long idx;
         T val;

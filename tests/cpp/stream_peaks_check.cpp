@@ -2,8 +2,10 @@
 // feeds a signal in the given segments, each with the candidates a scan of
 // the segment alone finds (threshold and local maximum with the end rules
 // applied at the segment's own ends, as the device scan does), then flushes.
-// Input file: int32 precision (0 float, 1 double), minDist, nSeg, n; double
-// threshold; int32 segment lengths[nSeg]; n samples.  Output file: int64
+// Input file: int32 precision (0 float, 1 double), minDist, nSeg, n, ckpt;
+// double threshold; int32 segment lengths[nSeg]; n samples.  With ckpt > 0
+// the state goes through save() into a new object's load() after every
+// ckpt-th segment (a checkpoint and resume).  Output file: int64
 // count, then the peaks in the order the pushes returned them, then int64
 // pushes-with-output-out-of-order (0 when every push's peaks follow the last).
 #include <cstdio>
@@ -34,7 +36,7 @@ static std::vector<std::pair<T, int>> scan(const T *x, int n, T thr)
 }
 
 template <typename T>
-static int run(FILE *f, FILE *g, int md, int nSeg, int n, double thr)
+static int run(FILE *f, FILE *g, int md, int nSeg, int n, int ckpt, double thr)
 {
     std::vector<int> seg(nSeg);
     std::vector<T> x(n);
@@ -52,6 +54,21 @@ static int run(FILE *f, FILE *g, int md, int nSeg, int n, double thr)
                 out);
         if (before > 0 && out.size() > before && out[before] <= out[before - 1]) { ++disorder; }
         g0 += m;
+        if (ckpt > 0 && (s + 1) % ckpt == 0) {
+            std::vector<unsigned char> blob;
+            sp.save(blob);
+            mfstream::StreamPeaks<T> resumed(T(0), 0);
+            if (!resumed.load(blob.data(), blob.size())) { return 4; }
+            std::vector<unsigned char> again;
+            resumed.save(again);
+            if (again != blob) { return 5; }
+            // a truncated or altered state is refused and leaves the object as it was
+            if (resumed.load(blob.data(), blob.size() - 1)) { return 6; }
+            std::vector<unsigned char> bad(blob);
+            bad[0] ^= 1u;
+            if (resumed.load(bad.data(), bad.size())) { return 6; }
+            sp = resumed;
+        }
     }
     if (g0 != n) { return 3; }
     const size_t before = out.size();
@@ -70,11 +87,11 @@ int main(int argc, char **argv)
     FILE *f = fopen(argv[1], "rb");
     FILE *g = fopen(argv[2], "wb");
     if (!f || !g) { return 1; }
-    int hdr[4];
+    int hdr[5];
     double thr;
-    if (fread(hdr, sizeof(int), 4, f) != 4 || fread(&thr, sizeof(double), 1, f) != 1) { return 2; }
-    const int rc = hdr[0] == 0 ? run<float>(f, g, hdr[1], hdr[2], hdr[3], thr)
-                               : run<double>(f, g, hdr[1], hdr[2], hdr[3], thr);
+    if (fread(hdr, sizeof(int), 5, f) != 5 || fread(&thr, sizeof(double), 1, f) != 1) { return 2; }
+    const int rc = hdr[0] == 0 ? run<float>(f, g, hdr[1], hdr[2], hdr[3], hdr[4], thr)
+                               : run<double>(f, g, hdr[1], hdr[2], hdr[3], hdr[4], thr);
     fclose(f);
     fclose(g);
     return rc;

@@ -27,11 +27,11 @@ def driver(tmp_path_factory):
     return exe
 
 
-def _stream(driver, tmp_path, x, segs, thr, md):
+def _stream(driver, tmp_path, x, segs, thr, md, ckpt=0):
     prec = 0 if x.dtype == np.float32 else 1
     inp, outp = tmp_path / "in.bin", tmp_path / "out.bin"
     with open(inp, "wb") as f:
-        np.array([prec, md, len(segs), len(x)], np.int32).tofile(f)
+        np.array([prec, md, len(segs), len(x), ckpt], np.int32).tofile(f)
         np.array([thr], np.float64).tofile(f)
         np.asarray(segs, np.int32).tofile(f)
         x.tofile(f)
@@ -89,3 +89,20 @@ def test_stream_long_chain_settles_late(driver, tmp_path):
     got, disorder = _stream(driver, tmp_path, x, [37] * (n // 37) + [n % 37], 0.5, 15)
     assert list(got) == list(O.peak_finder(x, 0.5, 15))
     assert disorder == 0
+
+
+@pytest.mark.parametrize("ckpt", [1, 3, 7])
+def test_stream_checkpoint_and_resume(driver, tmp_path, ckpt):
+    """The state saved after every ckpt-th segment and loaded into a new
+    finder (which then carries on) gives the uninterrupted list; the saved
+    bytes survive a round trip, and truncated or altered bytes are refused."""
+    rng = np.random.default_rng(40 + ckpt)
+    n = 900
+    x = np.round(rng.uniform(0, 1, n), 1)
+    for dtype in (np.float64, np.float32):
+        xs = x.astype(dtype)
+        segs = _segments(rng, n, "mixed")
+        for md in (0, 4, 40, 200):
+            got, disorder = _stream(driver, tmp_path, xs, segs, 0.3, md, ckpt=ckpt)
+            assert list(got) == list(O.peak_finder(xs, 0.3, md)), (md, dtype)
+            assert disorder == 0

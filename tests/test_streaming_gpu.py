@@ -468,3 +468,158 @@ def test_stream_peaks_retry_after_a_full_output_keeps_the_state():
     stack = np.concatenate(tails)[carry:]
     assert list(np.concatenate(got)) == list(O.peak_finder(stack, 0.0, 0))
     h.close()
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_stream_peaks_checkpoint_and_resume(dtype):
+    """A checkpoint after push 3 of 8 (every channel's carry, then every
+    group's stream-peak state) restored into a fresh handle: the restored
+    handle's peak lists for pushes 4..8 equal the original's, and the
+    original's lists over all pushes equal the oracle's PeakFinder on the
+    stream's whole stack.  A minDist longer than a push keeps candidates
+    pending across the checkpoint."""
+    rng = np.random.default_rng(107)
+    lens, shifts = [200, 150, 256, 200], [0, 11, 0, 5]
+    tpls = [rng.standard_normal(L) * np.hanning(L) for L in lens]
+    chunk, pushes, cut = 1500, 8, 3
+    sigs = [_signal(rng, chunk * pushes, 2.0 + 7 * c) for c in range(2)]
+    for c in range(2):
+        for pos in (1300, 4450, 4600, 7400, 10100):          # events around the checkpoint
+            sigs[c][pos:pos + 256] += 2.0 * np.hanning(256) * rng.standard_normal(256)
+    sigs = [s.astype(dtype) for s in sigs]
+
+    def fresh():
+        h = capi.Handle(dtype, "batched")
+        h.set_templates(tpls, shifts=shifts, weights=[1.0, 0.5, 1.0, 1.0])
+        h.set_batch_layout([chunk, chunk], [0, 1, 0, 1], [0, 0, 1, 1], 2)
+        h.set_streaming(True)
+        h.initialize()
+        return h
+
+    def push(h, p):
+        for c in range(2):
+            h.set_signal(c, sigs[c][p * chunk:(p + 1) * chunk])
+        h.apply()
+        return h.stream_peaks_all(0.1, 1700, last=(p == pushes - 1))
+
+    a = fresh()
+    carry = a.stream_carry()
+    got_a = [[] for _ in range(2)]
+    tails = [[] for _ in range(2)]
+    leads = [0, 0]
+    for p in range(pushes):
+        for g, pk in enumerate(push(a, p)):
+            got_a[g].append(pk)
+            st = a.shift_and_stack(g)
+            leads[g] = carry - (len(st) - chunk)
+            tails[g].append(st[-chunk:])
+        if p == cut - 1:
+            carries = [a.stream_state(c) for c in range(2)]
+            peak_states = [a.stream_peaks_state(g) for g in range(2)]
+    b = fresh()
+    for c in range(2):
+        b.set_stream_state(c, carries[c])
+    for g in range(2):
+        b.set_stream_peaks_state(g, peak_states[g])
+    for p in range(cut, pushes):
+        for g, pk in enumerate(push(b, p)):
+            assert np.array_equal(pk, got_a[g][p]), (p, g)
+    for g in range(2):
+        stack = np.concatenate(tails[g])[leads[g]:]
+        ref = O.peak_finder(stack, 0.1, 1700)
+        assert len(ref) >= 3
+        assert list(np.concatenate(got_a[g])) == list(ref), g
+    # bytes that are not a stream-peak state change nothing
+    with pytest.raises(ValueError):
+        b.set_stream_peaks_state(0, peak_states[0][:-3])
+    with pytest.raises(ValueError):
+        b.set_stream_peaks_state(0, b"\0" * len(peak_states[0]))
+    a.close()
+    b.close()
+
+
+def test_stream_peaks_reset_after_a_missed_call():
+    """A push whose mf_stream_peaks call was missed leaves the group's stream
+    stuck (the next call fails with BAD_STATE, last=1 included);
+    mf_stream_peaks_reset drops the stream and the next call starts a new
+    one, whose peaks equal PeakFinder over the stack from that push on.  A
+    carry restore (mf_set_stream_state) also ends every group's stream."""
+    rng = np.random.default_rng(109)
+    tpls = [rng.standard_normal(L) * np.hanning(L) for L in (220, 180)]
+    chunk = 2000
+    sigs = [_signal(rng, chunk * 9, 1.0 + 5 * c).astype(np.float32) for c in range(2)]
+    for c in range(2):
+        for pos in (9100, 12500, 15800):
+            sigs[c][pos:pos + len(tpls[c])] += 3.0 * tpls[c]
+    h = capi.Handle(np.float32, "multi")
+    h.set_templates(tpls, shifts=[0, 4], weights=[1.0, 1.0])
+    h.set_signal_size(chunk)
+    h.set_streaming(True)
+    h.initialize()
+    carry = h.stream_carry()
+
+    def push(p):
+        for c in range(2):
+            h.set_signal(c, sigs[c][p * chunk:(p + 1) * chunk])
+        h.apply()
+
+    push(0)
+    h.stream_peaks(0, 0.2, 300)
+    push(1)                                            # no stream_peaks call for this push
+    push(2)
+    with pytest.raises(RuntimeError):
+        h.stream_peaks(0, 0.2, 300)
+    with pytest.raises(RuntimeError):
+        h.stream_peaks(0, 0.2, 300, last=True)
+    h.stream_peaks_reset(0)
+    got, tails = [], []
+    for p in range(3, 9):
+        push(p)
+        got.append(h.stream_peaks(0, 0.2, 300, last=(p == 8)))
+        tails.append(h.shift_and_stack(0)[-chunk:])
+    stack = np.concatenate(tails)[carry:]               # lag 0 of the new stream: push 3's first window
+    ref = O.peak_finder(stack, 0.2, 300)
+    assert len(ref) >= 2
+    assert list(np.concatenate(got)) == list(ref)
+    # a carry restore mid-stream ends the group's stream: the next call opens a new one
+    push(0)
+    h.stream_peaks(0, 0.2, 300)
+    state = h.stream_state(0)
+    h.set_stream_state(0, state)
+    push(1)
+    push(2)
+    h.stream_peaks(0, 0.2, 300)                        # no BAD_STATE: a new stream
+    h.stream_peaks_reset(-1)
+    h.close()
+
+
+def test_batched_push_from_host_memory_equals_device_push():
+    """mf_push_chunks_device with host pointers goes through the copy engine
+    and gives the same rows as the two-kernel push of device chunks."""
+    rng = np.random.default_rng(113)
+    tpls = [rng.standard_normal(L) for L in (240, 180)]
+    chunk = 3000
+    sigs = [_signal(rng, chunk * 3, 3.0 * c).astype(np.float32) for c in range(2)]
+
+    def fresh():
+        h = capi.Handle(np.float32, "multi")
+        h.set_templates(tpls, shifts=[0, 5], weights=[1.0, 1.0])
+        h.set_signal_size(chunk)
+        h.set_streaming(True)
+        h.initialize()
+        return h
+
+    a, b = fresh(), fresh()
+    for p in range(3):
+        host = [np.ascontiguousarray(sigs[c][p * chunk:(p + 1) * chunk]) for c in range(2)]
+        dev = [DeviceArray(x) for x in host]
+        a.push_chunks_device([d.ptr for d in dev], chunk)
+        b.push_chunks_device([x.ctypes.data for x in host], chunk)
+        for d in dev:
+            d.free()
+        a.apply()
+        b.apply()
+        for r in range(2):
+            assert np.array_equal(a.rho(r), b.rho(r)), (p, r)
+    a.close()
+    b.close()
