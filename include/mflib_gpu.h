@@ -29,6 +29,7 @@ extern "C" {
 #define MF_ERR_OUT_OF_MEMORY (-3)
 #define MF_ERR_DEVICE (-4)
 #define MF_ERR_UNSUPPORTED (-5)
+#define MF_ERR_CANCELLED (-6)
 
 /* Arithmetic type of signals and correlations: MatchedFilter<float> /
  * MatchedFilter<double> (src/matchedFilter.cpp:1434-1435). */
@@ -420,8 +421,20 @@ int mf_server_get_rho_device(mf_server *s, long long requestId, int templateInde
 int mf_server_latency(mf_server *s, long long requestId, double *queuedMs, double *totalMs);
 /* Frees the request's HBM; waiting requests may then be admitted. */
 int mf_server_release(mf_server *s, long long requestId);
+/* Cancels a request that has not been launched yet: *cancelled = 1, its
+ * waiters return with status MF_ERR_CANCELLED, and requests queued behind
+ * it may be admitted.  A request already on the GPU (or done) runs to
+ * completion: *cancelled = 0.  Either way release it afterwards. */
+int mf_server_cancel(mf_server *s, long long requestId, int *cancelled);
 /* Batches launched, requests completed, and correlation samples completed. */
 int mf_server_stats(mf_server *s, long long *batches, long long *requests, double *corrSamples);
+
+/* No reference counterpart (tests of failure handling): the next n device
+ * allocations the library makes, in any handle or server of this process,
+ * fail with MF_ERR_OUT_OF_MEMORY as a failed hipMalloc would; n = 0
+ * disarms.  *remaining (may be NULL) receives the count still armed
+ * before this call. */
+int mf_debug_fail_allocations(int n, int *remaining);
 
 #ifdef __cplusplus
 }

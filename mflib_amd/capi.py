@@ -24,6 +24,7 @@ MF_ERR_BAD_STATE = -2
 MF_ERR_OUT_OF_MEMORY = -3
 MF_ERR_DEVICE = -4
 MF_ERR_UNSUPPORTED = -5
+MF_ERR_CANCELLED = -6
 PRECISION = {np.dtype(np.float32): 0, np.dtype(np.float64): 1}
 MODES = {"single": 0, "multi": 1, "batched": 2}
 
@@ -116,6 +117,8 @@ _SIGNATURES = {
                                  ctypes.c_int),
     "mf_server_latency": ([_vp, ctypes.c_longlong, _dp, _dp], ctypes.c_int),
     "mf_server_release": ([_vp, ctypes.c_longlong], ctypes.c_int),
+    "mf_server_cancel": ([_vp, ctypes.c_longlong, _ip], ctypes.c_int),
+    "mf_debug_fail_allocations": ([ctypes.c_int, _ip], ctypes.c_int),
     "mf_server_stats": ([_vp, ctypes.POINTER(ctypes.c_longlong), ctypes.POINTER(ctypes.c_longlong), _dp],
                         ctypes.c_int),
 }
@@ -153,6 +156,14 @@ def check(rc: int, what: str = "") -> None:
     if rc == MF_ERR_INVALID_ARGUMENT:
         raise ValueError(msg)
     raise RuntimeError(f"[{rc}] {msg}")
+
+
+def debug_fail_allocations(n: int) -> int:
+    """Arms n injected device-allocation failures (0 disarms); returns the
+    count that was still armed."""
+    rem = ctypes.c_int()
+    check(lib().mf_debug_fail_allocations(int(n), ctypes.byref(rem)), "mf_debug_fail_allocations")
+    return rem.value
 
 
 def compute_optimal_fft_and_block_length(nb: int, nx: int):
@@ -668,6 +679,20 @@ class Server:
     def release(self, rid: int) -> None:
         check(lib().mf_server_release(self._h, rid), "mf_server_release")
         self._keep.pop(rid, None)
+
+    def cancel(self, rid: int) -> bool:
+        """Cancels a request not yet launched; False when it already runs
+        (or ran) on the GPU."""
+        c = ctypes.c_int()
+        check(lib().mf_server_cancel(self._h, rid, ctypes.byref(c)), "mf_server_cancel")
+        return bool(c.value)
+
+    def status(self, rid: int) -> int:
+        """Waits for the request and returns its status code (MF_OK, or the
+        failure code, MF_ERR_CANCELLED for a cancelled request)."""
+        st = ctypes.c_int()
+        check(lib().mf_server_wait(self._h, rid, ctypes.byref(st)), "mf_server_wait")
+        return st.value
 
     def stats(self):
         b, r, c = ctypes.c_longlong(), ctypes.c_longlong(), ctypes.c_double()

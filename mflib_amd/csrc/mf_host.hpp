@@ -8,6 +8,7 @@
 #include "mf_kernels.hpp"
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <mutex>
 #include <new>
@@ -58,6 +59,25 @@ int guard(F &&f)
 }
 
 // RAII device allocation.
+// Allocation-failure injection for tests of the failure paths
+// (mf_debug_fail_allocations): while positive, each device allocation of
+// the library takes one and fails as out of memory.
+inline std::atomic<int> &injectedAllocFailures()
+{
+    static std::atomic<int> n{0};
+    return n;
+}
+
+inline bool takeInjectedAllocFailure()
+{
+    auto &n = injectedAllocFailures();
+    int v = n.load();
+    while (v > 0) {
+        if (n.compare_exchange_weak(v, v - 1)) { return true; }
+    }
+    return false;
+}
+
 struct DevBuf {
     void *p = nullptr;
     size_t bytes = 0;
@@ -81,6 +101,9 @@ struct DevBuf {
     {
         release();
         if (nb == 0) { return; }
+        if (takeInjectedAllocFailure()) {
+            fail(MF_ERR_OUT_OF_MEMORY, "hipMalloc of " + std::to_string(nb) + " bytes failed (injected failure)");
+        }
         hipError_t e = hipMalloc(&p, nb);
         if (e != hipSuccess) {
             p = nullptr;

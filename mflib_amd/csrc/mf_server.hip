@@ -16,6 +16,7 @@
 #include "mf_host.hpp"
 #include "mf_kernels.hpp"
 
+#include <algorithm>
 #include <chrono>
 #include <cstdint>
 #include <cstring>
@@ -37,7 +38,7 @@ double msBetween(Clock::time_point a, Clock::time_point b)
     return std::chrono::duration<double, std::milli>(b - a).count();
 }
 
-enum class ReqState { Pending, Launched, Done, Failed };
+enum class ReqState { Pending, Launched, Done, Failed };  // Failed includes cancelled
 
 struct LibTemplate {
     int Lt = 0;
@@ -650,6 +651,35 @@ int mf_server_release(mf_server *s, long long id)
         s->reqs.erase(id);
         ++s->memEpoch;
         s->cvSched.notify_one();
+    });
+}
+
+int mf_server_cancel(mf_server *s, long long id, int *cancelled)
+{
+    return guard([&] {
+        if (s == nullptr) { fail(MF_ERR_INVALID_ARGUMENT, "server is NULL"); }
+        std::lock_guard<std::mutex> lk(s->mu);
+        Request *r = findRequest(s, id);
+        const bool pending = r->state == ReqState::Pending;
+        if (pending) {
+            s->pending.erase(std::find(s->pending.begin(), s->pending.end(), r));
+            r->state = ReqState::Failed;
+            r->status = MF_ERR_CANCELLED;
+            r->tDone = Clock::now();
+            ++s->memEpoch;  // the queue head may have changed: admission retries
+            s->cvDone.notify_all();
+            s->cvSched.notify_one();
+        }
+        if (cancelled) { *cancelled = pending ? 1 : 0; }
+    });
+}
+
+int mf_debug_fail_allocations(int n, int *remaining)
+{
+    return guard([&] {
+        if (n < 0) { fail(MF_ERR_INVALID_ARGUMENT, "n must be non-negative"); }
+        const int before = injectedAllocFailures().exchange(n);
+        if (remaining) { *remaining = before; }
     });
 }
 

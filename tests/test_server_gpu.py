@@ -171,3 +171,85 @@ def test_request_errors():
     with pytest.raises(ValueError):
         srv.wait(rid)                                       # released
     srv.close()
+
+
+def test_cancel_pending_requests():
+    """A request queued behind one that holds the whole budget is cancelled
+    before launch: its waiters see MF_ERR_CANCELLED, the request behind it
+    is admitted once memory frees up and matches the oracle.  A request
+    that already ran cannot be cancelled."""
+    rng = np.random.default_rng(506)
+    tmpls = _library(rng, 3, 300, 600)
+    n = 200_000
+    # room for one request's rows (3 templates x 2 channels x n floats) at a time
+    srv = capi.Server(np.float32, budget_bytes=3 * 2 * n * 4 * 1.5 + (2 << 20), max_batch=4, max_inflight=1)
+    ids = srv.add_templates(tmpls)
+    chans = [_channels(rng, 2, n) for _ in range(3)]
+    a = srv.submit(chans[0], ids)
+    srv.wait(a)                                   # a holds the arena until released
+    b = srv.submit(chans[1], ids)
+    c = srv.submit(chans[2], ids)
+    assert not srv.done(b) and not srv.done(c)
+    assert srv.cancel(b)
+    assert srv.status(b) == capi.MF_ERR_CANCELLED
+    with pytest.raises(RuntimeError):
+        srv.rho(b, 0, 0)
+    assert not srv.cancel(a)                      # already done
+    srv.release(a)
+    srv.wait(c)
+    _check_against_oracle(srv, c, tmpls, ids, chans[2], lags_per_row=300, rng=rng)
+    srv.release(b)
+    srv.release(c)
+    b2, r2, _ = srv.stats()
+    assert r2 == 2
+    srv.close()
+
+
+def test_injected_allocation_failure_fails_one_batch_only():
+    """The first batch's descriptor allocation fails (injected from the
+    host): its request ends with MF_ERR_OUT_OF_MEMORY and the server keeps
+    serving; the next request is correct."""
+    rng = np.random.default_rng(507)
+    tmpls = _library(rng, 4, 200, 900)
+    srv = capi.Server(np.float32, budget_bytes=1 << 28, max_batch=8, max_inflight=1)
+    ids = srv.add_templates(tmpls)
+    chans = _channels(rng, 2, 30000)
+    try:
+        capi.debug_fail_allocations(1)
+        a = srv.submit(chans, ids)
+        assert srv.status(a) == capi.MF_ERR_OUT_OF_MEMORY
+    finally:
+        capi.debug_fail_allocations(0)
+    srv.release(a)
+    b = srv.submit(chans, ids)
+    srv.wait(b)
+    _check_against_oracle(srv, b, tmpls, ids, chans, lags_per_row=300, rng=rng)
+    srv.release(b)
+    srv.close()
+
+
+def test_injected_allocation_failure_in_initialize():
+    """A handle whose initialize fails on an allocation stays uninitialized
+    (runtime errors on use, as the reference's before initialize); the next
+    initialize succeeds and apply matches the oracle."""
+    rng = np.random.default_rng(508)
+    tpl = rng.standard_normal(300) * np.hanning(300)
+    x = (rng.standard_normal(20000) + 30).astype(np.float32)
+    h = capi.Handle(np.float32, "single")
+    h.set_templates([tpl])
+    h.set_signal_size(x.size)
+    try:
+        capi.debug_fail_allocations(2)
+        with pytest.raises(RuntimeError, match="injected"):
+            h.initialize()
+    finally:
+        left = capi.debug_fail_allocations(0)
+    assert left == 1
+    assert not h.is_initialized()
+    with pytest.raises(RuntimeError):
+        h.apply()
+    h.initialize()
+    h.set_signal(0, x)
+    h.apply()
+    assert np.abs(h.rho(0) - O.pearson(tpl, x)).max() < 1e-5
+    h.close()
