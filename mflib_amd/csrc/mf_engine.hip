@@ -97,7 +97,7 @@ struct mf_handle {
         int ch = 0, logM = 0, L = 0, nOut = 0;
         int td = 0, KT = 0;  // time-domain lane: kernel kind (3/4) and padded taps
         RowMix mix;
-        std::vector<int> rows;  // sorted by Lt
+        std::vector<int> rows;  // FFT lanes: by decreasing nOut, then Lt; time-domain lanes: by Lt
     };
     struct Group {
         std::vector<int> rows;
@@ -319,11 +319,11 @@ void initialize(mf_handle *h)
         std::vector<int> sorted = chRows[c];
         std::stable_sort(sorted.begin(), sorted.end(),
                          [&](int a, int b) { return h->rows[a].Lt < h->rows[b].Lt; });
-        long long chOut = 0;
         std::vector<int> lts;
+        std::vector<long long> outs;
         for (int r : sorted) {
-            chOut = std::max<long long>(chOut, h->rows[r].nOut);
             lts.push_back(h->rows[r].Lt);
+            outs.push_back(h->rows[r].nOut);
         }
         // time-domain path (variant 3/4, FP32): the rows of at most kTdMaxTaps
         // taps form one lane; longer rows keep FFT lanes
@@ -349,15 +349,22 @@ void initialize(mf_handle *h)
             h->lanes.push_back(std::move(ln));
             sorted.erase(sorted.begin(), sorted.begin() + nTd);
             lts.erase(lts.begin(), lts.begin() + nTd);
+            outs.erase(outs.begin(), outs.begin() + nTd);
             if (sorted.empty()) { continue; }
         }
-        for (const Bucket &bk : planBuckets(h->precision, h->userNfft == 0 ? 0 : h->userNfft, lts, chOut)) {
+        for (const Bucket &bk : planBuckets(h->precision, h->userNfft == 0 ? 0 : h->userNfft, lts, outs)) {
             mf_handle::Lane ln;
             ln.ch = c;
             ln.mix = bk.mix;
             ln.logM = bk.logM;
             ln.L = blockLength(ln.logM, bk.nb);
             ln.rows.assign(sorted.begin() + bk.begin, sorted.begin() + bk.end);
+            // decreasing nOut (then increasing length): the rows with
+            // outputs in a block are a prefix, which the block's item names
+            std::stable_sort(ln.rows.begin(), ln.rows.end(), [&](int a, int b) {
+                if (h->rows[a].nOut != h->rows[b].nOut) { return h->rows[a].nOut > h->rows[b].nOut; }
+                return h->rows[a].Lt < h->rows[b].Lt;
+            });
             for (int r : ln.rows) {
                 ln.nOut = std::max(ln.nOut, h->rows[r].nOut);
                 h->rows[r].lane = static_cast<int>(h->lanes.size());
@@ -557,7 +564,14 @@ void initialize(mf_handle *h)
         la.logM = kv.first.first;
         la.variant = kv.first.second;
         la.first = static_cast<int>(items.size());
-        appendXcdItems(kv.second, blocks, items);
+        const std::vector<int> &laneIds = kv.second;
+        appendXcdItems(laneIds, blocks, items, [&](int i, long long k) {
+            const auto &ln = h->lanes[laneIds[i]];
+            const long long start = k * ln.L;
+            int n = 0;
+            while (n < static_cast<int>(ln.rows.size()) && h->rows[ln.rows[n]].nOut > start) { ++n; }
+            return n;
+        });
         la.count = static_cast<int>(items.size()) - la.first;
         h->launches.push_back(la);
     }

@@ -10,6 +10,7 @@
 #include <algorithm>
 #include <atomic>
 #include <cmath>
+#include <functional>
 #include <mutex>
 #include <new>
 #include <stdexcept>
@@ -316,14 +317,59 @@ inline int staticVariant(int precision, int logM)
     return waveFaster && variantAvailable(precision, logM, 2) ? 2 : 1;
 }
 
-// Predicted cost of a lane (blocks x per-block cost) for an FFT size and a
-// variant (0 = the default variant), or -1 if the plan is impossible.
-inline double laneCost(int precision, int logM, int variant, int nb, long long nOut, RowMix mix)
+// The rows of a lane as the cost model sees them: each row's template
+// length and number of outputs.  A row works only in the blocks holding its
+// outputs, so a lane's cost is its blocks' fixed part plus, per row, the
+// blocks of that row.
+struct LaneRows {
+    std::vector<int> Lt;
+    std::vector<long long> nOut;
+    long long maxOut() const
+    {
+        long long m = 0;
+        for (long long v : nOut) { m = std::max(m, v); }
+        return m;
+    }
+    int nb() const
+    {
+        int m = 0;
+        for (int v : Lt) { m = std::max(m, v); }
+        return m;
+    }
+    RowMix mix() const { return rowMix(Lt); }
+};
+
+// Row blocks of a lane at block length L, split into rows that share a
+// normalisation scale and solo rows (as rowMix splits them).
+inline void laneRowBlocks(const LaneRows &rs, long long L, double &shared, double &solo)
+{
+    std::vector<int> order(rs.Lt.size());
+    for (size_t i = 0; i < order.size(); ++i) { order[i] = static_cast<int>(i); }
+    std::sort(order.begin(), order.end(), [&](int a, int b) { return rs.Lt[a] < rs.Lt[b]; });
+    shared = solo = 0;
+    for (size_t i = 0; i < order.size();) {
+        size_t j = i;
+        while (j < order.size() && rs.Lt[order[j]] == rs.Lt[order[i]]) { ++j; }
+        double rb = 0;
+        for (size_t k = i; k < j; ++k) {
+            rb += std::ceil(static_cast<double>(rs.nOut[order[k]]) / static_cast<double>(L));
+        }
+        (j - i >= 4 ? shared : solo) += rb;
+        i = j;
+    }
+}
+
+// Predicted cost of a lane for an FFT size and a variant (0 = the default
+// variant), or -1 if the plan is impossible: blocks x the per-block part
+// plus row blocks x the per-row part.
+inline double laneCost(int precision, int logM, int variant, const LaneRows &rs)
 {
     const long long N = 2LL << logM;
-    const long long L = N - nb + 1;
+    const long long L = N - rs.nb() + 1;
     if (L < 2) { return -1; }  // even block starts need L >= 2
-    const double blocks = std::ceil(static_cast<double>(nOut) / static_cast<double>(L));
+    const double blocks = std::ceil(static_cast<double>(rs.maxOut()) / static_cast<double>(L));
+    double shared = 0, solo = 0;
+    laneRowBlocks(rs, L, shared, solo);
     std::lock_guard<std::mutex> lk(costMutex());
     const CostTable &t = costTable(precision);
     if (t.measured) {
@@ -332,15 +378,17 @@ inline double laneCost(int precision, int logM, int variant, int nb, long long n
             double best = -1;
             for (int cand = 1; cand <= 2; ++cand) {
                 if (t.a[logM][cand] <= 0) { continue; }
-                const double c = t.a[logM][cand] + t.bShared[logM][cand] * mix.shared + t.bSolo[logM][cand] * mix.solo;
+                const double c = blocks * t.a[logM][cand] + t.bShared[logM][cand] * shared + t.bSolo[logM][cand] * solo;
                 if (best < 0 || c < best) { best = c; v = cand; }
             }
-            if (best >= 0) { return blocks * best; }
+            if (best >= 0) { return best; }
         } else if (t.a[logM][v] > 0) {
-            return blocks * (t.a[logM][v] + t.bShared[logM][v] * mix.shared + t.bSolo[logM][v] * mix.solo);
+            return blocks * t.a[logM][v] + t.bShared[logM][v] * shared + t.bSolo[logM][v] * solo;
         }
     }
-    return blocks * staticBlockCost(logM, mix, precision);
+    // static model: per-block part and per-row parts at the same scale
+    const double unit = staticBlockCost(logM, RowMix{0, 0}, precision);
+    return unit * (blocks + 1.15 * shared + 1.84 * solo);
 }
 
 // Correlation kernel variant of one lane: a requested variant if it exists
@@ -369,8 +417,9 @@ inline int laneVariant(int precision, int logM, int requested, RowMix mix)
 // FFT size (as logM, nfft = 2^(logM+1)) of a lane whose longest template
 // has nb samples and which produces nOut outputs for the rows in `mix`.  A
 // positive userNfft forces the size.
-inline int chooseLogM(int precision, int userNfft, int nb, long long nOut, RowMix mix)
+inline int chooseLogM(int precision, int userNfft, const LaneRows &rs)
 {
+    const int nb = rs.nb();
     const int hiLog = maxLogM(precision);
     if (userNfft > 0) {
         int lg = 0;
@@ -392,7 +441,7 @@ inline int chooseLogM(int precision, int userNfft, int nb, long long nOut, RowMi
     for (int lg = mfk::kMinLogM; lg <= hiLog; ++lg) {
         const long long N = 2LL << lg;
         if (N - nb + 1 < N / 4 && lg < hiLog) { continue; }  // keep >= 25% useful outputs
-        const double c = laneCost(precision, lg, 0, nb, nOut, mix);
+        const double c = laneCost(precision, lg, 0, rs);
         if (c < 0) { continue; }
         if (best < 0 || c < bestCost) { best = lg; bestCost = c; }
     }
@@ -418,30 +467,42 @@ struct Bucket {
     double cost = 0;
 };
 
-inline Bucket makeBucket(int precision, int userNfft, const std::vector<int> &sortedLt, int begin, int end,
-                         long long nOut)
+inline LaneRows laneRows(const std::vector<int> &sortedLt, const std::vector<long long> &nOut, int begin, int end)
+{
+    LaneRows rs;
+    rs.Lt.assign(sortedLt.begin() + begin, sortedLt.begin() + end);
+    rs.nOut.assign(nOut.begin() + begin, nOut.begin() + end);
+    return rs;
+}
+
+inline Bucket makeBucket(int precision, int userNfft, const std::vector<int> &sortedLt,
+                         const std::vector<long long> &nOut, int begin, int end)
 {
     Bucket b;
     b.begin = begin;
     b.end = end;
     b.nb = sortedLt[end - 1];
-    b.mix = rowMix(std::vector<int>(sortedLt.begin() + begin, sortedLt.begin() + end));
-    b.logM = chooseLogM(precision, userNfft, b.nb, nOut, b.mix);
-    b.cost = laneCost(precision, b.logM, 0, b.nb, nOut, b.mix);
+    const LaneRows rs = laneRows(sortedLt, nOut, begin, end);
+    b.mix = rs.mix();
+    b.logM = chooseLogM(precision, userNfft, rs);
+    b.cost = laneCost(precision, b.logM, 0, rs);
     return b;
 }
 
+// sortedLt: the channel's template lengths in increasing order; nOut[i]:
+// the outputs of row i of that order.
 inline std::vector<Bucket> planBuckets(int precision, int userNfft, const std::vector<int> &sortedLt,
-                                       long long nOut)
+                                       const std::vector<long long> &nOut)
 {
     const int n = static_cast<int>(sortedLt.size());
-    if (userNfft > 0 || n <= 1) { return {makeBucket(precision, userNfft, sortedLt, 0, n, nOut)}; }
+    if (userNfft > 0 || n <= 1) { return {makeBucket(precision, userNfft, sortedLt, nOut, 0, n)}; }
     std::vector<Bucket> bk;
+    auto alone = [&](int i) { return chooseLogM(precision, 0, laneRows(sortedLt, nOut, i, i + 1)); };
     for (int i = 0; i < n;) {
-        const int lg = chooseLogM(precision, 0, sortedLt[i], nOut, RowMix{0, 1});
+        const int lg = alone(i);
         int j = i + 1;
-        while (j < n && chooseLogM(precision, 0, sortedLt[j], nOut, RowMix{0, 1}) == lg) { ++j; }
-        bk.push_back(makeBucket(precision, 0, sortedLt, i, j, nOut));
+        while (j < n && alone(j) == lg) { ++j; }
+        bk.push_back(makeBucket(precision, 0, sortedLt, nOut, i, j));
         i = j;
     }
     while (bk.size() > 1) {
@@ -449,7 +510,7 @@ inline std::vector<Bucket> planBuckets(int precision, int userNfft, const std::v
         double bestGain = 0;
         Bucket bestM;
         for (size_t k = 0; k + 1 < bk.size(); ++k) {
-            const Bucket m = makeBucket(precision, 0, sortedLt, bk[k].begin, bk[k + 1].end, nOut);
+            const Bucket m = makeBucket(precision, 0, sortedLt, nOut, bk[k].begin, bk[k + 1].end);
             const double gain = bk[k].cost + bk[k + 1].cost - m.cost;
             if (gain > bestGain) {
                 bestGain = gain;
@@ -477,8 +538,12 @@ inline int blockLength(int logM, int nb)
 // lanes are binned to the 8 XCDs (largest first, least-loaded bin), and
 // the bins are interleaved so that workgroup i, which runs on XCD i mod 8,
 // takes its blocks from one bin.  All blocks of a lane then share one L2.
+// rowsAt(i, k), when given, is the number of leading rows of lane
+// laneIds[i] with outputs in its block k (the item's nRows); otherwise -1
+// (every row).
 inline void appendXcdItems(const std::vector<int> &laneIds, const std::vector<long long> &laneBlocks,
-                           std::vector<mfk::Item> &items)
+                           std::vector<mfk::Item> &items,
+                           const std::function<int(int, long long)> &rowsAt = nullptr)
 {
     constexpr int kXcd = 8;
     std::vector<int> order(laneIds.size());
@@ -488,7 +553,9 @@ inline void appendXcdItems(const std::vector<int> &laneIds, const std::vector<lo
     std::vector<long long> load(kXcd, 0);
     for (int i : order) {
         const int b = static_cast<int>(std::min_element(load.begin(), load.end()) - load.begin());
-        for (long long k = 0; k < laneBlocks[i]; ++k) { bins[b].push_back({laneIds[i], static_cast<int>(k)}); }
+        for (long long k = 0; k < laneBlocks[i]; ++k) {
+            bins[b].push_back({laneIds[i], static_cast<int>(k), rowsAt ? rowsAt(i, k) : -1, 0});
+        }
         load[b] += laneBlocks[i];
     }
     std::vector<size_t> pos(kXcd, 0);

@@ -249,9 +249,11 @@ __global__ __launch_bounds__(NT) void corr_kernel(const Item *__restrict__ items
     const int tid = threadIdx.x;
     const long long base = static_cast<long long>(item.block) * sg.L;
     const int nValidSeg = static_cast<int>(min(static_cast<long long>(sg.L), sg.nOut - base));
+    // the rows with outputs in this block (a prefix of the lane's rows)
+    const int rowEnd = item.nRows < 0 ? sg.rowEnd : min(sg.rowEnd, sg.rowBegin + item.nRows);
 
     if (sg.dead) {
-        for (int r = sg.rowBegin; r < sg.rowEnd; ++r) {
+        for (int r = sg.rowBegin; r < rowEnd; ++r) {
             const Row row = rows[r];
             const int nv = static_cast<int>(min(static_cast<long long>(sg.L), row.nOut - base));
             T *out = static_cast<T *>(row.out) + base;
@@ -322,7 +324,7 @@ __global__ __launch_bounds__(NT) void corr_kernel(const Item *__restrict__ items
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);  // wave-uniform: scalar row loads and branches
     const int lane = tid & 63;
     int curLt = -1;
-    for (int r = sg.rowBegin; r < sg.rowEnd; ++r) {
+    for (int r = sg.rowBegin; r < rowEnd; ++r) {
         const Row row = rows[r];
         const int nvRow = static_cast<int>(min(static_cast<long long>(sg.L), row.nOut - base));
         if (nvRow <= 0) { continue; }
@@ -507,9 +509,11 @@ __global__ __launch_bounds__(NT, (Corr2Waves<T, M>::value)) void corr2_kernel(co
     const int lane = tid & 63;
     const long long base = static_cast<long long>(item.block) * sg.L;
     const int nValidSeg = static_cast<int>(min(static_cast<long long>(sg.L), sg.nOut - base));
+    // the rows with outputs in this block (a prefix of the lane's rows)
+    const int rowEnd = item.nRows < 0 ? sg.rowEnd : min(sg.rowEnd, sg.rowBegin + item.nRows);
 
     if (sg.dead) {
-        for (int r = sg.rowBegin; r < sg.rowEnd; ++r) {
+        for (int r = sg.rowBegin; r < rowEnd; ++r) {
             const Row row = rows[r];
             const int nv = static_cast<int>(min(static_cast<long long>(sg.L), row.nOut - base));
             gptr<T> out = to_global(static_cast<T *>(row.out)) + base;
@@ -718,10 +722,10 @@ __global__ __launch_bounds__(NT, (Corr2Waves<T, M>::value)) void corr2_kernel(co
     // are "solo": each wave takes every NWAVE-th of them and computes its
     // row's scale itself, so no wave waits for a group it has no row in.
     bool fftDone = false;
-    for (int g0 = sg.rowBegin; g0 < sg.rowEnd;) {
+    for (int g0 = sg.rowBegin; g0 < rowEnd;) {
         const int Lt = rows[g0].Lt;
         int g1 = g0 + 1;
-        while (g1 < sg.rowEnd && rows[g1].Lt == Lt) { ++g1; }
+        while (g1 < rowEnd && rows[g1].Lt == Lt) { ++g1; }
         if (g1 - g0 >= NWAVE) {
             int myFlags = 0;
             if (!fftDone) {
@@ -764,10 +768,10 @@ __global__ __launch_bounds__(NT, (Corr2Waves<T, M>::value)) void corr2_kernel(co
         while (true) {
             if (sr >= sg1) {
                 sg0 = sg1;
-                if (sg0 >= sg.rowEnd) { return -1; }
+                if (sg0 >= rowEnd) { return -1; }
                 const int Lt = rows[sg0].Lt;
                 sg1 = sg0 + 1;
-                while (sg1 < sg.rowEnd && rows[sg1].Lt == Lt) { ++sg1; }
+                while (sg1 < rowEnd && rows[sg1].Lt == Lt) { ++sg1; }
                 sr = (sg1 - sg0 >= NWAVE) ? sg1 : sg0;
                 continue;
             }

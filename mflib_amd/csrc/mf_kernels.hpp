@@ -20,8 +20,8 @@ struct Seg {
     long long n;        // channel length
     int L;              // correlation outputs per block (nfft - nb + 1)
     int nOut;           // outputs needed on this lane (max over its rows)
-    int rowBegin;       // rows [rowBegin, rowEnd) of the row table,
-    int rowEnd;         //   sorted by template length
+    int rowBegin;       // rows [rowBegin, rowEnd) of the row table, by
+    int rowEnd;         //   decreasing nOut, then increasing template length
     int dead;           // dead trace: every row of the lane is zero
     float hmax;         // max |template spectrum| over the lane's rows
 };
@@ -35,10 +35,14 @@ struct Row {
     int Lt;               // template length
 };
 
-// One work item: block `block` of lane `seg`.
+// One work item: block `block` of lane `seg`, for the first nRows rows of
+// the lane (a lane's rows are ordered by decreasing nOut, so the rows with
+// outputs in a block are a prefix of them); nRows < 0: every row.
 struct Item {
     int seg;
     int block;
+    int nRows;
+    int pad;
 };
 
 // Template preparation job (one per row).

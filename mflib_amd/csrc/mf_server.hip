@@ -457,8 +457,9 @@ std::unique_ptr<Request> planRequest(mf_server *s, int nC, const int *lengths, c
                                               " samples, fewer than the template length " + std::to_string(r->nb));
         }
         r->nOut[c] = static_cast<int>(no);
-        r->buckets[c] = (c > 0 && lengths[c] == lengths[c - 1]) ? r->buckets[c - 1]
-                                                                 : planBuckets(s->precision, 0, lts, no);
+        r->buckets[c] = (c > 0 && lengths[c] == lengths[c - 1])
+                            ? r->buckets[c - 1]
+                            : planBuckets(s->precision, 0, lts, std::vector<long long>(lts.size(), no));
         maxOut = std::max(maxOut, r->nOut[c]);
         r->corrSamples += static_cast<double>(no) * nT;
         if (!r->onDevice) { r->inBytes += roundUp(static_cast<size_t>(lengths[c]) * s->tsize, kAlign); }
