@@ -302,16 +302,16 @@ __device__ __forceinline__ void wave_load(Cx<T> *v, const Cx<T> *buf, int lane)
 //   pass 2: butterfly j has k = j mod R1 = lane mod R1 (64 is a multiple
 //           of R1), the same for all of the lane's butterflies;
 //   pass 3: k = j mod R1 R2, which differs per butterfly when R1 R2 > 64.
-template <typename T, int M>
+// W3REG: pass-3 twiddle rows resident in registers (computed once per
+// block); otherwise looked up from the two-level table per transform, which
+// saves registers where they limit occupancy.
+template <typename T, int M, bool W3REG = true>
 struct WaveTwiddles {
     using P = WavePlan<M>;
     static constexpr int NS2 = P::R1;
     static constexpr int NS3 = P::R1 * P::R2;
     static constexpr int B3 = M / (P::R3 * 64);
-#ifndef MF_W3REG
-#define MF_W3REG 1
-#endif
-    static constexpr int W3 = MF_W3REG ? B3 : 1;
+    static constexpr int W3 = W3REG ? B3 : 1;
     Cx<T> w2[P::R2 - 1];  // pass 2, resident in registers
     Cx<T> w3[W3][P::R3 - 1];
     Twiddle<T> tab;       // pass 3, looked up per transform
@@ -322,7 +322,7 @@ struct WaveTwiddles {
         const int s2 = (2 * M / (NS2 * P::R2)) * k2;
 #pragma unroll
         for (int r = 1; r < P::R2; ++r) { w2[r - 1] = tw(r * s2); }
-        if constexpr (MF_W3REG) {
+        if constexpr (W3REG) {
 #pragma unroll
             for (int b = 0; b < B3; ++b) { lookup3(lane, b, w3[b]); }
         }
@@ -337,7 +337,7 @@ struct WaveTwiddles {
     // pass-3 twiddle row of butterfly b of this lane
     __device__ __forceinline__ void pass3(int lane, int b, Cx<T> *w) const
     {
-        if constexpr (MF_W3REG) {
+        if constexpr (W3REG) {
 #pragma unroll
             for (int r = 1; r < P::R3; ++r) { w[r - 1] = w3[b][r - 1]; }
         } else {
@@ -384,11 +384,10 @@ __device__ __forceinline__ void wave_store(const Cx<T> *v, Cx<T> *buf, int lane)
 
 // Passes 1..3 on data already in registers as pass-1 input.  On return the
 // lane holds output points j + r M/R3 (j = lane + 64 b) in v[b R3 + r].
-template <typename T, int M, bool INV>
-__device__ __forceinline__ void wave_fft_regs(Cx<T> *v, Cx<T> *buf, int lane, const WaveTwiddles<T, M> &tw)
+template <typename T, int M, bool INV, typename W>
+__device__ __forceinline__ void wave_fft_regs(Cx<T> *v, Cx<T> *buf, int lane, const W &tw)
 {
     using P = WavePlan<M>;
-    using W = WaveTwiddles<T, M>;
     wave_butterflies<T, M, P::R1, INV, false>(v, nullptr, 0);
     wave_store<T, M, P::R1, 1>(v, buf, lane);
     wave_lds_sync();
@@ -398,6 +397,100 @@ __device__ __forceinline__ void wave_fft_regs(Cx<T> *v, Cx<T> *buf, int lane, co
     wave_store<T, M, P::R2, P::R1>(v, buf, lane);
     wave_lds_sync();
     wave_load<T, M, P::R3>(v, buf, lane);
+#pragma unroll
+    for (int b = 0; b < W::B3; ++b) {
+        Cx<T> w3[P::R3 - 1];
+        tw.pass3(lane, b, w3);
+#pragma unroll
+        for (int r = 1; r < P::R3; ++r) {
+            Cx<T> t = w3[r - 1];
+            if (INV) { t.y = -t.y; }
+            v[b * P::R3 + r] = cmul(v[b * P::R3 + r], t);
+        }
+        mfcl::dftR<P::R3, INV>(&v[b * P::R3]);
+    }
+    wave_lds_sync();
+}
+
+// ---- half-size exchanges ------------------------------------------------
+// The exchanges of a row transform go through a wave buffer of half the
+// padded size (XHalf<M>::size), in two phases: first every value bound for
+// points [0, M/2), which the lanes then load, then those bound for
+// [M/2, M).  A half buffer holds point i + H M/2 at xpad(i), so the half-H
+// base is buf - cpad(H M/2) and every access keeps its full-size index
+// expression.  Halving the buffer is what lets more blocks share a CU's LDS.
+template <int M>
+struct XHalf {
+    static constexpr int size = M / 2 + M / 32;
+};
+
+// Stores of a pass's outputs (radix R, butterfly phase NS) bound for half
+// H.  With two or more butterflies per lane the half follows the butterfly
+// index b; with one, the lane: lanes 0-31 write half 0 and 32-63 half 1.
+template <typename T, int M, int R, int NS, int H>
+__device__ __forceinline__ void wave_store_half(const Cx<T> *v, Cx<T> *buf, int lane)
+{
+    constexpr int B = M / (R * 64);
+    static_assert(NS <= 64, "butterfly phase must depend on the lane only");
+    const int k = lane & (NS - 1);
+    Cx<T> *p = buf - cpad(H * (M / 2)) + xpad((lane - k) * R + k);
+    if constexpr (B >= 2) {
+#pragma unroll
+        for (int b = H * (B / 2); b < (H + 1) * (B / 2); ++b) {
+#pragma unroll
+            for (int r = 0; r < R; ++r) { p[cpad(64 * b * R + r * NS)] = v[b * R + r]; }
+        }
+    } else {
+        if ((lane >= 32) == (H == 1)) {
+#pragma unroll
+            for (int r = 0; r < R; ++r) { p[cpad(r * NS)] = v[r]; }
+        }
+    }
+}
+
+// Loads of the next pass's inputs (radix R) that lie in half H: input r of
+// butterfly b is point lane + 64 b + r M/R, in half 0 exactly when r < R/2.
+template <typename T, int M, int R, int H>
+__device__ __forceinline__ void wave_load_half(Cx<T> *v, const Cx<T> *buf, int lane)
+{
+    constexpr int B = M / (R * 64);
+    static_assert((M / R) % 16 == 0, "load offsets must be 16-aligned");
+    const Cx<T> *p = buf - cpad(H * (M / 2)) + xpad(lane);
+#pragma unroll
+    for (int b = 0; b < B; ++b) {
+#pragma unroll
+        for (int r = H * (R / 2); r < (H + 1) * (R / 2); ++r) { v[b * R + r] = p[cpad(64 * b + r * (M / R))]; }
+    }
+}
+
+// Pass outputs in v (radix RS, phase NS) to the next pass's inputs (radix
+// RL) in v, through the half buffer.
+template <typename T, int M, int RS, int NS, int RL>
+__device__ __forceinline__ void wave_exchange_half(Cx<T> *v, Cx<T> *buf, int lane)
+{
+    constexpr int Q = M / 64;
+    Cx<T> u[Q];
+    wave_store_half<T, M, RS, NS, 0>(v, buf, lane);
+    wave_lds_sync();
+    wave_load_half<T, M, RL, 0>(u, buf, lane);
+    wave_lds_sync();
+    wave_store_half<T, M, RS, NS, 1>(v, buf, lane);
+    wave_lds_sync();
+    wave_load_half<T, M, RL, 1>(u, buf, lane);
+    wave_lds_sync();
+#pragma unroll
+    for (int i = 0; i < Q; ++i) { v[i] = u[i]; }
+}
+
+// wave_fft_regs through a half buffer.
+template <typename T, int M, bool INV, typename W>
+__device__ __forceinline__ void wave_fft_regs_half(Cx<T> *v, Cx<T> *buf, int lane, const W &tw)
+{
+    using P = WavePlan<M>;
+    wave_butterflies<T, M, P::R1, INV, false>(v, nullptr, 0);
+    wave_exchange_half<T, M, P::R1, 1, P::R2>(v, buf, lane);
+    wave_butterflies<T, M, P::R2, INV, true>(v, tw.w2, 0);
+    wave_exchange_half<T, M, P::R2, P::R1, P::R3>(v, buf, lane);
 #pragma unroll
     for (int b = 0; b < W::B3; ++b) {
         Cx<T> w3[P::R3 - 1];

@@ -26,6 +26,7 @@
 #include "mf_device.hpp"
 #include "mf_kernels.hpp"
 #include <cfloat>
+#include <type_traits>
 
 using namespace mfdev;
 
@@ -147,10 +148,11 @@ __device__ __forceinline__ void window_sums(const T *raw, const double *ts1, con
 // outputs.  The FP32 scale is rsqrt(den^2) in FP32 plus one Newton step
 // (relative error ~2^-23, far inside the FP32 contract); the FP64 scale is
 // sqrt/division in FP64.  Returns this thread's number of flagged outputs.
+// Outputs [first, first + nValid) of the block (sc[i - first] for output i).
 template <typename T, int TILE, int UNROLL = 1>
 __device__ __forceinline__ int compute_scale(const T *raw, const double *ts1, const double *ts2,
                                              const double *tp1, const double *tp2, T *sc, double a, int nValid,
-                                             int Lt, double noise, int t, int nthr)
+                                             int Lt, double noise, int t, int nthr, int first = 0)
 {
     const double sqrtLt = sqrt(static_cast<double>(Lt));
     const double denThr = sqrtLt * noise;
@@ -159,10 +161,11 @@ __device__ __forceinline__ int compute_scale(const T *raw, const double *ts1, co
     // spreads the threads of a wave over all 32 LDS banks
     const int C = ((nValid + nthr - 1) / nthr) | 1;
     const double canc = 2.0e-9 * (Lt + C) * Lt;
-    const int i0 = t * C;
-    const int i1 = min(i0 + C, nValid);
+    const int i0 = first + t * C;
+    const int i1 = min(i0 + C, first + nValid);
     int nflag = 0;
     if (i0 >= i1) { return 0; }
+    sc -= first;
     double S1, S2;
     window_sums<T, TILE>(raw, ts1, ts2, tp1, tp2, a, i0, Lt, S1, S2);
     double S2max = S2;
@@ -410,12 +413,14 @@ __global__ __launch_bounds__(NT) void corr_kernel(const Item *__restrict__ items
 // scale of a group is shared by its rows.
 // ---------------------------------------------------------------------------
 
-// Exact FP64 two-pass Pearson for the flagged outputs of one row (one wave).
+// Exact FP64 two-pass Pearson for the flagged outputs [first, nv) of one
+// row (one wave); sc[i - first] is output i's scale (negative: flagged).
 template <typename T>
 __device__ __forceinline__ void fixup_row_wave(const T *raw, const T *sc, gptr<const double> tmpl, int Lt,
-                                               gptr<T> out, int nv, int lane)
+                                               gptr<T> out, int nv, int lane, int first = 0)
 {
-    for (int w0 = 0; w0 < nv; w0 += 64) {
+    sc -= first;
+    for (int w0 = first; w0 < nv; w0 += 64) {
         const int i = w0 + lane;
         const bool fl = (i < nv) && (sc[i] < T(0));
         unsigned long long m = __ballot(fl);
@@ -458,28 +463,38 @@ struct Vec2<double> {
     using type = double2;
 };
 
-// Waves per SIMD that the LDS footprint of corr2_kernel<T, M> allows; the
-// register budget is capped to match so registers never lower occupancy.
-#ifndef MF_CAP
-#define MF_CAP 1
-#endif
+// Shape of corr2_kernel<T, M>:
+//  - SHARED: a block-wide scale buffer for groups of rows of one template
+//    length (FP32 up to M = 1024, FP64 up to M = 512);
+//  - HALF (the larger blocks): every row computes its own scale, the
+//    per-wave exchange buffers are half size, the template spectrum of the
+//    next row is not prefetched into registers and the pass-3 twiddles come
+//    from the table.  That fits two blocks per CU in LDS and registers
+//    where the full layout fitted one;
+//  - TILE: samples per FP64 prefix-sum tile (FP32: 4, or 8 at M = 2048; FP64:
+//    16, whose windows sum whole tiles);
+//  - BLOCKS per CU that LDS and registers allow; the register budget is
+//    capped to match.
 template <typename T, int M>
-struct Corr2Waves {
-    static constexpr int value =
-        !MF_CAP ? 1 : (sizeof(T) * M <= 2048) ? 3 : (sizeof(T) * M <= 4096) ? 2 : 1;
+struct Corr2Shape {
+    static constexpr bool SHARED = sizeof(T) * M <= 4096;
+    static constexpr bool HALF = !SHARED;
+    static constexpr int TILE = sizeof(T) == 4 ? (M >= 2048 ? 8 : 4) : 16;
+    static constexpr int BLOCKS = sizeof(T) * M <= 2048 ? 3 : 2;
 };
 
 template <typename T, int M>
-__global__ __launch_bounds__(NT, (Corr2Waves<T, M>::value)) void corr2_kernel(const Item *__restrict__ items,
+__global__ __launch_bounds__(NT, (Corr2Shape<T, M>::BLOCKS)) void corr2_kernel(const Item *__restrict__ items,
                                                    const Seg *__restrict__ segs,
                                                    const Row *__restrict__ rows,
                                                    const Cx<T> *__restrict__ twloG,
                                                    const Cx<T> *__restrict__ twhiG)
 {
+    using Shape = Corr2Shape<T, M>;
     constexpr int N = 2 * M;
-    // tile of the FP64 prefix sums: 4 samples in FP32 (short partial tiles
-    // at each window's ends), 16 in FP64 (whose windows sum whole tiles)
-    constexpr int TILE = sizeof(T) == 4 ? 4 : 16;
+    constexpr int TILE = Shape::TILE;
+    constexpr bool SHARED = Shape::SHARED;
+    constexpr bool HALF = Shape::HALF;
     constexpr int NTILE = N / TILE;
     constexpr int SPT = N / NT;
     constexpr int LOGN = Log2<N>::value;
@@ -488,19 +503,29 @@ __global__ __launch_bounds__(NT, (Corr2Waves<T, M>::value)) void corr2_kernel(co
     // has a closed form (its twiddle is exactly -i) and is done apart
     constexpr int KP = M / 2 / 64;
     constexpr int NWAVE = NT / 64;
+    constexpr int XH = XHalf<M>::size;
+    constexpr int XW = HALF ? XH : XBuf<M>::size;  // exchange buffer of one wave
     using P = WavePlan<M>;
     static_assert(N % NT == 0 && NWAVE == 4, "block shape");
+    static_assert(2 * XH == XBuf<M>::size, "two half buffers make one full buffer");
     __shared__ T raw[N];
     __shared__ Cx<T> Xs[M + 1];
-    __shared__ __align__(16) T sc[N];
-    __shared__ Cx<T> xbuf[NWAVE][XBuf<M>::size];
-    __shared__ double ts1[NTILE];
-    __shared__ double ts2[NTILE];
+    __shared__ __align__(16) T sc[SHARED ? N : 1];
+    // one exchange buffer per wave (half size with HALF; then the first two
+    // together are the full buffer of the forward transform and of the
+    // packed input)
+    __shared__ Cx<T> xb[NWAVE * XW];
+    // FP64 tile prefix sums; FP64 correlation also keeps the tile sums (FP32
+    // builds the prefix in place over the sums)
     __shared__ double tp1[NTILE + 1];
     __shared__ double tp2[NTILE + 1];
+    __shared__ double tsd1[sizeof(T) == 8 ? NTILE : 1];
+    __shared__ double tsd2[sizeof(T) == 8 ? NTILE : 1];
     __shared__ Cx<T> twlo[64];
     __shared__ Cx<T> twhi[N / 64];
     __shared__ double red[NWAVE];
+    double *ts1 = sizeof(T) == 8 ? tsd1 : tp1;
+    double *ts2 = sizeof(T) == 8 ? tsd2 : tp2;
 
     const Item item = items[blockIdx.x];
     const Seg sg = segs[item.seg];
@@ -525,7 +550,7 @@ __global__ __launch_bounds__(NT, (Corr2Waves<T, M>::value)) void corr2_kernel(co
     for (int i = tid; i < N / 64; i += NT) { twhi[i] = twhiG[i]; }
     const Twiddle<T> tw{twlo, twhi};
 
-    // A. load, anchor, tile sums, energy; packed anchored input -> xbuf[0]
+    // A. load, anchor, tile sums, energy; packed anchored input -> xb (full)
     const gptr<const T> x = to_global(static_cast<const T *>(sg.x));
     const int nSamp = static_cast<int>(min(static_cast<long long>(N), sg.n - base));
     T v0[SPT];
@@ -541,7 +566,7 @@ __global__ __launch_bounds__(NT, (Corr2Waves<T, M>::value)) void corr2_kernel(co
     const double a = uniform(block_sum<NT>(s, red) / nSamp);  // scalar registers: used by every row
     const T aT = static_cast<T>(a);
     double e = 0;
-    T *xin = reinterpret_cast<T *>(xbuf[0]);
+    T *xin = reinterpret_cast<T *>(xb);
 #pragma unroll
     for (int q = 0; q < SPT; ++q) {
         const int j = tid + q * NT;
@@ -561,18 +586,18 @@ __global__ __launch_bounds__(NT, (Corr2Waves<T, M>::value)) void corr2_kernel(co
         xin[2 * xpad(j >> 1) + (j & 1)] = valid ? static_cast<T>(v0[q] - aT) : T(0);
     }
     const double energy = block_sum<NT>(e, red);
-    tile_prefix<NTILE>(ts1, ts2, tp1, tp2, tid);
+    tile_prefix<NTILE>(ts1, ts2, tp1, tp2, tid);  // in place for FP32 (each lane reads its tiles first)
     __syncthreads();
     const double rmsBlk = sqrt(energy / nSamp);
     const double noise = uniform(kNoiseK / Eps<T>::rhoTol * Eps<T>::value * sqrt(static_cast<double>(LOGN)) *
                                  static_cast<double>(sg.hmax) * rmsBlk);
-    WaveTwiddles<T, M> wt;
+    WaveTwiddles<T, M, !HALF> wt;
     wt.init(lane, tw);
     Cx<T> v[Q];
 
-    // B. forward FFT + unpack into Xs (wave 0)
+    // B. forward FFT + unpack into Xs (wave 0, full buffer)
     auto forwardFft = [&]() {
-        Cx<T> *buf = xbuf[0];
+        Cx<T> *buf = xb;
         wave_load<T, M, P::R1>(v, buf, lane);
         wave_lds_sync();
         wave_fft_regs<T, M, false>(v, buf, lane, wt);
@@ -615,7 +640,7 @@ __global__ __launch_bounds__(NT, (Corr2Waves<T, M>::value)) void corr2_kernel(co
     // finish(previous), start(r), and finally finish(last).  start(r)
     // loads the spectrum of the row after r once its product is done, so
     // the load is in flight through r's transform and epilogue; the
-    // epilogue issues the same 16 (range-checked) stores on every path, so
+    // epilogue issues the same (range-checked) stores on every path, so
     // the wait for that spectrum never waits for those stores.
     Cx<T> hk[KP], hm[KP];  // template spectrum pairs (H[k], H[M-k]), k < M/2
     Cx<T> hmid;            // H[M/2]
@@ -632,16 +657,52 @@ __global__ __launch_bounds__(NT, (Corr2Waves<T, M>::value)) void corr2_kernel(co
     auto rowValid = [&](const Row &row) {
         return static_cast<int>(max(0ll, min(static_cast<long long>(sg.L), row.nOut - base)));
     };
+    constexpr bool PREFETCH = !HALF;
     auto startRow = [&](const Row &row, const void *nextH) {
         if (rowValid(row) == 0) {
-            loadH(nextH);
+            if (PREFETCH) { loadH(nextH); }
             return;
         }
-        Cx<T> *buf = xbuf[wave];
-        // (k, M-k) pair per lane step (k = lane + 64 t; tw(k) = lo[lane] hi[t])
+        if (!PREFETCH) { loadH(row.H); }
+        Cx<T> *buf = xb + wave * XW;
+        if constexpr (!HALF) {
+            // (k, M-k) pair per lane step (k = lane + 64 t; tw(k) = lo[lane] hi[t])
+            const Cx<T> wlo = twlo[lane];
+            Cx<T> *pk = buf + xpad(lane);
+            Cx<T> *pm = buf + xpad(M - lane);
+#pragma unroll
+            for (int t = 0; t < KP; ++t) {
+                const int k = lane + 64 * t;
+                const Cx<T> yk = cmul(Xs[k], hk[t]);
+                const Cx<T> ym = cmul(Xs[M - k], hm[t]);
+                const Cx<T> S = cadd(yk, cconj(ym));
+                const Cx<T> D = csub(yk, cconj(ym));
+                const Cx<T> Pp = cmul(cconj(cmul(wlo, twhi[t])), D);
+                pk[68 * t] = cadd(S, cmuli(Pp));
+                if (k > 0) { pm[-68 * t] = cadd(cconj(S), cmuli(cconj(Pp))); }
+            }
+            // k = M/2 (y = X H): S = 2 Re y, D = 2i Im y, twiddle -i, so Z = 2 conj(y)
+            if (lane == 0) {
+                const Cx<T> y = cmul(Xs[M / 2], hmid);
+                buf[xpad(M / 2)] = Cx<T>{y.x + y.x, -(y.y + y.y)};
+            }
+            // the next row's spectrum is in flight through this row's
+            // transform and epilogue
+            loadH(nextH);
+            wave_lds_sync();
+            wave_load<T, M, P::R1>(v, buf, lane);
+            wave_lds_sync();
+            wave_fft_regs<T, M, true>(v, buf, lane, wt);
+            return;
+        }
+        // (k, M-k) pair per lane step (k = lane + 64 t; tw(k) = lo[lane] hi[t]).
+        // Points k < M/2 are the first half of the pass-1 input, M - k and
+        // M/2 the second: the first half goes out and comes back in while
+        // the second waits in registers.
         const Cx<T> wlo = twlo[lane];
         Cx<T> *pk = buf + xpad(lane);
-        Cx<T> *pm = buf + xpad(M - lane);
+        Cx<T> *pm = buf - cpad(M / 2) + xpad(M - lane);
+        Cx<T> zm[KP];
 #pragma unroll
         for (int t = 0; t < KP; ++t) {
             const int k = lane + 64 * t;
@@ -651,110 +712,168 @@ __global__ __launch_bounds__(NT, (Corr2Waves<T, M>::value)) void corr2_kernel(co
             const Cx<T> D = csub(yk, cconj(ym));
             const Cx<T> Pp = cmul(cconj(cmul(wlo, twhi[t])), D);
             pk[68 * t] = cadd(S, cmuli(Pp));
-            if (k > 0) { pm[-68 * t] = cadd(cconj(S), cmuli(cconj(Pp))); }
+            zm[t] = cadd(cconj(S), cmuli(cconj(Pp)));
         }
         // k = M/2 (y = X H): S = 2 Re y, D = 2i Im y, twiddle -i, so Z = 2 conj(y)
-        if (lane == 0) {
-            const Cx<T> y = cmul(Xs[M / 2], hmid);
-            buf[xpad(M / 2)] = Cx<T>{y.x + y.x, -(y.y + y.y)};
+        const Cx<T> ymid = cmul(Xs[M / 2], hmid);
+        wave_lds_sync();
+        wave_load_half<T, M, P::R1, 0>(v, buf, lane);
+        wave_lds_sync();
+#pragma unroll
+        for (int t = 0; t < KP; ++t) {
+            if (lane + 64 * t > 0) { pm[-68 * t] = zm[t]; }
         }
-        // the next row's spectrum is in flight through this row's transform
-        // and epilogue
-        loadH(nextH);
+        if (lane == 0) { buf[0] = Cx<T>{ymid.x + ymid.x, -(ymid.y + ymid.y)}; }  // point M/2 of half 1
         wave_lds_sync();
-        wave_load<T, M, P::R1>(v, buf, lane);
+        wave_load_half<T, M, P::R1, 1>(v, buf, lane);
         wave_lds_sync();
-        wave_fft_regs<T, M, true>(v, buf, lane, wt);
+        wave_fft_regs_half<T, M, true>(v, buf, lane, wt);
     };
-    // sharedScale is the row's block-shared scale, or null: the wave then
-    // computes the row's scale itself, into its own exchange buffer, which
-    // is free once the inverse FFT is in registers.
-    auto finishRow = [&](const Row &row, const T *sharedScale, bool anyFlagShared) {
-        const int nvRow = rowValid(row);
-        Cx<T> *buf = xbuf[wave];
-        const T *scale = sharedScale;
-        bool anyFlag = anyFlagShared && nvRow > 0;
-        if (scale == nullptr) {
-            T *own = reinterpret_cast<T *>(buf);
-            int nf = 0;
-            if (nvRow > 0) { nf = compute_scale<T, TILE, 4>(raw, ts1, ts2, tp1, tp2, own, a, nvRow, row.Lt, noise, lane, 64); }
-            anyFlag = __ballot(nf > 0) != 0ull;
-            wave_lds_sync();
-            scale = own;
-        }
-        // rho = c * scale, paired stores.  Flagged windows (scale -1) are
-        // overwritten below by the exact fix-up; for the others
-        // |rho| <= 1 + O(eps), so the reference's tooBig rule (|rho| > 1.1
-        // -> 0) cannot fire and is not evaluated.  The scale reads stay in
-        // bounds for every position (both scale buffers hold >= N entries).
-        // The store resource ends at the last whole pair of valid outputs,
-        // so invalid pairs are dropped; an odd last output is stored apart.
-        using V2 = typename Vec2<T>::type;
-        constexpr int B3 = M / (P::R3 * 64);
-        constexpr int NQ = B3 * P::R3;
-        T *outRow = static_cast<T *>(row.out) + base;
+    // rho = c * scale for the pass-3 outputs in half H of the block (the
+    // lane's points j + r M/R3 with r in [H R3/2, (H+1) R3/2): outputs
+    // [H N/2, (H+1) N/2)), paired stores; scl[i - first] is output i's
+    // scale.  Flagged windows (scale -1) are overwritten by the exact
+    // fix-up; for the others |rho| <= 1 + O(eps), so the reference's tooBig
+    // rule (|rho| > 1.1 -> 0) cannot fire and is not evaluated.  The scale
+    // reads stay inside the scale buffer for every position.  The store
+    // resource ends at the last whole pair of valid outputs, so invalid
+    // pairs are dropped; an odd last output is stored apart.
+    using V2 = typename Vec2<T>::type;
+    constexpr int B3 = M / (P::R3 * 64);
+    auto epilogueHalf = [&](auto hTag, const T *scl, int first, T *outRow, int nvRow) {
+        constexpr int H = decltype(hTag)::value;
+        constexpr int R0 = H * (P::R3 / 2), R1e = (H + 1) * (P::R3 / 2);
+        constexpr int NQH = B3 * (P::R3 / 2);
         const __amdgpu_buffer_rsrc_t rs = buffer_rsrc(outRow, static_cast<int>(sizeof(T)) * (nvRow & ~1));
-        const T *scl = scale + 2 * lane;
-        V2 f[NQ];
+        const T *sp = scl - first + 2 * lane;
+        V2 f[NQH];
 #pragma unroll
-        for (int q = 0; q < NQ; ++q) {
-            f[q] = *reinterpret_cast<const V2 *>(scl + 2 * (64 * (q / P::R3) + (q % P::R3) * (M / P::R3)));
+        for (int b = 0; b < B3; ++b) {
+#pragma unroll
+            for (int r = R0; r < R1e; ++r) {
+                f[b * (P::R3 / 2) + r - R0] = *reinterpret_cast<const V2 *>(sp + 2 * (64 * b + r * (M / P::R3)));
+            }
         }
 #pragma unroll
-        for (int q = 0; q < NQ; ++q) {
-            const int i0 = 2 * lane + 2 * (64 * (q / P::R3) + (q % P::R3) * (M / P::R3));
-            buffer_store_pair(v[q].x * f[q].x, v[q].y * f[q].y, rs, static_cast<int>(sizeof(T)) * i0);
+        for (int b = 0; b < B3; ++b) {
+#pragma unroll
+            for (int r = R0; r < R1e; ++r) {
+                const int q = b * P::R3 + r;
+                const V2 fq = f[b * (P::R3 / 2) + r - R0];
+                const int i0 = 2 * lane + 2 * (64 * b + r * (M / P::R3));
+                buffer_store_pair(v[q].x * fq.x, v[q].y * fq.y, rs, static_cast<int>(sizeof(T)) * i0);
+            }
         }
         if (nvRow & 1) {
             gptr<T> outG = to_global(outRow);
 #pragma unroll
-            for (int q = 0; q < NQ; ++q) {
-                const int i0 = 2 * lane + 2 * (64 * (q / P::R3) + (q % P::R3) * (M / P::R3));
-                if (i0 == nvRow - 1) { outG[i0] = v[q].x * f[q].x; }
+            for (int b = 0; b < B3; ++b) {
+#pragma unroll
+                for (int r = R0; r < R1e; ++r) {
+                    const int q = b * P::R3 + r;
+                    const int i0 = 2 * lane + 2 * (64 * b + r * (M / P::R3));
+                    if (i0 == nvRow - 1) { outG[i0] = v[q].x * f[b * (P::R3 / 2) + r - R0].x; }
+                }
             }
         }
-        if (anyFlag) { fixup_row_wave<T>(raw, scale, to_global(row.tmpl), row.Lt, to_global(outRow), nvRow, lane); }
-        if (sharedScale == nullptr) { wave_lds_sync(); }  // the next row reuses buf
+    };
+    using Half0 = std::integral_constant<int, 0>;
+    using Half1 = std::integral_constant<int, 1>;
+    // sharedScale is the row's block-shared scale, or null: the wave then
+    // computes the row's scale itself, half a block at a time, into its own
+    // exchange buffer, which is free once the inverse FFT is in registers.
+    auto finishRow = [&](const Row &row, const T *sharedScale, bool anyFlagShared) {
+        const int nvRow = rowValid(row);
+        T *outRow = static_cast<T *>(row.out) + base;
+        if (sharedScale != nullptr) {
+            epilogueHalf(Half0{}, sharedScale, 0, outRow, nvRow);
+            epilogueHalf(Half1{}, sharedScale, 0, outRow, nvRow);
+            if (anyFlagShared && nvRow > 0) {
+                fixup_row_wave<T>(raw, sharedScale, to_global(row.tmpl), row.Lt, to_global(outRow), nvRow, lane);
+            }
+            return;
+        }
+        T *own = reinterpret_cast<T *>(xb + wave * XW);
+        const gptr<const double> tmplG = to_global(row.tmpl);
+        if constexpr (!HALF) {
+            // the whole row's scale at once (the full buffer holds N values)
+            int nf = 0;
+            if (nvRow > 0) { nf = compute_scale<T, TILE, 4>(raw, ts1, ts2, tp1, tp2, own, a, nvRow, row.Lt, noise, lane, 64); }
+            const bool anyFlag = __ballot(nf > 0) != 0ull;
+            wave_lds_sync();
+            epilogueHalf(Half0{}, own, 0, outRow, nvRow);
+            epilogueHalf(Half1{}, own, 0, outRow, nvRow);
+            if (anyFlag) { fixup_row_wave<T>(raw, own, tmplG, row.Lt, to_global(outRow), nvRow, lane); }
+            wave_lds_sync();  // the next row reuses buf
+            return;
+        }
+        // half 0: outputs [0, N/2)
+        {
+            const int n0 = min(nvRow, N / 2);
+            int nf = 0;
+            if (n0 > 0) { nf = compute_scale<T, TILE, 4>(raw, ts1, ts2, tp1, tp2, own, a, n0, row.Lt, noise, lane, 64, 0); }
+            const bool anyFlag = __ballot(nf > 0) != 0ull;
+            wave_lds_sync();
+            epilogueHalf(Half0{}, own, 0, outRow, nvRow);
+            if (anyFlag) { fixup_row_wave<T>(raw, own, tmplG, row.Lt, to_global(outRow), n0, lane, 0); }
+            wave_lds_sync();  // the second half reuses the buffer
+        }
+        // half 1: outputs [N/2, N)
+        {
+            const int n1 = nvRow - N / 2;
+            int nf = 0;
+            if (n1 > 0) {
+                nf = compute_scale<T, TILE, 4>(raw, ts1, ts2, tp1, tp2, own, a, n1, row.Lt, noise, lane, 64, N / 2);
+            }
+            const bool anyFlag = __ballot(nf > 0) != 0ull;
+            wave_lds_sync();
+            epilogueHalf(Half1{}, own, N / 2, outRow, nvRow);
+            if (anyFlag) { fixup_row_wave<T>(raw, own, tmplG, row.Lt, to_global(outRow), nvRow, lane, N / 2); }
+            wave_lds_sync();  // the next row reuses the buffer
+        }
     };
 
-    // D. rows in groups of equal template length.  A group of NWAVE rows or
-    // more shares one scale computed by the block; rows of smaller groups
-    // are "solo": each wave takes every NWAVE-th of them and computes its
-    // row's scale itself, so no wave waits for a group it has no row in.
+    // D. rows in groups of equal template length.  With a shared scale
+    // buffer, a group of NWAVE rows or more shares one scale computed by
+    // the block; rows of smaller groups (every row without the buffer) are
+    // "solo": each wave takes every NWAVE-th of them and computes its row's
+    // scale itself, so no wave waits for a group it has no row in.
     bool fftDone = false;
-    for (int g0 = sg.rowBegin; g0 < rowEnd;) {
-        const int Lt = rows[g0].Lt;
-        int g1 = g0 + 1;
-        while (g1 < rowEnd && rows[g1].Lt == Lt) { ++g1; }
-        if (g1 - g0 >= NWAVE) {
-            int myFlags = 0;
-            if (!fftDone) {
-                // wave 0 transforms while waves 1..3 build the scale
-                if (wave == 0) {
-                    forwardFft();
+    if constexpr (SHARED) {
+        for (int g0 = sg.rowBegin; g0 < rowEnd;) {
+            const int Lt = rows[g0].Lt;
+            int g1 = g0 + 1;
+            while (g1 < rowEnd && rows[g1].Lt == Lt) { ++g1; }
+            if (g1 - g0 >= NWAVE) {
+                int myFlags = 0;
+                if (!fftDone) {
+                    // wave 0 transforms while waves 1..3 build the scale
+                    if (wave == 0) {
+                        forwardFft();
+                    } else {
+                        myFlags = compute_scale<T, TILE>(raw, ts1, ts2, tp1, tp2, sc, a, nValidSeg, Lt, noise,
+                                                         tid - 64, NT - 64);
+                    }
+                    fftDone = true;
                 } else {
-                    myFlags = compute_scale<T, TILE>(raw, ts1, ts2, tp1, tp2, sc, a, nValidSeg, Lt, noise, tid - 64,
-                                               NT - 64);
+                    myFlags = compute_scale<T, TILE>(raw, ts1, ts2, tp1, tp2, sc, a, nValidSeg, Lt, noise, tid, NT);
                 }
-                fftDone = true;
-            } else {
-                myFlags = compute_scale<T, TILE>(raw, ts1, ts2, tp1, tp2, sc, a, nValidSeg, Lt, noise, tid, NT);
-            }
-            const bool anyFlag = block_sum<NT>(static_cast<double>(myFlags), red) > 0.0;
-            int r = g0 + wave;
-            if (r < g1) {
-                loadH(rows[r].H);
-                startRow(rows[r], rows[r + NWAVE < g1 ? r + NWAVE : r].H);
-                for (int rn = r + NWAVE; rn < g1; rn += NWAVE) {
+                const bool anyFlag = block_sum<NT>(static_cast<double>(myFlags), red) > 0.0;
+                int r = g0 + wave;
+                if (r < g1) {
+                    if (PREFETCH) { loadH(rows[r].H); }
+                    startRow(rows[r], rows[r + NWAVE < g1 ? r + NWAVE : r].H);
+                    for (int rn = r + NWAVE; rn < g1; rn += NWAVE) {
+                        finishRow(rows[r], sc, anyFlag);
+                        startRow(rows[rn], rows[rn + NWAVE < g1 ? rn + NWAVE : rn].H);
+                        r = rn;
+                    }
                     finishRow(rows[r], sc, anyFlag);
-                    startRow(rows[rn], rows[rn + NWAVE < g1 ? rn + NWAVE : rn].H);
-                    r = rn;
                 }
-                finishRow(rows[r], sc, anyFlag);
+                __syncthreads();  // sc is rebuilt for the next group
             }
-            __syncthreads();  // sc is rebuilt for the next group
+            g0 = g1;
         }
-        g0 = g1;
     }
     if (!fftDone) {
         if (wave == 0) { forwardFft(); }
@@ -772,7 +891,7 @@ __global__ __launch_bounds__(NT, (Corr2Waves<T, M>::value)) void corr2_kernel(co
                 const int Lt = rows[sg0].Lt;
                 sg1 = sg0 + 1;
                 while (sg1 < rowEnd && rows[sg1].Lt == Lt) { ++sg1; }
-                sr = (sg1 - sg0 >= NWAVE) ? sg1 : sg0;
+                sr = (SHARED && sg1 - sg0 >= NWAVE) ? sg1 : sg0;
                 continue;
             }
             const int r = sr++;
@@ -781,7 +900,7 @@ __global__ __launch_bounds__(NT, (Corr2Waves<T, M>::value)) void corr2_kernel(co
     };
     int cur = nextSolo();
     if (cur >= 0) {
-        loadH(rows[cur].H);
+        if (PREFETCH) { loadH(rows[cur].H); }
         int nx = nextSolo();
         startRow(rows[cur], rows[nx >= 0 ? nx : cur].H);
         while (nx >= 0) {
