@@ -282,9 +282,35 @@ inline std::mutex &costMutex()
     return m;
 }
 
+// The FP32 default is mf_autotune's measurement on an MI355X (microseconds
+// per block for the whole GPU; variant 1 block-wide, 2 wave-level); FP64
+// starts from the static model below.
+inline CostTable defaultCostTable(int precision)
+{
+    CostTable t;
+#ifdef MF_STATIC_FORMULA
+    return t;
+#endif
+    if (precision != MF_PRECISION_FLOAT) { return t; }
+    struct E { int logM, v; double a, bS, bO; };
+    static const E kMi355x[] = {
+        {9, 1, 0.00949, 0.00331, 0.00551},  {9, 2, 0.01247, 0.00135, 0.00237},
+        {10, 1, 0.01468, 0.00583, 0.00882}, {10, 2, 0.02817, 0.00176, 0.00415},
+        {11, 1, 0.03286, 0.01595, 0.02210}, {11, 2, 0.05440, 0.00756, 0.00747},
+        {12, 1, 0.08531, 0.05636, 0.07194},
+    };
+    for (const E &e : kMi355x) {
+        t.a[e.logM][e.v] = e.a;
+        t.bShared[e.logM][e.v] = e.bS;
+        t.bSolo[e.logM][e.v] = e.bO;
+    }
+    t.measured = true;
+    return t;
+}
+
 inline CostTable &costTable(int precision)
 {
-    static CostTable t[2];
+    static CostTable t[2] = {defaultCostTable(MF_PRECISION_FLOAT), defaultCostTable(MF_PRECISION_DOUBLE)};
     return t[precision == MF_PRECISION_FLOAT ? 0 : 1];
 }
 
@@ -359,6 +385,32 @@ inline void laneRowBlocks(const LaneRows &rs, long long L, double &shared, doubl
     }
 }
 
+// Risk margin of larger FFT blocks: each doubling of the block over the
+// smallest size adds kBlockRisk to the predicted cost.  Two effects the
+// per-block table does not see grow with the block: windows recomputed
+// exactly (the FFT noise bound of a block holding a loud stretch flags
+// every quiet window of that block, so the flagged share of a row grows
+// with the block length), and the per-output scale work of rows whose
+// templates are much shorter than the calibration's N/4.  On the
+// precision-stress workload (gaps at other levels) nfft 1024 measured 2.1x
+// faster than 2048 and 3.1x faster than 4096, while clean data ran within
+// 20 % at all three.
+constexpr double kBlockRisk = 0.5;
+inline double blockRisk(int logM) { return 1.0 + kBlockRisk * (logM - mfk::kMinLogM); }
+
+// Extra cost of a lane whose template spectra do not stay in its XCD's L2
+// (4 MB): each row block then streams its row's spectrum, (M + 1) complex
+// values, from the MALL or HBM, priced at 4 TB/s for the whole GPU (half the
+// HBM peak: the reads compete with the rho writes).  The autotuner's
+// synthetic lanes (8 rows) never reach this, so the term is added on top.
+inline double spectrumPenalty(int precision, int logM, long long rows, double rowBlocks)
+{
+    constexpr double kL2Budget = 3.0 * (1 << 20);
+    const double specBytes = static_cast<double>((1LL << logM) + 1) * 2.0 * (precision == MF_PRECISION_FLOAT ? 4 : 8);
+    if (static_cast<double>(rows) * specBytes <= kL2Budget) { return 0.0; }
+    return rowBlocks * specBytes / 4.0e12 * 1.0e6;  // microseconds
+}
+
 // Predicted cost of a lane for an FFT size and a variant (0 = the default
 // variant), or -1 if the plan is impossible: blocks x the per-block part
 // plus row blocks x the per-row part.
@@ -370,6 +422,7 @@ inline double laneCost(int precision, int logM, int variant, const LaneRows &rs)
     const double blocks = std::ceil(static_cast<double>(rs.maxOut()) / static_cast<double>(L));
     double shared = 0, solo = 0;
     laneRowBlocks(rs, L, shared, solo);
+    const double penalty = spectrumPenalty(precision, logM, static_cast<long long>(rs.Lt.size()), shared + solo);
     std::lock_guard<std::mutex> lk(costMutex());
     const CostTable &t = costTable(precision);
     if (t.measured) {
@@ -381,14 +434,15 @@ inline double laneCost(int precision, int logM, int variant, const LaneRows &rs)
                 const double c = blocks * t.a[logM][cand] + t.bShared[logM][cand] * shared + t.bSolo[logM][cand] * solo;
                 if (best < 0 || c < best) { best = c; v = cand; }
             }
-            if (best >= 0) { return best; }
+            if (best >= 0) { return best * blockRisk(logM) + penalty; }
         } else if (t.a[logM][v] > 0) {
-            return blocks * t.a[logM][v] + t.bShared[logM][v] * shared + t.bSolo[logM][v] * solo;
+            return (blocks * t.a[logM][v] + t.bShared[logM][v] * shared + t.bSolo[logM][v] * solo) * blockRisk(logM) +
+                   penalty;
         }
     }
     // static model: per-block part and per-row parts at the same scale
     const double unit = staticBlockCost(logM, RowMix{0, 0}, precision);
-    return unit * (blocks + 1.15 * shared + 1.84 * solo);
+    return unit * (blocks + 1.15 * shared + 1.84 * solo) * blockRisk(logM) + penalty;
 }
 
 // Correlation kernel variant of one lane: a requested variant if it exists
@@ -453,12 +507,20 @@ inline int chooseLogM(int precision, int userNfft, const LaneRows &rs)
     return best;
 }
 
+// Block length of a lane: nfft - nb + 1, made even so that every block
+// start stays 8-byte aligned for the paired stores of the epilogue.
+inline int blockLength(int logM, int nb)
+{
+    int L = static_cast<int>((2LL << logM) - nb + 1);
+    if (L > 1 && (L & 1)) { L -= 1; }
+    return L;
+}
+
 // Length buckets of one channel's rows.  Rows (sorted by template length)
 // are split into contiguous buckets; each bucket becomes a lane with its own
-// FFT size, so short templates are not forced onto the long templates'
-// blocks.  Start: one bucket per FFT size a row would pick alone; then the
-// adjacent pair whose merge saves the most predicted cost is merged while
-// any merge saves (a merge shares the forward FFT and block loads).
+// FFT size and block length, so short templates are not forced onto the
+// long templates' short blocks, while a shared bucket saves the forward FFT
+// and block loads.  planBuckets finds the split of least predicted cost.
 struct Bucket {
     int begin = 0, end = 0;  // [begin, end) of the sorted rows
     int nb = 0;              // longest template of the bucket
@@ -496,42 +558,93 @@ inline std::vector<Bucket> planBuckets(int precision, int userNfft, const std::v
 {
     const int n = static_cast<int>(sortedLt.size());
     if (userNfft > 0 || n <= 1) { return {makeBucket(precision, userNfft, sortedLt, nOut, 0, n)}; }
-    std::vector<Bucket> bk;
-    auto alone = [&](int i) { return chooseLogM(precision, 0, laneRows(sortedLt, nOut, i, i + 1)); };
-    for (int i = 0; i < n;) {
-        const int lg = alone(i);
-        int j = i + 1;
-        while (j < n && alone(j) == lg) { ++j; }
-        bk.push_back(makeBucket(precision, 0, sortedLt, nOut, i, j));
-        i = j;
+    // Optimal split into contiguous buckets by dynamic programming over the
+    // boundaries between runs of equal length (a run stays in one bucket, so
+    // its rows keep sharing a scale): best[J] = min over I < J and FFT size
+    // of best[I] + cost(runs I..J-1), where a bucket's cost is its blocks
+    // times the per-block part plus its rows' blocks times the per-row
+    // parts, at the block length its longest template allows.
+    std::vector<int> bnd{0};
+    for (int i = 1; i < n; ++i) {
+        if (sortedLt[i] != sortedLt[i - 1]) { bnd.push_back(i); }
     }
-    while (bk.size() > 1) {
-        int bestK = -1;
-        double bestGain = 0;
-        Bucket bestM;
-        for (size_t k = 0; k + 1 < bk.size(); ++k) {
-            const Bucket m = makeBucket(precision, 0, sortedLt, nOut, bk[k].begin, bk[k + 1].end);
-            const double gain = bk[k].cost + bk[k + 1].cost - m.cost;
-            if (gain > bestGain) {
-                bestGain = gain;
-                bestK = static_cast<int>(k);
-                bestM = m;
+    bnd.push_back(n);
+    const int R = static_cast<int>(bnd.size()) - 1;
+    std::vector<char> shared(n, 0);
+    for (int k = 0; k < R; ++k) {
+        if (bnd[k + 1] - bnd[k] >= 4) { std::fill(shared.begin() + bnd[k], shared.begin() + bnd[k + 1], 1); }
+    }
+    CostTable t;
+    {
+        std::lock_guard<std::mutex> lk(costMutex());
+        t = costTable(precision);
+    }
+    const int hiLog = maxLogM(precision);
+    // per-block and per-row-block parts of an FFT size, best variant for
+    // the given mix of row blocks
+    auto cost = [&](int lg, double blocks, double sS, double sO, long long rows) {
+        double best = -1;
+        if (t.measured) {
+            for (int v = 1; v <= 2; ++v) {
+                if (t.a[lg][v] <= 0 || !variantAvailable(precision, lg, v)) { continue; }
+                const double c = blocks * t.a[lg][v] + sS * t.bShared[lg][v] + sO * t.bSolo[lg][v];
+                if (best < 0 || c < best) { best = c; }
             }
         }
-        if (bestK < 0) { break; }
-        bk[bestK] = bestM;
-        bk.erase(bk.begin() + bestK + 1);
+        if (best < 0) {
+            const double unit = staticBlockCost(lg, RowMix{0, 0}, precision);
+            best = unit * (blocks + 1.15 * sS + 1.84 * sO);
+        }
+        return best * blockRisk(lg) + spectrumPenalty(precision, lg, rows, sS + sO);
+    };
+    std::vector<double> best(R + 1, -1);
+    std::vector<int> fromRun(R + 1, -1), fromLog(R + 1, -1);
+    best[0] = 0;
+    for (int J = 1; J <= R; ++J) {
+        const int j = bnd[J];
+        const int nb = sortedLt[j - 1];
+        for (int lg = mfk::kMinLogM; lg <= hiLog; ++lg) {
+            const long long N = 2LL << lg;
+            if (N - nb + 1 < 2) { continue; }
+            if (N - nb + 1 < N / 4 && lg < hiLog) { continue; }  // keep >= 25% useful outputs
+            const double L = static_cast<double>(blockLength(lg, nb));
+            double sS = 0, sO = 0;
+            long long maxOut = 0;
+            for (int I = J - 1; I >= 0; --I) {
+                for (int r = bnd[I]; r < bnd[I + 1]; ++r) {
+                    const double rb = std::ceil(static_cast<double>(nOut[r]) / L);
+                    (shared[r] ? sS : sO) += rb;
+                    maxOut = std::max(maxOut, nOut[r]);
+                }
+                if (best[I] < 0) { continue; }
+                const double c = best[I] + cost(lg, std::ceil(static_cast<double>(maxOut) / L), sS, sO, j - bnd[I]);
+                if (best[J] < 0 || c < best[J]) {
+                    best[J] = c;
+                    fromRun[J] = I;
+                    fromLog[J] = lg;
+                }
+            }
+        }
+        if (best[J] < 0) {
+            fail(MF_ERR_UNSUPPORTED, "template length " + std::to_string(nb) +
+                                         " exceeds the largest supported FFT block (" + std::to_string(2 << hiLog) +
+                                         " samples)");
+        }
     }
+    std::vector<Bucket> bk;
+    for (int J = R; J > 0; J = fromRun[J]) {
+        Bucket b;
+        b.begin = bnd[fromRun[J]];
+        b.end = bnd[J];
+        b.nb = sortedLt[b.end - 1];
+        const LaneRows rs = laneRows(sortedLt, nOut, b.begin, b.end);
+        b.mix = rs.mix();
+        b.logM = fromLog[J];
+        b.cost = laneCost(precision, b.logM, 0, rs);
+        bk.push_back(b);
+    }
+    std::reverse(bk.begin(), bk.end());
     return bk;
-}
-
-// Block length of a lane: nfft - nb + 1, made even so that every block
-// start stays 8-byte aligned for the paired stores of the epilogue.
-inline int blockLength(int logM, int nb)
-{
-    int L = static_cast<int>((2LL << logM) - nb + 1);
-    if (L > 1 && (L & 1)) { L -= 1; }
-    return L;
 }
 
 // Work items of a set of lanes sharing one FFT size, in XCD-aware order:

@@ -59,6 +59,17 @@ struct Eps<double> {
 };
 constexpr double kNoiseK = 4.0;
 
+// Scale buffer encoding of a window: f > 0, rho = c f; -f (FP32 correlation:
+// a window whose FFT numerator may be off by the noise bound while its FP64
+// window sums hold): rho = (exact FP64 numerator) f; -infinity (the window
+// sums lost digits, or any flag in FP64 correlation): the whole Pearson
+// value is recomputed exactly.
+template <typename T>
+__device__ __forceinline__ T scale_exact()
+{
+    return -static_cast<T>(INFINITY);
+}
+
 // Exclusive prefix sums tp[q] = sum of ts[q'] for q' < q (q = 0..NTILE) of
 // the 16-sample tile sums, computed by wave 0 with a shuffle scan (fixed
 // order, deterministic).  The caller synchronises before reading tp.
@@ -175,26 +186,93 @@ __device__ __forceinline__ int compute_scale(const T *raw, const double *ts1, co
 #pragma unroll UNROLL
     for (int i = i0; i < i1; ++i) {
         const double den2 = Lt * S2 - S1 * S1;
-        const bool flag = !(den2 > canc * S2max) || !(den2 > denThr2);
-        nflag += flag ? 1 : 0;
+        const bool canc1 = !(den2 > canc * S2max);
+        const bool noisy = !(den2 > denThr2);
+        nflag += (canc1 || noisy) ? 1 : 0;
+        T f;
         if constexpr (sizeof(T) == 4) {
-            float f = -1.0f;
-            if (!flag) {
-                const float d2 = static_cast<float>(den2);
-                float y = __builtin_amdgcn_rsqf(d2);
-                y = y * (1.5f - 0.5f * d2 * y * y);
-                f = sqrtLtF * y;
-            }
-            sc[i] = f;
+            const float d2 = static_cast<float>(den2);
+            float y = __builtin_amdgcn_rsqf(d2);
+            y = y * (1.5f - 0.5f * d2 * y * y);
+            f = sqrtLtF * y;
         } else {
-            sc[i] = flag ? T(-1) : static_cast<T>(sqrtLt / sqrt(den2));
+            f = static_cast<T>(sqrtLt / sqrt(den2));
         }
+        // FP64 correlation recomputes every flagged window whole: its 1e-12
+        // tolerance leaves no room for the window sums' own rounding
+        sc[i] = (canc1 || (sizeof(T) == 8 && noisy)) ? scale_exact<T>() : (noisy ? -f : f);
         if (i + 1 < i1) {
             const double dn = static_cast<double>(raw[i + Lt]) - a;
             const double dd = static_cast<double>(raw[i]) - a;
             S1 += dn - dd;
             S2 += dn * dn - dd * dd;
             S2max = fmax(S2max, S2);
+        }
+    }
+    return nflag;
+}
+
+// The FP32 scales of one row's outputs [first, first + nValid) computed by
+// one wave, for rows that have no shared scale.  Lane t runs NCH
+// interleaved chains (independent work in every step), chain k over the
+// outputs [first + (64 k + t) Cc, ... + Cc) with Cc odd: FP64 window sums
+// from the tile prefix at the chain start, then one sliding update per
+// output, S1 += xn - xo and S2 += (xn - xo)(xn + xo - 2a).  A window is
+// flagged (scale -1, recomputed exactly later) when den^2 = Lt S2 - S1^2 is
+// not above max(canc S2span, denThr^2): S2span, the energy of the tiles the
+// chain's windows cover, bounds every window sum of the chain, so the
+// cancellation test of compute_scale holds with one compare per output.
+// Returns this lane's number of flagged outputs.
+template <int TILE, int NCH>
+__device__ __forceinline__ int wave_row_scale(const float *raw, const double *tp1, const double *tp2, float *sc,
+                                              double a, int nValid, int Lt, double noise, int lane, int first)
+{
+    const double sqrtLt = sqrt(static_cast<double>(Lt));
+    const double denThr2 = sqrtLt * noise * sqrtLt * noise;
+    const float sqrtLtF = static_cast<float>(sqrtLt);
+    const int Cc = ((nValid + 64 * NCH - 1) / (64 * NCH)) | 1;
+    const double canc = 2.0e-9 * (Lt + Cc) * Lt;
+    const int end = first + nValid;
+    const double a2 = a + a;
+    double S1[NCH], S2[NCH], thr[NCH], cancThr[NCH];
+    int i0[NCH];
+#pragma unroll
+    for (int k = 0; k < NCH; ++k) {
+        i0[k] = first + (64 * k + lane) * Cc;
+        S1[k] = S2[k] = 0;
+        thr[k] = denThr2;
+        cancThr[k] = 0;
+        if (i0[k] < end) {
+            window_sums<float, TILE>(raw, nullptr, nullptr, tp1, tp2, a, i0[k], Lt, S1[k], S2[k]);
+            const int last = min(i0[k] + Cc, end) - 1;
+            const int qa = i0[k] / TILE, qb = (last + Lt + TILE - 1) / TILE;
+            cancThr[k] = canc * (tp2[qb] - tp2[qa]);
+            thr[k] = fmax(cancThr[k], denThr2);
+        }
+    }
+    sc -= first;
+    int nflag = 0;
+    for (int j = 0; j < Cc; ++j) {
+#pragma unroll
+        for (int k = 0; k < NCH; ++k) {
+            const int i = i0[k] + j;
+            if (i < end) {
+                const double den2 = Lt * S2[k] - S1[k] * S1[k];
+                const bool flag = !(den2 > thr[k]);
+                nflag += flag ? 1 : 0;
+                const float d2 = static_cast<float>(den2);
+                const float y = __builtin_amdgcn_rsqf(d2);
+                const float h = fmaf(-0.5f * d2, y * y, 1.5f);
+                const float f = (sqrtLtF * y) * h;
+                // noise flag: the scale holds, the numerator is recomputed
+                const bool canc1 = !(den2 > cancThr[k]);
+                sc[i] = canc1 ? scale_exact<float>() : (flag ? -f : f);
+                const double xn = static_cast<double>(raw[i + Lt]);
+                const double xo = static_cast<double>(raw[i]);
+                const double e = xn - xo;
+                S1[k] += e;
+                S2[k] = fma(e, (xn + xo) - a2, S2[k]);
+            }
         }
     }
     return nflag;
@@ -413,39 +491,64 @@ __global__ __launch_bounds__(NT) void corr_kernel(const Item *__restrict__ items
 // scale of a group is shared by its rows.
 // ---------------------------------------------------------------------------
 
-// Exact FP64 two-pass Pearson for the flagged outputs [first, nv) of one
-// row (one wave); sc[i - first] is output i's scale (negative: flagged).
+// Fix-up of the flagged outputs [first, nv) of one row (one wave);
+// sc[i - first] < 0 flags output i (see scale_exact).  A noise flag (-f)
+// keeps the window's scale f and recomputes the numerator exactly in FP64
+// from the anchored samples, sum_m t[m] (x[i+m] - a) (the template has zero
+// mean, so the window mean drops out of the numerator); -infinity
+// recomputes the two-pass Pearson value.
 template <typename T>
 __device__ __forceinline__ void fixup_row_wave(const T *raw, const T *sc, gptr<const double> tmpl, int Lt,
-                                               gptr<T> out, int nv, int lane, int first = 0)
+                                               gptr<T> out, int nv, int lane, double a, int first = 0)
 {
     sc -= first;
     for (int w0 = first; w0 < nv; w0 += 64) {
         const int i = w0 + lane;
-        const bool fl = (i < nv) && (sc[i] < T(0));
+        const T s = i < nv ? sc[i] : T(1);
+        const bool fl = s < T(0);
         unsigned long long m = __ballot(fl);
         if (__popcll(m) >= kDenseFlags) {
-            // many flagged windows (a gap, a dead stretch): one lane per window
-            if (fl) { out[i] = static_cast<T>(exact_window<T>(raw, i, tmpl, Lt)); }
+            // many flagged windows (a gap, a loud stretch): one lane per window
+            if (fl) {
+                double rho;
+                if (isinf(s)) {
+                    rho = exact_window<T>(raw, i, tmpl, Lt);
+                } else {
+                    double dn = 0;
+                    for (int mm = 0; mm < Lt; ++mm) { dn = fma(tmpl[mm], static_cast<double>(raw[i + mm]) - a, dn); }
+                    rho = dn * static_cast<double>(-s);
+                    if (fabs(rho) > 1.1) { rho = 0; }
+                }
+                out[i] = static_cast<T>(rho);
+            }
             continue;
         }
         while (m != 0ull) {
             const int b = __ffsll(static_cast<long long>(m)) - 1;
             m &= m - 1ull;
             const int wi = w0 + b;
-            double sm = 0;
-            for (int mm = lane; mm < Lt; mm += 64) { sm += static_cast<double>(raw[wi + mm]); }
-            sm = wave_sum(sm);
-            const double mean = sm / Lt;
-            double e2 = 0, dn = 0;
-            for (int mm = lane; mm < Lt; mm += 64) {
-                const double d = static_cast<double>(raw[wi + mm]) - mean;
-                e2 += d * d;
-                dn += tmpl[mm] * d;
+            const T sw = sc[wi];
+            double rho;
+            if (isinf(sw)) {
+                double sm = 0;
+                for (int mm = lane; mm < Lt; mm += 64) { sm += static_cast<double>(raw[wi + mm]); }
+                sm = wave_sum(sm);
+                const double mean = sm / Lt;
+                double e2 = 0, dn = 0;
+                for (int mm = lane; mm < Lt; mm += 64) {
+                    const double d = static_cast<double>(raw[wi + mm]) - mean;
+                    e2 += d * d;
+                    dn += tmpl[mm] * d;
+                }
+                e2 = wave_sum(e2);
+                dn = wave_sum(dn);
+                rho = dn / fmax(DBL_EPSILON, sqrt(e2));
+            } else {
+                double dn = 0;
+                for (int mm = lane; mm < Lt; mm += 64) { dn = fma(tmpl[mm], static_cast<double>(raw[wi + mm]) - a, dn); }
+                dn = wave_sum(dn);
+                rho = dn * static_cast<double>(-sw);
             }
-            e2 = wave_sum(e2);
-            dn = wave_sum(dn);
-            double rho = dn / fmax(DBL_EPSILON, sqrt(e2));
             if (fabs(rho) > 1.1) { rho = 0; }
             if (lane == 0) { out[wi] = static_cast<T>(rho); }
         }
@@ -789,21 +892,29 @@ __global__ __launch_bounds__(NT, (Corr2Shape<T, M>::BLOCKS)) void corr2_kernel(c
             epilogueHalf(Half0{}, sharedScale, 0, outRow, nvRow);
             epilogueHalf(Half1{}, sharedScale, 0, outRow, nvRow);
             if (anyFlagShared && nvRow > 0) {
-                fixup_row_wave<T>(raw, sharedScale, to_global(row.tmpl), row.Lt, to_global(outRow), nvRow, lane);
+                fixup_row_wave<T>(raw, sharedScale, to_global(row.tmpl), row.Lt, to_global(outRow), nvRow, lane, a);
             }
             return;
         }
         T *own = reinterpret_cast<T *>(xb + wave * XW);
         const gptr<const double> tmplG = to_global(row.tmpl);
+        // a row's own scale: interleaved chains in FP32, compute_scale in FP64
+        auto ownScale = [&](int n, int first) {
+            if constexpr (sizeof(T) == 4) {
+                return wave_row_scale<TILE, 2>(raw, tp1, tp2, own, a, n, row.Lt, noise, lane, first);
+            } else {
+                return compute_scale<T, TILE, 4>(raw, ts1, ts2, tp1, tp2, own, a, n, row.Lt, noise, lane, 64, first);
+            }
+        };
         if constexpr (!HALF) {
             // the whole row's scale at once (the full buffer holds N values)
             int nf = 0;
-            if (nvRow > 0) { nf = compute_scale<T, TILE, 4>(raw, ts1, ts2, tp1, tp2, own, a, nvRow, row.Lt, noise, lane, 64); }
+            if (nvRow > 0) { nf = ownScale(nvRow, 0); }
             const bool anyFlag = __ballot(nf > 0) != 0ull;
             wave_lds_sync();
             epilogueHalf(Half0{}, own, 0, outRow, nvRow);
             epilogueHalf(Half1{}, own, 0, outRow, nvRow);
-            if (anyFlag) { fixup_row_wave<T>(raw, own, tmplG, row.Lt, to_global(outRow), nvRow, lane); }
+            if (anyFlag) { fixup_row_wave<T>(raw, own, tmplG, row.Lt, to_global(outRow), nvRow, lane, a); }
             wave_lds_sync();  // the next row reuses buf
             return;
         }
@@ -811,24 +922,22 @@ __global__ __launch_bounds__(NT, (Corr2Shape<T, M>::BLOCKS)) void corr2_kernel(c
         {
             const int n0 = min(nvRow, N / 2);
             int nf = 0;
-            if (n0 > 0) { nf = compute_scale<T, TILE, 4>(raw, ts1, ts2, tp1, tp2, own, a, n0, row.Lt, noise, lane, 64, 0); }
+            if (n0 > 0) { nf = ownScale(n0, 0); }
             const bool anyFlag = __ballot(nf > 0) != 0ull;
             wave_lds_sync();
             epilogueHalf(Half0{}, own, 0, outRow, nvRow);
-            if (anyFlag) { fixup_row_wave<T>(raw, own, tmplG, row.Lt, to_global(outRow), n0, lane, 0); }
+            if (anyFlag) { fixup_row_wave<T>(raw, own, tmplG, row.Lt, to_global(outRow), n0, lane, a, 0); }
             wave_lds_sync();  // the second half reuses the buffer
         }
         // half 1: outputs [N/2, N)
         {
             const int n1 = nvRow - N / 2;
             int nf = 0;
-            if (n1 > 0) {
-                nf = compute_scale<T, TILE, 4>(raw, ts1, ts2, tp1, tp2, own, a, n1, row.Lt, noise, lane, 64, N / 2);
-            }
+            if (n1 > 0) { nf = ownScale(n1, N / 2); }
             const bool anyFlag = __ballot(nf > 0) != 0ull;
             wave_lds_sync();
             epilogueHalf(Half1{}, own, N / 2, outRow, nvRow);
-            if (anyFlag) { fixup_row_wave<T>(raw, own, tmplG, row.Lt, to_global(outRow), nvRow, lane, N / 2); }
+            if (anyFlag) { fixup_row_wave<T>(raw, own, tmplG, row.Lt, to_global(outRow), nvRow, lane, a, N / 2); }
             wave_lds_sync();  // the next row reuses the buffer
         }
     };
