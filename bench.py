@@ -75,7 +75,7 @@ def serve_section(args, torch):
     log(f"serving: batched capacity {cap['corr_samples_per_s'] / 1e9:.1f} G corr samples/s")
     runs, audit = [], []
     for lam in args.serve_lambda:
-        n_req = max(200, int(lam * args.serve_seconds))
+        n_req = max(50, int(lam * args.serve_seconds))  # at least 50 requests: 5 s at 10/s
         r, a = loadgen.run_poisson(srv, wl, res, lib_ids, lam, n_req, seed=3 + int(lam))
         runs.append(r)
         audit += a
@@ -778,7 +778,7 @@ def main() -> int:
     ap.add_argument("--precision-only", action="store_true", help="only the precision-stress workload")
     ap.add_argument("--uneven-only", action="store_true", help="only the uneven batched workload")
     ap.add_argument("--serve-only", action="store_true", help="only the served-stream measurement")
-    ap.add_argument("--serve-lambda", type=float, nargs="+", default=[500.0, 1000.0])
+    ap.add_argument("--serve-lambda", type=float, nargs="+", default=[10.0, 100.0, 500.0, 1000.0, 2000.0])
     ap.add_argument("--serve-seconds", type=float, default=2.0)
     ap.add_argument("--serve-requests", type=int, default=400, help="requests of the capacity run")
     ap.add_argument("--serve-ref-requests", type=int, default=6)
