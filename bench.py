@@ -190,6 +190,16 @@ def single_channel_cpu(state, threads):
     }
 
 
+def _window_fracs(ws):
+    """Shares of a handle's windows (mf_get_window_stats) that were
+    recomputed exactly (flagged), flagged by the fast path's error bound,
+    recomputed in full (cancelled variance), and computed by solo rows."""
+    w = max(ws["windows"], 1)
+    return {"windows": ws["windows"], "flagged_frac": ws["flagged"] / w,
+            "bound_flag_frac": (ws["flagged"] - ws["full"]) / w, "full_recompute_frac": ws["full"] / w,
+            "solo_frac": ws["solo"] / w}
+
+
 def uneven_section(args, torch):
     """Uneven batched workload: 200 templates of 200-3000 samples, each on
     1-50 of 50 channels lasting 1-24 h; moveouts 0-500; 0-30 injections per
@@ -247,9 +257,11 @@ def uneven_section(args, torch):
         if len(wl.origins[t]):
             ptr, _ = h.stack_device(t)
             stack_at_origins[t] = _device_gather_f32(ptr, wl.origins[t])
+    ws = h.window_stats()
     h.close()
     log(f"uneven batch: {len(rows)} rows, {n_corr / 1e9:.2f} G corr samples, {dt * 1e3:.1f} ms per step "
-        f"(corr {corr_ms:.1f} ms), {n_corr / dt / 1e9:.1f} G corr samples/s, plans {dict(plans)}")
+        f"(corr {corr_ms:.1f} ms), {n_corr / dt / 1e9:.1f} G corr samples/s, plans {dict(plans)}; "
+        f"flagged windows {100.0 * ws['flagged'] / ws['windows']:.3f}%, solo {100.0 * ws['solo'] / ws['windows']:.1f}%")
     # plain per-template loop
     row_index = {rc: i for i, rc in enumerate(rows)}
     plain_rows = {}
@@ -295,6 +307,7 @@ def uneven_section(args, torch):
         "max_abs_drho_batched_vs_plain": diff,
         "injections_recovered": hits,
         "injections": n_inj,
+        "window_stats": _window_fracs(ws),
     }, state
 
 
@@ -465,7 +478,7 @@ def precision_section(args, torch):
             lags += [s0 - 100, s0 - 3, s0, s0 + 5, s0 + L // 2, s0 + L - 130, s0 + L - 2, s0 + L + 10]
         lag_sets[(c, t)] = np.clip(np.array(lags, dtype=np.int64), 0, n - 257)
     paths = {}
-    for name, variant in (("fft_fp32", 0), ("mfma_fp16_hilo", 3), ("mfma_bf16_hilo", 4)):
+    for name, variant in (("fft_fp32", 0), ("mfma_fp16_split2", 3), ("mfma_bf16_split3", 4)):
         h = capi.Handle(np.float32, "batched")
         h.set_templates([wl.templates[t] for c in range(C) for t in range(T)])
         h.set_batch_layout([n] * C, [c for c in range(C) for _ in range(T)], [c for c in range(C) for _ in range(T)], C)
@@ -482,6 +495,7 @@ def precision_section(args, torch):
             ms.append(h.timings()[0])
         n_corr = sum(h.filtered_length(r) for r in range(C * T))
         corr_ms = float(np.median(ms))
+        ws = h.window_stats()
         rows = {}
         for c, t in audit_rows:
             r = h.rho(c * T + t)
@@ -557,8 +571,11 @@ def precision_section(args, torch):
         plan = h.plan(0)
         h.close()
         paths[name] = {"corr_kernel_ms": corr_ms, "corr_samples_per_s": n_corr / (corr_ms * 1e-3),
-                       "plan_row0_nfft": plan[0], "rows": rows}
-        log(f"precision stress {name}: corr {corr_ms:.2f} ms, {n_corr / (corr_ms * 1e-3) / 1e9:.1f} G corr samples/s")
+                       "plan_row0_nfft": plan[0], "window_stats": _window_fracs(ws), "rows": rows}
+        fr = paths[name]["window_stats"]
+        log(f"precision stress {name}: corr {corr_ms:.2f} ms, {n_corr / (corr_ms * 1e-3) / 1e9:.1f} G corr samples/s, "
+            f"flagged windows {100.0 * fr['flagged_frac']:.3f}% (error bound {100.0 * fr['bound_flag_frac']:.3f}%, "
+            f"cancelled variance {100.0 * fr['full_recompute_frac']:.3f}%), solo {100.0 * fr['solo_frac']:.1f}%")
     # normalization scans: window scales of every audited (channel, template
     # length) from FP64 prefix scans and from compensated FP32 scans, kept at
     # the audit lags; throughput on a 64M-sample signal
@@ -891,8 +908,10 @@ def main() -> int:
     for e in range(E):
         det = np.asarray(peaks[e])
         hits += sum(1 for o in wl.origins[e] if det.size and np.min(np.abs(det - o)) <= 2)
+    ws = h.window_stats()
     log(f"{ms_per_step:.2f} ms/step, corr kernels {corr_avg:.2f} ms, "
-        f"{value / 1e9:.1f} G corr samples/s; detections {hits}/{E * len(wl.origins[0])}")
+        f"{value / 1e9:.1f} G corr samples/s; detections {hits}/{E * len(wl.origins[0])}; "
+        f"flagged windows {100.0 * ws['flagged'] / ws['windows']:.4f}%, solo {100.0 * ws['solo'] / ws['windows']:.1f}%")
 
     out = {
         "metric": METRIC,
@@ -924,6 +943,7 @@ def main() -> int:
             "avg_launch_ms": corr_avg,
         },
         "stack_ms_all_groups": float(np.mean(stack_ms)),
+        "window_stats": _window_fracs(ws),
         "detections_recovered": hits,
         "detections_injected": E * len(wl.origins[0]),
     }

@@ -372,6 +372,22 @@ int mf_autotune(int precision, double *table, int cap, int *n);
  * kernel of the last mf_shift_and_stack*. */
 int mf_get_timings(mf_handle *h, double *corrMs, double *stackMs);
 
+/* No reference counterpart: window statistics of the correlation launches
+ * since mf_initialize, summed over applies.  *windows: correlation outputs
+ * computed (every row's length per apply).  *flagged: windows whose fast
+ * value was replaced by an exact FP64 recomputation, counted on the
+ * device: the fast path's error bound exceeded the tolerance, or the
+ * window's variance lost too many digits.  *full: the flagged windows
+ * recomputed in full (two-pass Pearson value: the cancelled variances, and
+ * every flagged window of an FP64 handle); the others had their numerator
+ * recomputed with the window norm kept.  *solo: windows of rows that
+ * computed their window scale on their own wave instead of sharing a
+ * block-wide one (rows whose template length fewer than four rows of the
+ * same block share).  Any pointer may be NULL.  Waits for the handle's
+ * stream. */
+int mf_get_window_stats(mf_handle *h, long long *windows, long long *flagged, long long *full,
+                        long long *solo);
+
 /* ---- request server: no reference counterpart ---------------------------
  * Serving form of the path: a template library resident in HBM (FP64
  * templates and their spectra at every supported FFT size) and a request

@@ -102,6 +102,7 @@ _SIGNATURES = {
     "mf_detection_products": ([_vp, ctypes.c_int, _ip, _ip, _dp, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                ctypes.c_int, _dp, _dp, _dp, _dp, _ip], ctypes.c_int),
     "mf_get_timings": ([_vp, _dp, _dp], ctypes.c_int),
+    "mf_get_window_stats": ([_vp] + [ctypes.POINTER(ctypes.c_longlong)] * 4, ctypes.c_int),
     "mf_autotune": ([ctypes.c_int, _dp, ctypes.c_int, _ip], ctypes.c_int),
     "mf_server_create": ([ctypes.POINTER(_vp), ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_int],
                          ctypes.c_int),
@@ -590,6 +591,16 @@ class Handle:
         a, b = ctypes.c_double(), ctypes.c_double()
         check(lib().mf_get_timings(self._h, ctypes.byref(a), ctypes.byref(b)))
         return a.value, b.value
+
+    def window_stats(self):
+        """Window statistics summed over the applies since initialize:
+        windows (outputs computed), flagged (outputs recomputed exactly in
+        FP64), full (the flagged ones recomputed as a two-pass Pearson
+        value), solo (outputs of rows that computed their window scale
+        alone)."""
+        v = [ctypes.c_longlong() for _ in range(4)]
+        check(lib().mf_get_window_stats(self._h, *[ctypes.byref(x) for x in v]))
+        return dict(zip(("windows", "flagged", "full", "solo"), (x.value for x in v)))
 
 
 class Server:

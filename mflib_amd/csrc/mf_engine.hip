@@ -35,6 +35,10 @@
 
 using namespace mfh;
 
+// blocks of the exact-recomputation launch after the time-domain
+// correlation (grid-stride over the queued windows: 8 blocks per CU)
+constexpr int kTdFixBlocks = 2048;
+
 struct mf_handle {
     int precision = MF_PRECISION_FLOAT;
     int mode = MF_MODE_MULTI_CHANNEL;
@@ -118,7 +122,14 @@ struct mf_handle {
     // device state
     DevBuf dRho, dSpec, dTmpl, dTmplRaw, dHmax;
     DevBuf dSegs, dRowTab, dItems;
+    // window statistics (mf_get_window_stats): the device's running counts
+    // of windows recomputed exactly (all, in full), and per apply the
+    // windows computed and the windows of solo rows (both fixed by the plan)
+    DevBuf dFlagged;
+    long long winPerApply = 0, soloPerApply = 0, statApplies = 0;
     DevBuf dFrag, dTdRowTab, dTdItems;  // time-domain lanes
+    DevBuf dTdFix;                      //   their queue of flagged windows (count, then entries)
+    mfk::TdFixQueue tdFix{};
     DevBuf dStackRows, dStackOut;
     DevBuf dReduceRows, dRedDet, dRedId, dRedPk, dRedPkId;  // Detector reduce step
     DevBuf dProdJobs, dProdOut;                             // per-detection products
@@ -198,6 +209,8 @@ void dropDevice(mf_handle *h)
     h->dItems.release();
     h->dFrag.release();
     h->dTdRowTab.release();
+    h->dTdFix.release();
+    h->tdFix = mfk::TdFixQueue{};
     h->dTdItems.release();
     h->dStackRows.release();
     h->stacksReady = false;
@@ -250,6 +263,37 @@ void deriveLayout(mf_handle *h, std::vector<int> &rowCh, std::vector<int> &rowGr
         rowCh.assign(nT, 0);
         chLen.assign(1, h->signalSize);
     }
+}
+
+// Windows of the launches whose row computes its window scale on its own
+// wave, grouped as corr2_kernel groups a block's rows (runs of equal
+// template length in the row table; runs of kCorr2SharedGroup or more share
+// one scale where the kernel keeps the shared buffer).  The block-wide
+// kernel computes every scale for the whole block: no solo rows.
+long long soloWindows(const mf_handle *h, const std::vector<mfk::Item> &items, const std::vector<mfk::Row> &rowTab)
+{
+    long long solo = 0;
+    for (const auto &la : h->launches) {
+        if (la.KT > 0 || la.variant != 2) { continue; }
+        const bool shared = mfk::corr2_shared_scale(static_cast<int>(h->tsize), 1 << la.logM);
+        for (int k = la.first; k < la.first + la.count; ++k) {
+            const mfk::Item &it = items[k];
+            const mfk::Seg &sg = h->segHost[it.seg];
+            const long long base = static_cast<long long>(it.block) * sg.L;
+            const int end = it.nRows < 0 ? sg.rowEnd : std::min(sg.rowEnd, sg.rowBegin + it.nRows);
+            for (int g0 = sg.rowBegin; g0 < end;) {
+                int g1 = g0 + 1;
+                while (g1 < end && rowTab[g1].Lt == rowTab[g0].Lt) { ++g1; }
+                if (!(shared && g1 - g0 >= mfk::kCorr2SharedGroup)) {
+                    for (int r = g0; r < g1; ++r) {
+                        solo += std::max(0ll, std::min(static_cast<long long>(sg.L), rowTab[r].nOut - base));
+                    }
+                }
+                g0 = g1;
+            }
+        }
+    }
+    return solo;
 }
 
 void initialize(mf_handle *h)
@@ -443,9 +487,15 @@ void initialize(mf_handle *h)
     std::vector<mfk::Row> rowTab;
     h->rowTabIndex.assign(nT, 0);
     h->segHost.clear();
+    h->dFlagged.alloc(2 * sizeof(unsigned long long));
+    MF_HIP(hipMemset(h->dFlagged.p, 0, 2 * sizeof(unsigned long long)));
+    h->statApplies = 0;
+    h->winPerApply = 0;
+    for (int r = 0; r < nT; ++r) { h->winPerApply += h->rows[r].nOut; }
     for (size_t li = 0; li < h->lanes.size(); ++li) {
         const auto &ln = h->lanes[li];
         mfk::Seg s{};
+        s.flagged = h->dFlagged.as<unsigned long long>();
         s.n = h->ch[ln.ch].n;
         s.L = ln.L;
         s.nOut = ln.nOut;
@@ -485,11 +535,12 @@ void initialize(mf_handle *h)
     // time-domain lanes: split taps, row table, work items
     std::vector<mfk::TdRow> tdRows;
     std::vector<mfk::TdItem> tdItems;
+    long long tdWindows = 0;
     std::map<std::pair<int, int>, std::pair<int, int>> tdLaunch;  // (kind, KT) -> (first, count)
     {
         size_t fragBytes = 0;
         for (const auto &ln : h->lanes) {
-            if (ln.td) { fragBytes += ln.rows.size() * 4 * static_cast<size_t>(ln.KT); }
+            if (ln.td) { fragBytes += ln.rows.size() * mfk::td_frag_bytes(ln.td, ln.KT); }
         }
         if (fragBytes > 0) {
             h->dFrag.alloc(fragBytes);
@@ -508,9 +559,15 @@ void initialize(mf_handle *h)
                     tr.out = static_cast<char *>(h->dRho.p) + rp.rhoOff * h->tsize;
                     tr.nOut = rp.nOut;
                     tr.Lt = rp.Lt;
+                    const double eq = mfk::td_eps(ln.td, ln.KT) / mfk::kTdRhoTol;
+                    tr.sqrtLt = static_cast<float>(std::sqrt(static_cast<double>(rp.Lt)));
+                    tr.kc = static_cast<float>(2.0e-9 * rp.Lt * rp.Lt);
+                    tr.kn = static_cast<float>(eq * eq * rp.Lt);
+                    tr.x = static_cast<const float *>(h->ch[ln.ch].buf.p);
                     tdRows.push_back(tr);
+                    tdWindows += rp.nOut;
                     jobs[{ln.td, ln.KT}].push_back({tr.tmpl, rp.Lt, 0, const_cast<void *>(tr.frag)});
-                    off += 4 * static_cast<size_t>(ln.KT);
+                    off += mfk::td_frag_bytes(ln.td, ln.KT);
                 }
                 const int nRows = static_cast<int>(ln.rows.size());
                 for (long long s0 = 0; s0 < ln.nOut; s0 += mfk::kTdOutputsPerItem) {
@@ -539,6 +596,14 @@ void initialize(mf_handle *h)
         }
     }
     h->dTdRowTab.alloc(std::max<size_t>(tdRows.size(), 1) * sizeof(mfk::TdRow));
+    if (!tdRows.empty()) {
+        // room for 1 in 32 windows flagged (more are recomputed in place)
+        const long long cap = std::min<long long>(std::max<long long>(tdWindows / 32, 1 << 16), 1 << 25);
+        h->dTdFix.alloc(256 + static_cast<size_t>(cap) * 8);
+        h->tdFix.count = h->dTdFix.as<unsigned long long>();
+        h->tdFix.entries = reinterpret_cast<int *>(static_cast<char *>(h->dTdFix.p) + 256);
+        h->tdFix.cap = static_cast<unsigned long long>(cap);
+    }
     h->dTdItems.alloc(std::max<size_t>(tdItems.size(), 1) * sizeof(mfk::TdItem));
     if (!tdRows.empty()) {
         MF_HIP(hipMemcpy(h->dTdRowTab.p, tdRows.data(), tdRows.size() * sizeof(mfk::TdRow), hipMemcpyHostToDevice));
@@ -575,6 +640,7 @@ void initialize(mf_handle *h)
         la.count = static_cast<int>(items.size()) - la.first;
         h->launches.push_back(la);
     }
+    h->soloPerApply = soloWindows(h, items, rowTab);
     for (const auto &kv : tdLaunch) {
         mf_handle::Launch la;
         la.logM = 0;
@@ -642,10 +708,17 @@ void apply(mf_handle *h)
         h->tablesDirty = false;
     }
     auto launchAll = [&]() {
+        bool td = false;
         for (const auto &la : h->launches) {
             if (la.KT > 0) {
+                if (!td) {
+                    // the queue of flagged windows starts empty (a fill
+                    // kernel: no memset node in a captured graph)
+                    MF_HIP(mfk::launch_fill_int(reinterpret_cast<int *>(h->tdFix.count), 2, 0, h->stream));
+                    td = true;
+                }
                 MF_HIP(mfk::launch_td(la.variant, la.KT, h->dTdItems.as<mfk::TdItem>() + la.first, la.count,
-                                      h->dSegs.as<mfk::Seg>(), h->dTdRowTab.as<mfk::TdRow>(), h->stream));
+                                      h->dSegs.as<mfk::Seg>(), h->dTdRowTab.as<mfk::TdRow>(), h->tdFix, h->stream));
                 continue;
             }
             const TwTables &t = h->tw.at(la.logM);
@@ -654,6 +727,9 @@ void apply(mf_handle *h)
             MF_HIP(mfk::launch_corr(h->precision, la.logM, la.variant, h->dItems.as<mfk::Item>() + la.first,
                                     la.count, h->dSegs.as<mfk::Seg>(), h->dRowTab.as<mfk::Row>(), lo, hi,
                                     h->stream));
+        }
+        if (td) {
+            MF_HIP(mfk::launch_td_fix(h->dTdRowTab.as<mfk::TdRow>(), h->tdFix, kTdFixBlocks, h->stream));
         }
     };
     if (h->useGraph && h->graphExec == nullptr && !h->launches.empty()) {
@@ -692,6 +768,7 @@ void apply(mf_handle *h)
     MF_HIP(hipEventRecord(h->evB, h->stream));
     h->applied = true;
     ++h->applies;
+    ++h->statApplies;
 }
 
 // Stack weights exactly as shiftAndStack forms them
@@ -2679,6 +2756,21 @@ int mf_autotune(int precision, double *table, int cap, int *n)
             std::copy(out.begin(), out.begin() + std::min<size_t>(out.size(), static_cast<size_t>(std::max(cap, 0)) * 5),
                       table);
         }
+    });
+}
+
+int mf_get_window_stats(mf_handle *h, long long *windows, long long *flagged, long long *full, long long *solo)
+{
+    return guard([&] {
+        requireHandle(h);
+        requireInit(h);
+        unsigned long long nf[2] = {0, 0};
+        MF_HIP(hipStreamSynchronize(h->stream));
+        MF_HIP(hipMemcpy(nf, h->dFlagged.p, sizeof(nf), hipMemcpyDeviceToHost));
+        if (windows) { *windows = h->winPerApply * h->statApplies; }
+        if (flagged) { *flagged = static_cast<long long>(nf[0]); }
+        if (full) { *full = static_cast<long long>(nf[1]); }
+        if (solo) { *solo = h->soloPerApply * h->statApplies; }
     });
 }
 

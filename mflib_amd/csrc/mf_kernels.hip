@@ -447,10 +447,13 @@ __global__ __launch_bounds__(NT) void corr_kernel(const Item *__restrict__ items
             const int slice = (nvRow + NW - 1) / NW;
             const int s0 = wave * slice;
             const int s1 = min(s0 + slice, nvRow);
+            unsigned long long nFlag = 0, nCanc = 0;
             for (int w0 = s0; w0 < s1; w0 += 64) {
                 const int i = w0 + lane;
                 const bool fl = (i < s1) && (sc[i] < T(0));
                 unsigned long long m = __ballot(fl);
+                nFlag += __popcll(m);
+                nCanc += __popcll(__ballot(fl && isinf(sc[i])));
                 if (__popcll(m) >= kDenseFlags) {
                     if (fl) { out[i] = static_cast<T>(exact_window<T>(raw, i, to_global(row.tmpl), Lt)); }
                     continue;
@@ -476,6 +479,10 @@ __global__ __launch_bounds__(NT) void corr_kernel(const Item *__restrict__ items
                     if (lane == 0) { out[wi] = static_cast<T>(rho); }
                 }
             }
+            if (sg.flagged != nullptr && nFlag != 0ull && lane == 0) {
+                atomicAdd(sg.flagged, nFlag);
+                if (nCanc != 0ull) { atomicAdd(sg.flagged + 1, nCanc); }
+            }
         }
         __syncthreads();
     }
@@ -499,14 +506,18 @@ __global__ __launch_bounds__(NT) void corr_kernel(const Item *__restrict__ items
 // recomputes the two-pass Pearson value.
 template <typename T>
 __device__ __forceinline__ void fixup_row_wave(const T *raw, const T *sc, gptr<const double> tmpl, int Lt,
-                                               gptr<T> out, int nv, int lane, double a, int first = 0)
+                                               gptr<T> out, int nv, int lane, double a, int first,
+                                               unsigned long long *flagged)
 {
     sc -= first;
+    unsigned long long nFlag = 0, nCanc = 0;
     for (int w0 = first; w0 < nv; w0 += 64) {
         const int i = w0 + lane;
         const T s = i < nv ? sc[i] : T(1);
         const bool fl = s < T(0);
         unsigned long long m = __ballot(fl);
+        nFlag += __popcll(m);
+        nCanc += __popcll(__ballot(fl && isinf(s)));
         if (__popcll(m) >= kDenseFlags) {
             // many flagged windows (a gap, a loud stretch): one lane per window
             if (fl) {
@@ -553,6 +564,10 @@ __device__ __forceinline__ void fixup_row_wave(const T *raw, const T *sc, gptr<c
             if (lane == 0) { out[wi] = static_cast<T>(rho); }
         }
     }
+    if (flagged != nullptr && nFlag != 0ull && lane == 0) {
+        atomicAdd(flagged, nFlag);
+        if (nCanc != 0ull) { atomicAdd(flagged + 1, nCanc); }
+    }
 }
 
 template <typename T>
@@ -580,7 +595,7 @@ struct Vec2<double> {
 //    capped to match.
 template <typename T, int M>
 struct Corr2Shape {
-    static constexpr bool SHARED = sizeof(T) * M <= 4096;
+    static constexpr bool SHARED = corr2_shared_scale(static_cast<int>(sizeof(T)), M);
     static constexpr bool HALF = !SHARED;
     static constexpr int TILE = sizeof(T) == 4 ? (M >= 2048 ? 8 : 4) : 16;
     static constexpr int BLOCKS = sizeof(T) * M <= 2048 ? 3 : 2;
@@ -892,7 +907,8 @@ __global__ __launch_bounds__(NT, (Corr2Shape<T, M>::BLOCKS)) void corr2_kernel(c
             epilogueHalf(Half0{}, sharedScale, 0, outRow, nvRow);
             epilogueHalf(Half1{}, sharedScale, 0, outRow, nvRow);
             if (anyFlagShared && nvRow > 0) {
-                fixup_row_wave<T>(raw, sharedScale, to_global(row.tmpl), row.Lt, to_global(outRow), nvRow, lane, a);
+                fixup_row_wave<T>(raw, sharedScale, to_global(row.tmpl), row.Lt, to_global(outRow), nvRow, lane, a, 0,
+                                  sg.flagged);
             }
             return;
         }
@@ -914,7 +930,7 @@ __global__ __launch_bounds__(NT, (Corr2Shape<T, M>::BLOCKS)) void corr2_kernel(c
             wave_lds_sync();
             epilogueHalf(Half0{}, own, 0, outRow, nvRow);
             epilogueHalf(Half1{}, own, 0, outRow, nvRow);
-            if (anyFlag) { fixup_row_wave<T>(raw, own, tmplG, row.Lt, to_global(outRow), nvRow, lane, a); }
+            if (anyFlag) { fixup_row_wave<T>(raw, own, tmplG, row.Lt, to_global(outRow), nvRow, lane, a, 0, sg.flagged); }
             wave_lds_sync();  // the next row reuses buf
             return;
         }
@@ -926,7 +942,7 @@ __global__ __launch_bounds__(NT, (Corr2Shape<T, M>::BLOCKS)) void corr2_kernel(c
             const bool anyFlag = __ballot(nf > 0) != 0ull;
             wave_lds_sync();
             epilogueHalf(Half0{}, own, 0, outRow, nvRow);
-            if (anyFlag) { fixup_row_wave<T>(raw, own, tmplG, row.Lt, to_global(outRow), n0, lane, a, 0); }
+            if (anyFlag) { fixup_row_wave<T>(raw, own, tmplG, row.Lt, to_global(outRow), n0, lane, a, 0, sg.flagged); }
             wave_lds_sync();  // the second half reuses the buffer
         }
         // half 1: outputs [N/2, N)
@@ -937,7 +953,9 @@ __global__ __launch_bounds__(NT, (Corr2Shape<T, M>::BLOCKS)) void corr2_kernel(c
             const bool anyFlag = __ballot(nf > 0) != 0ull;
             wave_lds_sync();
             epilogueHalf(Half1{}, own, N / 2, outRow, nvRow);
-            if (anyFlag) { fixup_row_wave<T>(raw, own, tmplG, row.Lt, to_global(outRow), nvRow, lane, a, N / 2); }
+            if (anyFlag) {
+                fixup_row_wave<T>(raw, own, tmplG, row.Lt, to_global(outRow), nvRow, lane, a, N / 2, sg.flagged);
+            }
             wave_lds_sync();  // the next row reuses the buffer
         }
     };
@@ -953,7 +971,7 @@ __global__ __launch_bounds__(NT, (Corr2Shape<T, M>::BLOCKS)) void corr2_kernel(c
             const int Lt = rows[g0].Lt;
             int g1 = g0 + 1;
             while (g1 < rowEnd && rows[g1].Lt == Lt) { ++g1; }
-            if (g1 - g0 >= NWAVE) {
+            if (g1 - g0 >= kCorr2SharedGroup) {
                 int myFlags = 0;
                 if (!fftDone) {
                     // wave 0 transforms while waves 1..3 build the scale
@@ -1000,7 +1018,7 @@ __global__ __launch_bounds__(NT, (Corr2Shape<T, M>::BLOCKS)) void corr2_kernel(c
                 const int Lt = rows[sg0].Lt;
                 sg1 = sg0 + 1;
                 while (sg1 < rowEnd && rows[sg1].Lt == Lt) { ++sg1; }
-                sr = (SHARED && sg1 - sg0 >= NWAVE) ? sg1 : sg0;
+                sr = (SHARED && sg1 - sg0 >= kCorr2SharedGroup) ? sg1 : sg0;
                 continue;
             }
             const int r = sr++;

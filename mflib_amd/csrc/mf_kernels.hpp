@@ -3,6 +3,7 @@
 // structs, copies them to device memory and calls the launchers.
 #pragma once
 #include <hip/hip_runtime.h>
+#include <cmath>
 #include <cstdint>
 
 namespace mfk {
@@ -24,7 +25,18 @@ struct Seg {
     int rowEnd;         //   decreasing nOut, then increasing template length
     int dead;           // dead trace: every row of the lane is zero
     float hmax;         // max |template spectrum| over the lane's rows
+    unsigned long long *flagged;  // running counts of windows recomputed
+                                  //   exactly: [0] all, [1] those whose
+                                  //   variance cancelled (null: not counted)
 };
+
+// corr2_kernel keeps a block-shared window-scale buffer (rows of a block
+// in groups of four or more of equal template length share one scale) when
+// a block's N = 2 M outputs of element size tsize fit in 8 KB of LDS; rows of
+// smaller groups, and every row without the buffer, compute their scale on
+// their own wave ("solo" rows).
+constexpr bool corr2_shared_scale(int tsize, int M) { return tsize * M <= 4096; }
+constexpr int kCorr2SharedGroup = 4;
 
 // One template row.
 struct Row {
@@ -76,19 +88,58 @@ struct StackGroup {
 };
 
 // ---- time-domain correlation of short templates on MFMA -----------------
-constexpr int kTdFp16 = 3;              // kernel variant: FP16 hi/lo operands
-constexpr int kTdBf16 = 4;              // kernel variant: BF16 hi/lo operands
+constexpr int kTdFp16 = 3;              // kernel variant: FP16 operands split in two parts
+constexpr int kTdBf16 = 4;              // kernel variant: BF16 operands split in three parts
 constexpr int kTdOutputsPerItem = 512;  // outputs per work item
 constexpr int kTdMaxTaps = 256;         // longest template of the path
 
-// One template row of a time-domain lane.
+// 16-bit parts each operand is split into (hi, lo for FP16; hi, mid, lo for
+// BF16, whose 8-bit significand needs three parts for the FP32 contract).
+constexpr int td_parts(int kind) { return kind == kTdBf16 ? 3 : 2; }
+// bytes of one row's split taps
+constexpr size_t td_frag_bytes(int kind, int KT) { return static_cast<size_t>(td_parts(kind)) * 2 * KT; }
+
+// Tolerance of the fast value of a time-domain window, and the relative
+// error eps of the split MFMA numerator: the dropped products and split
+// residuals (3 u^2 for the FP16 pair, u = 2^-11; 3 u^3 for the BF16
+// triple, u = 2^-8), FP32 accumulation over KT / 16 k-steps and the FP32
+// epilogue.  A window's fast value is kept when eps sqrt(Lt S2) / den is
+// within the tolerance.
+constexpr double kTdRhoTol = 2.5e-6;
+inline double td_eps(int kind, int KT)
+{
+    const double u = kind == kTdBf16 ? 1.0 / 256.0 : 1.0 / 2048.0;
+    const double split = kind == kTdBf16 ? 3.0 * u * u * u : 3.0 * u * u;
+    return split + (4.0 * std::sqrt(KT / 16.0) + 4.0) * 5.9604644775390625e-08;
+}
+
+// One template row of a time-domain lane.  The epilogue's per-row
+// constants: with S1, S2 the window's sums of d = x - a and d^2 and
+// den2 = Lt S2 - S1^2, a window is recomputed in full when
+// den2 <= kc S2 + 1e-12 Lt E (E the block's sum of d^2: the variance lost
+// its digits) and its numerator is recomputed when den2 < kn S2 (the
+// error bound exceeds the tolerance).
 struct TdRow {
-    const void *frag;     // split taps: KT hi parts, then KT lo parts (16-bit)
+    const void *frag;     // split taps: KT taps of each part in turn (16-bit)
     const double *tmpl;   // demeaned unit-norm template (FP64), Lt samples
     void *out;            // correlation output row (FP32)
     long long nOut;       // outputs of this row
     int Lt;               // template length
-    int pad;
+    float sqrtLt;         // sqrt(Lt)
+    float kc;             // 2e-9 Lt^2
+    float kn;             // (eps / tol)^2 Lt
+    const float *x;       // the row's channel samples
+    void *pad;
+};
+
+// Queue of flagged time-domain windows for the exact recomputation kernel:
+// 64-bit entries, output index << 32 | row | cancelled << 31.  The correlation launches
+// append while *count stays within cap (the rest are recomputed in place);
+// *count is cleared before them.
+struct TdFixQueue {
+    unsigned long long *count;
+    int *entries;  // 8 bytes per entry
+    unsigned long long cap;
 };
 
 // One work item: outputs [seg0, seg0 + kTdOutputsPerItem) of lane `seg`
@@ -113,7 +164,9 @@ bool td_supported(int nb);
 int td_taps(int nb);  // padded tap count KT (64, 128 or 256)
 hipError_t launch_td_prep(int kind, const TdPrepJob *dJobs, int nJobs, int KT, hipStream_t stream);
 hipError_t launch_td(int kind, int KT, const TdItem *dItems, int nItems, const Seg *dSegs, const TdRow *dRows,
-                     hipStream_t stream);
+                     TdFixQueue queue, hipStream_t stream);
+// exact recomputation of the queued windows (after the correlation launches)
+hipError_t launch_td_fix(const TdRow *dRows, TdFixQueue queue, int blocks, hipStream_t stream);
 
 // Correlation + normalisation: one launch per FFT size.  Variant 1 runs
 // block-wide LDS Stockham FFTs; variant 2 runs wave-level register FFTs

@@ -405,7 +405,7 @@ def test_graph_launch_replays_match_eager():
 @pytest.mark.parametrize("variant,tol", [(3, 1e-5), (4, 1e-5)])
 def test_time_domain_mfma_path_matches_oracle(variant, tol):
     """Short templates (64-256 taps, several lengths) on the MFMA
-    time-domain path, FP16 or BF16 hi/lo operands: every row within the
+    time-domain path, FP16 two-part or BF16 three-part operands: every row within the
     FP32 contract of the FP64 oracle, including a constant gap with a DC
     offset (windows recomputed exactly) and a dead channel."""
     rng = np.random.default_rng(700 + variant)
