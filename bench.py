@@ -82,6 +82,8 @@ def serve_section(args, torch):
         log(f"serving: lambda {lam:g}/s: p50 {r['p50_ms']:.2f} ms, p99 {r['p99_ms']:.2f} ms, "
             f"{r['corr_samples_per_s'] / 1e9:.1f} G corr samples/s")
     b, n, _ = srv.stats()
+    ws = _window_fracs(srv.window_stats())
+    log(f"serving: flagged windows {100.0 * ws['flagged_frac']:.4f}%, solo {100.0 * ws['solo_frac']:.1f}%")
     srv.close()
     one = capi.Server(np.float32, budget_bytes=min(budget, 64e9), max_batch=1, max_inflight=1)
     lib1 = one.add_templates(wl.library)
@@ -104,6 +106,7 @@ def serve_section(args, torch):
         "poisson": runs,
         "batches": b,
         "requests_completed": n,
+        "window_stats": ws,
     }
     return out, (wl, audit)
 
@@ -211,7 +214,9 @@ def uneven_section(args, torch):
     from mflib_amd import capi, synth
     log("uneven batch: generating 200 templates x 1-50 channels x 1-24 h")
     wl = synth.uneven_batch()
-    table = capi.autotune(np.float32)
+    # the measured autotuner, or (for profiling, where the profiler skews
+    # the calibration launches) the built-in MI355X cost table
+    table = None if args.no_autotune else capi.autotune(np.float32)
     rows = wl.rows()
     ns = [s.size for s in wl.signals]
     dev = [torch.from_numpy(s).to("cuda:0") for s in wl.signals]
@@ -785,6 +790,8 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--samples", type=int, default=8_640_000, help="samples per channel")
     ap.add_argument("--nfft", type=int, default=0, help="force the FFT length (0 = autotuner)")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="uneven batch: plan from the built-in cost table (reproducible under a profiler)")
     ap.add_argument("--variant", type=int, default=0, help="correlation kernel variant (0 = best)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)

@@ -442,6 +442,15 @@ int mf_server_release(mf_server *s, long long requestId);
  * it may be admitted.  A request already on the GPU (or done) runs to
  * completion: *cancelled = 0.  Either way release it afterwards. */
 int mf_server_cancel(mf_server *s, long long requestId, int *cancelled);
+
+/* Window statistics of the server's correlation launches since
+ * mf_server_create, as mf_get_window_stats gives them for a handle: windows
+ * of the launched batches, windows recomputed exactly (counted on the
+ * device; final for a request once mf_server_wait has returned for it),
+ * those recomputed in full, and windows of solo rows.  Any pointer may be
+ * NULL. */
+int mf_server_window_stats(mf_server *s, long long *windows, long long *flagged, long long *full,
+                           long long *solo);
 /* Batches launched, requests completed, and correlation samples completed. */
 int mf_server_stats(mf_server *s, long long *batches, long long *requests, double *corrSamples);
 

@@ -103,6 +103,7 @@ _SIGNATURES = {
                                ctypes.c_int, _dp, _dp, _dp, _dp, _ip], ctypes.c_int),
     "mf_get_timings": ([_vp, _dp, _dp], ctypes.c_int),
     "mf_get_window_stats": ([_vp] + [ctypes.POINTER(ctypes.c_longlong)] * 4, ctypes.c_int),
+    "mf_server_window_stats": ([_vp] + [ctypes.POINTER(ctypes.c_longlong)] * 4, ctypes.c_int),
     "mf_autotune": ([ctypes.c_int, _dp, ctypes.c_int, _ip], ctypes.c_int),
     "mf_server_create": ([ctypes.POINTER(_vp), ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_int],
                          ctypes.c_int),
@@ -697,6 +698,13 @@ class Server:
         c = ctypes.c_int()
         check(lib().mf_server_cancel(self._h, rid, ctypes.byref(c)), "mf_server_cancel")
         return bool(c.value)
+
+    def window_stats(self):
+        """Window statistics of the launched batches (see
+        Handle.window_stats)."""
+        v = [ctypes.c_longlong() for _ in range(4)]
+        check(lib().mf_server_window_stats(self._h, *[ctypes.byref(x) for x in v]), "mf_server_window_stats")
+        return dict(zip(("windows", "flagged", "full", "solo"), (x.value for x in v)))
 
     def status(self, rid: int) -> int:
         """Waits for the request and returns its status code (MF_OK, or the

@@ -266,32 +266,15 @@ void deriveLayout(mf_handle *h, std::vector<int> &rowCh, std::vector<int> &rowGr
 }
 
 // Windows of the launches whose row computes its window scale on its own
-// wave, grouped as corr2_kernel groups a block's rows (runs of equal
-// template length in the row table; runs of kCorr2SharedGroup or more share
-// one scale where the kernel keeps the shared buffer).  The block-wide
-// kernel computes every scale for the whole block: no solo rows.
+// wave (corr2_kernel's solo rows); the block-wide kernel computes every
+// scale for the whole block: no solo rows.
 long long soloWindows(const mf_handle *h, const std::vector<mfk::Item> &items, const std::vector<mfk::Row> &rowTab)
 {
     long long solo = 0;
     for (const auto &la : h->launches) {
         if (la.KT > 0 || la.variant != 2) { continue; }
-        const bool shared = mfk::corr2_shared_scale(static_cast<int>(h->tsize), 1 << la.logM);
-        for (int k = la.first; k < la.first + la.count; ++k) {
-            const mfk::Item &it = items[k];
-            const mfk::Seg &sg = h->segHost[it.seg];
-            const long long base = static_cast<long long>(it.block) * sg.L;
-            const int end = it.nRows < 0 ? sg.rowEnd : std::min(sg.rowEnd, sg.rowBegin + it.nRows);
-            for (int g0 = sg.rowBegin; g0 < end;) {
-                int g1 = g0 + 1;
-                while (g1 < end && rowTab[g1].Lt == rowTab[g0].Lt) { ++g1; }
-                if (!(shared && g1 - g0 >= mfk::kCorr2SharedGroup)) {
-                    for (int r = g0; r < g1; ++r) {
-                        solo += std::max(0ll, std::min(static_cast<long long>(sg.L), rowTab[r].nOut - base));
-                    }
-                }
-                g0 = g1;
-            }
-        }
+        solo += mfh::soloWindows(items.data(), la.first, la.count, h->segHost.data(), rowTab.data(),
+                                 mfk::corr2_shared_scale(static_cast<int>(h->tsize), 1 << la.logM));
     }
     return solo;
 }

@@ -385,6 +385,34 @@ inline void laneRowBlocks(const LaneRows &rs, long long L, double &shared, doubl
     }
 }
 
+// Windows of the work items [first, first + count) of one corr2_kernel
+// launch whose row computes its window scale on its own wave, grouped as
+// the kernel groups a block's rows (runs of equal template length in the
+// row table; runs of kCorr2SharedGroup or more share one scale where the
+// kernel keeps the shared buffer).
+inline long long soloWindows(const mfk::Item *items, int first, int count, const mfk::Seg *segs,
+                             const mfk::Row *rows, bool sharedBuffer)
+{
+    long long solo = 0;
+    for (int k = first; k < first + count; ++k) {
+        const mfk::Item &it = items[k];
+        const mfk::Seg &sg = segs[it.seg];
+        const long long base = static_cast<long long>(it.block) * sg.L;
+        const int end = it.nRows < 0 ? sg.rowEnd : std::min(sg.rowEnd, sg.rowBegin + it.nRows);
+        for (int g0 = sg.rowBegin; g0 < end;) {
+            int g1 = g0 + 1;
+            while (g1 < end && rows[g1].Lt == rows[g0].Lt) { ++g1; }
+            if (!(sharedBuffer && g1 - g0 >= mfk::kCorr2SharedGroup)) {
+                for (int r = g0; r < g1; ++r) {
+                    solo += std::max(0ll, std::min(static_cast<long long>(sg.L), rows[r].nOut - base));
+                }
+            }
+            g0 = g1;
+        }
+    }
+    return solo;
+}
+
 // Risk margin of larger FFT blocks: each doubling of the block over the
 // smallest size adds kBlockRisk to the predicted cost.  Two effects the
 // per-block table does not see grow with the block: windows recomputed

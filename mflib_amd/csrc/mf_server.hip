@@ -17,6 +17,7 @@
 #include "mf_kernels.hpp"
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cstdint>
 #include <cstring>
@@ -107,6 +108,11 @@ struct mf_server {
     long long nBatches = 0, nDone = 0;
     double doneSamples = 0;
     hipStream_t copyStream = nullptr;
+    // window statistics (mf_server_window_stats): the device's running
+    // counts of windows recomputed exactly (all, in full), and the windows
+    // and solo-row windows of the launched batches
+    DevBuf dFlagged;
+    std::atomic<long long> statWindows{0}, statSolo{0};
 
     ~mf_server();
 };
@@ -162,6 +168,7 @@ void launchBatch(mf_server *s, Slot &slot)
 {
     std::vector<mfk::Seg> segs;
     std::vector<mfk::Row> rows;
+    long long windows = 0;
     // one launch per (FFT size, kernel variant); each lane's variant depends
     // on the lane alone
     std::map<std::pair<int, int>, std::vector<int>> segsByPlan;
@@ -186,6 +193,7 @@ void launchBatch(mf_server *s, Slot &slot)
                 const int lg = bk.logM;
                 const int L = blockLength(lg, bk.nb);
                 mfk::Seg sg{};
+                sg.flagged = s->dFlagged.as<unsigned long long>();
                 sg.x = x;
                 sg.n = r->len[c];
                 sg.L = L;
@@ -203,6 +211,7 @@ void launchBatch(mf_server *s, Slot &slot)
                     row.nOut = r->nOut[c];
                     row.Lt = t.Lt;
                     rows.push_back(row);
+                    windows += row.nOut;
                     hmax = std::max(hmax, t.hmax[lg]);
                 }
                 sg.rowEnd = static_cast<int>(rows.size());
@@ -226,6 +235,14 @@ void launchBatch(mf_server *s, Slot &slot)
         la.count = static_cast<int>(items.size()) - la.first;
         launches.push_back(la);
     }
+    long long solo = 0;
+    for (const auto &la : launches) {
+        if (la.variant != 2) { continue; }
+        solo += soloWindows(items.data(), la.first, la.count, segs.data(), rows.data(),
+                            mfk::corr2_shared_scale(static_cast<int>(s->tsize), 1 << la.logM));
+    }
+    s->statWindows += windows;
+    s->statSolo += solo;
     const size_t segBytes = roundUp(segs.size() * sizeof(mfk::Seg), kAlign);
     const size_t rowBytes = roundUp(rows.size() * sizeof(mfk::Row), kAlign);
     const size_t itemBytes = roundUp(std::max<size_t>(items.size(), 1) * sizeof(mfk::Item), kAlign);
@@ -533,6 +550,8 @@ int mf_server_create(mf_server **out, int precision, double budgetBytes, int max
         }
         s->arena.alloc(want);
         s->freeList[0] = want;
+        s->dFlagged.alloc(2 * sizeof(unsigned long long));
+        MF_HIP(hipMemset(s->dFlagged.p, 0, 2 * sizeof(unsigned long long)));
         // twiddle tables of every FFT size, read-only from here on
         for (int lg = mfk::kMinLogM; lg <= maxLogM(precision); ++lg) {
             makeTwiddles(s->tw[lg], lg, precision == MF_PRECISION_FLOAT);
@@ -672,6 +691,19 @@ int mf_server_cancel(mf_server *s, long long id, int *cancelled)
             s->cvSched.notify_one();
         }
         if (cancelled) { *cancelled = pending ? 1 : 0; }
+    });
+}
+
+int mf_server_window_stats(mf_server *s, long long *windows, long long *flagged, long long *full, long long *solo)
+{
+    return guard([&] {
+        if (s == nullptr) { fail(MF_ERR_INVALID_ARGUMENT, "server is NULL"); }
+        unsigned long long nf[2] = {0, 0};
+        MF_HIP(hipMemcpy(nf, s->dFlagged.p, sizeof(nf), hipMemcpyDeviceToHost));
+        if (windows) { *windows = s->statWindows.load(); }
+        if (flagged) { *flagged = static_cast<long long>(nf[0]); }
+        if (full) { *full = static_cast<long long>(nf[1]); }
+        if (solo) { *solo = s->statSolo.load(); }
     });
 }
 

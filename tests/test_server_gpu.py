@@ -111,6 +111,38 @@ def test_results_identical_alone_and_in_the_mix():
     srv.close()
 
 
+def test_window_stats_count_windows_and_constant_stretch():
+    """mf_server_window_stats: the windows of a request are its rows'
+    lengths; a stationary channel flags none, a channel with an exactly
+    constant stretch flags at least the windows inside it (recomputed in
+    full), and the rows stay within the FP32 contract of the oracle."""
+    rng = np.random.default_rng(509)
+    tmpls = _library(rng, 6, 300, 600)
+    srv = capi.Server(np.float32, budget_bytes=1 << 30, max_batch=8, max_inflight=1)
+    ids = srv.add_templates(tmpls)
+    n = 30000
+    chans = [(rng.standard_normal(n) * 2 + 50).astype(np.float32) for _ in range(2)]
+    rid = srv.submit(chans, ids)
+    srv.wait(rid)
+    nb = max(len(t) for t in tmpls)
+    ws = srv.window_stats()
+    assert ws["windows"] == len(ids) * len(chans) * (n - nb + 1)
+    assert ws["flagged"] == 0 and ws["full"] == 0, ws
+    assert 0 <= ws["solo"] <= ws["windows"]
+    srv.release(rid)
+    g0, G = 10000, 3000
+    chans[1][g0:g0 + G] = np.float32(57.0)
+    rid = srv.submit(chans, ids)
+    srv.wait(rid)
+    ws2 = srv.window_stats()
+    assert ws2["windows"] == 2 * ws["windows"]
+    inside = sum(G - len(t) + 1 for t in tmpls)
+    assert ws2["full"] >= inside and ws2["flagged"] >= ws2["full"], (ws2, inside)
+    _check_against_oracle(srv, rid, tmpls, ids, chans, rng=rng)
+    srv.release(rid)
+    srv.close()
+
+
 def test_admission_under_small_budget():
     rng = np.random.default_rng(503)
     tmpls = _library(rng, 6, 300, 600)
