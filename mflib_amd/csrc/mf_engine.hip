@@ -35,9 +35,9 @@
 
 using namespace mfh;
 
-// blocks of the exact-recomputation launch after the time-domain
-// correlation (grid-stride over the queued windows: 8 blocks per CU)
-constexpr int kTdFixBlocks = 2048;
+// blocks of the exact-recomputation launches after the correlation
+// launches (grid-stride over the queued windows: 8 blocks per CU)
+constexpr int kFixBlocks = 2048;
 
 struct mf_handle {
     int precision = MF_PRECISION_FLOAT;
@@ -129,7 +129,9 @@ struct mf_handle {
     long long winPerApply = 0, soloPerApply = 0, statApplies = 0;
     DevBuf dFrag, dTdRowTab, dTdItems;  // time-domain lanes
     DevBuf dTdFix;                      //   their queue of flagged windows (count, then entries)
-    mfk::TdFixQueue tdFix{};
+    mfk::FixQueue tdFix{};
+    DevBuf dFftFix;                     // queue of flagged windows of the FP32 wave-kernel rows
+    mfk::FixQueue fftFix{};
     DevBuf dStackRows, dStackOut;
     DevBuf dReduceRows, dRedDet, dRedId, dRedPk, dRedPkId;  // Detector reduce step
     DevBuf dProdJobs, dProdOut;                             // per-detection products
@@ -210,7 +212,9 @@ void dropDevice(mf_handle *h)
     h->dFrag.release();
     h->dTdRowTab.release();
     h->dTdFix.release();
-    h->tdFix = mfk::TdFixQueue{};
+    h->tdFix = mfk::FixQueue{};
+    h->dFftFix.release();
+    h->fftFix = mfk::FixQueue{};
     h->dTdItems.release();
     h->dStackRows.release();
     h->stacksReady = false;
@@ -501,6 +505,7 @@ void initialize(mf_handle *h)
             row.out = static_cast<char *>(h->dRho.p) + rp.rhoOff * h->tsize;
             row.nOut = rp.nOut;
             row.Lt = rp.Lt;
+            row.x = h->ch[ln.ch].buf.p;
             h->rowTabIndex[r] = static_cast<int>(rowTab.size());
             rowTab.push_back(row);
             hmax = std::max(hmax, rp.hmax);
@@ -624,6 +629,26 @@ void initialize(mf_handle *h)
         h->launches.push_back(la);
     }
     h->soloPerApply = soloWindows(h, items, rowTab);
+    // FP32 wave-kernel rows: flagged outputs are written as NaN and their
+    // row blocks recorded (at most two records per row block: a block's two
+    // halves at M = 2048) for one exact recomputation launch
+    if (h->precision == MF_PRECISION_FLOAT) {
+        long long rowBlocks = 0;
+        for (const auto &la : h->launches) {
+            if (la.KT > 0 || la.variant != 2) { continue; }
+            for (int k = la.first; k < la.first + la.count; ++k) {
+                const mfk::Item &it = items[k];
+                const mfk::Seg &sg = h->segHost[it.seg];
+                const int end = it.nRows < 0 ? sg.rowEnd : std::min(sg.rowEnd, sg.rowBegin + it.nRows);
+                rowBlocks += end - sg.rowBegin;
+            }
+        }
+        if (rowBlocks > 0) {
+            h->dFftFix.alloc(256 + static_cast<size_t>(2 * rowBlocks) * 16);
+            h->fftFix.overCount = h->dFftFix.as<unsigned long long>();
+            h->fftFix.over = reinterpret_cast<int *>(static_cast<char *>(h->dFftFix.p) + 256);
+        }
+    }
     for (const auto &kv : tdLaunch) {
         mf_handle::Launch la;
         la.logM = 0;
@@ -692,6 +717,9 @@ void apply(mf_handle *h)
     }
     auto launchAll = [&]() {
         bool td = false;
+        if (h->fftFix.over != nullptr) {
+            MF_HIP(mfk::launch_fill_int(reinterpret_cast<int *>(h->fftFix.overCount), 2, 0, h->stream));
+        }
         for (const auto &la : h->launches) {
             if (la.KT > 0) {
                 if (!td) {
@@ -709,11 +737,12 @@ void apply(mf_handle *h)
             const void *hi = h->precision == MF_PRECISION_FLOAT ? t.hi.p : t.hi64.p;
             MF_HIP(mfk::launch_corr(h->precision, la.logM, la.variant, h->dItems.as<mfk::Item>() + la.first,
                                     la.count, h->dSegs.as<mfk::Seg>(), h->dRowTab.as<mfk::Row>(), lo, hi,
-                                    h->stream));
+                                    h->fftFix, h->stream));
         }
         if (td) {
-            MF_HIP(mfk::launch_td_fix(h->dTdRowTab.as<mfk::TdRow>(), h->tdFix, kTdFixBlocks, h->stream));
+            MF_HIP(mfk::launch_fix(h->dTdRowTab.as<mfk::TdRow>(), h->tdFix, kFixBlocks, h->stream));
         }
+        MF_HIP(mfk::launch_fix(h->dRowTab.as<mfk::Row>(), h->fftFix, kFixBlocks, h->stream));
     };
     if (h->useGraph && h->graphExec == nullptr && !h->launches.empty()) {
         // capture the correlation launches once; their descriptor tables and

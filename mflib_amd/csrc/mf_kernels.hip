@@ -160,7 +160,10 @@ __device__ __forceinline__ void window_sums(const T *raw, const double *ts1, con
 // (relative error ~2^-23, far inside the FP32 contract); the FP64 scale is
 // sqrt/division in FP64.  Returns this thread's number of flagged outputs.
 // Outputs [first, first + nValid) of the block (sc[i - first] for output i).
-template <typename T, int TILE, int UNROLL = 1>
+// NANFLAG: flagged outputs get NaN instead (the epilogue then writes NaN
+// for them, which the recomputation launch looks for), and the return value
+// packs the flagged count with the cancelled count << 16.
+template <typename T, int TILE, int UNROLL = 1, bool NANFLAG = false>
 __device__ __forceinline__ int compute_scale(const T *raw, const double *ts1, const double *ts2,
                                              const double *tp1, const double *tp2, T *sc, double a, int nValid,
                                              int Lt, double noise, int t, int nthr, int first = 0)
@@ -200,7 +203,12 @@ __device__ __forceinline__ int compute_scale(const T *raw, const double *ts1, co
         }
         // FP64 correlation recomputes every flagged window whole: its 1e-12
         // tolerance leaves no room for the window sums' own rounding
-        sc[i] = (canc1 || (sizeof(T) == 8 && noisy)) ? scale_exact<T>() : (noisy ? -f : f);
+        if constexpr (NANFLAG) {
+            sc[i] = (canc1 || noisy) ? __builtin_nanf("") : f;
+            nflag += canc1 ? (1 << 16) : 0;
+        } else {
+            sc[i] = (canc1 || (sizeof(T) == 8 && noisy)) ? scale_exact<T>() : (noisy ? -f : f);
+        }
         if (i + 1 < i1) {
             const double dn = static_cast<double>(raw[i + Lt]) - a;
             const double dd = static_cast<double>(raw[i]) - a;
@@ -223,7 +231,7 @@ __device__ __forceinline__ int compute_scale(const T *raw, const double *ts1, co
 // chain's windows cover, bounds every window sum of the chain, so the
 // cancellation test of compute_scale holds with one compare per output.
 // Returns this lane's number of flagged outputs.
-template <int TILE, int NCH>
+template <int TILE, int NCH, bool NANFLAG = false>
 __device__ __forceinline__ int wave_row_scale(const float *raw, const double *tp1, const double *tp2, float *sc,
                                               double a, int nValid, int Lt, double noise, int lane, int first)
 {
@@ -266,7 +274,12 @@ __device__ __forceinline__ int wave_row_scale(const float *raw, const double *tp
                 const float f = (sqrtLtF * y) * h;
                 // noise flag: the scale holds, the numerator is recomputed
                 const bool canc1 = !(den2 > cancThr[k]);
-                sc[i] = canc1 ? scale_exact<float>() : (flag ? -f : f);
+                if constexpr (NANFLAG) {
+                    sc[i] = flag ? __builtin_nanf("") : f;
+                    nflag += canc1 ? (1 << 16) : 0;
+                } else {
+                    sc[i] = canc1 ? scale_exact<float>() : (flag ? -f : f);
+                }
                 const double xn = static_cast<double>(raw[i + Lt]);
                 const double xo = static_cast<double>(raw[i]);
                 const double e = xn - xo;
@@ -498,75 +511,89 @@ __global__ __launch_bounds__(NT) void corr_kernel(const Item *__restrict__ items
 // scale of a group is shared by its rows.
 // ---------------------------------------------------------------------------
 
-// Fix-up of the flagged outputs [first, nv) of one row (one wave);
-// sc[i - first] < 0 flags output i (see scale_exact).  A noise flag (-f)
-// keeps the window's scale f and recomputes the numerator exactly in FP64
-// from the anchored samples, sum_m t[m] (x[i+m] - a) (the template has zero
-// mean, so the window mean drops out of the numerator); -infinity
-// recomputes the two-pass Pearson value.
+// In-place exact recomputation of the flagged outputs of one chunk of 64
+// (mask m, output w0 + lane for this lane) of a row, sc[i] < 0 flagging
+// output i (see scale_exact): a noise flag (-f) keeps the window's scale f
+// and recomputes the numerator exactly in FP64 from the anchored samples,
+// sum_m t[m] (x[i+m] - a) (the template has zero mean, so the window mean
+// drops out of the numerator); -infinity recomputes the two-pass Pearson
+// value.  Many flags: one lane per window; few: the wave on each window.
+template <typename T>
+__device__ __forceinline__ void fixup_chunk(const T *raw, const T *sc, gptr<const double> tmpl, int Lt, gptr<T> out,
+                                            int w0, int nv, int lane, double a, unsigned long long m)
+{
+    const int i = w0 + lane;
+    const T s = i < nv ? sc[i] : T(1);
+    const bool fl = s < T(0);
+    if (__popcll(m) >= kDenseFlags) {
+        // many flagged windows (a gap, a loud stretch): one lane per window
+        if (fl) {
+            double rho;
+            if (isinf(s)) {
+                rho = exact_window<T>(raw, i, tmpl, Lt);
+            } else {
+                double dn = 0;
+                for (int mm = 0; mm < Lt; ++mm) { dn = fma(tmpl[mm], static_cast<double>(raw[i + mm]) - a, dn); }
+                rho = dn * static_cast<double>(-s);
+                if (fabs(rho) > 1.1) { rho = 0; }
+            }
+            out[i] = static_cast<T>(rho);
+        }
+        return;
+    }
+    while (m != 0ull) {
+        const int b = __ffsll(static_cast<long long>(m)) - 1;
+        m &= m - 1ull;
+        const int wi = w0 + b;
+        const T sw = sc[wi];
+        double rho;
+        if (isinf(sw)) {
+            double sm = 0;
+            for (int mm = lane; mm < Lt; mm += 64) { sm += static_cast<double>(raw[wi + mm]); }
+            sm = wave_sum(sm);
+            const double mean = sm / Lt;
+            double e2 = 0, dn = 0;
+            for (int mm = lane; mm < Lt; mm += 64) {
+                const double d = static_cast<double>(raw[wi + mm]) - mean;
+                e2 += d * d;
+                dn += tmpl[mm] * d;
+            }
+            e2 = wave_sum(e2);
+            dn = wave_sum(dn);
+            rho = dn / fmax(DBL_EPSILON, sqrt(e2));
+        } else {
+            double dn = 0;
+            for (int mm = lane; mm < Lt; mm += 64) { dn = fma(tmpl[mm], static_cast<double>(raw[wi + mm]) - a, dn); }
+            dn = wave_sum(dn);
+            rho = dn * static_cast<double>(-sw);
+        }
+        if (fabs(rho) > 1.1) { rho = 0; }
+        if (lane == 0) { out[wi] = static_cast<T>(rho); }
+    }
+}
+
+// Flagged outputs [first, nv) of one row (one wave), sc[i - first] < 0
+// flagging output i (see scale_exact), recomputed in place chunk by chunk;
+// their count goes to the window statistics.
 template <typename T>
 __device__ __forceinline__ void fixup_row_wave(const T *raw, const T *sc, gptr<const double> tmpl, int Lt,
                                                gptr<T> out, int nv, int lane, double a, int first,
                                                unsigned long long *flagged)
 {
     sc -= first;
-    unsigned long long nFlag = 0, nCanc = 0;
+    unsigned nFlag = 0, nCanc = 0;
     for (int w0 = first; w0 < nv; w0 += 64) {
         const int i = w0 + lane;
         const T s = i < nv ? sc[i] : T(1);
         const bool fl = s < T(0);
-        unsigned long long m = __ballot(fl);
+        const unsigned long long m = __ballot(fl);
         nFlag += __popcll(m);
         nCanc += __popcll(__ballot(fl && isinf(s)));
-        if (__popcll(m) >= kDenseFlags) {
-            // many flagged windows (a gap, a loud stretch): one lane per window
-            if (fl) {
-                double rho;
-                if (isinf(s)) {
-                    rho = exact_window<T>(raw, i, tmpl, Lt);
-                } else {
-                    double dn = 0;
-                    for (int mm = 0; mm < Lt; ++mm) { dn = fma(tmpl[mm], static_cast<double>(raw[i + mm]) - a, dn); }
-                    rho = dn * static_cast<double>(-s);
-                    if (fabs(rho) > 1.1) { rho = 0; }
-                }
-                out[i] = static_cast<T>(rho);
-            }
-            continue;
-        }
-        while (m != 0ull) {
-            const int b = __ffsll(static_cast<long long>(m)) - 1;
-            m &= m - 1ull;
-            const int wi = w0 + b;
-            const T sw = sc[wi];
-            double rho;
-            if (isinf(sw)) {
-                double sm = 0;
-                for (int mm = lane; mm < Lt; mm += 64) { sm += static_cast<double>(raw[wi + mm]); }
-                sm = wave_sum(sm);
-                const double mean = sm / Lt;
-                double e2 = 0, dn = 0;
-                for (int mm = lane; mm < Lt; mm += 64) {
-                    const double d = static_cast<double>(raw[wi + mm]) - mean;
-                    e2 += d * d;
-                    dn += tmpl[mm] * d;
-                }
-                e2 = wave_sum(e2);
-                dn = wave_sum(dn);
-                rho = dn / fmax(DBL_EPSILON, sqrt(e2));
-            } else {
-                double dn = 0;
-                for (int mm = lane; mm < Lt; mm += 64) { dn = fma(tmpl[mm], static_cast<double>(raw[wi + mm]) - a, dn); }
-                dn = wave_sum(dn);
-                rho = dn * static_cast<double>(-sw);
-            }
-            if (fabs(rho) > 1.1) { rho = 0; }
-            if (lane == 0) { out[wi] = static_cast<T>(rho); }
-        }
+        if (m != 0ull) { fixup_chunk<T>(raw, sc, tmpl, Lt, out, w0, nv, lane, a, m); }
     }
-    if (flagged != nullptr && nFlag != 0ull && lane == 0) {
-        atomicAdd(flagged, nFlag);
-        if (nCanc != 0ull) { atomicAdd(flagged + 1, nCanc); }
+    if (flagged != nullptr && nFlag != 0u && lane == 0) {
+        atomicAdd(flagged, static_cast<unsigned long long>(nFlag));
+        if (nCanc != 0u) { atomicAdd(flagged + 1, static_cast<unsigned long long>(nCanc)); }
     }
 }
 
@@ -601,12 +628,12 @@ struct Corr2Shape {
     static constexpr int BLOCKS = sizeof(T) * M <= 2048 ? 3 : 2;
 };
 
-template <typename T, int M>
+template <typename T, int M, bool QUEUE>
 __global__ __launch_bounds__(NT, (Corr2Shape<T, M>::BLOCKS)) void corr2_kernel(const Item *__restrict__ items,
                                                    const Seg *__restrict__ segs,
                                                    const Row *__restrict__ rows,
                                                    const Cx<T> *__restrict__ twloG,
-                                                   const Cx<T> *__restrict__ twhiG)
+                                                   const Cx<T> *__restrict__ twhiG, FixQueue queue)
 {
     using Shape = Corr2Shape<T, M>;
     constexpr int N = 2 * M;
@@ -900,15 +927,43 @@ __global__ __launch_bounds__(NT, (Corr2Shape<T, M>::BLOCKS)) void corr2_kernel(c
     // sharedScale is the row's block-shared scale, or null: the wave then
     // computes the row's scale itself, half a block at a time, into its own
     // exchange buffer, which is free once the inverse FFT is in registers.
-    auto finishRow = [&](const Row &row, const T *sharedScale, bool anyFlagShared) {
+    // QUEUE: a row block with flagged outputs (written as NaN by the
+    // epilogue) is recorded once for the recomputation launch, and the
+    // window statistics are added (flag counts packed as flagged + cancelled
+    // << 16; a shared scale's counts cover the block's longest row)
+    auto recordRow = [&](int rowIdx, int first, int n) {
+        if (lane == 0) {
+            const unsigned long long k = atomicAdd(queue.overCount, 1ull);
+            gptr<int> rec = to_global(queue.over) + 4 * k;
+            rec[0] = rowIdx;
+            rec[1] = static_cast<int>(base + first);
+            rec[2] = n;
+            rec[3] = 0;
+        }
+    };
+    auto countFlags = [&](unsigned long long nf, unsigned long long nc) {
+        if (sg.flagged != nullptr) {
+            atomicAdd(sg.flagged, nf);
+            if (nc != 0ull) { atomicAdd(sg.flagged + 1, nc); }
+        }
+    };
+    auto finishRow = [&](const Row &row, int rowIdx, const T *sharedScale, bool anyFlagShared, double flagSum) {
         const int nvRow = rowValid(row);
         T *outRow = static_cast<T *>(row.out) + base;
         if (sharedScale != nullptr) {
             epilogueHalf(Half0{}, sharedScale, 0, outRow, nvRow);
             epilogueHalf(Half1{}, sharedScale, 0, outRow, nvRow);
             if (anyFlagShared && nvRow > 0) {
-                fixup_row_wave<T>(raw, sharedScale, to_global(row.tmpl), row.Lt, to_global(outRow), nvRow, lane, a, 0,
-                                  sg.flagged);
+                if constexpr (QUEUE) {
+                    recordRow(rowIdx, 0, nvRow);
+                    if (lane == 0) {
+                        const unsigned long long packed = static_cast<unsigned long long>(flagSum);
+                        countFlags(packed & 0xffffull, packed >> 16);
+                    }
+                } else {
+                    fixup_row_wave<T>(raw, sharedScale, to_global(row.tmpl), row.Lt, to_global(outRow), nvRow, lane,
+                                      a, 0, sg.flagged);
+                }
             }
             return;
         }
@@ -917,9 +972,30 @@ __global__ __launch_bounds__(NT, (Corr2Shape<T, M>::BLOCKS)) void corr2_kernel(c
         // a row's own scale: interleaved chains in FP32, compute_scale in FP64
         auto ownScale = [&](int n, int first) {
             if constexpr (sizeof(T) == 4) {
-                return wave_row_scale<TILE, 2>(raw, tp1, tp2, own, a, n, row.Lt, noise, lane, first);
+                return wave_row_scale<TILE, 2, QUEUE>(raw, tp1, tp2, own, a, n, row.Lt, noise, lane, first);
             } else {
                 return compute_scale<T, TILE, 4>(raw, ts1, ts2, tp1, tp2, own, a, n, row.Lt, noise, lane, 64, first);
+            }
+        };
+        // after the epilogue of outputs [first, end) with this lane's flag
+        // count nf: record / count (QUEUE) or recompute in place
+        auto flagged = [&](int nf, bool anyFlag, int first, int end) {
+            if constexpr (QUEUE) {
+                if (anyFlag) {
+                    // the wave's packed counts (each lane's low part is at
+                    // most a few hundred, so the sum stays packed), then one
+                    // atomic per counter: per-lane atomics on one address
+                    // serialize
+                    int sum = nf;
+#pragma unroll
+                    for (int o = 32; o > 0; o >>= 1) { sum += __shfl_xor(sum, o, 64); }
+                    if (lane == 0) {
+                        countFlags(static_cast<unsigned long long>(sum & 0xffff), static_cast<unsigned long long>(sum >> 16));
+                    }
+                    recordRow(rowIdx, first, end - first);
+                }
+            } else {
+                if (anyFlag) { fixup_row_wave<T>(raw, own, tmplG, row.Lt, to_global(outRow), end, lane, a, first, sg.flagged); }
             }
         };
         if constexpr (!HALF) {
@@ -930,7 +1006,7 @@ __global__ __launch_bounds__(NT, (Corr2Shape<T, M>::BLOCKS)) void corr2_kernel(c
             wave_lds_sync();
             epilogueHalf(Half0{}, own, 0, outRow, nvRow);
             epilogueHalf(Half1{}, own, 0, outRow, nvRow);
-            if (anyFlag) { fixup_row_wave<T>(raw, own, tmplG, row.Lt, to_global(outRow), nvRow, lane, a, 0, sg.flagged); }
+            flagged(nf, anyFlag, 0, nvRow);
             wave_lds_sync();  // the next row reuses buf
             return;
         }
@@ -942,7 +1018,7 @@ __global__ __launch_bounds__(NT, (Corr2Shape<T, M>::BLOCKS)) void corr2_kernel(c
             const bool anyFlag = __ballot(nf > 0) != 0ull;
             wave_lds_sync();
             epilogueHalf(Half0{}, own, 0, outRow, nvRow);
-            if (anyFlag) { fixup_row_wave<T>(raw, own, tmplG, row.Lt, to_global(outRow), n0, lane, a, 0, sg.flagged); }
+            flagged(nf, anyFlag, 0, n0);
             wave_lds_sync();  // the second half reuses the buffer
         }
         // half 1: outputs [N/2, N)
@@ -953,9 +1029,7 @@ __global__ __launch_bounds__(NT, (Corr2Shape<T, M>::BLOCKS)) void corr2_kernel(c
             const bool anyFlag = __ballot(nf > 0) != 0ull;
             wave_lds_sync();
             epilogueHalf(Half1{}, own, N / 2, outRow, nvRow);
-            if (anyFlag) {
-                fixup_row_wave<T>(raw, own, tmplG, row.Lt, to_global(outRow), nvRow, lane, a, N / 2, sg.flagged);
-            }
+            flagged(nf, anyFlag, N / 2, nvRow);
             wave_lds_sync();  // the next row reuses the buffer
         }
     };
@@ -978,24 +1052,26 @@ __global__ __launch_bounds__(NT, (Corr2Shape<T, M>::BLOCKS)) void corr2_kernel(c
                     if (wave == 0) {
                         forwardFft();
                     } else {
-                        myFlags = compute_scale<T, TILE>(raw, ts1, ts2, tp1, tp2, sc, a, nValidSeg, Lt, noise,
+                        myFlags = compute_scale<T, TILE, 1, QUEUE>(raw, ts1, ts2, tp1, tp2, sc, a, nValidSeg, Lt, noise,
                                                          tid - 64, NT - 64);
                     }
                     fftDone = true;
                 } else {
-                    myFlags = compute_scale<T, TILE>(raw, ts1, ts2, tp1, tp2, sc, a, nValidSeg, Lt, noise, tid, NT);
+                    myFlags = compute_scale<T, TILE, 1, QUEUE>(raw, ts1, ts2, tp1, tp2, sc, a, nValidSeg, Lt, noise, tid,
+                                                               NT);
                 }
-                const bool anyFlag = block_sum<NT>(static_cast<double>(myFlags), red) > 0.0;
+                const double flagSum = uniform(block_sum<NT>(static_cast<double>(myFlags), red));  // scalar registers
+                const bool anyFlag = flagSum > 0.0;
                 int r = g0 + wave;
                 if (r < g1) {
                     if (PREFETCH) { loadH(rows[r].H); }
                     startRow(rows[r], rows[r + NWAVE < g1 ? r + NWAVE : r].H);
                     for (int rn = r + NWAVE; rn < g1; rn += NWAVE) {
-                        finishRow(rows[r], sc, anyFlag);
+                        finishRow(rows[r], r, sc, anyFlag, flagSum);
                         startRow(rows[rn], rows[rn + NWAVE < g1 ? rn + NWAVE : rn].H);
                         r = rn;
                     }
-                    finishRow(rows[r], sc, anyFlag);
+                    finishRow(rows[r], r, sc, anyFlag, flagSum);
                 }
                 __syncthreads();  // sc is rebuilt for the next group
             }
@@ -1032,12 +1108,12 @@ __global__ __launch_bounds__(NT, (Corr2Shape<T, M>::BLOCKS)) void corr2_kernel(c
         startRow(rows[cur], rows[nx >= 0 ? nx : cur].H);
         while (nx >= 0) {
             const int nn = nextSolo();
-            finishRow(rows[cur], nullptr, false);
+            finishRow(rows[cur], cur, nullptr, false, 0.0);
             startRow(rows[nx], rows[nn >= 0 ? nn : nx].H);
             cur = nx;
             nx = nn;
         }
-        finishRow(rows[cur], nullptr, false);
+        finishRow(rows[cur], cur, nullptr, false, 0.0);
     }
 }
 
@@ -1456,11 +1532,82 @@ hipError_t corr_launch(const Item *dItems, int nItems, const Seg *dSegs, const R
 
 template <typename T, int M>
 hipError_t corr2_launch(const Item *dItems, int nItems, const Seg *dSegs, const Row *dRows,
-                        const void *lo, const void *hi, hipStream_t stream)
+                        const void *lo, const void *hi, FixQueue queue, hipStream_t stream)
 {
-    hipLaunchKernelGGL((corr2_kernel<T, M>), dim3(nItems), dim3(NT), 0, stream, dItems, dSegs,
-                       dRows, static_cast<const Cx<T> *>(lo), static_cast<const Cx<T> *>(hi));
+    if (sizeof(T) == 4 && queue.over != nullptr) {
+        hipLaunchKernelGGL((corr2_kernel<T, M, sizeof(T) == 4>), dim3(nItems), dim3(NT), 0, stream, dItems, dSegs, dRows,
+                           static_cast<const Cx<T> *>(lo), static_cast<const Cx<T> *>(hi), queue);
+    } else {
+        hipLaunchKernelGGL((corr2_kernel<T, M, false>), dim3(nItems), dim3(NT), 0, stream, dItems, dSegs, dRows,
+                           static_cast<const Cx<T> *>(lo), static_cast<const Cx<T> *>(hi), FixQueue{});
+    }
     return hipGetLastError();
+}
+
+// Exact FP64 value of each queued FP32 window, one window per thread: one
+// pass over d = x - x0 (x0 the window's first sample, so every d is exact
+// and a cancelled variance keeps its digits; a constant window gives 0):
+// rho = sum t d / sqrt(sum d^2 - (sum d)^2 / Lt), the template's zero mean
+// making sum t d the centred numerator; |rho| > 1.1 gives 0 (the
+// reference's tooBig rule).  Rows of either table (FFT rows, time-domain
+// rows): their channel samples, FP64 template, output row and length.
+template <typename RowT>
+__global__ __launch_bounds__(NT) void fix_kernel(const RowT *__restrict__ rows, FixQueue queue)
+{
+    const unsigned long long n = min(*queue.count, queue.cap);
+    const gptr<const unsigned long long> e = to_global(reinterpret_cast<const unsigned long long *>(queue.entries));
+    for (unsigned long long k = blockIdx.x * NT + threadIdx.x; k < n; k += static_cast<unsigned long long>(gridDim.x) * NT) {
+        const unsigned long long v = e[k];
+        const RowT row = rows[static_cast<unsigned>(v) & 0x7fffffffu];
+        const int pos = static_cast<int>(v >> 32);
+        const gptr<const float> x = to_global(static_cast<const float *>(row.x)) + pos;
+        const gptr<const double> t = to_global(row.tmpl);
+        const int Lt = row.Lt;
+        const double shift = x[0];
+        double s1 = 0, s2 = 0, dn = 0;
+#pragma unroll 8
+        for (int m = 0; m < Lt; ++m) {
+            const double d = static_cast<double>(x[m]) - shift;
+            s1 += d;
+            s2 = fma(d, d, s2);
+            dn = fma(t[m], d, dn);
+        }
+        double rho = dn / fmax(DBL_EPSILON, sqrt(fmax(0.0, s2 - s1 * s1 / Lt)));
+        if (fabs(rho) > 1.1) { rho = 0; }
+        to_global(static_cast<float *>(row.out))[pos] = static_cast<float>(rho);
+    }
+}
+
+// The recorded row blocks of a queue-mode wave-kernel launch: each NaN
+// output (a flagged window) recomputed exactly, as fix_kernel does; one
+// record per workgroup.
+__global__ __launch_bounds__(NT) void fix_blocks_kernel(const Row *__restrict__ rows, FixQueue queue)
+{
+    const unsigned long long n = *queue.overCount;
+    for (unsigned long long k = blockIdx.x; k < n; k += gridDim.x) {
+        const gptr<const int> rec = to_global(static_cast<const int *>(queue.over)) + 4 * k;
+        const Row row = rows[rec[0]];
+        const int p0 = rec[1], cnt = rec[2];
+        const gptr<const double> t = to_global(row.tmpl);
+        const int Lt = row.Lt;
+        gptr<float> out = to_global(static_cast<float *>(row.out));
+        for (int q = threadIdx.x; q < cnt; q += NT) {
+            if (!isnan(out[p0 + q])) { continue; }
+            const gptr<const float> x = to_global(static_cast<const float *>(row.x)) + p0 + q;
+            const double shift = x[0];
+            double s1 = 0, s2 = 0, dn = 0;
+#pragma unroll 8
+            for (int m = 0; m < Lt; ++m) {
+                const double d = static_cast<double>(x[m]) - shift;
+                s1 += d;
+                s2 = fma(d, d, s2);
+                dn = fma(t[m], d, dn);
+            }
+            double rho = dn / fmax(DBL_EPSILON, sqrt(fmax(0.0, s2 - s1 * s1 / Lt)));
+            if (fabs(rho) > 1.1) { rho = 0; }
+            out[p0 + q] = static_cast<float>(rho);
+        }
+    }
 }
 
 template <int M>
@@ -1483,21 +1630,23 @@ bool corr_variant_available(int precision, int logM, int variant)
 
 hipError_t launch_corr(int precision, int logM, int variant, const Item *dItems, int nItems,
                        const Seg *dSegs, const Row *dRows, const void *dTwLo, const void *dTwHi,
-                       hipStream_t stream)
+                       FixQueue queue, hipStream_t stream)
 {
     if (nItems <= 0) { return hipSuccess; }
     if (variant == 2) {
         if (precision == 0) {
             switch (logM) {
-            case 9: return corr2_launch<float, 512>(dItems, nItems, dSegs, dRows, dTwLo, dTwHi, stream);
-            case 10: return corr2_launch<float, 1024>(dItems, nItems, dSegs, dRows, dTwLo, dTwHi, stream);
-            case 11: return corr2_launch<float, 2048>(dItems, nItems, dSegs, dRows, dTwLo, dTwHi, stream);
+            case 9: return corr2_launch<float, 512>(dItems, nItems, dSegs, dRows, dTwLo, dTwHi, queue, stream);
+            case 10: return corr2_launch<float, 1024>(dItems, nItems, dSegs, dRows, dTwLo, dTwHi, queue, stream);
+            case 11: return corr2_launch<float, 2048>(dItems, nItems, dSegs, dRows, dTwLo, dTwHi, queue, stream);
             default: return hipErrorInvalidValue;
             }
         }
+        // FP64 rows are recomputed in place (their tolerance wants the
+        // block's own anchored samples)
         switch (logM) {
-        case 9: return corr2_launch<double, 512>(dItems, nItems, dSegs, dRows, dTwLo, dTwHi, stream);
-        case 10: return corr2_launch<double, 1024>(dItems, nItems, dSegs, dRows, dTwLo, dTwHi, stream);
+        case 9: return corr2_launch<double, 512>(dItems, nItems, dSegs, dRows, dTwLo, dTwHi, FixQueue{}, stream);
+        case 10: return corr2_launch<double, 1024>(dItems, nItems, dSegs, dRows, dTwLo, dTwHi, FixQueue{}, stream);
         default: return hipErrorInvalidValue;
         }
     }
@@ -1516,6 +1665,20 @@ hipError_t launch_corr(int precision, int logM, int variant, const Item *dItems,
     case 11: return corr_launch<double, 2048>(dItems, nItems, dSegs, dRows, dTwLo, dTwHi, stream);
     default: return hipErrorInvalidValue;
     }
+}
+
+hipError_t launch_fix(const Row *dRows, FixQueue queue, int blocks, hipStream_t stream)
+{
+    if (blocks <= 0 || queue.over == nullptr) { return hipSuccess; }
+    hipLaunchKernelGGL(fix_blocks_kernel, dim3(blocks), dim3(NT), 0, stream, dRows, queue);
+    return hipGetLastError();
+}
+
+hipError_t launch_fix(const TdRow *dRows, FixQueue queue, int blocks, hipStream_t stream)
+{
+    if (blocks <= 0 || queue.entries == nullptr) { return hipSuccess; }
+    hipLaunchKernelGGL(fix_kernel<TdRow>, dim3(blocks), dim3(NT), 0, stream, dRows, queue);
+    return hipGetLastError();
 }
 
 hipError_t launch_tmpl(int precision, int logM, const TmplJob *dJobs, int nJobs,

@@ -45,6 +45,25 @@ struct Row {
     void *out;            // correlation output row
     int nOut;             // outputs of this row
     int Lt;               // template length
+    const void *x;        // the row's channel samples
+};
+
+// Queue of flagged windows for the exact recomputation kernel: 64-bit
+// entries, output index << 32 | row | cancelled << 31 (row: index into the
+// launch's row table).  The correlation launches append while *count stays
+// within cap (the rest are recomputed in place); *count is cleared before
+// them.  A null `entries` means no queue: every flagged window is
+// recomputed in place.
+struct FixQueue {
+    unsigned long long *count;
+    int *entries;  // 8 bytes per entry
+    unsigned long long cap;
+    // row blocks whose flagged windows did not fit the queue: (row, first
+    // output, outputs) per record, recomputed whole; sized by the host for
+    // every row block of the launches (two records per row block at most),
+    // so it cannot overflow.  Only the FFT wave-kernel queue has them.
+    unsigned long long *overCount;
+    int *over;  // 4 ints per record
 };
 
 // One work item: block `block` of lane `seg`, for the first nRows rows of
@@ -132,16 +151,6 @@ struct TdRow {
     void *pad;
 };
 
-// Queue of flagged time-domain windows for the exact recomputation kernel:
-// 64-bit entries, output index << 32 | row | cancelled << 31.  The correlation launches
-// append while *count stays within cap (the rest are recomputed in place);
-// *count is cleared before them.
-struct TdFixQueue {
-    unsigned long long *count;
-    int *entries;  // 8 bytes per entry
-    unsigned long long cap;
-};
-
 // One work item: outputs [seg0, seg0 + kTdOutputsPerItem) of lane `seg`
 // for rows [rowBegin, rowBegin + nRows) (nRows <= 128) of the row table.
 struct TdItem {
@@ -164,9 +173,11 @@ bool td_supported(int nb);
 int td_taps(int nb);  // padded tap count KT (64, 128 or 256)
 hipError_t launch_td_prep(int kind, const TdPrepJob *dJobs, int nJobs, int KT, hipStream_t stream);
 hipError_t launch_td(int kind, int KT, const TdItem *dItems, int nItems, const Seg *dSegs, const TdRow *dRows,
-                     TdFixQueue queue, hipStream_t stream);
-// exact recomputation of the queued windows (after the correlation launches)
-hipError_t launch_td_fix(const TdRow *dRows, TdFixQueue queue, int blocks, hipStream_t stream);
+                     FixQueue queue, hipStream_t stream);
+// exact FP32 recomputation of the queued windows (after the correlation
+// launches that filled the queue), for either row table
+hipError_t launch_fix(const Row *dRows, FixQueue queue, int blocks, hipStream_t stream);  // and its overflow records
+hipError_t launch_fix(const TdRow *dRows, FixQueue queue, int blocks, hipStream_t stream);
 
 // Correlation + normalisation: one launch per FFT size.  Variant 1 runs
 // block-wide LDS Stockham FFTs; variant 2 runs wave-level register FFTs
@@ -174,7 +185,7 @@ hipError_t launch_td_fix(const TdRow *dRows, TdFixQueue queue, int blocks, hipSt
 bool corr_variant_available(int precision, int logM, int variant);
 hipError_t launch_corr(int precision, int logM, int variant, const Item *dItems, int nItems,
                        const Seg *dSegs, const Row *dRows, const void *dTwLo,
-                       const void *dTwHi, hipStream_t stream);
+                       const void *dTwHi, FixQueue queue, hipStream_t stream);
 // Template demean/normalise + spectra (FP64 arithmetic): one block per job.
 hipError_t launch_tmpl(int precision, int logM, const TmplJob *dJobs, int nJobs,
                        const void *dTwLo64, const void *dTwHi64, hipStream_t stream);

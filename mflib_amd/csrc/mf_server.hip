@@ -210,6 +210,7 @@ void launchBatch(mf_server *s, Slot &slot)
                     row.out = out + (static_cast<size_t>(i) * r->nC + c) * r->ldOut * s->tsize;
                     row.nOut = r->nOut[c];
                     row.Lt = t.Lt;
+                    row.x = x;
                     rows.push_back(row);
                     windows += row.nOut;
                     hmax = std::max(hmax, t.hmax[lg]);
@@ -267,7 +268,7 @@ void launchBatch(mf_server *s, Slot &slot)
         const bool f = s->precision == MF_PRECISION_FLOAT;
         MF_HIP(mfk::launch_corr(s->precision, la.logM, la.variant, dItems + la.first,
                                 la.count, dSegs, dRows, f ? t.lo.p : t.lo64.p, f ? t.hi.p : t.hi64.p,
-                                slot.stream));
+                                mfk::FixQueue{}, slot.stream));
     }
     MF_HIP(hipEventRecord(slot.done, slot.stream));
 }

@@ -142,7 +142,7 @@ constexpr int td_blocks_per_cu() { return TdTraits<MT>::parts * KT / 4 <= 128 ? 
 template <typename MT, int KT>
 __global__ __launch_bounds__(NT, (td_blocks_per_cu<MT, KT>())) void tdcorr_kernel(const TdItem *__restrict__ items,
                                                        const Seg *__restrict__ segs,
-                                                       const TdRow *__restrict__ rows, TdFixQueue queue)
+                                                       const TdRow *__restrict__ rows, FixQueue queue)
 {
     using Tr = TdTraits<MT>;
     using V8 = typename Tr::vec8;
@@ -394,7 +394,7 @@ __global__ __launch_bounds__(NT, (td_blocks_per_cu<MT, KT>())) void tdcorr_kerne
     // 6. flagged windows, after the tile loop (the taps are dead here).
     //    Exactly constant windows (a cancelled variance over equal
     //    samples) are 0 at once; the others go to the device queue, whose
-    //    windows td_fix_kernel recomputes after the launch.  When the queue
+    //    windows fix_kernel recomputes after the launch.  When the queue
     //    is full they go to the wave's list, which is recomputed here one
     //    window per lane in passes of 64 (a few left at the end: the whole
     //    wave on one window at a time).  List entry: tile
@@ -530,35 +530,6 @@ __global__ __launch_bounds__(NT) void td_prep_kernel(const TdPrepJob *__restrict
     }
 }
 
-// Exact FP64 value of each queued window, one window per thread: one pass
-// over d = x - x0 (x0 the window's first sample), as exact_window_td's
-// cancelled case; |rho| > 1.1 gives 0.
-__global__ __launch_bounds__(NT) void td_fix_kernel(const TdRow *__restrict__ rows, TdFixQueue queue)
-{
-    const unsigned long long n = min(*queue.count, queue.cap);
-    const gptr<const unsigned long long> e = to_global(reinterpret_cast<const unsigned long long *>(queue.entries));
-    for (unsigned long long k = blockIdx.x * NT + threadIdx.x; k < n; k += static_cast<unsigned long long>(gridDim.x) * NT) {
-        const unsigned long long v = e[k];
-        const TdRow row = rows[static_cast<unsigned>(v) & 0x7fffffffu];
-        const int pos = static_cast<int>(v >> 32);
-        const gptr<const float> x = to_global(row.x) + pos;
-        const gptr<const double> t = to_global(row.tmpl);
-        const int Lt = row.Lt;
-        const double shift = x[0];
-        double s1 = 0, s2 = 0, dn = 0;
-#pragma unroll 8
-        for (int m = 0; m < Lt; ++m) {
-            const double d = static_cast<double>(x[m]) - shift;
-            s1 += d;
-            s2 = fma(d, d, s2);
-            dn = fma(t[m], d, dn);
-        }
-        double rho = dn / fmax(DBL_EPSILON, sqrt(fmax(0.0, s2 - s1 * s1 / Lt)));
-        if (fabs(rho) > 1.1) { rho = 0; }
-        to_global(static_cast<float *>(row.out))[pos] = static_cast<float>(rho);
-    }
-}
-
 }  // namespace
 
 bool td_supported(int nb) { return nb >= 1 && nb <= kTdMaxTaps; }
@@ -576,15 +547,8 @@ hipError_t launch_td_prep(int kind, const TdPrepJob *dJobs, int nJobs, int KT, h
     return hipGetLastError();
 }
 
-hipError_t launch_td_fix(const TdRow *dRows, TdFixQueue queue, int blocks, hipStream_t stream)
-{
-    if (blocks <= 0) { return hipSuccess; }
-    hipLaunchKernelGGL(td_fix_kernel, dim3(blocks), dim3(NT), 0, stream, dRows, queue);
-    return hipGetLastError();
-}
-
 hipError_t launch_td(int kind, int KT, const TdItem *dItems, int nItems, const Seg *dSegs, const TdRow *dRows,
-                     TdFixQueue queue, hipStream_t stream)
+                     FixQueue queue, hipStream_t stream)
 {
     if (nItems <= 0) { return hipSuccess; }
 #define MF_TD_LAUNCH(MT, K) \
