@@ -506,4 +506,20 @@ __device__ __forceinline__ void wave_fft_regs_half(Cx<T> *v, Cx<T> *buf, int lan
     wave_lds_sync();
 }
 
+// True when the Lt samples of the window at i are all equal (its exact
+// value is then 0: zero numerator over the floor of the norm), from the
+// equal-neighbour bits.
+__device__ __forceinline__ bool constant_window(const unsigned *eqw, int i, int Lt)
+{
+    const int end = i + Lt - 1;  // bits [i, end)
+    for (int p = i; p < end;) {
+        const int b = p & 31;
+        const int take = min(32 - b, end - p);
+        const unsigned mask = (take == 32 ? ~0u : ((1u << take) - 1u)) << b;
+        if ((eqw[p >> 5] & mask) != mask) { return false; }
+        p += take;
+    }
+    return true;
+}
+
 }  // namespace mfdev

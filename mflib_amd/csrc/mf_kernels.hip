@@ -1580,20 +1580,63 @@ __global__ __launch_bounds__(NT) void fix_kernel(const RowT *__restrict__ rows, 
 
 // The recorded row blocks of a queue-mode wave-kernel launch: each NaN
 // output (a flagged window) recomputed exactly, as fix_kernel does; one
-// record per workgroup.
+// record per workgroup.  The record's samples are staged in LDS with their
+// equal-neighbour bits, so an exactly constant window (a gap) is 0 at once
+// (zero numerator over the floor of the norm); records longer than the
+// staging buffer read their samples from global memory.
+constexpr int kFixSpan = 6144;  // samples of a record staged in LDS: outputs + Lt - 1
 __global__ __launch_bounds__(NT) void fix_blocks_kernel(const Row *__restrict__ rows, FixQueue queue)
 {
+    __shared__ float xs[kFixSpan];
+    __shared__ unsigned eqw[kFixSpan / 32];
     const unsigned long long n = *queue.overCount;
+    const int tid = threadIdx.x;
     for (unsigned long long k = blockIdx.x; k < n; k += gridDim.x) {
         const gptr<const int> rec = to_global(static_cast<const int *>(queue.over)) + 4 * k;
         const Row row = rows[rec[0]];
         const int p0 = rec[1], cnt = rec[2];
         const gptr<const double> t = to_global(row.tmpl);
         const int Lt = row.Lt;
-        gptr<float> out = to_global(static_cast<float *>(row.out));
-        for (int q = threadIdx.x; q < cnt; q += NT) {
-            if (!isnan(out[p0 + q])) { continue; }
-            const gptr<const float> x = to_global(static_cast<const float *>(row.x)) + p0 + q;
+        const gptr<const float> xg = to_global(static_cast<const float *>(row.x)) + p0;
+        gptr<float> out = to_global(static_cast<float *>(row.out)) + p0;
+        const int span = cnt + Lt - 1;
+        if (span <= kFixSpan) {  // uniform over the workgroup
+            __syncthreads();     // the previous record's reads are done
+            for (int j = tid; j < span; j += NT) { xs[j] = xg[j]; }
+            __syncthreads();
+            for (int w = tid; w < (span + 31) / 32; w += NT) {
+                unsigned bits = 0;
+                for (int b = 0; b < 32; ++b) {
+                    const int j = 32 * w + b;
+                    if (j + 1 < span && xs[j] == xs[j + 1]) { bits |= 1u << b; }
+                }
+                eqw[w] = bits;
+            }
+            __syncthreads();
+            for (int q = tid; q < cnt; q += NT) {
+                if (!isnan(out[q])) { continue; }
+                if (constant_window(eqw, q, Lt)) {
+                    out[q] = 0.0f;
+                    continue;
+                }
+                const double shift = xs[q];
+                double s1 = 0, s2 = 0, dn = 0;
+#pragma unroll 8
+                for (int m = 0; m < Lt; ++m) {
+                    const double d = static_cast<double>(xs[q + m]) - shift;
+                    s1 += d;
+                    s2 = fma(d, d, s2);
+                    dn = fma(t[m], d, dn);
+                }
+                double rho = dn / fmax(DBL_EPSILON, sqrt(fmax(0.0, s2 - s1 * s1 / Lt)));
+                if (fabs(rho) > 1.1) { rho = 0; }
+                out[q] = static_cast<float>(rho);
+            }
+            continue;
+        }
+        for (int q = tid; q < cnt; q += NT) {
+            if (!isnan(out[q])) { continue; }
+            const gptr<const float> x = xg + q;
             const double shift = x[0];
             double s1 = 0, s2 = 0, dn = 0;
 #pragma unroll 8
@@ -1605,7 +1648,7 @@ __global__ __launch_bounds__(NT) void fix_blocks_kernel(const Row *__restrict__ 
             }
             double rho = dn / fmax(DBL_EPSILON, sqrt(fmax(0.0, s2 - s1 * s1 / Lt)));
             if (fabs(rho) > 1.1) { rho = 0; }
-            out[p0 + q] = static_cast<float>(rho);
+            out[q] = static_cast<float>(rho);
         }
     }
 }

@@ -78,22 +78,6 @@ struct TdTraits<__bf16> {
 constexpr int kTdFixList = 64 + 16 * 64;  // list capacity of a wave's flagged windows
 constexpr int kTdCoopFlags = 4;           // at most this many left at the end: the wave shares each window
 
-// True when the Lt samples of the window at i are all equal (its exact
-// value is then 0: zero numerator over the floor of the norm), from the
-// equal-neighbour bits.
-__device__ bool constant_window(const unsigned *eqw, int i, int Lt)
-{
-    const int end = i + Lt - 1;  // bits [i, end)
-    for (int p = i; p < end;) {
-        const int b = p & 31;
-        const int take = min(32 - b, end - p);
-        const unsigned mask = (take == 32 ? ~0u : ((1u << take) - 1u)) << b;
-        if ((eqw[p >> 5] & mask) != mask) { return false; }
-        p += take;
-    }
-    return true;
-}
-
 // Exact FP64 value of one flagged window at item offset i, over taps
 // m = m0, m0 + step, ... (one lane: 0, 1; a whole wave: lane, 64, summed
 // across the wave).
