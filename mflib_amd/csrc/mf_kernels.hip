@@ -1298,56 +1298,70 @@ __global__ __launch_bounds__(NT) void reduce_kernel(const void *const *__restric
 // setCorrelations (src/singleChannel/detector.cpp:252-284) sees a row:
 // clamped to [-1, 1] (NaN -> 1, as std::max(-1, std::min(1, x))), of rho
 // or of |rho|.
-constexpr int kPeakPerThread = 8;  // grid-stride iterations a launch aims for
+constexpr int kPeakVectors = 4;  // 16-byte vectors per thread per pass
 constexpr int kPeakRaw = 0;
 constexpr int kPeakClamp = 1;
 constexpr int kPeakClampAbs = 2;
 template <int XF, typename T>
-__device__ __forceinline__ T peak_value(gptr<const T> x, long long i)
+__device__ __forceinline__ T peak_xf(T v)
 {
-    T v = x[i];
     if constexpr (XF == kPeakClampAbs) { v = fabs(v); }
     if constexpr (XF != kPeakRaw) { v = fmax(T(-1), fmin(T(1), v)); }
     return v;
 }
 
-// Candidates of src/peakFinder.cpp:500-521: strictly above threshold and
-// a local maximum (plateaus keep their last sample; end-point rules).
+template <int XF, typename T>
+__device__ __forceinline__ T peak_value(gptr<const T> x, long long i)
+{
+    return peak_xf<XF>(x[i]);
+}
+
+// The candidate rule of src/peakFinder.cpp:500-521 for sample i of n, from
+// its value and its neighbours': strictly above the threshold and a local
+// maximum (a plateau keeps its last sample); an end point compares with its
+// one neighbour.
+template <typename T>
+__device__ __forceinline__ bool peak_rule(T xi, T left, T right, long long i, long long n, T thr)
+{
+    if (!(xi > thr)) { return false; }
+    if (n == 1) { return true; }
+    if (i == 0) { return right < xi; }
+    if (i == n - 1) { return xi > left; }
+    return (xi >= left) && (right < xi);
+}
+
+// Candidates of x[0, n).  Each sample is read once: the body, from the
+// first 16-byte boundary on, as 16-byte vectors, consecutive lanes holding
+// consecutive vectors, so a vector's outer neighbours come from the lanes
+// beside it (the first and last lane of a wave, and the last vector, load
+// theirs, only when a sample of theirs is above the threshold).  The
+// samples before the boundary and after the last whole vector (fewer than
+// two vectors' worth) are tested by block 0 one per thread.  Candidates are
+// counted per lane, scanned across the wave and appended with one atomic
+// per wave; their order in idx/val is arbitrary.
 template <typename T, int XF = kPeakRaw>
 __device__ __forceinline__ void peak_scan(gptr<const T> x, long long n, T thr, int *count, gptr<int> idx,
                                           gptr<T> val, int cap)
 {
-    const long long stride = static_cast<long long>(gridDim.x) * NT;
-    const long long start = static_cast<long long>(blockIdx.x) * NT + threadIdx.x;
-    // every lane of a wave runs the same number of iterations (ballot below)
-    const long long nIter = (n + stride - 1) / stride;
-    for (long long it = 0; it < nIter; ++it) {
-        const long long i = start + it * stride;
-        bool cand = false;
-        T xi = T(0);
-        if (i < n) {
-            xi = peak_value<XF>(x, i);
-            if (xi > thr) {
-                if (n == 1) {
-                    cand = true;
-                } else if (i == 0) {
-                    cand = peak_value<XF>(x, 1) < xi;
-                } else if (i == n - 1) {
-                    cand = xi > peak_value<XF>(x, n - 2);
-                } else {
-                    cand = (xi >= peak_value<XF>(x, i - 1)) && (peak_value<XF>(x, i + 1) < xi);
-                }
-            }
-        }
-        const unsigned long long mask = __ballot(cand);
-        if (mask != 0ull) {
-            const int lane = threadIdx.x & 63;
-            const int leader = __ffsll(static_cast<long long>(mask)) - 1;
-            int basePos = 0;
-            if (lane == leader) { basePos = atomicAdd(count, __popcll(mask)); }
-            basePos = __shfl(basePos, leader, 64);
-            if (cand) {
-                const int pos = basePos + __popcll(mask & ((1ull << lane) - 1ull));
+    constexpr int VEC = 16 / static_cast<int>(sizeof(T));
+    constexpr int U = kPeakVectors;
+    static_assert(U * VEC <= 32, "candidate bits");
+    using V = T __attribute__((ext_vector_type(VEC)));
+    const int lane = threadIdx.x & 63;
+    const unsigned mis = static_cast<unsigned>(reinterpret_cast<unsigned long long>(x) & 15ull);
+    long long head = static_cast<long long>(((16u - mis) & 15u) / sizeof(T));
+    if (head > n) { head = n; }
+    const long long nVec = (n - head) / VEC;
+    const long long tail0 = head + nVec * VEC;
+    const long long nEdge = head + (n - tail0);
+    if (blockIdx.x == 0 && threadIdx.x < nEdge) {
+        const long long i = threadIdx.x < head ? threadIdx.x : tail0 + (threadIdx.x - head);
+        const T xi = peak_value<XF>(x, i);
+        if (xi > thr) {
+            const T left = i > 0 ? peak_value<XF>(x, i - 1) : T(0);
+            const T right = i + 1 < n ? peak_value<XF>(x, i + 1) : T(0);
+            if (peak_rule(xi, left, right, i, n, thr)) {
+                const int pos = atomicAdd(count, 1);
                 if (pos < cap) {
                     idx[pos] = static_cast<int>(i);
                     val[pos] = xi;
@@ -1355,6 +1369,77 @@ __device__ __forceinline__ void peak_scan(gptr<const T> x, long long n, T thr, i
             }
         }
     }
+    const gptr<const V> xv = reinterpret_cast<gptr<const V>>(x + head);
+    const long long span = static_cast<long long>(NT) * U;
+    for (long long base = static_cast<long long>(blockIdx.x) * span; base < nVec;
+         base += static_cast<long long>(gridDim.x) * span) {
+        V v[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const long long k = base + u * NT + threadIdx.x;
+            v[u] = k < nVec ? xv[k] : V(T(0));
+        }
+        unsigned bits = 0u;
+        T s[U][VEC];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const long long k = base + u * NT + threadIdx.x;
+            bool any = false;
+#pragma unroll
+            for (int e = 0; e < VEC; ++e) {
+                s[u][e] = peak_xf<XF>(v[u][e]);
+                any = any || (s[u][e] > thr);
+            }
+            T left = __shfl_up(s[u][VEC - 1], 1, 64);
+            T right = __shfl_down(s[u][0], 1, 64);
+            if (k < nVec && any) {
+                const long long i0 = head + k * VEC;
+                if (lane == 0 && i0 > 0) { left = peak_value<XF>(x, i0 - 1); }
+                if ((lane == 63 || k + 1 >= nVec) && i0 + VEC < n) { right = peak_value<XF>(x, i0 + VEC); }
+#pragma unroll
+                for (int e = 0; e < VEC; ++e) {
+                    const T l = e > 0 ? s[u][e - 1] : left;
+                    const T r = e + 1 < VEC ? s[u][e + 1] : right;
+                    if (peak_rule(s[u][e], l, r, i0 + e, n, thr)) { bits |= 1u << (u * VEC + e); }
+                }
+            }
+        }
+        if (__ballot(bits != 0u) != 0ull) {
+            const int c = __popc(bits);
+            int incl = c;
+#pragma unroll
+            for (int o = 1; o < 64; o <<= 1) {
+                const int t = __shfl_up(incl, o, 64);
+                if (lane >= o) { incl += t; }
+            }
+            int basePos = 0;
+            if (lane == 63) { basePos = atomicAdd(count, incl); }
+            int pos = __shfl(basePos, 63, 64) + incl - c;
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+#pragma unroll
+                for (int e = 0; e < VEC; ++e) {
+                    if ((bits >> (u * VEC + e)) & 1u) {
+                        if (pos < cap) {
+                            idx[pos] = static_cast<int>(head + (base + u * NT + threadIdx.x) * VEC + e);
+                            val[pos] = s[u][e];
+                        }
+                        ++pos;
+                    }
+                }
+            }
+        }
+    }
+}
+
+// Workgroups of a peak scan over at most maxLen samples of tsize bytes:
+// one pass of kPeakVectors vectors per thread, at least one block (block 0
+// also tests the unaligned ends), at most `cap` blocks (grid-stride beyond).
+inline int peak_grid(long long maxLen, int tsize, int cap)
+{
+    const long long vecs = maxLen / (16 / tsize);
+    const long long want = (vecs + NT * kPeakVectors - 1) / (NT * kPeakVectors);
+    return static_cast<int>(want < 1 ? 1 : (want < cap ? want : cap));
 }
 
 // Streaming push of every channel, in 32-bit words, two launches: first
@@ -1814,9 +1899,7 @@ hipError_t launch_peak_groups(int precision, const StackGroup *dGroups, int nGro
                               double threshold, hipStream_t stream)
 {
     if (maxLen <= 0 || nGroups <= 0) { return hipSuccess; }
-    // about kPeakPerThread samples per thread: far fewer workgroups to dispatch
-    const long long want = (maxLen + NT * kPeakPerThread - 1) / (NT * kPeakPerThread);
-    const int grid = static_cast<int>(want < 8192 ? want : 8192);
+    const int grid = peak_grid(maxLen, precision == 0 ? 4 : 8, 8192);
     if (precision == 0) {
         hipLaunchKernelGGL(peak_groups_kernel<float>, dim3(grid, nGroups), dim3(NT), 0, stream, dGroups,
                            static_cast<float>(threshold));
@@ -1832,9 +1915,7 @@ hipError_t launch_peak_rows(int precision, const StackGroup *dGroups, int nGroup
 {
     if (maxLen <= 0 || nGroups <= 0) { return hipSuccess; }
     if (nGroups > 65535) { return hipErrorInvalidValue; }
-    // about kPeakPerThread samples per thread: far fewer workgroups to dispatch
-    const long long want = (maxLen + NT * kPeakPerThread - 1) / (NT * kPeakPerThread);
-    const int grid = static_cast<int>(want < 8192 ? want : 8192);
+    const int grid = peak_grid(maxLen, precision == 0 ? 4 : 8, 8192);
     if (precision == 0) {
         if (absval) {
             hipLaunchKernelGGL((peak_groups_kernel<float, kPeakClampAbs>), dim3(grid, nGroups), dim3(NT), 0, stream,
@@ -1885,8 +1966,7 @@ hipError_t launch_peaks(int precision, const void *dx, long long n, double thres
                         int *dIdx, void *dVal, int cap, hipStream_t stream)
 {
     if (n <= 0) { return hipSuccess; }
-    const long long want = (n + NT * kPeakPerThread - 1) / (NT * kPeakPerThread);
-    const int grid = static_cast<int>(want < 65536 ? want : 65536);
+    const int grid = peak_grid(n, precision == 0 ? 4 : 8, 65536);
     if (precision == 0) {
         hipLaunchKernelGGL(peak_kernel<float>, dim3(grid), dim3(NT), 0, stream,
                            static_cast<const float *>(dx), n, static_cast<float>(threshold), dCount,
