@@ -13,19 +13,21 @@
 //    lo = round(v - hi - mid); the six products hi*hi, hi*mid, mid*hi,
 //    hi*lo, lo*hi and mid*mid (the dropped ones, mid*lo, lo*mid and lo*lo,
 //    are below 2^-24 of |t||x|).
-// v_mfma_f32_32x32x16 accumulates them in FP32: hi*hi in its own
+// v_mfma_f32_16x16x32 accumulates them in FP32: hi*hi in its own
 // accumulator, the smaller products in a second one.
 // rho = s * D * scale with the window normalisation from FP64 prefix sums
 // of the block.
 //
 // Work item: IS = 512 consecutive outputs of the lane and up to 128 rows.
-// Each of the 4 waves owns 32 rows: their split taps stay in registers
-// (weight-stationary, parts * KT / 4 VGPRs) while the wave walks the 16
-// output tiles of 32; the Hankel operand is read from LDS, where entry p
-// holds the 8 samples x~[p .. p+7] (one 16-byte read per part and k-step).
+// Each of the 4 waves takes 16 rows at a time: their split taps stay in
+// registers (weight-stationary, parts * KT / 8 VGPRs) while the wave walks
+// the 32 output tiles of 16; the Hankel operand is read from LDS, where
+// entry p holds the 8 samples x~[p .. p+7] (one 16-byte read per part and
+// k-step of 32).
 //
 // Accuracy: per window |d rho| <= eps_m * sqrt(Lt S2) / den with
-// eps_m = eps_split + (4 sqrt(KT/16) + 4) 2^-24 (FP32 accumulation and the
+// eps_m = eps_split + (4 sqrt(KT/16) + 4) 2^-24 (FP32 accumulation over at
+// most KT / 16 accumulator updates -- the kernel makes KT / 32 -- and the
 // FP32 epilogue on the FP64 window sums), S2 the
 // window's sum of (x - a)^2 and den its centred norm; eps_split is the
 // relative size of the dropped products and split residuals: 3 * 2^-22 for
@@ -47,36 +49,35 @@ namespace {
 
 constexpr int NT = kThreads;
 constexpr int IS = kTdOutputsPerItem;  // outputs per work item
-constexpr int NTILE_I = IS / 32;       // output tiles of 32 per item
 
-typedef float f32x16 __attribute__((ext_vector_type(16)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
 typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef int i32x4 __attribute__((ext_vector_type(4)));
 
+// The 16-bit operand type's parts and its v_mfma_f32_16x16x32 form.
 template <typename MT>
 struct TdTraits;
 template <>
 struct TdTraits<_Float16> {
     using vec8 = f16x8;
     static constexpr int parts = 2;
-    __device__ static f32x16 mfma(vec8 a, vec8 b, f32x16 c)
+    __device__ static f32x4 mfma(vec8 a, vec8 b, f32x4 c)
     {
-        return __builtin_amdgcn_mfma_f32_32x32x16_f16(a, b, c, 0, 0, 0);
+        return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0);
     }
 };
 template <>
 struct TdTraits<__bf16> {
     using vec8 = bf16x8;
     static constexpr int parts = 3;
-    __device__ static f32x16 mfma(vec8 a, vec8 b, f32x16 c)
+    __device__ static f32x4 mfma(vec8 a, vec8 b, f32x4 c)
     {
-        return __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, c, 0, 0, 0);
+        return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
     }
 };
 
-constexpr int kTdFixList = 64 + 16 * 64;  // list capacity of a wave's flagged windows
-constexpr int kTdCoopFlags = 4;           // at most this many left at the end: the wave shares each window
+constexpr int kTdCoopFlags = 4;  // at most this many left at the end: the wave shares each window
 
 // Exact FP64 value of one flagged window at item offset i, over taps
 // m = m0, m0 + step, ... (one lane: 0, 1; a whole wave: lane, 64, summed
@@ -117,57 +118,41 @@ __device__ double exact_window_td(const float *raw, const double *P1, const doub
     return fabs(rho) > 1.1 ? 0.0 : rho;
 }
 
-// Blocks per CU the register budget is capped for: two while the split taps
-// (parts * KT / 4 VGPRs) leave room for the accumulators and the epilogue
-// in 256 VGPRs, else one (the BF16 triple at KT = 256: 192 VGPRs of taps).
+// One work item's staged samples in LDS, shared by both tile shapes.
 template <typename MT, int KT>
-constexpr int td_blocks_per_cu() { return TdTraits<MT>::parts * KT / 4 <= 128 ? 2 : 1; }
+struct TdStage {
+    static constexpr int NP = TdTraits<MT>::parts;
+    static constexpr int SPAN = IS + KT;  // samples one item reads
+    MT x8[NP][SPAN][8];                   // entry p of part q: x~[p .. p+7]
+    float raw[SPAN];                      // samples (FP32), for exact windows
+    double P1[SPAN + 1];                  // prefix sums of d = x - a
+    double P2[SPAN + 1];                  //   and of d^2
+    double red[NT / 64];
+    double wsum1[NT / 64], wsum2[NT / 64];
+    unsigned eqw[(SPAN + 31) / 32];       // bit p: sample p equals sample p + 1 (constant windows)
+};
 
+// What the epilogue needs of the staged block: the anchor a, the scale s
+// of x~ (FP32) and the cancellation floor 1e-12 E (per tap of Lt).
+struct TdBlock {
+    double a;
+    float scF;
+    float flK;
+};
+
+// Stages outputs [base, base + IS) of lane sg: loads the samples, takes the
+// block anchor and RMS, splits x~ into the 8-wide Hankel entries of every
+// part, builds the FP64 prefix sums and the equal-neighbour bits.  The
+// caller synchronises the block before reading any of it.
 template <typename MT, int KT>
-__global__ __launch_bounds__(NT, (td_blocks_per_cu<MT, KT>())) void tdcorr_kernel(const TdItem *__restrict__ items,
-                                                       const Seg *__restrict__ segs,
-                                                       const TdRow *__restrict__ rows, FixQueue queue)
+__device__ __forceinline__ TdBlock td_stage(TdStage<MT, KT> &sm, const Seg &sg, long long base)
 {
-    using Tr = TdTraits<MT>;
-    using V8 = typename Tr::vec8;
-    constexpr int NP = Tr::parts;
-    constexpr int NS = KT / 16;        // k-steps of 16
-    constexpr int SPAN = IS + KT;      // samples one item reads
+    constexpr int NP = TdTraits<MT>::parts;
+    constexpr int SPAN = TdStage<MT, KT>::SPAN;
     constexpr int SPT = (SPAN + NT - 1) / NT;
-    __shared__ __align__(16) MT x8[NP][SPAN][8];  // entry p of part q: x~[p .. p+7]
-    __shared__ float raw[SPAN];                   // samples (FP32), for exact windows
-    __shared__ double P1[SPAN + 1];               // prefix sums of d = x - a
-    __shared__ double P2[SPAN + 1];               //   and of d^2
-    __shared__ double red[NT / 64];
-    __shared__ double wsum1[NT / 64], wsum2[NT / 64];
-    // flagged windows of each wave's tiles: bit r of the low half flags the
-    // lane's window r, bit r of the high half marks a cancelled variance
-    __shared__ unsigned fq[NT / 64][NTILE_I][64];
-    // bit p: sample p equals sample p + 1 (exactly constant windows)
-    __shared__ unsigned eqw[(SPAN + 31) / 32];
-    // each wave's list of flagged windows to recompute (under 64 carried
-    // over plus one tile's 1024)
-    __shared__ unsigned short fixList[NT / 64][kTdFixList];
-
-    const TdItem item = items[blockIdx.x];
-    const Seg sg = segs[item.seg];
     const int tid = threadIdx.x;
-    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);  // wave-uniform: scalar row loads and branches
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int lane = tid & 63;
-    const int half = lane >> 5;
-    const int col = lane & 31;
-    const long long base = item.seg0;
-
-    if (sg.dead) {
-        for (int r = 0; r < item.nRows; ++r) {
-            const TdRow row = rows[item.rowBegin + r];
-            const int nv = static_cast<int>(min(static_cast<long long>(IS), row.nOut - base));
-            gptr<float> out = to_global(static_cast<float *>(row.out)) + base;
-            for (int i = tid; i < nv; i += NT) { out[i] = 0.0f; }
-        }
-        return;
-    }
-
     // 1. samples, anchor and scale
     const gptr<const float> x = to_global(static_cast<const float *>(sg.x));
     const int nSamp = static_cast<int>(min(static_cast<long long>(SPAN), sg.n - base));
@@ -178,10 +163,10 @@ __global__ __launch_bounds__(NT, (td_blocks_per_cu<MT, KT>())) void tdcorr_kerne
         const int p = tid * SPT + q;  // contiguous run per thread (for the scan)
         const float val = (p < nSamp) ? x[base + p] : 0.0f;
         v[q] = val;
-        if (p < SPAN) { raw[p] = val; }
+        if (p < SPAN) { sm.raw[p] = val; }
         s += static_cast<double>(val);
     }
-    const double a = block_sum<NT>(s, red) / nSamp;
+    const double a = block_sum<NT>(s, sm.red) / nSamp;
     double e = 0, l1 = 0, l2 = 0;
     double d1[SPT], d2[SPT];
 #pragma unroll
@@ -194,11 +179,9 @@ __global__ __launch_bounds__(NT, (td_blocks_per_cu<MT, KT>())) void tdcorr_kerne
         l2 += d * d;
         e += d * d;
     }
-    const double energy = block_sum<NT>(e, red);
+    const double energy = block_sum<NT>(e, sm.red);
     const double sc = energy > 0 ? sqrt(energy / nSamp) : 1.0;
     const float inv = static_cast<float>(1.0 / sc);
-    const float scF = static_cast<float>(sc);
-    const float flK = static_cast<float>(1.0e-12 * energy);  // times Lt: the cancellation floor
     // 2. split samples into the 8-wide Hankel entries: sample p goes to
     //    entry p - k, slot k (k = 0..7), of every part.  The FP32 value
     //    xs is the sum of its parts exactly (each remainder is exact in
@@ -214,7 +197,7 @@ __global__ __launch_bounds__(NT, (td_blocks_per_cu<MT, KT>())) void tdcorr_kerne
                 rest -= static_cast<float>(part);
 #pragma unroll
                 for (int k = 0; k < 8; ++k) {
-                    if (p - k >= 0) { x8[pt][p - k][k] = part; }
+                    if (p - k >= 0) { sm.x8[pt][p - k][k] = part; }
                 }
             }
         }
@@ -223,7 +206,7 @@ __global__ __launch_bounds__(NT, (td_blocks_per_cu<MT, KT>())) void tdcorr_kerne
     for (int p = SPAN - 7 + tid; p < SPAN; p += NT) {
         for (int k = SPAN - p; k < 8; ++k) {
 #pragma unroll
-            for (int pt = 0; pt < NP; ++pt) { x8[pt][p][k] = static_cast<MT>(0.0f); }
+            for (int pt = 0; pt < NP; ++pt) { sm.x8[pt][p][k] = static_cast<MT>(0.0f); }
         }
     }
     // 3. FP64 prefix sums P[p] = sum_{q < p} d_q (thread runs, then waves)
@@ -238,256 +221,315 @@ __global__ __launch_bounds__(NT, (td_blocks_per_cu<MT, KT>())) void tdcorr_kerne
         }
     }
     if (lane == 63) {
-        wsum1[wave] = x1;
-        wsum2[wave] = x2;
+        sm.wsum1[wave] = x1;
+        sm.wsum2[wave] = x2;
     }
     __syncthreads();
     if (tid < (SPAN + 31) / 32) {
         unsigned bits = 0;
         for (int b = 0; b < 32; ++b) {
             const int p = 32 * tid + b;
-            if (p + 1 < SPAN && raw[p] == raw[p + 1]) { bits |= 1u << b; }
+            if (p + 1 < SPAN && sm.raw[p] == sm.raw[p + 1]) { bits |= 1u << b; }
         }
-        eqw[tid] = bits;
+        sm.eqw[tid] = bits;
     }
     double w1 = 0, w2 = 0;
     for (int w = 0; w < wave; ++w) {
-        w1 += wsum1[w];
-        w2 += wsum2[w];
+        w1 += sm.wsum1[w];
+        w2 += sm.wsum2[w];
     }
     const double e1 = w1 + x1 - l1, e2 = w2 + x2 - l2;  // exclusive prefix of this thread's run
 #pragma unroll
     for (int q = 0; q < SPT; ++q) {
         const int p = tid * SPT + q;
         if (p < SPAN) {
-            P1[p] = e1 + d1[q];
-            P2[p] = e2 + d2[q];
+            sm.P1[p] = e1 + d1[q];
+            sm.P2[p] = e2 + d2[q];
         }
     }
     if (tid == NT - 1) {
-        P1[SPAN] = w1 + x1;
-        P2[SPAN] = w2 + x2;
+        sm.P1[SPAN] = w1 + x1;
+        sm.P2[SPAN] = w2 + x2;
     }
+    return TdBlock{a, static_cast<float>(sc), static_cast<float>(1.0e-12 * energy)};
+}
 
-    // 4. this wave's 32 rows: split taps in registers (A operand: row =
-    //    lane % 32, taps 16 s + 8 half + 0..7 of each part)
-    const int rowBase = item.rowBegin + 32 * wave;
-    const int nMine = min(32, item.nRows - 32 * wave);
-    V8 ap[NP][NS];
-    {
-        const bool has = col < nMine;
-        const MT *f = has ? static_cast<const MT *>(rows[rowBase + col].frag) : nullptr;
+// A dead lane's item: zeros in every row.
+__device__ __forceinline__ void td_dead(const TdItem &item, const TdRow *__restrict__ rows)
+{
+    const long long base = item.seg0;
+    for (int r = 0; r < item.nRows; ++r) {
+        const TdRow row = rows[item.rowBegin + r];
+        const int nv = static_cast<int>(min(static_cast<long long>(IS), row.nOut - base));
+        gptr<float> out = to_global(static_cast<float *>(row.out)) + base;
+        for (int i = threadIdx.x; i < nv; i += NT) { out[i] = 0.0f; }
+    }
+}
+
+// Tiles of 16 rows x 16 outputs on v_mfma_f32_16x16x32: each wave holds the
+// split taps of 16 rows (parts * KT / 8 VGPRs: 96 for the BF16 triple at
+// KT = 256; tiles of 32 rows would need 192 and leave one wave per SIMD)
+// and walks the item's 32 output tiles; the 4 waves take 64 rows at a
+// time, in two row groups when the item has more.  A lane holds column
+// i = lane % 16 and rows 4 (lane / 16) + r, r = 0..3: their epilogue
+// constants stay in registers for the row group, and each quarter wave
+// stores 16 consecutive outputs of its row.
+constexpr int kTdTiles = IS / 16;        // output tiles of 16 per item
+constexpr int kTdFixList = 64 + 4 * 64;  // carried over plus one tile's windows
+
+// Blocks per CU the register budget is capped for: three for the FP16 pair
+// (64 VGPRs of taps at KT = 256, 51 KB of LDS per block), two for the BF16
+// triple (96 VGPRs of taps, 63 KB of LDS).
+template <typename MT>
+constexpr int td_blocks_per_cu() { return TdTraits<MT>::parts == 2 ? 3 : 2; }
+
+template <typename MT, int KT>
+__global__ __launch_bounds__(NT, (td_blocks_per_cu<MT>())) void tdcorr_kernel(const TdItem *__restrict__ items,
+                                                       const Seg *__restrict__ segs,
+                                                       const TdRow *__restrict__ rows, FixQueue queue)
+{
+    using Tr = TdTraits<MT>;
+    using V8 = typename Tr::vec8;
+    constexpr int NP = Tr::parts;
+    constexpr int NS = KT / 32;  // k-steps of 32
+    __shared__ __align__(16) TdStage<MT, KT> sm;
+    // flagged windows of each wave's tiles: bit r flags the lane's window
+    // r, bit 4 + r marks a cancelled variance
+    __shared__ unsigned char fq[NT / 64][kTdTiles][64];
+    // each wave's list of flagged windows to recompute when the device
+    // queue is full
+    __shared__ unsigned short fixList[NT / 64][kTdFixList];
+
+    const TdItem item = items[blockIdx.x];
+    const Seg sg = segs[item.seg];
+    const int tid = threadIdx.x;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int lane = tid & 63;
+    const int quad = lane >> 4;  // k-group of the operands, row group of the result
+    const int c16 = lane & 15;   // output column
+    const long long base = item.seg0;
+
+    if (sg.dead) {
+        td_dead(item, rows);
+        return;
+    }
+    const TdBlock blk = td_stage(sm, sg, base);
+    const double a = blk.a;
+    const float scF = blk.scF;
+    const float flK = blk.flK;
+    __syncthreads();  // Hankel entries, prefix sums and raw samples ready
+
+    unsigned long long nFlag = 0, nCanc = 0;  // windows recomputed by this wave, in full
+    for (int g = 0; 64 * g < item.nRows; ++g) {
+        const int rowBase = item.rowBegin + 64 * g + 16 * wave;
+        const int nMine = min(16, item.nRows - 64 * g - 16 * wave);
+        if (nMine <= 0) { break; }
+        // split taps (A operand: row = lane % 16, taps 32 s + 8 quad + 0..7
+        // of each part)
+        V8 ap[NP][NS];
+        {
+            const bool has = c16 < nMine;
+            const MT *f = has ? static_cast<const MT *>(rows[rowBase + c16].frag) : nullptr;
 #pragma unroll
-        for (int pt = 0; pt < NP; ++pt) {
+            for (int pt = 0; pt < NP; ++pt) {
 #pragma unroll
-            for (int st = 0; st < NS; ++st) {
-                if (has) {
-                    const gptr<const i32x4> fh = to_global(reinterpret_cast<const i32x4 *>(f));
-                    ap[pt][st] = __builtin_bit_cast(V8, fh[pt * 2 * NS + 2 * st + half]);
-                } else {
-                    ap[pt][st] = __builtin_bit_cast(V8, i32x4{0, 0, 0, 0});
+                for (int st = 0; st < NS; ++st) {
+                    if (has) {
+                        const gptr<const i32x4> fh = to_global(reinterpret_cast<const i32x4 *>(f));
+                        ap[pt][st] = __builtin_bit_cast(V8, fh[pt * (KT / 8) + 4 * st + quad]);
+                    } else {
+                        ap[pt][st] = __builtin_bit_cast(V8, i32x4{0, 0, 0, 0});
+                    }
                 }
             }
         }
-    }
-    __syncthreads();  // Hankel entries, prefix sums and raw samples ready
-    if (nMine <= 0) { return; }
-
-    unsigned long long nFlag = 0, nCanc = 0;  // windows recomputed by this wave, in full
-    unsigned tiles = 0;            // tiles with flagged windows (wave-uniform)
-
-    // 5. output tiles: D[j][i], lane holds column i = col and rows
-    //    j = 8 (r / 4) + 4 half + r % 4
-    for (int tile = 0; tile < NTILE_I; ++tile) {
-        const int i0 = tile * 32;
-        f32x16 acc0 = {}, acc1 = {};
-        const i32x4 *b0p = reinterpret_cast<const i32x4 *>(&x8[0][i0 + col + 8 * half][0]);
-        const i32x4 *b1p = reinterpret_cast<const i32x4 *>(&x8[1][i0 + col + 8 * half][0]);
-        const i32x4 *b2p = reinterpret_cast<const i32x4 *>(&x8[NP - 1][i0 + col + 8 * half][0]);
+        // the epilogue constants of the lane's 4 rows (absent rows: no
+        // outputs, and a length that keeps the prefix reads in the span)
+        int LtR[4], nvR[4];
+        float kcR[4], knR[4], slR[4];
+        gptr<float> outR[4];
 #pragma unroll
-        for (int st = 0; st < NS; ++st) {
-            const V8 b0 = __builtin_bit_cast(V8, b0p[16 * st]);  // entry + 16 st
-            const V8 b1 = __builtin_bit_cast(V8, b1p[16 * st]);
-            acc0 = Tr::mfma(ap[0][st], b0, acc0);
-            acc1 = Tr::mfma(ap[0][st], b1, acc1);
-            acc1 = Tr::mfma(ap[1][st], b0, acc1);
-            if constexpr (NP == 3) {
-                const V8 b2 = __builtin_bit_cast(V8, b2p[16 * st]);
-                acc1 = Tr::mfma(ap[0][st], b2, acc1);
-                acc1 = Tr::mfma(ap[NP - 1][st], b0, acc1);
-                acc1 = Tr::mfma(ap[1][st], b1, acc1);
+        for (int r = 0; r < 4; ++r) {
+            const int j = 4 * quad + r;
+            if (j < nMine) {
+                const TdRow row = rows[rowBase + j];
+                LtR[r] = row.Lt;
+                nvR[r] = static_cast<int>(max(0ll, min(static_cast<long long>(IS), row.nOut - base)));
+                kcR[r] = row.kc;
+                knR[r] = row.kn;
+                slR[r] = row.sqrtLt;
+                outR[r] = to_global(static_cast<float *>(row.out)) + base;
+            } else {
+                LtR[r] = 1;
+                nvR[r] = 0;
+                kcR[r] = 0.0f;
+                knR[r] = 0.0f;
+                slR[r] = 0.0f;
+                outR[r] = nullptr;
             }
         }
-        // epilogue: normalise, flag, store; flagged windows (bit r of
-        // flg, with bit r of cnc when the variance cancelled) are
-        // recomputed below
-        const int i = i0 + col;
-        const double p1i = P1[i], p2i = P2[i];
-        unsigned flg = 0, cnc = 0;
-        // Branch-free over the rows: the two rows of step r (one per
-        // half-wave) are wave-uniform (scalar loads); each half stores
-        // through its row's range-checked buffer, so the other half's
-        // lanes, flagged windows and outputs past the row's end (offset
-        // out of range) are dropped by the hardware.
-        // (the row loads stay inside the tile loop: hoisted out of it,
-        // 32 rows' descriptors would not fit in scalar registers)
-        int rowT = rowBase, nT = nMine;
-        asm volatile("" : "+s"(rowT), "+s"(nT));
+        unsigned tiles = 0;  // tiles with flagged windows (wave-uniform)
+        for (int tile = 0; tile < kTdTiles; ++tile) {
+            const int i0 = 16 * tile;
+            f32x4 acc0 = {}, acc1 = {};
+            // B operand: column i0 + c16, taps 32 s + 8 quad + 0..7 = entry
+            // i0 + c16 + 8 quad + 32 s
+            const i32x4 *b0p = reinterpret_cast<const i32x4 *>(&sm.x8[0][i0 + c16 + 8 * quad][0]);
+            const i32x4 *b1p = reinterpret_cast<const i32x4 *>(&sm.x8[1][i0 + c16 + 8 * quad][0]);
+            const i32x4 *b2p = reinterpret_cast<const i32x4 *>(&sm.x8[NP - 1][i0 + c16 + 8 * quad][0]);
 #pragma unroll
-        for (int r = 0; r < 16; ++r) {
-            const int jA = 8 * (r >> 2) + (r & 3), jB = jA + 4;
-            const TdRow rA = rows[rowT + min(jA, nT - 1)];
-            const TdRow rB = rows[rowT + min(jB, nT - 1)];
-            const int nvA = jA < nT ? static_cast<int>(max(0ll, min(static_cast<long long>(IS), rA.nOut - base))) : 0;
-            const int nvB = jB < nT ? static_cast<int>(max(0ll, min(static_cast<long long>(IS), rB.nOut - base))) : 0;
-            const int Lt = half ? rB.Lt : rA.Lt;
-            const float kc = half ? rB.kc : rA.kc;
-            const float kn = half ? rB.kn : rA.kn;
-            const float sl = half ? rB.sqrtLt : rA.sqrtLt;
-            const int nv = half ? nvB : nvA;
-            // window sums in FP64 (differences of the block's prefix sums),
-            // the rest in FP32
-            const double S1 = P1[i + Lt] - p1i;
-            const double S2 = P2[i + Lt] - p2i;
-            const float d2 = static_cast<float>(fma(-S1, S1, Lt * S2));
-            const float s2 = static_cast<float>(S2);
-            const bool canc = !(d2 > fmaf(kc, s2, flK * static_cast<float>(Lt)));
-            const bool flag = canc || kn * s2 > d2;
-            float y = __builtin_amdgcn_rsqf(d2);
-            y = y * (1.5f - 0.5f * d2 * y * y);
-            const unsigned rho = __float_as_uint((acc0[r] + acc1[r]) * (scF * sl * y));
-            constexpr int kDrop = 0x7ffffff0;  // beyond every row's range
-            const int off = flag ? kDrop : 4 * i;
-            __builtin_amdgcn_raw_buffer_store_b32(rho, buffer_rsrc(static_cast<float *>(rA.out) + base, 4 * nvA),
-                                                  half ? kDrop : off, 0, 0);
-            __builtin_amdgcn_raw_buffer_store_b32(rho, buffer_rsrc(static_cast<float *>(rB.out) + base, 4 * nvB),
-                                                  half ? off : kDrop, 0, 0);
-            const bool valid = i < nv;
-            flg |= (valid && flag ? 1u : 0u) << r;
-            cnc |= (valid && canc ? 1u : 0u) << r;
-            // two steps' loads in flight at a time: the scheduler may not
-            // hoist more (their registers would not fit beside the taps)
-            if ((r & 1) == 1) { __builtin_amdgcn_sched_barrier(0); }
-        }
-        const unsigned long long any = __ballot(flg != 0u);
-        if (any != 0ull) {
-            fq[wave][tile][lane] = flg | (cnc << 16);
-            tiles |= 1u << tile;
-        }
-    }
-    // 6. flagged windows, after the tile loop (the taps are dead here).
-    //    Exactly constant windows (a cancelled variance over equal
-    //    samples) are 0 at once; the others go to the device queue, whose
-    //    windows fix_kernel recomputes after the launch.  When the queue
-    //    is full they go to the wave's list, which is recomputed here one
-    //    window per lane in passes of 64 (a few left at the end: the whole
-    //    wave on one window at a time).  List entry: tile
-    //    (4 bits), step r (4 bits), lane (6 bits), cancelled (1 bit); in
-    //    tile, step, lane order.
-    unsigned short *list = fixList[wave];
-    int nList = 0;  // wave-uniform
-    auto entryWindow = [&](unsigned e, int &wi, TdRow &row, bool &cr) {
-        const int t = e >> 10, r = (e >> 6) & 15, l = e & 63;
-        wi = 32 * t + (l & 31);
-        row = rows[rowBase + 8 * (r >> 2) + 4 * (l >> 5) + (r & 3)];
-        cr = (e >> 14) & 1u;
-    };
-    auto runList = [&](bool all) {
-        int done = 0;
-        while (nList - done >= 64 || (all && nList - done > kTdCoopFlags)) {
-            const int n = min(64, nList - done);
-            if (lane < n) {
-                int wi;
-                TdRow row;
-                bool cr;
-                entryWindow(list[done + lane], wi, row, cr);
-                const double rho = exact_window_td(raw, P1, P2, wi, to_global(row.tmpl), row.Lt, a, cr, 0, 1);
-                to_global(static_cast<float *>(row.out))[base + wi] = static_cast<float>(rho);
+            for (int st = 0; st < NS; ++st) {
+                const V8 b0 = __builtin_bit_cast(V8, b0p[32 * st]);
+                const V8 b1 = __builtin_bit_cast(V8, b1p[32 * st]);
+                acc0 = Tr::mfma(ap[0][st], b0, acc0);
+                acc1 = Tr::mfma(ap[0][st], b1, acc1);
+                acc1 = Tr::mfma(ap[1][st], b0, acc1);
+                if constexpr (NP == 3) {
+                    const V8 b2 = __builtin_bit_cast(V8, b2p[32 * st]);
+                    acc1 = Tr::mfma(ap[0][st], b2, acc1);
+                    acc1 = Tr::mfma(ap[NP - 1][st], b0, acc1);
+                    acc1 = Tr::mfma(ap[1][st], b1, acc1);
+                }
             }
-            done += n;
-        }
-        if (all) {
-            for (; done < nList; ++done) {
-                int wi;
-                TdRow row;
-                bool cr;
-                entryWindow(list[done], wi, row, cr);  // uniform entry: the wave shares it
-                const double rho = exact_window_td(raw, P1, P2, wi, to_global(row.tmpl), row.Lt, a, cr, lane, 64);
-                if (lane == 0) { to_global(static_cast<float *>(row.out))[base + wi] = static_cast<float>(rho); }
-            }
-        } else if (done > 0) {
-            // the remainder (under 64 entries) moves to the front: every
-            // lane reads before any lane writes (one instruction each)
-            const int rest = nList - done;
-            const unsigned short v = lane < rest ? list[done + lane] : 0;
-            if (lane < rest) { list[lane] = v; }
-        }
-        nList -= done;
-    };
-    while (tiles != 0u) {
-        const int tile = __ffs(tiles) - 1;
-        tiles &= tiles - 1u;
-        const int i = tile * 32 + col;
-        const unsigned word = fq[wave][tile][lane];
-        const unsigned flg = word & 0xffffu;
-        const unsigned cnc = word >> 16;
-        int total = __popc(flg), full = __popc(flg & cnc);
+            // epilogue: normalise, flag, store
+            const int i = i0 + c16;
+            const double p1i = sm.P1[i], p2i = sm.P2[i];
+            unsigned bits = 0;
 #pragma unroll
-        for (int o = 32; o > 0; o >>= 1) {
-            total += __shfl_xor(total, o, 64);
-            full += __shfl_xor(full, o, 64);
-        }
-        nFlag += static_cast<unsigned long long>(__builtin_amdgcn_readfirstlane(total));
-        nCanc += static_cast<unsigned long long>(__builtin_amdgcn_readfirstlane(full));
-        // constant windows now; the others counted for the list
-        unsigned keep = flg;
-        for (unsigned m = flg & cnc; m != 0u; m &= m - 1u) {
-            const int r = __ffs(m) - 1;
-            const TdRow row = rows[rowBase + 8 * (r >> 2) + 4 * half + (r & 3)];
-            if (constant_window(eqw, i, row.Lt)) {
-                to_global(static_cast<float *>(row.out))[base + i] = 0.0f;
-                keep &= ~(1u << r);
+            for (int r = 0; r < 4; ++r) {
+                const int Lt = LtR[r];
+                // window sums in FP64 (differences of the block's prefix
+                // sums), the rest in FP32
+                const double S1 = sm.P1[i + Lt] - p1i;
+                const double S2 = sm.P2[i + Lt] - p2i;
+                const float d2 = static_cast<float>(fma(-S1, S1, Lt * S2));
+                const float s2 = static_cast<float>(S2);
+                const bool canc = !(d2 > fmaf(kcR[r], s2, flK * static_cast<float>(Lt)));
+                const bool flag = canc || knR[r] * s2 > d2;
+                float y = __builtin_amdgcn_rsqf(d2);
+                y = y * (1.5f - 0.5f * d2 * y * y);
+                const float rho = (acc0[r] + acc1[r]) * (scF * slR[r] * y);
+                const bool valid = i < nvR[r];
+                if (valid && !flag) { outR[r][i] = rho; }
+                bits |= (valid && flag ? 1u : 0u) << r;
+                bits |= (valid && canc ? 1u : 0u) << (4 + r);
+            }
+            if (__ballot(bits != 0u) != 0ull) {
+                fq[wave][tile][lane] = static_cast<unsigned char>(bits);
+                tiles |= 1u << tile;
             }
         }
-        // to the device queue when it has room for the tile's windows
-        // (step by step, lanes in order)
-        int n = 0;
-        for (int r = 0; r < 16; ++r) { n += __popcll(__ballot((keep >> r) & 1u)); }
-        if (n == 0) { continue; }
-        unsigned long long q0 = 0;
-        if (lane == 0) { q0 = atomicAdd(queue.count, static_cast<unsigned long long>(n)); }
-        q0 = __shfl(q0, 0, 64);
-        if (q0 <= queue.cap && static_cast<unsigned long long>(n) <= queue.cap - q0) {
-            for (int r = 0; r < 16; ++r) {
+        // flagged windows of the row group, after its tile loop: exactly
+        // constant windows are 0 at once, the others go to the device
+        // queue (fix_kernel recomputes them after the launch), or, when it
+        // is full, to the wave's list, recomputed here one window per lane
+        // in passes of 64 (a few left at the end: the whole wave on one
+        // window at a time).  List entry: tile (5 bits), step r (2 bits),
+        // lane (6 bits), cancelled (1 bit).
+        unsigned short *list = fixList[wave];
+        int nList = 0;  // wave-uniform
+        auto entryWindow = [&](unsigned e, int &wi, TdRow &row, bool &cr) {
+            const int t = (e >> 8) & 31, r = (e >> 6) & 3, l = e & 63;
+            wi = 16 * t + (l & 15);
+            row = rows[rowBase + 4 * (l >> 4) + r];
+            cr = (e >> 13) & 1u;
+        };
+        auto runList = [&](bool all) {
+            int done = 0;
+            while (nList - done >= 64 || (all && nList - done > kTdCoopFlags)) {
+                const int n = min(64, nList - done);
+                if (lane < n) {
+                    int wi;
+                    TdRow row;
+                    bool cr;
+                    entryWindow(list[done + lane], wi, row, cr);
+                    const double rho =
+                        exact_window_td(sm.raw, sm.P1, sm.P2, wi, to_global(row.tmpl), row.Lt, a, cr, 0, 1);
+                    to_global(static_cast<float *>(row.out))[base + wi] = static_cast<float>(rho);
+                }
+                done += n;
+            }
+            if (all) {
+                for (; done < nList; ++done) {
+                    int wi;
+                    TdRow row;
+                    bool cr;
+                    entryWindow(list[done], wi, row, cr);  // uniform entry: the wave shares it
+                    const double rho =
+                        exact_window_td(sm.raw, sm.P1, sm.P2, wi, to_global(row.tmpl), row.Lt, a, cr, lane, 64);
+                    if (lane == 0) { to_global(static_cast<float *>(row.out))[base + wi] = static_cast<float>(rho); }
+                }
+            } else if (done > 0) {
+                // the remainder (under 64 entries) moves to the front: every
+                // lane reads before any lane writes (one instruction each)
+                const int rest = nList - done;
+                const unsigned short v = lane < rest ? list[done + lane] : 0;
+                if (lane < rest) { list[lane] = v; }
+            }
+            nList -= done;
+        };
+        while (tiles != 0u) {
+            const int tile = __ffs(tiles) - 1;
+            tiles &= tiles - 1u;
+            const int i = 16 * tile + c16;
+            const unsigned word = fq[wave][tile][lane];
+            const unsigned flg = word & 15u;
+            const unsigned cnc = word >> 4;
+            int total = __popc(flg), full = __popc(flg & cnc);
+#pragma unroll
+            for (int o = 32; o > 0; o >>= 1) {
+                total += __shfl_xor(total, o, 64);
+                full += __shfl_xor(full, o, 64);
+            }
+            nFlag += static_cast<unsigned long long>(__builtin_amdgcn_readfirstlane(total));
+            nCanc += static_cast<unsigned long long>(__builtin_amdgcn_readfirstlane(full));
+            unsigned keep = flg;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                if (((flg & cnc) >> r) & 1u) {
+                    if (constant_window(sm.eqw, i, LtR[r])) {
+                        outR[r][i] = 0.0f;
+                        keep &= ~(1u << r);
+                    }
+                }
+            }
+            int n = 0;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) { n += __popcll(__ballot((keep >> r) & 1u)); }
+            if (n == 0) { continue; }
+            unsigned long long q0 = 0;
+            if (lane == 0) { q0 = atomicAdd(queue.count, static_cast<unsigned long long>(n)); }
+            q0 = __shfl(q0, 0, 64);
+            if (q0 <= queue.cap && static_cast<unsigned long long>(n) <= queue.cap - q0) {
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const bool mine = (keep >> r) & 1u;
+                    const unsigned long long bal = __ballot(mine);
+                    if (mine) {
+                        const unsigned long long at = q0 + __popcll(bal & ((1ull << lane) - 1ull));
+                        const unsigned rowCode =
+                            static_cast<unsigned>(rowBase + 4 * quad + r) | (((cnc >> r) & 1u) << 31);
+                        gptr<unsigned long long> e = to_global(reinterpret_cast<unsigned long long *>(queue.entries));
+                        e[at] = (static_cast<unsigned long long>(base + i) << 32) | rowCode;
+                    }
+                    q0 += __popcll(bal);
+                }
+                continue;
+            }
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
                 const bool mine = (keep >> r) & 1u;
                 const unsigned long long bal = __ballot(mine);
                 if (mine) {
-                    const unsigned long long at = q0 + __popcll(bal & ((1ull << lane) - 1ull));
-                    const int j = 8 * (r >> 2) + 4 * half + (r & 3);
-                    const unsigned rowCode = static_cast<unsigned>(rowBase + j) | (((cnc >> r) & 1u) << 31);
-                    gptr<unsigned long long> e = to_global(reinterpret_cast<unsigned long long *>(queue.entries));
-                    e[at] = (static_cast<unsigned long long>(base + i) << 32) | rowCode;
+                    const int at = nList + __popcll(bal & ((1ull << lane) - 1ull));
+                    list[at] = static_cast<unsigned short>((tile << 8) | (r << 6) | lane | (((cnc >> r) & 1u) << 13));
                 }
-                q0 += __popcll(bal);
+                nList += __popcll(bal);
             }
-            continue;
+            runList(false);
         }
-        // queue full: the wave's own list, step by step (lanes in order):
-        // the windows of a pass then share one or two template rows, whose
-        // loads are broadcasts
-        for (int r = 0; r < 16; ++r) {
-            const bool mine = (keep >> r) & 1u;
-            const unsigned long long bal = __ballot(mine);
-            if (bal == 0ull) { continue; }
-            if (mine) {
-                const int at = nList + __popcll(bal & ((1ull << lane) - 1ull));
-                list[at] = static_cast<unsigned short>((tile << 10) | (r << 6) | lane | (((cnc >> r) & 1u) << 14));
-            }
-            nList += __popcll(bal);
-        }
-        runList(false);
+        runList(true);
     }
-    runList(true);
     if (sg.flagged != nullptr && nFlag != 0ull && lane == 0) {
         atomicAdd(sg.flagged, nFlag);
         if (nCanc != 0ull) { atomicAdd(sg.flagged + 1, nCanc); }

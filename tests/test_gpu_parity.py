@@ -437,3 +437,30 @@ def test_time_domain_mfma_path_matches_oracle(variant, tol):
             continue
         ref = O.pearson(t, x0)[:nout]
         assert np.abs(row.astype(np.float64) - ref).max() < tol, (r, len(t))
+
+
+@pytest.mark.parametrize("variant", [3, 4])
+def test_time_domain_many_rows_per_lane_match_oracle(variant):
+    """150 short templates of 20-256 taps on one channel: work items of 128
+    and 22 rows, so every wave of a block holds a partial or empty row set
+    and the 16-row tiles take two row groups; every row within the FP32
+    contract of the FP64 oracle."""
+    rng = np.random.default_rng(740 + variant)
+    n = 6_000
+    x = (rng.standard_normal(n) * 2 + 300.0).astype(np.float32)
+    lens = [int(v) for v in rng.integers(20, 257, size=150)]
+    tpls = [rng.standard_normal(L) * np.hanning(L) for L in lens]
+    h = capi.Handle(np.float32, "batched")
+    h.set_templates(tpls, shifts=[0] * len(tpls), weights=[1.0] * len(tpls))
+    h.set_batch_layout([n], [0] * len(tpls), [0] * len(tpls), 1)
+    h.set_kernel_variant(variant)
+    h.initialize()
+    h.set_signal(0, x)
+    h.apply()
+    nout = n - max(lens) + 1
+    for r, t in enumerate(tpls):
+        assert h.plan(r)[0] == 0
+        row = h.rho(r)
+        assert row.size == nout
+        ref = O.pearson(t, x)[:nout]
+        assert np.abs(row.astype(np.float64) - ref).max() < 1e-5, (r, len(t))
