@@ -306,6 +306,12 @@ __global__ __launch_bounds__(NT, (td_blocks_per_cu<MT>())) void tdcorr_kernel(co
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int lane = tid & 63;
     const int quad = lane >> 4;  // k-group of the operands, row group of the result
+    // The taps of k-group quad: 8 kq + 0..7 of each k-step of 32, with kq =
+    // 0, 2, 1, 3 for quad = 0..3 (A and B alike, so the products summed are
+    // the same): each half wave then reads 32 consecutive Hankel entries
+    // (quads 0 and 1: i0 + 0..31; 2 and 3: i0 + 8..39) instead of two
+    // overlapping runs of 16 in the same LDS banks.
+    const int kq = ((quad & 1) << 1) | (quad >> 1);
     const int c16 = lane & 15;   // output column
     const long long base = item.seg0;
 
@@ -324,7 +330,7 @@ __global__ __launch_bounds__(NT, (td_blocks_per_cu<MT>())) void tdcorr_kernel(co
         const int rowBase = item.rowBegin + 64 * g + 16 * wave;
         const int nMine = min(16, item.nRows - 64 * g - 16 * wave);
         if (nMine <= 0) { break; }
-        // split taps (A operand: row = lane % 16, taps 32 s + 8 quad + 0..7
+        // split taps (A operand: row = lane % 16, taps 32 s + 8 kq + 0..7
         // of each part)
         V8 ap[NP][NS];
         {
@@ -336,7 +342,7 @@ __global__ __launch_bounds__(NT, (td_blocks_per_cu<MT>())) void tdcorr_kernel(co
                 for (int st = 0; st < NS; ++st) {
                     if (has) {
                         const gptr<const i32x4> fh = to_global(reinterpret_cast<const i32x4 *>(f));
-                        ap[pt][st] = __builtin_bit_cast(V8, fh[pt * (KT / 8) + 4 * st + quad]);
+                        ap[pt][st] = __builtin_bit_cast(V8, fh[pt * (KT / 8) + 4 * st + kq]);
                     } else {
                         ap[pt][st] = __builtin_bit_cast(V8, i32x4{0, 0, 0, 0});
                     }
@@ -372,11 +378,11 @@ __global__ __launch_bounds__(NT, (td_blocks_per_cu<MT>())) void tdcorr_kernel(co
         for (int tile = 0; tile < kTdTiles; ++tile) {
             const int i0 = 16 * tile;
             f32x4 acc0 = {}, acc1 = {};
-            // B operand: column i0 + c16, taps 32 s + 8 quad + 0..7 = entry
-            // i0 + c16 + 8 quad + 32 s
-            const i32x4 *b0p = reinterpret_cast<const i32x4 *>(&sm.x8[0][i0 + c16 + 8 * quad][0]);
-            const i32x4 *b1p = reinterpret_cast<const i32x4 *>(&sm.x8[1][i0 + c16 + 8 * quad][0]);
-            const i32x4 *b2p = reinterpret_cast<const i32x4 *>(&sm.x8[NP - 1][i0 + c16 + 8 * quad][0]);
+            // B operand: column i0 + c16, taps 32 s + 8 kq + 0..7 = entry
+            // i0 + c16 + 8 kq + 32 s
+            const i32x4 *b0p = reinterpret_cast<const i32x4 *>(&sm.x8[0][i0 + c16 + 8 * kq][0]);
+            const i32x4 *b1p = reinterpret_cast<const i32x4 *>(&sm.x8[1][i0 + c16 + 8 * kq][0]);
+            const i32x4 *b2p = reinterpret_cast<const i32x4 *>(&sm.x8[NP - 1][i0 + c16 + 8 * kq][0]);
 #pragma unroll
             for (int st = 0; st < NS; ++st) {
                 const V8 b0 = __builtin_bit_cast(V8, b0p[32 * st]);
