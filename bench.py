@@ -783,6 +783,39 @@ def cpu_baseline(wl, threads: int, accuracy_rows, gpu_rho_rows, sample_samples: 
     }, {"max_abs_drho_vs_fp64": worst, "audited_samples": n_aud}
 
 
+def hbm_probe() -> int:
+    """Plain device streams of 1 and 4 GiB, one JSON line each: a copy (read +
+    write), a sum (read only) and the maximum over 256 rows (read, 1/256
+    written; the access pattern of the Detector reduce and of the stack).
+    Each is warmed up once and timed over 5 calls with HIP events."""
+    import torch
+
+    def timed(f, reps=5):
+        f()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            f()
+        e1.record()
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / reps
+
+    for gib in (1, 4):
+        n = gib * (1 << 30) // 4
+        a = torch.ones(n, dtype=torch.float32, device="cuda:0")
+        b = torch.empty_like(a)
+        rows = a.view(256, -1)
+        for name, f, nbytes in (("copy", lambda: b.copy_(a), 8 * n),
+                                ("sum", lambda: a.sum(), 4 * n),
+                                ("max over 256 rows", lambda: torch.amax(rows, dim=0), 4 * n + 4 * n // 256)):
+            ms = timed(f)
+            print(json.dumps({"hbm_probe": name, "GiB": gib, "ms": round(ms, 4),
+                              "TB_per_s": round(nbytes / (ms * 1e-3) / 1e12, 3)}), flush=True)
+        del a, b, rows
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
@@ -809,6 +842,9 @@ def main() -> int:
     ap.add_argument("--serve-batch", type=int, default=256)
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="only copy BYTES device-to-device 3 times (calibrates rocprof HBM counters)")
+    ap.add_argument("--hbm-probe", action="store_true",
+                    help="only time plain device streams of 1 and 4 GiB (copy, sum, max over 256 rows): "
+                         "the practical HBM rates the stack and reduce kernels compare with")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: runs on one GPU only (asked for --gpus {args.gpus}); sharding the path across GPUs "
@@ -829,6 +865,9 @@ def main() -> int:
         print(json.dumps({"copy_probe_bytes": a.numel() * 4, "copies": 3, "ms_per_copy": ms,
                           "copy_GBps_read_plus_write": 2 * a.numel() * 4 / (ms * 1e-3) / 1e9}))
         return 0
+
+    if args.hbm_probe:
+        return hbm_probe()
 
     import torch  # before the engine library: one HIP runtime per process
     from mflib_amd import capi, synth
