@@ -369,7 +369,8 @@ int mf_autotune(int precision, double *table, int cap, int *n);
 
 /* No reference counterpart: per-apply kernel timings (ms) measured with
  * HIP events on the handle's stream: correlation kernels and the stack
- * kernel of the last mf_shift_and_stack*. */
+ * kernel of the last mf_shift_and_stack* (for mf_detect, the one launch
+ * that stacks and finds the candidates). */
 int mf_get_timings(mf_handle *h, double *corrMs, double *stackMs);
 
 /* No reference counterpart: window statistics of the correlation launches

@@ -1040,7 +1040,7 @@ void peaksDevice(const void *dx, long long n, double threshold, int minDist, int
     pruneCandidates<T>(cand, minDist, idx, cap, nPeaks);
 }
 
-// Stack every group and find the peaks of every stack (one launch each),
+// Stack every group and find the candidates of every stack (one launch),
 // then prune per group on the host.
 template <typename T>
 void detectT(mf_handle *h, double threshold, int minDist, int *idx, int cap, int *offsets)
@@ -1049,9 +1049,11 @@ void detectT(mf_handle *h, double threshold, int minDist, int *idx, int cap, int
     long long maxLen = 0;
     for (const auto &gi : h->stackInfo) { maxLen = std::max(maxLen, gi.len); }
     MF_HIP(hipMemsetAsync(h->dCounts.p, 0, nG * sizeof(int), h->stream));
-    stackGroups(h, -1);
-    MF_HIP(mfk::launch_peak_groups(h->precision, h->dGroupTab.as<mfk::StackGroup>(), nG, maxLen, threshold,
-                                   h->stream));
+    // stacks and their candidates in one pass (the stacks are not read back)
+    MF_HIP(hipEventRecord(h->evC, h->stream));
+    MF_HIP(mfk::launch_stack_peak_groups(h->precision, h->dGroupTab.as<mfk::StackGroup>(), nG, maxLen, h->absval,
+                                         threshold, h->stream));
+    MF_HIP(hipEventRecord(h->evD, h->stream));
     MF_HIP(mfk::launch_pack_candidates(h->precision, h->dGroupTab.as<mfk::StackGroup>(), nG, h->dPackIdx.as<int>(),
                                        h->dPackVal.p, h->stream));
     // one readback of the counts and the first kPrefix candidates (usually

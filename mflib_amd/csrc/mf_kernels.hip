@@ -1442,6 +1442,104 @@ inline int peak_grid(long long maxLen, int tsize, int cap)
     return static_cast<int>(want < 1 ? 1 : (want < cap ? want : cap));
 }
 
+// Stack and candidates of every group in one launch (the detect path).  As
+// stack_kernel, each block owns kStackChunk outputs (one pass of U = 8 per
+// thread) and writes them; the block's sums also go to LDS, where each
+// output above the threshold finds its neighbours.  An output at the
+// block's edge recomputes its outside neighbour from the rows: the same
+// sum, in the same order, as the block that owns it.  Candidates as in
+// peak_scan (the rule of src/peakFinder.cpp:500-521, one atomic per wave).
+template <typename T>
+__global__ __launch_bounds__(NT) void stack_peak_kernel(const StackGroup *__restrict__ groups, int absval, T thr)
+{
+#pragma clang fp contract(off)
+    constexpr int U = 8;
+    static_assert(kStackChunk == NT * U, "one pass per block");
+    __shared__ T sv[kStackChunk];
+    const StackGroup grp = groups[blockIdx.y];
+    const StackRow *__restrict__ srows = grp.rows;
+    const int nRows = grp.nRows;
+    const long long len = grp.len;
+    const long long chunk0 = static_cast<long long>(blockIdx.x) * kStackChunk;
+    if (chunk0 >= len) { return; }  // uniform per block
+    const long long k0 = chunk0 + threadIdx.x;
+    T acc[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) { acc[u] = T(0); }
+#pragma unroll 8
+    for (int r = 0; r < nRows; ++r) {
+        const StackRow sr = srows[r];
+        const gptr<const T> p = to_global(static_cast<const T *>(sr.rho)) + sr.shift;
+        const T w = static_cast<T>(sr.weight);
+        T v[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const long long k = k0 + u * NT;
+            v[u] = k < len ? p[k] : T(0);
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const T a = absval ? fabs(v[u]) : v[u];
+            acc[u] = acc[u] + w * a;
+        }
+    }
+    gptr<T> o = to_global(static_cast<T *>(grp.out));
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        const long long k = k0 + u * NT;
+        if (k < len) { o[k] = acc[u]; }
+        sv[u * NT + threadIdx.x] = acc[u];
+    }
+    __syncthreads();
+    // the stack at an output outside the block (0 <= k < len)
+    const auto stackAt = [&](long long k) {
+        T sum = T(0);
+        for (int r = 0; r < nRows; ++r) {
+            const StackRow sr = srows[r];
+            const T x = to_global(static_cast<const T *>(sr.rho))[sr.shift + k];
+            sum = sum + static_cast<T>(sr.weight) * (absval ? fabs(x) : x);
+        }
+        return sum;
+    };
+    unsigned bits = 0u;
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        const int j = u * NT + threadIdx.x;
+        const long long k = chunk0 + j;
+        const T xi = acc[u];
+        if (k < len && xi > thr) {
+            const T left = j > 0 ? sv[j - 1] : (k > 0 ? stackAt(k - 1) : T(0));
+            const T right = k + 1 < len ? (j + 1 < kStackChunk ? sv[j + 1] : stackAt(k + 1)) : T(0);
+            if (peak_rule(xi, left, right, k, len, thr)) { bits |= 1u << u; }
+        }
+    }
+    const int lane = threadIdx.x & 63;
+    if (__ballot(bits != 0u) != 0ull) {
+        const int c = __popc(bits);
+        int incl = c;
+#pragma unroll
+        for (int o2 = 1; o2 < 64; o2 <<= 1) {
+            const int t = __shfl_up(incl, o2, 64);
+            if (lane >= o2) { incl += t; }
+        }
+        int basePos = 0;
+        if (lane == 63) { basePos = atomicAdd(grp.count, incl); }
+        int pos = __shfl(basePos, 63, 64) + incl - c;
+        const gptr<int> idx = to_global(grp.candIdx);
+        const gptr<T> val = to_global(static_cast<T *>(grp.candVal));
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            if ((bits >> u) & 1u) {
+                if (pos < grp.candCap) {
+                    idx[pos] = static_cast<int>(chunk0 + u * NT + threadIdx.x);
+                    val[pos] = acc[u];
+                }
+                ++pos;
+            }
+        }
+    }
+}
+
 // Streaming push of every channel, in 32-bit words, two launches: first
 // each channel's last `carry` words (the buffer's tail) go to its slot of
 // the scratch area, then each buffer is rewritten as carry + new chunk.
@@ -1892,6 +1990,23 @@ hipError_t launch_gather_int(const int *dSrc, const int *dIdx, int n, int *dOut,
 {
     if (n <= 0) { return hipSuccess; }
     hipLaunchKernelGGL(gather_int_kernel, dim3((n + NT - 1) / NT), dim3(NT), 0, stream, dSrc, dIdx, n, dOut);
+    return hipGetLastError();
+}
+
+hipError_t launch_stack_peak_groups(int precision, const StackGroup *dGroups, int nGroups, long long maxLen,
+                                    int absval, double threshold, hipStream_t stream)
+{
+    if (maxLen <= 0 || nGroups <= 0) { return hipSuccess; }
+    const long long chunks = (maxLen + kStackChunk - 1) / kStackChunk;
+    if (chunks > 0x7fffffffLL || nGroups > 65535) { return hipErrorInvalidValue; }
+    const int grid = static_cast<int>(chunks);
+    if (precision == 0) {
+        hipLaunchKernelGGL(stack_peak_kernel<float>, dim3(grid, nGroups), dim3(NT), 0, stream, dGroups, absval,
+                           static_cast<float>(threshold));
+    } else {
+        hipLaunchKernelGGL(stack_peak_kernel<double>, dim3(grid, nGroups), dim3(NT), 0, stream, dGroups, absval,
+                           threshold);
+    }
     return hipGetLastError();
 }
 

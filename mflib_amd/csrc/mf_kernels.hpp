@@ -190,10 +190,15 @@ hipError_t launch_corr(int precision, int logM, int variant, const Item *dItems,
 hipError_t launch_tmpl(int precision, int logM, const TmplJob *dJobs, int nJobs,
                        const void *dTwLo64, const void *dTwHi64, hipStream_t stream);
 // Shift and stack of nGroups groups (grid.y = group); maxLen bounds the
-// grid.  With peaks != 0 the same launch sequence also finds the
-// threshold/local-maximum candidates of every stack.
+// grid.
 hipError_t launch_stack_groups(int precision, const StackGroup *dGroups, int nGroups, long long maxLen,
                                int absval, hipStream_t stream);
+// The same launch also finding the threshold/local-maximum candidates of
+// every stack (into each group's count/candIdx/candVal, counts cleared by
+// the caller): the stacks are written and tested in one pass.
+hipError_t launch_stack_peak_groups(int precision, const StackGroup *dGroups, int nGroups, long long maxLen,
+                                    int absval, double threshold, hipStream_t stream);
+// Candidates of every group's stack (group.out), read back from memory.
 hipError_t launch_peak_groups(int precision, const StackGroup *dGroups, int nGroups, long long maxLen,
                               double threshold, hipStream_t stream);
 // The same over template rows as the Detector's per-template policy sees

@@ -80,3 +80,15 @@ def read_elements(ptr: int, dtype, indices) -> np.ndarray:
         if rc != 0:
             raise RuntimeError(f"hipMemcpy failed ({rc})")
     return out
+
+
+def read_array(ptr: int, dtype, n: int) -> np.ndarray:
+    """Copies the first n elements of a device array of `dtype` back to the
+    host (one copy)."""
+    dt = np.dtype(dtype)
+    out = np.empty(int(n), dtype=dt)
+    if n > 0:
+        rc = hip().hipMemcpy(out.ctypes.data, int(ptr), out.nbytes, 2)
+        if rc != 0:
+            raise RuntimeError(f"hipMemcpy failed ({rc})")
+    return out

@@ -10,6 +10,7 @@ import pytest
 
 import oracle as O
 import refcases as R
+from hipmem import read_array
 from mflib_amd import capi
 
 pytestmark = pytest.mark.gpu
@@ -245,6 +246,9 @@ def test_batched_layout_detect_matches_oracle():
         members = [r for r in range(len(tpls)) if groups[r] == g]
         st_ref = O.shift_and_stack(np.stack([rows[r] for r in members]), [shifts[r] for r in members],
                                    [weights[r] for r in members])
+        # the stack mf_detect wrote in the same launch as its candidates
+        ptr, slen = h.stack_device(g)
+        assert np.array_equal(read_array(ptr, np.float32, slen), st_ref)
         st = h.shift_and_stack(g)
         assert np.array_equal(st, st_ref)
         assert np.array_equal(peaks[g], O.peak_finder(st_ref, 0.3, 25))
