@@ -675,29 +675,69 @@ inline std::vector<Bucket> planBuckets(int precision, int userNfft, const std::v
     return bk;
 }
 
-// Work items of a set of lanes sharing one FFT size, in XCD-aware order:
-// lanes are binned to the 8 XCDs (largest first, least-loaded bin), and
-// the bins are interleaved so that workgroup i, which runs on XCD i mod 8,
-// takes its blocks from one bin.  All blocks of a lane then share one L2.
-// rowsAt(i, k), when given, is the number of leading rows of lane
-// laneIds[i] with outputs in its block k (the item's nRows); otherwise -1
-// (every row).
+// Work items of a set of lanes sharing one FFT size, in XCD-aware order.
+// Workgroup i runs on XCD i mod 8, so the items are dealt from 8 bins in
+// turn and all items of a bin share one L2 (its lanes' template spectra
+// stay there).  Lanes (or pieces of them) go to the least-loaded bin,
+// largest first:
+//  - a launch of many lanes (at least 4 per XCD) balances the bins by
+//    block count, so they hold equal numbers of items and the dealing keeps
+//    every item on its bin's XCD;
+//  - a launch of few lanes balances them by work, a block weighing
+//    kBlockRows row units for its own part (load, statistics, forward
+//    transform) plus one per row it computes (rowsAt).
+// A lane of more than half (by blocks) or a quarter (by work) of a bin's
+// share is cut into consecutive block ranges of at most that size first,
+// so a launch of few lanes still spreads over every XCD (the pieces of one
+// lane then read its spectra into more than one L2).  rowsAt(i, k), when
+// given, is the number of leading rows of lane laneIds[i] with outputs in
+// its block k (the item's nRows); otherwise -1 (every row).
 inline void appendXcdItems(const std::vector<int> &laneIds, const std::vector<long long> &laneBlocks,
                            std::vector<mfk::Item> &items,
                            const std::function<int(int, long long)> &rowsAt = nullptr)
 {
     constexpr int kXcd = 8;
-    std::vector<int> order(laneIds.size());
-    for (size_t i = 0; i < order.size(); ++i) { order[i] = static_cast<int>(i); }
-    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return laneBlocks[a] > laneBlocks[b]; });
-    std::vector<std::vector<mfk::Item>> bins(kXcd);
-    std::vector<long long> load(kXcd, 0);
-    for (int i : order) {
-        const int b = static_cast<int>(std::min_element(load.begin(), load.end()) - load.begin());
+    constexpr double kBlockRows = 6.0;  // measured per-block part over per-row part of a block: about 6
+    struct Piece {
+        int lane;
+        long long k0, k1;
+        double w;
+    };
+    const int nL = static_cast<int>(laneIds.size());
+    const bool byWork = rowsAt != nullptr && nL < 4 * kXcd;
+    std::vector<std::vector<int>> rows(nL);
+    double total = 0;
+    for (int i = 0; i < nL; ++i) {
+        rows[i].resize(static_cast<size_t>(laneBlocks[i]));
         for (long long k = 0; k < laneBlocks[i]; ++k) {
-            bins[b].push_back({laneIds[i], static_cast<int>(k), rowsAt ? rowsAt(i, k) : -1, 0});
+            rows[i][k] = rowsAt ? rowsAt(i, k) : -1;
+            total += byWork ? kBlockRows + rows[i][k] : 1.0;
         }
-        load[b] += laneBlocks[i];
+    }
+    const double cap = std::max(1.0, total / (byWork ? 4 * kXcd : 2 * kXcd));
+    std::vector<Piece> pieces;
+    for (int i = 0; i < nL; ++i) {
+        Piece p{i, 0, 0, 0.0};
+        for (long long k = 0; k < laneBlocks[i]; ++k) {
+            const double w = byWork ? kBlockRows + rows[i][k] : 1.0;
+            if (p.k1 > p.k0 && p.w + w > cap) {
+                pieces.push_back(p);
+                p = {i, k, k, 0.0};
+            }
+            p.k1 = k + 1;
+            p.w += w;
+        }
+        if (p.k1 > p.k0) { pieces.push_back(p); }
+    }
+    std::stable_sort(pieces.begin(), pieces.end(), [](const Piece &x, const Piece &y) { return x.w > y.w; });
+    std::vector<std::vector<mfk::Item>> bins(kXcd);
+    std::vector<double> load(kXcd, 0.0);
+    for (const Piece &p : pieces) {
+        const int b = static_cast<int>(std::min_element(load.begin(), load.end()) - load.begin());
+        for (long long k = p.k0; k < p.k1; ++k) {
+            bins[b].push_back({laneIds[p.lane], static_cast<int>(k), rows[p.lane][k], 0});
+        }
+        load[b] += p.w;
     }
     std::vector<size_t> pos(kXcd, 0);
     bool any = true;
