@@ -677,67 +677,70 @@ inline std::vector<Bucket> planBuckets(int precision, int userNfft, const std::v
 
 // Work items of a set of lanes sharing one FFT size, in XCD-aware order.
 // Workgroup i runs on XCD i mod 8, so the items are dealt from 8 bins in
-// turn and all items of a bin share one L2 (its lanes' template spectra
-// stay there).  Lanes (or pieces of them) go to the least-loaded bin,
-// largest first:
-//  - a launch of many lanes (at least 4 per XCD) balances the bins by
-//    block count, so they hold equal numbers of items and the dealing keeps
-//    every item on its bin's XCD;
-//  - a launch of few lanes balances them by work, a block weighing
-//    kBlockRows row units for its own part (load, statistics, forward
-//    transform) plus one per row it computes (rowsAt).
-// A lane of more than half (by blocks) or a quarter (by work) of a bin's
-// share is cut into consecutive block ranges of at most that size first,
-// so a launch of few lanes still spreads over every XCD (the pieces of one
-// lane then read its spectra into more than one L2).  rowsAt(i, k), when
-// given, is the number of leading rows of lane laneIds[i] with outputs in
-// its block k (the item's nRows); otherwise -1 (every row).
+// turn: while every bin has items, item i comes from bin i mod 8.
+//  - Lanes are first kept whole in one bin each, so their template spectra
+//    stay in one L2: lanes go to the bin with the fewest blocks, largest
+//    first, and the bins hold equal numbers of items.  A lane of more than
+//    half a bin's share is cut into block ranges first.
+//  - A launch of few lanes (under 4 per XCD) whose bins then differ in work
+//    by more than 5 % is dealt by work instead.  A block weighs kBlockRows
+//    row units for its own part (load, statistics, forward transform) plus
+//    one per row it computes (rowsAt).  The blocks, heaviest first, go to
+//    the bins in snake order (0..7, 7..0, ...), so the bins hold equal
+//    numbers of items (to one) and about equal work.  Each XCD then reads
+//    the spectra of every lane, which few lanes afford.
+// rowsAt(i, k), when given, is the number of leading rows of lane
+// laneIds[i] with outputs in its block k (the item's nRows); otherwise -1
+// (every row).
 inline void appendXcdItems(const std::vector<int> &laneIds, const std::vector<long long> &laneBlocks,
                            std::vector<mfk::Item> &items,
                            const std::function<int(int, long long)> &rowsAt = nullptr)
 {
     constexpr int kXcd = 8;
-    constexpr double kBlockRows = 6.0;  // measured per-block part over per-row part of a block: about 6
+    constexpr int kBlockRows = 6;  // measured per-block part over per-row part of a block: about 6
+    const int nL = static_cast<int>(laneIds.size());
+    std::vector<std::vector<mfk::Item>> bins(kXcd);
     struct Piece {
         int lane;
         long long k0, k1;
-        double w;
     };
-    const int nL = static_cast<int>(laneIds.size());
-    const bool byWork = rowsAt != nullptr && nL < 4 * kXcd;
-    std::vector<std::vector<int>> rows(nL);
-    double total = 0;
-    for (int i = 0; i < nL; ++i) {
-        rows[i].resize(static_cast<size_t>(laneBlocks[i]));
-        for (long long k = 0; k < laneBlocks[i]; ++k) {
-            rows[i][k] = rowsAt ? rowsAt(i, k) : -1;
-            total += byWork ? kBlockRows + rows[i][k] : 1.0;
-        }
-    }
-    const double cap = std::max(1.0, total / (byWork ? 4 * kXcd : 2 * kXcd));
+    long long total = 0;
+    for (long long nb : laneBlocks) { total += nb; }
+    const long long cap = std::max<long long>(1, total / (2 * kXcd));
     std::vector<Piece> pieces;
     for (int i = 0; i < nL; ++i) {
-        Piece p{i, 0, 0, 0.0};
-        for (long long k = 0; k < laneBlocks[i]; ++k) {
-            const double w = byWork ? kBlockRows + rows[i][k] : 1.0;
-            if (p.k1 > p.k0 && p.w + w > cap) {
-                pieces.push_back(p);
-                p = {i, k, k, 0.0};
-            }
-            p.k1 = k + 1;
-            p.w += w;
+        for (long long k0 = 0; k0 < laneBlocks[i]; k0 += cap) {
+            pieces.push_back({i, k0, std::min(laneBlocks[i], k0 + cap)});
         }
-        if (p.k1 > p.k0) { pieces.push_back(p); }
     }
-    std::stable_sort(pieces.begin(), pieces.end(), [](const Piece &x, const Piece &y) { return x.w > y.w; });
-    std::vector<std::vector<mfk::Item>> bins(kXcd);
-    std::vector<double> load(kXcd, 0.0);
+    std::stable_sort(pieces.begin(), pieces.end(),
+                     [](const Piece &x, const Piece &y) { return x.k1 - x.k0 > y.k1 - y.k0; });
+    std::vector<long long> load(kXcd, 0), work(kXcd, 0);
     for (const Piece &p : pieces) {
         const int b = static_cast<int>(std::min_element(load.begin(), load.end()) - load.begin());
         for (long long k = p.k0; k < p.k1; ++k) {
-            bins[b].push_back({laneIds[p.lane], static_cast<int>(k), rows[p.lane][k], 0});
+            const int r = rowsAt ? rowsAt(p.lane, k) : -1;
+            bins[b].push_back({laneIds[p.lane], static_cast<int>(k), r, 0});
+            work[b] += kBlockRows + std::max(r, 0);
         }
-        load[b] += p.w;
+        load[b] += p.k1 - p.k0;
+    }
+    long long workSum = 0, workMax = 0;
+    for (long long w : work) {
+        workSum += w;
+        workMax = std::max(workMax, w);
+    }
+    if (rowsAt != nullptr && nL < 4 * kXcd && workMax * kXcd > 1.05 * workSum) {
+        std::vector<std::pair<int, mfk::Item>> all;
+        for (auto &bin : bins) {
+            for (const mfk::Item &it : bin) { all.push_back({kBlockRows + it.nRows, it}); }
+            bin.clear();
+        }
+        std::stable_sort(all.begin(), all.end(), [](const auto &x, const auto &y) { return x.first > y.first; });
+        for (size_t q = 0; q < all.size(); ++q) {
+            const int c = static_cast<int>(q % kXcd);
+            bins[(q / kXcd) % 2 == 0 ? c : kXcd - 1 - c].push_back(all[q].second);
+        }
     }
     std::vector<size_t> pos(kXcd, 0);
     bool any = true;
