@@ -984,7 +984,8 @@ def main() -> int:
             "frac": achieved / HBM_PEAK, "traffic": traffic,
             "traffic_unit": "bytes per launch (rocprofv3 FETCH_SIZE x2.0 + WRITE_SIZE x1.0)",
             "traffic_source": traffic_src,
-            "kernel": f"{kernel}<float,{nfft // 2}> (FFT correlation + fused normalization)",
+            "kernel": (f"{kernel}<float,{nfft // 2}{',true' if kernel == 'corr2_kernel' else ''}> "
+                       "(FFT correlation + fused normalization)"),
             "algorithmic_bytes_per_corr_sample": 4.0 / E + 4.0,
             "avg_launch_ms": corr_avg,
         },
