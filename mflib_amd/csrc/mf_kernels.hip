@@ -1741,6 +1741,7 @@ __global__ __launch_bounds__(NT) void fix_kernel(const RowT *__restrict__ rows, 
     const gptr<const unsigned long long> e = to_global(reinterpret_cast<const unsigned long long *>(queue.entries));
     for (unsigned long long k = blockIdx.x * NT + threadIdx.x; k < n; k += static_cast<unsigned long long>(gridDim.x) * NT) {
         const unsigned long long v = e[k];
+        if (v == kFixEmpty) { continue; }  // a slot left by a reservation that straddled cap
         const RowT row = rows[static_cast<unsigned>(v) & 0x7fffffffu];
         const int pos = static_cast<int>(v >> 32);
         const gptr<const float> x = to_global(static_cast<const float *>(row.x)) + pos;

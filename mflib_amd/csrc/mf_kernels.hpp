@@ -52,8 +52,10 @@ struct Row {
 // entries, output index << 32 | row | cancelled << 31 (row: index into the
 // launch's row table).  The correlation launches append while *count stays
 // within cap (the rest are recomputed in place); *count is cleared before
-// them.  A null `entries` means no queue: every flagged window is
-// recomputed in place.
+// them.  A wave whose reservation straddles cap fills the slots it leaves
+// below cap with kFixEmpty, which the recomputation skips.  A null
+// `entries` means no queue: every flagged window is recomputed in place.
+constexpr unsigned long long kFixEmpty = ~0ull;
 struct FixQueue {
     unsigned long long *count;
     int *entries;  // 8 bytes per entry

@@ -522,6 +522,13 @@ __global__ __launch_bounds__(NT, (td_blocks_per_cu<MT>())) void tdcorr_kernel(co
                 }
                 continue;
             }
+            // The queue is full.  A reservation that straddles its end
+            // leaves the slots [q0, cap) unused: they get the empty entry,
+            // which fix_kernel skips (it reads min(count, cap) entries).
+            if (q0 < queue.cap) {
+                gptr<unsigned long long> e = to_global(reinterpret_cast<unsigned long long *>(queue.entries));
+                for (unsigned long long k = q0 + lane; k < queue.cap; k += 64) { e[k] = kFixEmpty; }
+            }
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
                 const bool mine = (keep >> r) & 1u;

@@ -468,3 +468,41 @@ def test_time_domain_many_rows_per_lane_match_oracle(variant):
         assert row.size == nout
         ref = O.pearson(t, x)[:nout]
         assert np.abs(row.astype(np.float64) - ref).max() < 1e-5, (r, len(t))
+
+
+@pytest.mark.parametrize("variant", [3, 4])
+def test_time_domain_flag_queue_overflow_matches_oracle(variant):
+    """More flagged windows than the time-domain flag queue holds: a signal
+    alternating loud noise with near-flat stretches (1e4 plus or minus one
+    ulp), whose windows lose their variance's digits without being exactly
+    constant, so every one is queued for exact recomputation.  The queue
+    holds max(windows / 32, 65536) entries; the rest are recomputed by the
+    MFMA kernel's own per-wave lists.  Every row within the FP32 contract of
+    the FP64 oracle."""
+    rng = np.random.default_rng(760 + variant)
+    n, period = 140_000, 700
+    x = np.empty(n, dtype=np.float32)
+    ulp = np.spacing(np.float32(1e4))
+    for p0 in range(0, n, period):
+        q = min(n, p0 + period // 2)
+        x[p0:q] = (rng.standard_normal(q - p0) * 1e3).astype(np.float32)
+        r = min(n, p0 + period)
+        x[q:r] = np.float32(1e4) + ulp * rng.integers(-1, 2, size=r - q).astype(np.float32)
+    lens = [int(v) for v in rng.integers(64, 257, size=16)]
+    tpls = [rng.standard_normal(L) * np.hanning(L) for L in lens]
+    h = capi.Handle(np.float32, "batched")
+    h.set_templates(tpls, shifts=[0] * len(tpls), weights=[1.0] * len(tpls))
+    h.set_batch_layout([n], [0] * len(tpls), [0] * len(tpls), 1)
+    h.set_kernel_variant(variant)
+    h.initialize()
+    h.set_signal(0, x)
+    h.apply()
+    nout = n - max(lens) + 1
+    ws = h.window_stats()
+    # far more than the queue holds (16 rows x 139,745 windows / 32 = 69,872)
+    assert ws["full"] > 300_000, ws
+    for r, t in enumerate(tpls):
+        assert h.plan(r)[0] == 0
+        row = h.rho(r)
+        ref = O.pearson(t, x)[:nout]
+        assert np.abs(row.astype(np.float64) - ref).max() < 1e-5, (r, len(t))
