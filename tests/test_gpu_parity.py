@@ -210,6 +210,38 @@ def test_kernel_variants_match_oracle(dtype, nfft):
         h.close()
 
 
+@pytest.mark.parametrize("slen", [1, 2, 3, 2047, 2048, 2049, 4097])
+def test_detect_stack_lengths_around_the_block(slen):
+    """mf_detect's one launch of stacks and candidates at stack lengths of 1,
+    2, 3 and around its 2048-output blocks: the stacks it wrote equal the
+    oracle's shiftAndStack bit for bit, and the peaks equal the oracle
+    PeakFinder's at threshold 0 (every local maximum, so the block edges'
+    recomputed neighbours and the end-point rules all matter)."""
+    rng = np.random.default_rng(900 + slen)
+    C, L = 3, 40
+    shifts = [0, 7, 3]
+    n = L - 1 + slen + max(shifts)
+    sigs = [(rng.standard_normal(n) + rng.uniform(-5, 5)).astype(np.float32) for _ in range(C)]
+    tpls = [rng.standard_normal(L) * np.hanning(L) for _ in range(C)]
+    weights = [0.5, 0.25, 0.25]
+    h = capi.Handle(np.float32, "batched")
+    h.set_templates(tpls, shifts=shifts, weights=weights)
+    h.set_batch_layout([n] * C, list(range(C)), [0] * C, 1)
+    h.initialize()
+    for c in range(C):
+        h.set_signal(c, sigs[c])
+    h.apply()
+    rows = [h.rho(r) for r in range(C)]
+    st_ref = O.shift_and_stack(np.stack(rows), shifts, weights)
+    assert st_ref.size == slen
+    peaks = h.detect(0.0, 0)
+    ptr, got_len = h.stack_device(0)
+    assert got_len == slen
+    assert np.array_equal(read_array(ptr, np.float32, slen), st_ref)
+    assert np.array_equal(peaks[0], O.peak_finder(st_ref, 0.0, 0))
+    h.close()
+
+
 def test_batched_layout_detect_matches_oracle():
     """Batched layout (templates of two groups share three channels): each
     row equals the single-instance result, mf_detect's stacks equal the
