@@ -134,6 +134,28 @@ def test_peak_finder_many_candidates_vs_oracle():
     assert len(O.peak_finder(x, 0.0, 0)) > 8192
 
 
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_peak_finder_device_unaligned_starts_and_short_lengths(dtype):
+    """The candidate scan reads whole 16-byte vectors from the first 16-byte
+    boundary on and tests the samples before it and after the last whole
+    vector one by one: signals starting 0-3 elements past a boundary, of
+    lengths 1-9, 63-65 and 4099, with plateaus, against the oracle."""
+    from hipmem import DeviceArray
+    rng = np.random.default_rng(31)
+    base = np.round(rng.uniform(0, 1, 5000), 1).astype(dtype)   # plateaus and ties
+    d = DeviceArray(base)
+    try:
+        step = np.dtype(dtype).itemsize
+        for off in range(4 if dtype == np.float32 else 2):
+            for n in list(range(1, 10)) + [63, 64, 65, 4099]:
+                x = base[off:off + n]
+                for thr, md in ((0.0, 0), (0.45, 3)):
+                    got = capi.peaks_device(d.ptr + off * step, n, dtype, thr, md)
+                    assert np.array_equal(got, O.peak_finder(x, thr, md)), (off, n, thr, md)
+    finally:
+        d.free()
+
+
 @pytest.mark.parametrize("seed", [1, 2, 3])
 def test_peak_finder_random_vs_oracle(seed):
     rng = np.random.default_rng(seed)
