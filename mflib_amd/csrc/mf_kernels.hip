@@ -1734,6 +1734,51 @@ hipError_t corr2_launch(const Item *dItems, int nItems, const Seg *dSegs, const 
 // making sum t d the centred numerator; |rho| > 1.1 gives 0 (the
 // reference's tooBig rule).  Rows of either table (FFT rows, time-domain
 // rows): their channel samples, FP64 template, output row and length.
+// Sums of one window over d = x[m] - shift, m = 0..Lt-1 in order (s1, s2)
+// and against the FP64 taps t (dn).  The taps are read two per 16-byte
+// load and, from global memory (GLOBAL), the samples four per load; the
+// addresses are only 8- and 4-byte aligned, which gfx950 serves for global
+// loads.  The sums keep the order of a tap-by-tap loop.
+typedef float f32x4u __attribute__((ext_vector_type(4), aligned(4)));
+typedef double f64x2u __attribute__((ext_vector_type(2), aligned(8)));
+template <bool GLOBAL, typename XPtr>
+__device__ __forceinline__ void window_tap_sums(XPtr x, gptr<const double> t, int Lt, double shift, double &s1,
+                                                double &s2, double &dn)
+{
+    s1 = 0;
+    s2 = 0;
+    dn = 0;
+    int m = 0;
+#pragma unroll 2
+    for (; m + 4 <= Lt; m += 4) {
+        const f64x2u ta = *reinterpret_cast<const __attribute__((address_space(1))) f64x2u *>(t + m);
+        const f64x2u tb = *reinterpret_cast<const __attribute__((address_space(1))) f64x2u *>(t + m + 2);
+        float xv[4];
+        if constexpr (GLOBAL) {
+            const f32x4u v = *reinterpret_cast<const __attribute__((address_space(1))) f32x4u *>(x + m);
+#pragma unroll
+            for (int e = 0; e < 4; ++e) { xv[e] = v[e]; }
+        } else {
+#pragma unroll
+            for (int e = 0; e < 4; ++e) { xv[e] = x[m + e]; }
+        }
+        const double tv[4] = {ta[0], ta[1], tb[0], tb[1]};
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            const double d = static_cast<double>(xv[e]) - shift;
+            s1 += d;
+            s2 = fma(d, d, s2);
+            dn = fma(tv[e], d, dn);
+        }
+    }
+    for (; m < Lt; ++m) {
+        const double d = static_cast<double>(x[m]) - shift;
+        s1 += d;
+        s2 = fma(d, d, s2);
+        dn = fma(t[m], d, dn);
+    }
+}
+
 template <typename RowT>
 __global__ __launch_bounds__(NT) void fix_kernel(const RowT *__restrict__ rows, FixQueue queue)
 {
@@ -1747,15 +1792,8 @@ __global__ __launch_bounds__(NT) void fix_kernel(const RowT *__restrict__ rows, 
         const gptr<const float> x = to_global(static_cast<const float *>(row.x)) + pos;
         const gptr<const double> t = to_global(row.tmpl);
         const int Lt = row.Lt;
-        const double shift = x[0];
-        double s1 = 0, s2 = 0, dn = 0;
-#pragma unroll 8
-        for (int m = 0; m < Lt; ++m) {
-            const double d = static_cast<double>(x[m]) - shift;
-            s1 += d;
-            s2 = fma(d, d, s2);
-            dn = fma(t[m], d, dn);
-        }
+        double s1, s2, dn;
+        window_tap_sums<true>(x, t, Lt, static_cast<double>(x[0]), s1, s2, dn);
         double rho = dn / fmax(DBL_EPSILON, sqrt(fmax(0.0, s2 - s1 * s1 / Lt)));
         if (fabs(rho) > 1.1) { rho = 0; }
         to_global(static_cast<float *>(row.out))[pos] = static_cast<float>(rho);
@@ -1803,15 +1841,8 @@ __global__ __launch_bounds__(NT) void fix_blocks_kernel(const Row *__restrict__ 
                     out[q] = 0.0f;
                     continue;
                 }
-                const double shift = xs[q];
-                double s1 = 0, s2 = 0, dn = 0;
-#pragma unroll 8
-                for (int m = 0; m < Lt; ++m) {
-                    const double d = static_cast<double>(xs[q + m]) - shift;
-                    s1 += d;
-                    s2 = fma(d, d, s2);
-                    dn = fma(t[m], d, dn);
-                }
+                double s1, s2, dn;
+                window_tap_sums<false>(xs + q, t, Lt, static_cast<double>(xs[q]), s1, s2, dn);
                 double rho = dn / fmax(DBL_EPSILON, sqrt(fmax(0.0, s2 - s1 * s1 / Lt)));
                 if (fabs(rho) > 1.1) { rho = 0; }
                 out[q] = static_cast<float>(rho);
@@ -1821,15 +1852,8 @@ __global__ __launch_bounds__(NT) void fix_blocks_kernel(const Row *__restrict__ 
         for (int q = tid; q < cnt; q += NT) {
             if (!isnan(out[q])) { continue; }
             const gptr<const float> x = xg + q;
-            const double shift = x[0];
-            double s1 = 0, s2 = 0, dn = 0;
-#pragma unroll 8
-            for (int m = 0; m < Lt; ++m) {
-                const double d = static_cast<double>(x[m]) - shift;
-                s1 += d;
-                s2 = fma(d, d, s2);
-                dn = fma(t[m], d, dn);
-            }
+            double s1, s2, dn;
+            window_tap_sums<true>(x, t, Lt, static_cast<double>(x[0]), s1, s2, dn);
             double rho = dn / fmax(DBL_EPSILON, sqrt(fmax(0.0, s2 - s1 * s1 / Lt)));
             if (fabs(rho) > 1.1) { rho = 0; }
             out[q] = static_cast<float>(rho);
