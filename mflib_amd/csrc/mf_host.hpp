@@ -682,6 +682,13 @@ inline std::vector<Bucket> planBuckets(int precision, int userNfft, const std::v
 //    stay in one L2: lanes go to the bin with the fewest blocks, largest
 //    first, and the bins hold equal numbers of items.  A lane of more than
 //    half a bin's share is cut into block ranges first.
+//  - A launch of many lanes (at least 4 per XCD) whose bins then differ in
+//    work by more than 1 % is binned by work instead: whole lanes, or
+//    pieces of at most a quarter of a bin's work, to the bin with the least
+//    work, heaviest first; then the shorter bins are padded with empty
+//    items (nRows 0, which the correlation kernels return from at once), so
+//    all bins hold equal numbers of items and every item stays on its
+//    lane's XCD.
 //  - A launch of few lanes (under 4 per XCD) whose bins then differ in work
 //    by more than 5 % is dealt by work instead.  A block weighs kBlockRows
 //    row units for its own part (load, statistics, forward transform) plus
@@ -730,7 +737,43 @@ inline void appendXcdItems(const std::vector<int> &laneIds, const std::vector<lo
         workSum += w;
         workMax = std::max(workMax, w);
     }
-    if (rowsAt != nullptr && nL < 4 * kXcd && workMax * kXcd > 1.05 * workSum) {
+    if (rowsAt != nullptr && nL >= 4 * kXcd && workMax * kXcd > 1.01 * workSum) {
+        for (auto &bin : bins) { bin.clear(); }
+        struct WPiece {
+            int lane;
+            long long k0, k1;
+            long long w;
+        };
+        const long long wcap = std::max<long long>(1, workSum / (4 * kXcd));
+        std::vector<WPiece> wp;
+        for (int i = 0; i < nL; ++i) {
+            WPiece p{i, 0, 0, 0};
+            for (long long k = 0; k < laneBlocks[i]; ++k) {
+                const long long w = kBlockRows + std::max(rowsAt(i, k), 0);
+                if (p.k1 > p.k0 && p.w + w > wcap) {
+                    wp.push_back(p);
+                    p = {i, k, k, 0};
+                }
+                p.k1 = k + 1;
+                p.w += w;
+            }
+            if (p.k1 > p.k0) { wp.push_back(p); }
+        }
+        std::stable_sort(wp.begin(), wp.end(), [](const WPiece &x, const WPiece &y) { return x.w > y.w; });
+        std::vector<long long> wl(kXcd, 0);
+        for (const WPiece &p : wp) {
+            const int b = static_cast<int>(std::min_element(wl.begin(), wl.end()) - wl.begin());
+            for (long long k = p.k0; k < p.k1; ++k) {
+                bins[b].push_back({laneIds[p.lane], static_cast<int>(k), rowsAt(p.lane, k), 0});
+            }
+            wl[b] += p.w;
+        }
+        size_t most = 0;
+        for (const auto &bin : bins) { most = std::max(most, bin.size()); }
+        for (auto &bin : bins) {
+            while (bin.size() < most) { bin.push_back({laneIds[0], 0, 0, 0}); }
+        }
+    } else if (rowsAt != nullptr && nL < 4 * kXcd && workMax * kXcd > 1.05 * workSum) {
         std::vector<std::pair<int, mfk::Item>> all;
         for (auto &bin : bins) {
             for (const mfk::Item &it : bin) { all.push_back({kBlockRows + it.nRows, it}); }

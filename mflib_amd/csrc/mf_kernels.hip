@@ -339,6 +339,7 @@ __global__ __launch_bounds__(NT) void corr_kernel(const Item *__restrict__ items
     __shared__ double red[NT / 64];
 
     const Item item = items[blockIdx.x];
+    if (item.nRows == 0) { return; }  // an empty item that keeps the XCD order
     const Seg sg = segs[item.seg];
     const int tid = threadIdx.x;
     const long long base = static_cast<long long>(item.block) * sg.L;
@@ -673,6 +674,7 @@ __global__ __launch_bounds__(NT, (Corr2Shape<T, M>::BLOCKS)) void corr2_kernel(c
     double *ts2 = sizeof(T) == 8 ? tsd2 : tp2;
 
     const Item item = items[blockIdx.x];
+    if (item.nRows == 0) { return; }  // an empty item that keeps the XCD order
     const Seg sg = segs[item.seg];
     const int tid = threadIdx.x;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);  // wave-uniform: scalar row loads and branches

@@ -1,10 +1,10 @@
 // CPU check of mfh::appendXcdItems (mflib_amd/csrc/mf_host.hpp), the order
 // of a launch's work items over the 8 XCDs (workgroup i runs on XCD i mod
 // 8).  For generated sets of lanes it checks that every block of every lane
-// appears exactly once with its rowsAt count, that a launch of many lanes
-// keeps all items of an unsplit lane on one XCD, that a launch of few
-// lanes gives every XCD about the same work, and that equal lanes stay on
-// one XCD each.  Prints "ok" or the first
+// appears exactly once with its rowsAt count (empty items only pad the
+// bins of a many-lane launch), that a launch of many lanes keeps all items
+// of an unsplit lane on one XCD, that the XCDs get about the same work,
+// and that equal lanes stay on one XCD each.  Prints "ok" or the first
 // failure and exits non-zero.
 #include <cstdio>
 #include <cstdlib>
@@ -48,15 +48,24 @@ int main()
             } else {
                 mfh::appendXcdItems(laneIds, blocks, items);
             }
-            long long total = 0;
-            for (long long b : blocks) { total += b; }
-            if (static_cast<long long>(items.size()) != total) { return fail("item count", seed); }
+            long long total = 0, totalWork = 0;
+            for (int i = 0; i < nL; ++i) {
+                total += blocks[i];
+                for (long long k = 0; k < blocks[i]; ++k) { totalWork += 6 + rowsAt(i, k); }
+            }
             std::map<int, int> laneOf;
             for (int i = 0; i < nL; ++i) { laneOf[laneIds[i]] = i; }
             std::set<std::pair<int, int>> seen;
             std::map<int, std::set<int>> xcds;  // lane -> XCDs its items run on
+            double xcdWork[8] = {};
+            long long padding = 0;
             for (size_t p = 0; p < items.size(); ++p) {
                 const mfk::Item &it = items[p];
+                if (it.nRows == 0) {  // an empty item: only to pad the bins of a many-lane launch
+                    if (!withRows || nL < 32) { return fail("empty item", seed); }
+                    ++padding;
+                    continue;
+                }
                 const auto li = laneOf.find(it.seg);
                 if (li == laneOf.end()) { return fail("unknown lane", seed); }
                 if (it.block < 0 || it.block >= blocks[li->second]) { return fail("block out of range", seed); }
@@ -64,8 +73,27 @@ int main()
                 const int want = withRows ? rowsAt(li->second, it.block) : -1;
                 if (it.nRows != want) { return fail("nRows", seed); }
                 xcds[li->second].insert(static_cast<int>(p % 8));
+                xcdWork[p % 8] += 6.0 + std::max(it.nRows, 0);
             }
-            if (nL >= 32) {
+            if (static_cast<long long>(items.size()) - padding != total) { return fail("item count", seed); }
+            if (padding > 0 && items.size() % 8 != 0) { return fail("padded bins unequal", seed); }
+            if (withRows && nL >= 32 && padding > 0) {
+                // padded work bins: every lane under a quarter of a bin's work
+                // stays on one XCD, and the XCDs' work is within a quarter of the
+                // mean
+                double sum = 0, mx = 0;
+                for (double v : xcdWork) {
+                    sum += v;
+                    mx = std::max(mx, v);
+                }
+                if (mx > 1.25 * sum / 8) { return fail("XCD work unbalanced (many lanes)", seed); }
+                for (int i = 0; i < nL; ++i) {
+                    long long w = 0;
+                    for (long long k = 0; k < blocks[i]; ++k) { w += 6 + rowsAt(i, k); }
+                    if (w <= totalWork / 32 && xcds[i].size() > 1) { return fail("padded lane spread", seed); }
+                }
+            }
+            if (nL >= 32 && padding == 0) {
                 // balanced by blocks: a lane under half an XCD's share is never cut,
                 // and its items stay on one XCD unless they fall in the tail after
                 // the shortest bin ran out
