@@ -498,6 +498,40 @@ def test_time_domain_mfma_path_matches_oracle(variant, tol):
 
 
 @pytest.mark.parametrize("variant", [3, 4])
+def test_time_domain_short_tap_lanes_match_oracle(variant):
+    """Lanes whose longest template pads to 64 and to 128 taps (the kernel's
+    KT = 64 and 128 forms: 2 and 4 k-steps of 32), with a DC offset and a
+    near-flat stretch: every row within the FP32 contract of the FP64
+    oracle."""
+    rng = np.random.default_rng(780 + variant)
+    n = 30_000
+    sigs = []
+    for c in range(2):
+        x = (rng.standard_normal(n) * 2 + 500.0 * (c + 1)).astype(np.float32)
+        x[9000:9600] = np.float32(500.0 * (c + 1)) + (rng.standard_normal(600) * 1e-3).astype(np.float32)
+        sigs.append(x)
+    lens = [[16, 33, 50, 64, 64], [65, 80, 100, 127, 128]]   # channel 0: KT 64; channel 1: KT 128
+    tpls, rch = [], []
+    for c in range(2):
+        for L in lens[c]:
+            tpls.append(rng.standard_normal(L) * np.hanning(L))
+            rch.append(c)
+    h = capi.Handle(np.float32, "batched")
+    h.set_templates(tpls, shifts=[0] * len(tpls), weights=[1.0] * len(tpls))
+    h.set_batch_layout([n, n], rch, [0] * len(tpls), 1)
+    h.set_kernel_variant(variant)
+    h.initialize()
+    for c in range(2):
+        h.set_signal(c, sigs[c])
+    h.apply()
+    for r, (t, c) in enumerate(zip(tpls, rch)):
+        assert h.plan(r)[0] == 0
+        row = h.rho(r)
+        ref = O.pearson(t, sigs[c])[:row.size]
+        assert np.abs(row.astype(np.float64) - ref).max() < 1e-5, (r, len(t))
+
+
+@pytest.mark.parametrize("variant", [3, 4])
 def test_time_domain_many_rows_per_lane_match_oracle(variant):
     """150 short templates of 20-256 taps on one channel: work items of 128
     and 22 rows, so every wave of a block holds a partial or empty row set
