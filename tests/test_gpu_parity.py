@@ -232,8 +232,9 @@ def test_kernel_variants_match_oracle(dtype, nfft):
         h.close()
 
 
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
 @pytest.mark.parametrize("slen", [1, 2, 3, 2047, 2048, 2049, 4097])
-def test_detect_stack_lengths_around_the_block(slen):
+def test_detect_stack_lengths_around_the_block(slen, dtype):
     """mf_detect's one launch of stacks and candidates at stack lengths of 1,
     2, 3 and around its 2048-output blocks: the stacks it wrote equal the
     oracle's shiftAndStack bit for bit, and the peaks equal the oracle
@@ -243,10 +244,10 @@ def test_detect_stack_lengths_around_the_block(slen):
     C, L = 3, 40
     shifts = [0, 7, 3]
     n = L - 1 + slen + max(shifts)
-    sigs = [(rng.standard_normal(n) + rng.uniform(-5, 5)).astype(np.float32) for _ in range(C)]
+    sigs = [(rng.standard_normal(n) + rng.uniform(-5, 5)).astype(dtype) for _ in range(C)]
     tpls = [rng.standard_normal(L) * np.hanning(L) for _ in range(C)]
     weights = [0.5, 0.25, 0.25]
-    h = capi.Handle(np.float32, "batched")
+    h = capi.Handle(dtype, "batched")
     h.set_templates(tpls, shifts=shifts, weights=weights)
     h.set_batch_layout([n] * C, list(range(C)), [0] * C, 1)
     h.initialize()
@@ -259,7 +260,7 @@ def test_detect_stack_lengths_around_the_block(slen):
     peaks = h.detect(0.0, 0)
     ptr, got_len = h.stack_device(0)
     assert got_len == slen
-    assert np.array_equal(read_array(ptr, np.float32, slen), st_ref)
+    assert np.array_equal(read_array(ptr, dtype, slen), st_ref)
     assert np.array_equal(peaks[0], O.peak_finder(st_ref, 0.0, 0))
     h.close()
 
